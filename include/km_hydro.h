@@ -1,0 +1,240 @@
+/*
+ * km_hydro.h -- C ABI of libkm_hydro.so, the MI355X (gfx950) implementation of
+ * kamayan's hydro/MHD flux-and-update path.
+ *
+ * Every entry point below replaces one task of the reference's hydro unit or grid
+ * update; the reference interface it stands in for is cited beside it
+ * (paths relative to the kamayan source tree).
+ *
+ * Conventions
+ *  - All pointers held in km_pack_t (and the dt pointers) are DEVICE pointers for the
+ *    km_* functions; the CPU twin used for checking takes host pointers in the same
+ *    structs.
+ *  - Field layout is Parthenon's MeshData block pack: pack(b, var, k, j, i), one
+ *    (E3, E2, E1) array per (block, variable), i fastest, with ng ghost zones on each
+ *    side of every active dimension (E = nx + 2*ng; E = 1 for inactive dimensions).
+ *  - The flux through face i-1/2 of cell i is stored at index i of the flux array,
+ *    which has the shape of the cell array (hydro_add_flux_tasks.cpp:99-103).
+ *  - Face- and edge-centred arrays (CT face B and EMFs) all use one staggered shape
+ *    (E3 + (ndim>2), E2 + (ndim>1), E1 + 1); element (k, j, i) of an x1-face array is
+ *    the face at x1_{i-1/2} of cell (k, j, i).
+ *  - Return value 0 == TaskStatus::complete; nonzero codes below; km_last_error()
+ *    gives a thread-local message.
+ *  - Kernels are enqueued on the given hipStream_t (passed as void*; NULL = default
+ *    stream) and do not block the host, except where a function says so.
+ */
+#ifndef KM_HYDRO_H_
+#define KM_HYDRO_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------------ */
+#define KM_OK 0
+#define KM_EINVAL 1  /* unsupported option / bad argument (dispatcher.hpp:122-130) */
+#define KM_EHIP 2    /* HIP runtime error */
+#define KM_ENOMEM 3  /* allocation failed */
+#define KM_ENODEV 4  /* no GPU visible */
+
+/* ---- runtime options; value order follows the reference's POLYMORPHIC_PARM -------- */
+enum { KM_RECON_FOG = 0, KM_RECON_PLM = 1, KM_RECON_PPM = 2, KM_RECON_WENOZ = 3 }; /* hydro_types.hpp:18 */
+enum { KM_LIM_MINMOD = 0, KM_LIM_VAN_LEER = 1, KM_LIM_MC = 2 };                  /* hydro_types.hpp:19 */
+enum { KM_RIEMANN_HLL = 0, KM_RIEMANN_HLLC = 1, KM_RIEMANN_HLLD = 2 };           /* hydro_types.hpp:20 */
+enum { KM_RECONVARS_PRIMITIVE = 0 };                                             /* hydro_types.hpp:21 */
+enum { KM_STRATEGY_SCRATCHPAD = 0, KM_STRATEGY_SCRATCHVAR = 1 };                 /* hydro_types.hpp:22 */
+enum { KM_EMF_ARITHMETIC = 0 };                                                  /* hydro_types.hpp:24 */
+enum { KM_FLUID_ONET = 0, KM_FLUID_THREET = 1 };                                 /* physics_types.hpp:7 */
+enum { KM_MHD_OFF = 0, KM_MHD_CT = 1 };                                          /* physics_types.hpp:8 */
+enum { KM_GEOM_CARTESIAN = 0, KM_GEOM_CYLINDRICAL = 1 };                         /* geometry_types.hpp:5 */
+/* FP64: every operation in FP64, rounded in the reference's order (-ffp-contract=off);
+ * FP32_MIXED: reconstruction + Riemann in FP32, everything else FP64;
+ * FP64_FMA: FP64 with fused multiply-adds (agreement with FP64 to ~1e-14 relative). */
+enum { KM_PREC_FP64 = 0, KM_PREC_FP32_MIXED = 1, KM_PREC_FP64_FMA = 2 };
+enum { KM_BC_PERIODIC = 0, KM_BC_OUTFLOW = 1, KM_BC_REFLECT = 2 };
+
+/* Parsed option set. Built from the reference's input-deck keys by km_cfg_set /
+ * km_cfg_parse_deck (hydro.cpp:55-90, physics.cpp:18-24, kamayan_driver.cpp:25-27,
+ * eos.cpp:37-39, grid.cpp:35-37). */
+typedef struct km_cfg_t {
+  int32_t recon;         /* hydro/reconstruction         default fog    */
+  int32_t limiter;       /* hydro/slope_limiter          default minmod */
+  int32_t riemann;       /* hydro/riemann                default hll    */
+  int32_t recon_vars;    /* hydro/ReconstructionVars     default primitive */
+  int32_t strategy;      /* hydro/ReconstructionStrategy default scratchpad */
+  int32_t emf_averaging; /* hydro/EMF_averaging          default arithmetic */
+  int32_t fluid;         /* physics/fluid                default oneT   */
+  int32_t mhd;           /* physics/MHD                  default off    */
+  int32_t geometry;      /* geometry/geometry            default cartesian */
+  int32_t nstages;       /* parthenon/time/integrator rk1|rk2|rk3 -> 1|2|3, default rk2 */
+  int32_t precision;     /* kamayan_amd/precision fp64|fp32_mixed|fp64_fma (this library's own key) */
+  int32_t ndim;          /* from parthenon/mesh nx2, nx3 */
+  double gamma;          /* material/single/gamma        default 5/3    */
+  double abar;           /* material/single/Abar         default 1      */
+  double cfl;            /* hydro/cfl                    default 0.8    */
+} km_cfg_t;
+
+/* ---- cell-variable slots of a hydro state pack ------------------------------------
+ * Conserved first (the reference's Conserved TypeList order, hydro_types.hpp:61-74):
+ *   DENS, MOMENTUM[3], ENER, and MAGC[3] when MHD == ct   -> ncons = 5 or 8
+ * then the primitives / derived fields (NonFlux + EOS outputs, hydro_types.hpp:65;
+ * eos.cpp:67):
+ *   VELOCITY[3], PRES, BMOD, EINT, TEMP, and DIVB when MHD == ct */
+#define KM_DENS 0
+#define KM_MOM0 1
+#define KM_ENER 4
+#define KM_MAGC0 5
+#define KM_NCONS(mhd) ((mhd) == KM_MHD_CT ? 8 : 5)
+#define KM_VEL0(mhd) (KM_NCONS(mhd) + 0)
+#define KM_PRES(mhd) (KM_NCONS(mhd) + 3)
+#define KM_BMOD(mhd) (KM_NCONS(mhd) + 4)
+#define KM_EINT(mhd) (KM_NCONS(mhd) + 5)
+#define KM_TEMP(mhd) (KM_NCONS(mhd) + 6)
+#define KM_DIVB(mhd) (KM_NCONS(mhd) + 7)
+#define KM_NVAR(mhd) ((mhd) == KM_MHD_CT ? 16 : 12)
+
+/* Per-block uniform-Cartesian metric: the values parthenon::UniformCartesian gives
+ * to Dx<axis>(), FaceArea<axis>() and CellVolume() (geometry.hpp:35-167). */
+typedef struct km_block_t {
+  double xmin[3]; /* lower corner of the block's interior */
+  double dx[3];   /* cell widths                          */
+  double area[3]; /* dx2*dx3, dx1*dx3, dx1*dx2           */
+  double vol;     /* dx1*dx2*dx3                          */
+  int32_t level;  /* refinement level, 0 = root grid      */
+  int32_t lx[3];  /* logical location at that level       */
+} km_block_t;
+
+/* MeshData block pack (grid.hpp:36-131). Tables are indexed [b*nvar + v] for cell,
+ * [b*nflux + v] for flux, [b] for face/edge; unused tables may be NULL. */
+typedef struct km_pack_t {
+  int32_t nblocks;
+  int32_t ndim;
+  int32_t ng;      /* ghost zones per side (parthenon/mesh/nghost, grid.cpp:56) */
+  int32_t nx[3];   /* interior cells per block along x1, x2, x3 (1 if inactive) */
+  int32_t nvar;    /* entries per block in cell[] */
+  int32_t nflux;   /* entries per block in flux[d] */
+  const km_block_t *blocks;
+  double *const *cell;
+  double *const *flux[3];
+  double *const *face[3];
+  double *const *edge[3];
+} km_pack_t;
+
+/* ---- configuration ---------------------------------------------------------------- */
+/* Reference defaults (hydro.cpp:55-90; eos.cpp:37-39; kamayan_driver.cpp:25-27). */
+int km_cfg_default(km_cfg_t *cfg);
+/* Set one "block/key" = value, e.g. ("hydro/reconstruction", "wenoz"). Unknown keys
+ * are ignored (return KM_OK, as Parthenon keeps unrelated deck keys); an invalid value
+ * for a known key returns KM_EINVAL like the reference's rule check
+ * (runtime_parameters.hpp:157-169). */
+int km_cfg_set(km_cfg_t *cfg, const char *block_key, const char *value);
+/* Parse an input deck ("<block>" headers, "key = value" lines, '#' comments). */
+int km_cfg_parse_deck(km_cfg_t *cfg, const char *deck_text);
+/* Minimum ghost width the options need (python/kamayan/code_units/Hydro.py:14,35;
+ * +1 for CT, hydro_add_flux_tasks.cpp:59-70). */
+int km_cfg_min_ghost(const km_cfg_t *cfg);
+
+/* ---- hot-path tasks (device pointers; enqueued on `stream`) --------------------------- */
+/* hydro::CalculateFluxes -- Dispatcher<CalculateFluxesNested|Scratch>
+ * (hydro_add_flux_tasks.cpp:29-385, called from AddFluxTasks :444-476).
+ * Reads the reconstructed set DENS, VELOCITY, PRES, BMOD, EINT (+MAGC, face B for CT)
+ * from u->cell / u->face and writes u->flux[d] for every conserved variable. */
+int km_calculate_fluxes(const km_cfg_t *cfg, const km_pack_t *u, void *stream);
+/* hydro::CalculateEMF (hydro_add_flux_tasks.cpp:387-442): edge EMFs from the MAGC
+ * face fluxes into u->edge[d]. No-op unless MHD == ct and ndim >= 2. */
+int km_calculate_emf(const km_cfg_t *cfg, const km_pack_t *u, void *stream);
+/* grid::FluxesToDuDt (grid.cpp:179-211 -> FluxDivergence grid_update.hpp:15-57 and
+ * FluxStokes grid_update.hpp:80-137): dudt->cell[v] for the conserved cell vars and,
+ * for CT, dudt->face[d]. */
+int km_fluxes_to_dudt(const km_cfg_t *cfg, const km_pack_t *u, const km_pack_t *dudt,
+                      void *stream);
+/* grid::ApplyDuDt (grid.cpp:213-283):
+ *   u0 <- beta*base + (1-beta)*u0 ;  u1 <- u0 + (beta*dt)*dudt
+ * over the interior of every conserved cell var and (CT) every face element. */
+int km_apply_dudt(const km_cfg_t *cfg, const km_pack_t *base, const km_pack_t *u0,
+                  const km_pack_t *u1, const km_pack_t *dudt, double beta, double dt,
+                  void *stream);
+/* PreparePrimitive callbacks in DAG order: hydro PreparePrimitive_impl
+ * (primconsflux.cpp:82-142) then eos EosWrapped(ener) (eos.cpp:71-114,164). */
+int km_prepare_primitive(const km_cfg_t *cfg, const km_pack_t *u, void *stream);
+/* PostMeshInitialization in DAG order: eos PrepareConserved with mode_init=dens_pres
+ * (eos.cpp:165-168) then hydro ConvertToConserved_impl (primconsflux.cpp:19-79). */
+int km_prepare_conserved(const km_cfg_t *cfg, const km_pack_t *u, void *stream);
+/* hydro EstimateTimeStep (hydro_time_step.cpp:19-65): writes (cfl/ndim)*min(dx/(|v|+cf))
+ * over the interior to *dt_dev (device). Does not block the host. */
+int km_estimate_dt(const km_cfg_t *cfg, const km_pack_t *u, double *dt_dev, void *stream);
+/* hydro FillDerived_impl (hydro.cpp:150-194): DIVB from face B (CT only). */
+int km_fill_divb(const km_cfg_t *cfg, const km_pack_t *u, void *stream);
+
+/* ---- ghost fill (Parthenon AddBoundaryExchangeTasks, kamayan_driver.cpp:150-151) ---- */
+typedef struct km_ghost_plan_t km_ghost_plan_t; /* opaque, device-resident */
+/* Same-level ghost fill plan for a uniform grid of nb[0] x nb[1] x nb[2] blocks stored
+ * in x-fastest order; bc = {ix1, ox1, ix2, ox2, ix3, ox3} of KM_BC_*. */
+int km_ghost_plan_uniform(const km_pack_t *shape, const int32_t nb[3], const int32_t bc[6],
+                          km_ghost_plan_t **plan);
+int km_ghost_plan_destroy(km_ghost_plan_t *plan);
+/* Fill every ghost zone (faces, edges, corners) of cell vars [v0, v1) and, when
+ * with_faces != 0, of the CT face arrays. */
+int km_ghost_fill(const km_ghost_plan_t *plan, const km_pack_t *u, int32_t v0, int32_t v1,
+                  int32_t with_faces, void *stream);
+
+/* ---- whole-mesh driver (KamayanDriver RK stages, kamayan_driver.cpp:121-248) ----- */
+typedef struct km_mesh_desc_t {
+  int32_t ndim;
+  int32_t ng;
+  int32_t nx[3];    /* cells per block  (parthenon/meshblock/nx*) */
+  int32_t nb[3];    /* blocks per dimension on the root grid */
+  double xmin[3];   /* parthenon/mesh/x*min */
+  double xmax[3];   /* parthenon/mesh/x*max */
+  int32_t bc[6];    /* ix1, ox1, ix2, ox2, ix3, ox3 */
+} km_mesh_desc_t;
+
+enum {
+  KM_PATH_TASKS = 0, /* one kernel per reference task, same sequence as BuildTaskListRKStage */
+  KM_PATH_FUSED = 1  /* fused directional sweeps per RK stage (hydro) */
+};
+
+typedef struct km_sim_t km_sim_t; /* opaque: device mesh, stage containers, dt state */
+int km_sim_create(const km_cfg_t *cfg, const km_mesh_desc_t *mesh, int32_t path,
+                  km_sim_t **sim);
+int km_sim_destroy(km_sim_t *sim);
+/* Host buffers in pack layout: cell = [nblocks][KM_NVAR][E3][E2][E1], face[d] =
+ * [nblocks][S3][S2][S1] (NULL unless CT). km_sim_set_primitive takes DENS, VELOCITY,
+ * PRES (+MAGC, face B) on the interior, as a problem generator leaves them, and runs
+ * PostMeshInitialization, the initial ghost fill and the initial dt estimate. */
+int km_sim_set_primitive(km_sim_t *sim, const double *cell, const double *const face[3]);
+int km_sim_get_state(km_sim_t *sim, double *cell, double *const face[3]);
+/* Advance ncycles cycles (or until t >= tlim if tlim > 0). Blocks the host. */
+int km_sim_advance(km_sim_t *sim, int32_t ncycles, double tlim);
+int km_sim_info(const km_sim_t *sim, double *time, double *dt, int32_t *ncycle);
+/* Override the next step's dt (used to replay a fixed dt sequence). */
+int km_sim_set_dt(km_sim_t *sim, double dt);
+
+/* ---- problem generators (host-side; fill primitives on the interior) -------------- */
+/* names: "isentropic_vortex" (isentropic_vortex.cpp:105-175), "smooth3d", "mhd_random",
+ * "kelvin_helmholtz", "sedov" (sedov.cpp:75-143, 3-D form). Buffers as for
+ * km_sim_set_primitive; `seed` feeds the random modes where a problem has them. */
+int km_problem_fill(const char *name, const km_cfg_t *cfg, const km_mesh_desc_t *mesh,
+                    uint64_t seed, const double *params, int32_t nparams, double *cell,
+                    double *const face[3]);
+/* Block metric table for a uniform mesh (blocks x-fastest). */
+int km_mesh_blocks(const km_mesh_desc_t *mesh, km_block_t *blocks_host);
+
+/* ---- device memory helpers (so callers need no other runtime) --------------------- */
+int km_device_count(int32_t *count);
+int km_malloc(void **ptr, size_t bytes);
+int km_free(void *ptr);
+int km_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int km_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int km_synchronize(void);
+
+const char *km_last_error(void);
+const char *km_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KM_HYDRO_H_ */

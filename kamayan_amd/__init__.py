@@ -1,0 +1,298 @@
+"""kamayan_amd -- MI355X (gfx950) hydro/MHD flux-and-update path of kamayan.
+
+Python is only the test/benchmark harness here: every computation runs in
+libkm_hydro.so (C ABI declared in include/km_hydro.h). Loading fails loudly when the
+library has not been built; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+
+import numpy as np
+
+from ._abi import (BC, KM_OK, LIMITER, MHD, PATH_FUSED, PATH_TASKS, PRECISION, RECON,
+                   RIEMANN, KmBlock, KmCfg, KmMeshDesc, KmPack, Layout, f64ptr, mesh_desc,
+                   ncons, nvar, slot)
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = HERE / "libkm_hydro.so"
+
+__all__ = [
+    "lib", "Config", "Simulation", "DevicePack", "problem_fill", "Layout", "mesh_desc",
+    "KmCfg", "KmPack", "KmBlock", "KmMeshDesc", "ncons", "nvar", "slot", "PATH_TASKS",
+    "PATH_FUSED", "LIB_PATH", "device_count", "block_table",
+]
+
+_lib = None
+
+
+class KmError(RuntimeError):
+    pass
+
+
+def lib() -> C.CDLL:
+    """The loaded libkm_hydro.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build() or "
+                          "python kamayan_amd/build.py")
+    L = C.CDLL(str(LIB_PATH))
+    vp, i32, dbl, u64 = C.c_void_p, C.c_int32, C.c_double, C.c_uint64
+    P = C.POINTER
+    sig = {
+        "km_cfg_default": [P(KmCfg)],
+        "km_cfg_set": [P(KmCfg), C.c_char_p, C.c_char_p],
+        "km_cfg_parse_deck": [P(KmCfg), C.c_char_p],
+        "km_cfg_min_ghost": [P(KmCfg)],
+        "km_calculate_fluxes": [P(KmCfg), P(KmPack), vp],
+        "km_calculate_emf": [P(KmCfg), P(KmPack), vp],
+        "km_fluxes_to_dudt": [P(KmCfg), P(KmPack), P(KmPack), vp],
+        "km_apply_dudt": [P(KmCfg), P(KmPack), P(KmPack), P(KmPack), P(KmPack), dbl, dbl, vp],
+        "km_prepare_primitive": [P(KmCfg), P(KmPack), vp],
+        "km_prepare_conserved": [P(KmCfg), P(KmPack), vp],
+        "km_estimate_dt": [P(KmCfg), P(KmPack), vp, vp],
+        "km_fill_divb": [P(KmCfg), P(KmPack), vp],
+        "km_ghost_plan_uniform": [P(KmPack), P(i32), P(i32), P(vp)],
+        "km_ghost_plan_destroy": [vp],
+        "km_ghost_fill": [vp, P(KmPack), i32, i32, i32, vp],
+        "km_sim_create": [P(KmCfg), P(KmMeshDesc), i32, P(vp)],
+        "km_sim_destroy": [vp],
+        "km_sim_set_primitive": [vp, vp, P(vp)],
+        "km_sim_get_state": [vp, vp, P(vp)],
+        "km_sim_advance": [vp, i32, dbl],
+        "km_sim_info": [vp, P(dbl), P(dbl), P(i32)],
+        "km_sim_set_dt": [vp, dbl],
+        "km_problem_fill": [C.c_char_p, P(KmCfg), P(KmMeshDesc), u64, P(dbl), i32, vp, P(vp)],
+        "km_mesh_blocks": [P(KmMeshDesc), vp],
+        "km_device_count": [P(i32)],
+        "km_malloc": [P(vp), C.c_size_t],
+        "km_free": [vp],
+        "km_memcpy_h2d": [vp, vp, C.c_size_t],
+        "km_memcpy_d2h": [vp, vp, C.c_size_t],
+        "km_synchronize": [],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    L.km_last_error.restype = C.c_char_p
+    L.km_last_error.argtypes = []
+    L.km_version.restype = C.c_char_p
+    L.km_version.argtypes = []
+    _lib = L
+    return L
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != KM_OK:
+        msg = lib().km_last_error().decode()
+        raise KmError(f"{what} failed (code {rc}): {msg}")
+
+
+def device_count() -> int:
+    n = C.c_int32(0)
+    lib().km_device_count(C.byref(n))
+    return n.value
+
+
+class Config:
+    """km_cfg_t built from the reference's input-deck keys."""
+
+    def __init__(self, deck: str | None = None, **keys):
+        self.c = KmCfg()
+        check(lib().km_cfg_default(C.byref(self.c)), "km_cfg_default")
+        if deck:
+            check(lib().km_cfg_parse_deck(C.byref(self.c), deck.encode()), "parse deck")
+        for k, v in keys.items():
+            self.set(k.replace("__", "/"), v)
+
+    def set(self, key: str, value) -> "Config":
+        check(lib().km_cfg_set(C.byref(self.c), key.encode(), str(value).encode()), key)
+        return self
+
+    def __getattr__(self, name):
+        return getattr(self.__dict__["c"], name)
+
+
+def block_table(layout: Layout, xmin, xmax) -> np.ndarray:
+    m = mesh_desc(layout, xmin, xmax)
+    arr = (KmBlock * layout.nblocks)()
+    check(lib().km_mesh_blocks(C.byref(m), C.cast(arr, C.c_void_p)), "km_mesh_blocks")
+    return arr
+
+
+def problem_fill(name: str, cfg: Config, desc: KmMeshDesc, layout: Layout, seed: int = 0,
+                 params=None):
+    """Primitive initial state (cell array, face arrays or None) of a problem generator."""
+    mhd = cfg.c.mhd
+    cell = np.zeros(layout.cell_shape(nvar(mhd)))
+    faces = None
+    face_ptrs = (C.c_void_p * 3)()
+    if mhd == 1:
+        faces = [np.zeros(layout.stag_shape()) for _ in range(3)]
+        for d in range(3):
+            face_ptrs[d] = faces[d].ctypes.data
+    par = None
+    npar = 0
+    if params is not None:
+        params = np.ascontiguousarray(params, dtype=np.float64)
+        par = params.ctypes.data_as(C.POINTER(C.c_double))
+        npar = len(params)
+    check(lib().km_problem_fill(name.encode(), C.byref(cfg.c), C.byref(desc), seed, par, npar,
+                                f64ptr(cell), face_ptrs), f"problem {name}")
+    return cell, faces
+
+
+class Simulation:
+    """Whole-mesh driver on the GPU (km_sim_*)."""
+
+    def __init__(self, cfg: Config, desc: KmMeshDesc, layout: Layout, path: int = PATH_TASKS):
+        self.cfg, self.desc, self.layout = cfg, desc, layout
+        self.h = C.c_void_p()
+        check(lib().km_sim_create(C.byref(cfg.c), C.byref(desc), path, C.byref(self.h)),
+              "km_sim_create")
+
+    def close(self):
+        if self.h:
+            lib().km_sim_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_primitive(self, cell: np.ndarray, faces=None):
+        fp = (C.c_void_p * 3)()
+        if faces is not None:
+            for d in range(3):
+                fp[d] = faces[d].ctypes.data
+        check(lib().km_sim_set_primitive(self.h, f64ptr(cell), fp), "set_primitive")
+
+    def state(self):
+        mhd = self.cfg.c.mhd
+        cell = np.zeros(self.layout.cell_shape(nvar(mhd)))
+        fp = (C.c_void_p * 3)()
+        faces = None
+        if mhd == 1:
+            faces = [np.zeros(self.layout.stag_shape()) for _ in range(3)]
+            for d in range(3):
+                fp[d] = faces[d].ctypes.data
+        check(lib().km_sim_get_state(self.h, f64ptr(cell), fp), "get_state")
+        return cell, faces
+
+    def advance(self, ncycles: int, tlim: float = 0.0):
+        check(lib().km_sim_advance(self.h, ncycles, tlim), "advance")
+
+    def set_dt(self, dt: float):
+        check(lib().km_sim_set_dt(self.h, dt), "set_dt")
+
+    def info(self):
+        t, dt, n = C.c_double(), C.c_double(), C.c_int32()
+        check(lib().km_sim_info(self.h, C.byref(t), C.byref(dt), C.byref(n)), "info")
+        return t.value, dt.value, n.value
+
+
+class DevicePack:
+    """Device copies of block-pack arrays plus the pointer tables a km_pack_t needs."""
+
+    def __init__(self, layout: Layout, nv: int, nflux: int = 0, faces: bool = False,
+                 edges: bool = False, blocks=None):
+        self.layout, self.nv, self.nflux = layout, nv, nflux
+        self._allocs = []
+        self.pack = KmPack()
+        p = self.pack
+        p.nblocks = layout.nblocks
+        p.ndim = layout.ndim
+        p.ng = layout.ng
+        for d in range(3):
+            p.nx[d] = layout.nx[d] if d < layout.ndim else 1
+        p.nvar = nv
+        p.nflux = nflux
+        e1, e2, e3 = layout.E
+        self.ncell = e1 * e2 * e3
+        s1, s2, s3 = layout.S
+        self.nstag = s1 * s2 * s3
+        nb = layout.nblocks
+        self.cell = self._alloc(nb * nv * self.ncell * 8)
+        p.cell = self._table([self.cell + 8 * self.ncell * t for t in range(nb * nv)])
+        self.flux = [None] * 3
+        if nflux:
+            for d in range(layout.ndim):
+                self.flux[d] = self._alloc(nb * nflux * self.ncell * 8)
+                p.flux[d] = self._table([self.flux[d] + 8 * self.ncell * t
+                                         for t in range(nb * nflux)])
+        self.face = [None] * 3
+        if faces:
+            for d in range(3):
+                self.face[d] = self._alloc(nb * self.nstag * 8)
+                p.face[d] = self._table([self.face[d] + 8 * self.nstag * b for b in range(nb)])
+        self.edge = [None] * 3
+        if edges:
+            for d in range(3):
+                self.edge[d] = self._alloc(nb * self.nstag * 8)
+                p.edge[d] = self._table([self.edge[d] + 8 * self.nstag * b for b in range(nb)])
+        if blocks is not None:
+            bptr = self._alloc(C.sizeof(blocks))
+            check(lib().km_memcpy_h2d(C.c_void_p(bptr), C.cast(blocks, C.c_void_p),
+                                      C.sizeof(blocks)), "h2d")
+            p.blocks = bptr
+
+    def _alloc(self, nbytes: int) -> int:
+        ptr = C.c_void_p()
+        check(lib().km_malloc(C.byref(ptr), max(nbytes, 8)), "km_malloc")
+        self._allocs.append(ptr.value)
+        zeros = np.zeros(max(nbytes, 8) // 8)
+        check(lib().km_memcpy_h2d(ptr, f64ptr(zeros), zeros.nbytes), "h2d")
+        return ptr.value
+
+    def _table(self, ptrs) -> int:
+        arr = np.array(ptrs, dtype=np.uint64)
+        dev = self._alloc(arr.nbytes)
+        check(lib().km_memcpy_h2d(C.c_void_p(dev), C.c_void_p(arr.ctypes.data), arr.nbytes),
+              "h2d")
+        return dev
+
+    def upload(self, cell=None, flux=None, face=None, edge=None):
+        def put(dev, a):
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            check(lib().km_memcpy_h2d(C.c_void_p(dev), f64ptr(a), a.nbytes), "h2d")
+        if cell is not None:
+            put(self.cell, cell)
+        for d in range(3):
+            if flux is not None and flux[d] is not None and self.flux[d]:
+                put(self.flux[d], flux[d])
+            if face is not None and face[d] is not None and self.face[d]:
+                put(self.face[d], face[d])
+            if edge is not None and edge[d] is not None and self.edge[d]:
+                put(self.edge[d], edge[d])
+
+    def download(self):
+        L = self.layout
+
+        def get(dev, shape):
+            a = np.empty(shape)
+            check(lib().km_memcpy_d2h(f64ptr(a), C.c_void_p(dev), a.nbytes), "d2h")
+            return a
+        check(lib().km_synchronize(), "sync")
+        cell = get(self.cell, L.cell_shape(self.nv))
+        flux = [get(self.flux[d], L.cell_shape(self.nflux)) if self.flux[d] else None
+                for d in range(3)]
+        face = [get(self.face[d], L.stag_shape()) if self.face[d] else None for d in range(3)]
+        edge = [get(self.edge[d], L.stag_shape()) if self.edge[d] else None for d in range(3)]
+        return cell, flux, face, edge
+
+    def close(self):
+        for p in self._allocs:
+            lib().km_free(C.c_void_p(p))
+        self._allocs = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

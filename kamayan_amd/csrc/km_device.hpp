@@ -1,0 +1,119 @@
+// km_device.hpp -- device-side pack geometry and the register-window sweep shared by the
+// flux kernels.
+//
+// Sweep design (gfx950): one lane owns one pencil of cells along the sweep direction and
+// marches along it, keeping the reconstruction stencil of every reconstructed variable
+// in a 5-slot register window. Each step loads one new cell, reconstructs one cell (both
+// face values at once) and solves the Riemann problem at the face it shares with the
+// previous cell, whose upper state is still in registers. Every cell is reconstructed
+// exactly once per direction and every face is solved exactly once, as in the
+// reference's pencil loops (hydro_add_flux_tasks.cpp:79-261), but with no scratch
+// memory and no barriers. For the x2/x3 sweeps consecutive lanes own consecutive x1
+// columns, so each window load is one coalesced 512-byte wavefront access.
+#pragma once
+
+#include "../../include/km_hydro.h"
+#include "km_physics.hpp"
+
+namespace km {
+
+struct Geo {
+  int ndim, ng, nx[3], E[3], S[3];
+  int lo[3], hi[3];  // interior cell bounds per dimension (inclusive)
+  long sc[3];        // cell-array strides
+  long ss[3];        // staggered-array strides
+  KM_HD explicit Geo(const km_pack_t &p) {
+    ndim = p.ndim;
+    ng = p.ng;
+    for (int d = 0; d < 3; ++d) nx[d] = p.nx[d];
+    E[0] = nx[0] + 2 * ng;
+    E[1] = ndim > 1 ? nx[1] + 2 * ng : 1;
+    E[2] = ndim > 2 ? nx[2] + 2 * ng : 1;
+    S[0] = E[0] + 1;
+    S[1] = E[1] + (ndim > 1 ? 1 : 0);
+    S[2] = E[2] + (ndim > 2 ? 1 : 0);
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = d < ndim ? ng : 0;
+      hi[d] = d < ndim ? ng + nx[d] - 1 : 0;
+    }
+    sc[0] = 1;
+    sc[1] = E[0];
+    sc[2] = (long)E[0] * E[1];
+    ss[0] = 1;
+    ss[1] = S[0];
+    ss[2] = (long)S[0] * S[1];
+  }
+  KM_HD long c(int k, int j, int i) const { return ((long)k * E[1] + j) * E[0] + i; }
+  KM_HD long s(int k, int j, int i) const { return ((long)k * S[1] + j) * S[0] + i; }
+  KM_HD long ncell() const { return (long)E[0] * E[1] * E[2]; }
+  KM_HD long nstag() const { return (long)S[0] * S[1] * S[2]; }
+};
+
+KM_HD double *cell_ptr(const km_pack_t &p, int b, int v) { return p.cell[(long)b * p.nvar + v]; }
+KM_HD double *flux_ptr(const km_pack_t &p, int d, int b, int v) {
+  return p.flux[d][(long)b * p.nflux + v];
+}
+
+// slot of recon variable r in a hydro state pack
+template <int MHD>
+KM_HD int recon_slot(int r) {
+  constexpr int nc = MHD ? 8 : 5;
+  if (r == R_DENS) return 0;
+  if (r < R_PRES) return nc + (r - R_VEL);
+  if (r == R_PRES) return nc + 3;
+  if (r == R_BMOD) return nc + 4;
+  if (r == R_EINT) return nc + 5;
+  return 5 + (r - R_MAGC);
+}
+
+// runtime limiter choice (uniform across a wavefront) over a compile-time method
+template <int RECON, class T>
+KM_HD void reconstruct_rt(int lim, const T *w, T &vM, T &vP) {
+  if constexpr (RECON == RECON_PLM || RECON == RECON_PPM) {
+    if (lim == LIM_MC)
+      reconstruct<RECON, LIM_MC>(w, vM, vP);
+    else if (lim == LIM_VAN_LEER)
+      reconstruct<RECON, LIM_VAN_LEER>(w, vM, vP);
+    else
+      reconstruct<RECON, LIM_MINMOD>(w, vM, vP);
+  } else {
+    reconstruct<RECON, LIM_MINMOD>(w, vM, vP);
+  }
+}
+
+// March along one pencil: reconstruct cells c0..c1 and call face(c, vL, vR) for every
+// face c in (c0, c1], the face between cells c-1 and c (stored at index c).
+// load(c, T p[NR]) supplies the recon variables of cell c.
+template <class T, int RECON, int NR, class Load, class Face>
+__device__ __forceinline__ void march(int lim, int c0, int c1, Load &&load, Face &&face) {
+  constexpr int W = ReconWidth<RECON>::value;
+  T win[NR][5];
+#pragma unroll
+  for (int o = -W; o < W; ++o) {
+    T p[NR];
+    load(c0 + o, p);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) win[r][2 + o] = p[r];
+  }
+  T vPprev[NR];
+  for (int c = c0; c <= c1; ++c) {
+    {
+      T p[NR];
+      load(c + W, p);
+#pragma unroll
+      for (int r = 0; r < NR; ++r) win[r][2 + W] = p[r];
+    }
+    T vM[NR], vP[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) reconstruct_rt<RECON>(lim, win[r], vM[r], vP[r]);
+    if (c > c0) face(c, vPprev, vM);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      vPprev[r] = vP[r];
+#pragma unroll
+      for (int o = 2 - W; o < 2 + W; ++o) win[r][o] = win[r][o + 1];
+    }
+  }
+}
+
+}  // namespace km

@@ -1,0 +1,9 @@
+// km_fused_fma.hip -- fused RK-stage sweeps, fma form (see km_fused.inc).
+#include <hip/hip_runtime.h>
+
+#include "km_device.hpp"
+#include "km_internal.hpp"
+
+#define KM_FUSED_NS fma
+#define KM_FUSED_T double
+#include "km_fused.inc"

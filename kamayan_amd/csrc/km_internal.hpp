@@ -1,0 +1,83 @@
+// km_internal.hpp -- declarations shared by the host runtime and the kernel translation
+// units of libkm_hydro.so (not part of the public C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/km_hydro.h"
+#include "km_physics.hpp"
+
+namespace km {
+
+// inclusive cell-index box per dimension (x1, x2, x3)
+struct Box {
+  int lo[3], hi[3];
+};
+
+// block grid of a single-level mesh, blocks stored x-fastest
+struct UniformGrid {
+  int nb[3];
+  int bc[6];
+};
+
+inline Eos make_eos(const km_cfg_t &cfg) {
+  constexpr double kboltz = 1.380649e-16;  // equation_of_state.hpp:44
+  Eos e;
+  e.gm1 = cfg.gamma - 1.0;
+  e.gp1gm1 = (e.gm1 + 1) * e.gm1;
+  e.cv = cfg.gamma * kboltz / cfg.abar;
+  return e;
+}
+
+// stage arguments of the fused sweeps
+struct FusedArgs {
+  km_pack_t u0;        // stage input: conserved slots 0..4, ghost zones filled
+  km_pack_t base;      // RK base state (conserved)
+  km_pack_t u1;        // stage output (conserved, interior written)
+  double *du;          // [nblocks][5][nx3][nx2][nx1] accumulator (interior only)
+  const double *dt;    // device: step dt
+  double *dt_est;      // device: CFL minimum of u1 (bit-ordered), or null
+  double beta;
+  int lim;
+  Eos eos;
+};
+
+// task launchers (km_tasks.hip)
+void launch_calculate_fluxes(const km_cfg_t &cfg, const km_pack_t &u, const Box &bx,
+                             hipStream_t st);
+void launch_calculate_emf(const km_pack_t &u, hipStream_t st);
+void launch_fluxes_to_dudt(const km_cfg_t &cfg, const km_pack_t &u, const km_pack_t &dudt,
+                           hipStream_t st);
+void launch_apply_dudt(const km_cfg_t &cfg, const km_pack_t &base, const km_pack_t &u0,
+                       const km_pack_t &u1, const km_pack_t &dudt, double beta, double dt,
+                       hipStream_t st);
+void launch_prepare_primitive(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st);
+void launch_prepare_conserved(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st);
+void launch_estimate_dt(const km_cfg_t &cfg, const km_pack_t &u, double *dt_dev, hipStream_t st);
+void launch_fill_divb(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st);
+void launch_ghost_fill(const UniformGrid &gr, const km_pack_t &u, int v0, int v1, int nc,
+                       bool faces, hipStream_t st);
+
+// fused stage sweeps: one translation unit per arithmetic form
+//   exact : FP64, -ffp-contract=off (bitwise the reference's operation order)
+//   fma   : FP64, -ffp-contract=fast
+//   fp32  : FP32 reconstruction/Riemann, FP64 conversion, accumulation and stage combine
+namespace exact {
+int launch_fused_stage(const km_cfg_t &cfg, const FusedArgs &a, int ndim, bool with_dt,
+                       int tile, hipStream_t st);
+}
+namespace fma {
+int launch_fused_stage(const km_cfg_t &cfg, const FusedArgs &a, int ndim, bool with_dt,
+                       int tile, hipStream_t st);
+}
+namespace fp32 {
+int launch_fused_stage(const km_cfg_t &cfg, const FusedArgs &a, int ndim, bool with_dt,
+                       int tile, hipStream_t st);
+}
+
+// thread-local error message for km_last_error()
+void set_error(const std::string &msg);
+
+}  // namespace km

@@ -1,0 +1,703 @@
+// km_runtime.hip -- host runtime of libkm_hydro.so: the C-ABI task entry points, the
+// option dispatcher, the device mesh and the RK-stage driver.
+//
+// The driver mirrors KamayanDriver::MakeTaskCollection / BuildTaskListRKStage
+// (kamayan_driver.cpp:121-248) for a single MeshData partition on one GPU:
+//   stage s: fluxes(md0) -> EMF(md0) -> FluxesToDuDt -> ApplyDuDt(base, md0, md1)
+//            -> PreparePrimitive(md1) [-> EstimateTimestep, FillDerived on the last stage]
+//            -> ghost fill(md1)
+// with stage containers stage_name = {base, 1, .., n-1, base} and the SSP-RK beta table
+// used by Parthenon's multi-stage integrators. Conserved variables (and CT face B) are
+// independent per container; primitives are shared between containers, as the hydro
+// unit's field metadata makes them (hydro.cpp:96-105).
+//
+// The fused path (KM_PATH_FUSED, hydro) replaces the per-task kernels of a stage by one
+// sweep per direction (km_fused.inc) and keeps dt on the device, so a whole cycle can be
+// captured once in a hipGraph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/km_hydro.h"
+#include "km_device.hpp"
+#include "km_internal.hpp"
+
+namespace km {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+
+#define KM_HIP(call)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess) {                                                           \
+      set_error(std::string(#call) + ": " + hipGetErrorString(e_));                   \
+      return KM_EHIP;                                                                 \
+    }                                                                                 \
+  } while (0)
+
+static int check_launch() {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error(std::string("kernel launch: ") + hipGetErrorString(e));
+    return KM_EHIP;
+  }
+  return KM_OK;
+}
+
+// Dispatcher<...>(cfg): the option combinations the kernels are instantiated for. An
+// unsupported value fails the way PolymorphicDispatch does (dispatcher.hpp:122-130).
+static int check_cfg(const km_cfg_t *cfg, const km_pack_t *u) {
+  if (!cfg || !u) {
+    set_error("null argument");
+    return KM_EINVAL;
+  }
+  if (cfg->recon < 0 || cfg->recon > 3) {
+    set_error("dispatch parm [Reconstruction] not handled");
+    return KM_EINVAL;
+  }
+  if (cfg->limiter < 0 || cfg->limiter > 2) {
+    set_error("dispatch parm [SlopeLimiter] not handled");
+    return KM_EINVAL;
+  }
+  if (cfg->riemann != KM_RIEMANN_HLL && cfg->riemann != KM_RIEMANN_HLLC) {
+    set_error("dispatch parm [RiemannSolver] not handled");
+    return KM_EINVAL;
+  }
+  if (cfg->fluid != KM_FLUID_ONET) {
+    set_error("dispatch parm [Fluid] not handled");
+    return KM_EINVAL;
+  }
+  if (cfg->geometry != KM_GEOM_CARTESIAN) {
+    set_error("dispatch parm [Geometry] not handled: only cartesian is built");
+    return KM_EINVAL;
+  }
+  if (cfg->mhd != KM_MHD_OFF && cfg->mhd != KM_MHD_CT) {
+    set_error("dispatch parm [Mhd] not handled");
+    return KM_EINVAL;
+  }
+  if (u->ndim < 1 || u->ndim > 3 || u->nblocks < 0) {
+    set_error("bad pack shape");
+    return KM_EINVAL;
+  }
+  if (cfg->mhd == KM_MHD_CT && u->ndim < 2) {
+    set_error("constrained transport needs ndim >= 2");
+    return KM_EINVAL;
+  }
+  if (u->ng < km_cfg_min_ghost(cfg)) {
+    set_error("nghost " + std::to_string(u->ng) + " below the " +
+              std::to_string(km_cfg_min_ghost(cfg)) + " the options need");
+    return KM_EINVAL;
+  }
+  if (u->nvar < KM_NVAR(cfg->mhd)) {
+    set_error("pack has fewer cell variables than the hydro state needs");
+    return KM_EINVAL;
+  }
+  return KM_OK;
+}
+
+// flux bounds of CalculateFluxesNested (hydro_add_flux_tasks.cpp:56-70)
+static Box flux_box(const km_cfg_t &cfg, const km_pack_t &u) {
+  const Geo g(u);
+  Box bx;
+  for (int d = 0; d < 3; ++d) {
+    bx.lo[d] = g.lo[d];
+    bx.hi[d] = g.hi[d];
+  }
+  if (cfg.mhd == KM_MHD_CT) {
+    const int w[3] = {g.ndim > 1, g.ndim > 1, g.ndim > 2};
+    for (int d = 0; d < 3; ++d) {
+      bx.lo[d] -= w[d];
+      bx.hi[d] += w[d];
+    }
+  }
+  return bx;
+}
+
+static hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+__global__ void k_set(double *p, double v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+__global__ void k_scale(double *p, double s) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = *p * s;
+}
+// Parthenon time-step rule: time += dt; dt = min(2 dt, estimate) unless overridden
+__global__ void k_dt_update(double *dt, double *time, const double *est, int keep) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const double d = *dt;
+    *time = *time + d;
+    if (!keep) *dt = kmin(2.0 * d, *est);
+  }
+}
+
+}  // namespace km
+
+using namespace km;
+
+struct km_ghost_plan_t {
+  UniformGrid grid;
+};
+
+// ---------------------------------------------------------------------------------
+// device mesh
+// ---------------------------------------------------------------------------------
+namespace {
+
+struct DevBuf {
+  void *p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+struct Cont {
+  std::vector<double *> cell_h, face_h[3], flux_h[3], edge_h[3];
+  double **cell_d = nullptr, **face_d[3] = {nullptr, nullptr, nullptr};
+  double **flux_d[3] = {nullptr, nullptr, nullptr}, **edge_d[3] = {nullptr, nullptr, nullptr};
+  km_pack_t pack;
+};
+
+}  // namespace
+
+struct km_sim_t {
+  km_cfg_t cfg;
+  km_mesh_desc_t mesh;
+  int path = KM_PATH_TASKS;
+  int nb = 0, nvar = 0, nc = 0;
+  long ncell = 0, nstag = 0;
+  std::vector<std::unique_ptr<DevBuf>> bufs;
+  km_block_t *blocks_d = nullptr;
+  Cont base, stage[2], dudt, scratch;
+  double *du = nullptr;
+  UniformGrid grid;
+  hipStream_t stream = nullptr;
+  double *dt_d = nullptr, *est_d = nullptr, *time_d = nullptr;
+  int ncycle = 0;
+  bool prims_from_ic = false;  // first stage must read the initial primitives
+  bool dt_keep = false;
+  int tile = 256;
+  bool use_graph = true;
+  hipGraphExec_t graph_exec = nullptr;
+  hipGraph_t graph = nullptr;
+
+  template <class T>
+  int alloc(T **out, size_t count) {
+    auto b = std::make_unique<DevBuf>();
+    KM_HIP(hipMalloc(&b->p, count * sizeof(T) + 256));
+    KM_HIP(hipMemset(b->p, 0, count * sizeof(T) + 256));
+    *out = static_cast<T *>(b->p);
+    bufs.push_back(std::move(b));
+    return KM_OK;
+  }
+  int upload_table(double ***dst, const std::vector<double *> &h) {
+    int rc = alloc(dst, h.size());
+    if (rc) return rc;
+    KM_HIP(hipMemcpy(*dst, h.data(), h.size() * sizeof(double *), hipMemcpyHostToDevice));
+    return KM_OK;
+  }
+  ~km_sim_t() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+int make_cont(km_sim_t &s, Cont &c, bool full_cells, const Cont *share, bool faces,
+              bool fluxes) {
+  std::memset(&c.pack, 0, sizeof(c.pack));
+  c.pack.nblocks = s.nb;
+  c.pack.ndim = s.mesh.ndim;
+  c.pack.ng = s.mesh.ng;
+  for (int d = 0; d < 3; ++d) c.pack.nx[d] = d < s.mesh.ndim ? s.mesh.nx[d] : 1;
+  c.pack.nvar = s.nvar;
+  c.pack.nflux = s.nc;
+  c.pack.blocks = s.blocks_d;
+  const int own = full_cells ? s.nvar : s.nc;
+  double *arena = nullptr;
+  int rc = s.alloc(&arena, (size_t)s.nb * own * s.ncell);
+  if (rc) return rc;
+  c.cell_h.assign((size_t)s.nb * s.nvar, nullptr);
+  for (int b = 0; b < s.nb; ++b)
+    for (int v = 0; v < s.nvar; ++v) {
+      const size_t t = (size_t)b * s.nvar + v;
+      if (v < own)
+        c.cell_h[t] = arena + ((size_t)b * own + v) * s.ncell;
+      else if (share)
+        c.cell_h[t] = share->cell_h[t];
+    }
+  if ((rc = s.upload_table(&c.cell_d, c.cell_h))) return rc;
+  c.pack.cell = c.cell_d;
+  const bool ct = s.cfg.mhd == KM_MHD_CT;
+  if (faces && ct) {
+    for (int d = 0; d < 3; ++d) {
+      double *fa = nullptr;
+      if ((rc = s.alloc(&fa, (size_t)s.nb * s.nstag))) return rc;
+      c.face_h[d].resize(s.nb);
+      for (int b = 0; b < s.nb; ++b) c.face_h[d][b] = fa + (size_t)b * s.nstag;
+      if ((rc = s.upload_table(&c.face_d[d], c.face_h[d]))) return rc;
+      c.pack.face[d] = c.face_d[d];
+    }
+  }
+  if (fluxes) {
+    for (int d = 0; d < 3; ++d) {
+      if (d >= s.mesh.ndim) continue;
+      double *fa = nullptr;
+      if ((rc = s.alloc(&fa, (size_t)s.nb * s.nc * s.ncell))) return rc;
+      c.flux_h[d].resize((size_t)s.nb * s.nc);
+      for (size_t t = 0; t < c.flux_h[d].size(); ++t) c.flux_h[d][t] = fa + t * s.ncell;
+      if ((rc = s.upload_table(&c.flux_d[d], c.flux_h[d]))) return rc;
+      c.pack.flux[d] = c.flux_d[d];
+      if (ct) {
+        double *ea = nullptr;
+        if ((rc = s.alloc(&ea, (size_t)s.nb * s.nstag))) return rc;
+        c.edge_h[d].resize(s.nb);
+        for (int b = 0; b < s.nb; ++b) c.edge_h[d][b] = ea + (size_t)b * s.nstag;
+        if ((rc = s.upload_table(&c.edge_d[d], c.edge_h[d]))) return rc;
+        c.pack.edge[d] = c.edge_d[d];
+      }
+    }
+    if (ct && s.mesh.ndim < 3) {  // E3 is needed in 2-D too
+      double *ea = nullptr;
+      if ((rc = s.alloc(&ea, (size_t)s.nb * s.nstag))) return rc;
+      c.edge_h[2].resize(s.nb);
+      for (int b = 0; b < s.nb; ++b) c.edge_h[2][b] = ea + (size_t)b * s.nstag;
+      if ((rc = s.upload_table(&c.edge_d[2], c.edge_h[2]))) return rc;
+      c.pack.edge[2] = c.edge_d[2];
+    }
+  }
+  return KM_OK;
+}
+
+const double kBetaRK1[1] = {1.0};
+const double kBetaRK2[2] = {1.0, 0.5};
+const double kBetaRK3[3] = {1.0, 0.25, 2.0 / 3.0};
+
+Cont &container(km_sim_t &s, int idx) {
+  if (idx == 0 || idx == s.cfg.nstages) return s.base;
+  return s.stage[idx - 1];
+}
+
+// one RK stage as the reference's task list
+int task_stage(km_sim_t &s, int stage, double beta, double dt) {
+  const km_cfg_t &cfg = s.cfg;
+  Cont &md0 = container(s, stage - 1), &md1 = container(s, stage);
+  km_pack_t p0 = md0.pack;
+  for (int d = 0; d < 3; ++d) {
+    p0.flux[d] = s.scratch.pack.flux[d];
+    p0.edge[d] = s.scratch.pack.edge[d];
+  }
+  const bool ct = cfg.mhd == KM_MHD_CT;
+  launch_calculate_fluxes(cfg, p0, flux_box(cfg, p0), s.stream);
+  if (ct) launch_calculate_emf(p0, s.stream);
+  launch_fluxes_to_dudt(cfg, p0, s.dudt.pack, s.stream);
+  launch_apply_dudt(cfg, s.base.pack, p0, md1.pack, s.dudt.pack, beta, dt, s.stream);
+  launch_prepare_primitive(cfg, md1.pack, s.stream);
+  if (stage == cfg.nstages) {
+    launch_estimate_dt(cfg, s.base.pack, s.est_d, s.stream);
+    if (ct) launch_fill_divb(cfg, s.base.pack, s.stream);
+  }
+  // FillGhost fields: conserved, VELOCITY, PRES, BMOD, EINT, face B (TEMP is not)
+  launch_ghost_fill(s.grid, md1.pack, 0, KM_TEMP(cfg.mhd), s.nc, ct, s.stream);
+  return check_launch();
+}
+
+int fused_stage(km_sim_t &s, int stage, double beta) {
+  const km_cfg_t &cfg = s.cfg;
+  Cont &md0 = container(s, stage - 1), &md1 = container(s, stage);
+  const bool last = stage == cfg.nstages;
+  FusedArgs a;
+  a.u0 = md0.pack;
+  a.base = s.base.pack;
+  a.u1 = md1.pack;
+  a.du = s.du;
+  a.dt = s.dt_d;
+  a.dt_est = s.est_d;
+  a.beta = beta;
+  a.lim = cfg.limiter;
+  a.eos = make_eos(cfg);
+  if (last) hipLaunchKernelGGL(k_set, dim3(1), dim3(64), 0, s.stream, s.est_d, DBL_MAX);
+  int rc;
+  if (cfg.precision == KM_PREC_FP32_MIXED)
+    rc = fp32::launch_fused_stage(cfg, a, s.mesh.ndim, last, s.tile, s.stream);
+  else if (cfg.precision == KM_PREC_FP64_FMA)
+    rc = fma::launch_fused_stage(cfg, a, s.mesh.ndim, last, s.tile, s.stream);
+  else
+    rc = exact::launch_fused_stage(cfg, a, s.mesh.ndim, last, s.tile, s.stream);
+  if (rc) return rc;
+  if (last)
+    hipLaunchKernelGGL(k_scale, dim3(1), dim3(64), 0, s.stream, s.est_d,
+                       cfg.cfl / static_cast<double>(s.mesh.ndim));
+  launch_ghost_fill(s.grid, md1.pack, 0, s.nc, s.nc, false, s.stream);
+  return check_launch();
+}
+
+// one cycle; dt_host is the step dt when the task path needs it by value
+int run_cycle(km_sim_t &s, double dt_host) {
+  const int n = s.cfg.nstages;
+  const double *betas = n == 1 ? kBetaRK1 : n == 2 ? kBetaRK2 : kBetaRK3;
+  for (int stage = 1; stage <= n; ++stage) {
+    int rc;
+    if (s.path == KM_PATH_TASKS || s.prims_from_ic)
+      rc = task_stage(s, stage, betas[stage - 1], dt_host);
+    else
+      rc = fused_stage(s, stage, betas[stage - 1]);
+    s.prims_from_ic = false;
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(k_dt_update, dim3(1), dim3(64), 0, s.stream, s.dt_d, s.time_d, s.est_d,
+                     s.dt_keep ? 1 : 0);
+  s.dt_keep = false;
+  s.ncycle += 1;
+  return check_launch();
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *km_last_error(void) { return g_err.c_str(); }
+const char *km_version(void) { return "kamayan_amd 0.1 (gfx950)"; }
+
+// ---- memory helpers ----
+int km_device_count(int32_t *count) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  *count = e == hipSuccess ? n : 0;
+  return KM_OK;
+}
+int km_malloc(void **ptr, size_t bytes) {
+  KM_HIP(hipMalloc(ptr, bytes));
+  return KM_OK;
+}
+int km_free(void *ptr) {
+  KM_HIP(hipFree(ptr));
+  return KM_OK;
+}
+int km_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+  KM_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return KM_OK;
+}
+int km_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+  KM_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return KM_OK;
+}
+int km_synchronize(void) {
+  KM_HIP(hipDeviceSynchronize());
+  return KM_OK;
+}
+
+// ---- tasks ----
+int km_calculate_fluxes(const km_cfg_t *cfg, const km_pack_t *u, void *stream) {
+  if (int rc = check_cfg(cfg, u)) return rc;
+  for (int d = 0; d < u->ndim; ++d)
+    if (!u->flux[d]) {
+      set_error("flux tables missing");
+      return KM_EINVAL;
+    }
+  if (cfg->mhd == KM_MHD_CT)
+    for (int d = 0; d < u->ndim; ++d)
+      if (!u->face[d]) {
+        set_error("CT needs face B tables");
+        return KM_EINVAL;
+      }
+  launch_calculate_fluxes(*cfg, *u, flux_box(*cfg, *u), as_stream(stream));
+  return check_launch();
+}
+
+int km_calculate_emf(const km_cfg_t *cfg, const km_pack_t *u, void *stream) {
+  if (int rc = check_cfg(cfg, u)) return rc;
+  if (cfg->mhd != KM_MHD_CT || u->ndim < 2) return KM_OK;
+  if (!u->edge[2] || (u->ndim > 2 && (!u->edge[0] || !u->edge[1]))) {
+    set_error("edge tables missing");
+    return KM_EINVAL;
+  }
+  launch_calculate_emf(*u, as_stream(stream));
+  return check_launch();
+}
+
+int km_fluxes_to_dudt(const km_cfg_t *cfg, const km_pack_t *u, const km_pack_t *dudt,
+                      void *stream) {
+  if (int rc = check_cfg(cfg, u)) return rc;
+  if (!dudt) return KM_EINVAL;
+  launch_fluxes_to_dudt(*cfg, *u, *dudt, as_stream(stream));
+  return check_launch();
+}
+
+int km_apply_dudt(const km_cfg_t *cfg, const km_pack_t *base, const km_pack_t *u0,
+                  const km_pack_t *u1, const km_pack_t *dudt, double beta, double dt,
+                  void *stream) {
+  if (int rc = check_cfg(cfg, u0)) return rc;
+  if (!base || !u1 || !dudt) return KM_EINVAL;
+  launch_apply_dudt(*cfg, *base, *u0, *u1, *dudt, beta, dt, as_stream(stream));
+  return check_launch();
+}
+
+int km_prepare_primitive(const km_cfg_t *cfg, const km_pack_t *u, void *stream) {
+  if (int rc = check_cfg(cfg, u)) return rc;
+  launch_prepare_primitive(*cfg, *u, as_stream(stream));
+  return check_launch();
+}
+
+int km_prepare_conserved(const km_cfg_t *cfg, const km_pack_t *u, void *stream) {
+  if (int rc = check_cfg(cfg, u)) return rc;
+  launch_prepare_conserved(*cfg, *u, as_stream(stream));
+  return check_launch();
+}
+
+int km_estimate_dt(const km_cfg_t *cfg, const km_pack_t *u, double *dt_dev, void *stream) {
+  if (int rc = check_cfg(cfg, u)) return rc;
+  if (!dt_dev) return KM_EINVAL;
+  launch_estimate_dt(*cfg, *u, dt_dev, as_stream(stream));
+  return check_launch();
+}
+
+int km_fill_divb(const km_cfg_t *cfg, const km_pack_t *u, void *stream) {
+  if (int rc = check_cfg(cfg, u)) return rc;
+  if (cfg->mhd != KM_MHD_CT) return KM_OK;
+  launch_fill_divb(*cfg, *u, as_stream(stream));
+  return check_launch();
+}
+
+int km_ghost_plan_uniform(const km_pack_t *shape, const int32_t nb[3], const int32_t bc[6],
+                          km_ghost_plan_t **plan) {
+  if (!shape || !nb || !bc || !plan) return KM_EINVAL;
+  if ((long)nb[0] * nb[1] * nb[2] != shape->nblocks) {
+    set_error("block grid does not match the pack's block count");
+    return KM_EINVAL;
+  }
+  for (int d = 0; d < shape->ndim; ++d) {
+    const bool per_lo = bc[2 * d] == KM_BC_PERIODIC, per_hi = bc[2 * d + 1] == KM_BC_PERIODIC;
+    if (per_lo != per_hi) {
+      set_error("periodic boundaries must be paired");
+      return KM_EINVAL;
+    }
+    if (shape->ng > shape->nx[d] && nb[d] > 1) {
+      set_error("ghost width larger than a block");
+      return KM_EINVAL;
+    }
+  }
+  auto *p = new km_ghost_plan_t;
+  for (int d = 0; d < 3; ++d) p->grid.nb[d] = nb[d];
+  for (int d = 0; d < 6; ++d) p->grid.bc[d] = bc[d];
+  *plan = p;
+  return KM_OK;
+}
+
+int km_ghost_plan_destroy(km_ghost_plan_t *plan) {
+  delete plan;
+  return KM_OK;
+}
+
+int km_ghost_fill(const km_ghost_plan_t *plan, const km_pack_t *u, int32_t v0, int32_t v1,
+                  int32_t with_faces, void *stream) {
+  if (!plan || !u || v0 < 0 || v1 > u->nvar || v0 > v1) return KM_EINVAL;
+  const int nc = u->nvar >= 16 ? 8 : 5;
+  if (with_faces && (!u->face[0] || !u->face[1])) {
+    set_error("face tables missing");
+    return KM_EINVAL;
+  }
+  launch_ghost_fill(plan->grid, *u, v0, v1, nc, with_faces != 0, as_stream(stream));
+  return check_launch();
+}
+
+// ---- whole-mesh driver ----
+int km_sim_create(const km_cfg_t *cfg, const km_mesh_desc_t *mesh, int32_t path,
+                  km_sim_t **out) {
+  if (!cfg || !mesh || !out) return KM_EINVAL;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    set_error("no GPU visible");
+    return KM_ENODEV;
+  }
+  auto s = std::make_unique<km_sim_t>();
+  s->cfg = *cfg;
+  s->cfg.ndim = mesh->ndim;
+  s->mesh = *mesh;
+  s->path = path;
+  if (path == KM_PATH_FUSED && cfg->mhd != KM_MHD_OFF) {
+    set_error("the fused stage path is built for hydro (MHD off)");
+    return KM_EINVAL;
+  }
+  if (cfg->nstages < 1 || cfg->nstages > 3) {
+    set_error("integrator must be rk1, rk2 or rk3");
+    return KM_EINVAL;
+  }
+  s->nb = mesh->nb[0] * mesh->nb[1] * mesh->nb[2];
+  s->nvar = KM_NVAR(cfg->mhd);
+  s->nc = KM_NCONS(cfg->mhd);
+  km_pack_t shape;
+  std::memset(&shape, 0, sizeof(shape));
+  shape.nblocks = s->nb;
+  shape.ndim = mesh->ndim;
+  shape.ng = mesh->ng;
+  for (int d = 0; d < 3; ++d) shape.nx[d] = d < mesh->ndim ? mesh->nx[d] : 1;
+  shape.nvar = s->nvar;
+  if (int rc = check_cfg(&s->cfg, &shape)) return rc;
+  const Geo g(shape);
+  s->ncell = g.ncell();
+  s->nstag = g.nstag();
+  for (int d = 0; d < 3; ++d) s->grid.nb[d] = mesh->nb[d];
+  for (int d = 0; d < 6; ++d) s->grid.bc[d] = mesh->bc[d];
+  KM_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  std::vector<km_block_t> blocks(s->nb);
+  km_mesh_blocks(mesh, blocks.data());
+  int rc;
+  if ((rc = s->alloc(&s->blocks_d, s->nb))) return rc;
+  KM_HIP(hipMemcpy(s->blocks_d, blocks.data(), s->nb * sizeof(km_block_t), hipMemcpyHostToDevice));
+  if ((rc = make_cont(*s, s->base, true, nullptr, true, false))) return rc;
+  for (int i = 0; i < 2; ++i)
+    if ((rc = make_cont(*s, s->stage[i], false, &s->base, true, false))) return rc;
+  if ((rc = make_cont(*s, s->dudt, false, nullptr, true, false))) return rc;
+  if ((rc = make_cont(*s, s->scratch, false, nullptr, false, true))) return rc;
+  if (path == KM_PATH_FUSED) {
+    const long nint = (long)shape.nx[0] * shape.nx[1] * shape.nx[2];
+    if ((rc = s->alloc(&s->du, (size_t)s->nb * 5 * nint))) return rc;
+  }
+  if ((rc = s->alloc(&s->dt_d, 4))) return rc;
+  s->est_d = s->dt_d + 1;
+  s->time_d = s->dt_d + 2;
+  *out = s.release();
+  return KM_OK;
+}
+
+int km_sim_destroy(km_sim_t *s) {
+  if (s) (void)hipStreamSynchronize(s->stream);
+  delete s;
+  return KM_OK;
+}
+
+static int copy_state(km_sim_t *s, const double *cell, const double *const face[3],
+                      double *cell_out, double *const face_out[3]) {
+  for (int b = 0; b < s->nb; ++b) {
+    for (int v = 0; v < s->nvar; ++v) {
+      double *dev = s->base.cell_h[(size_t)b * s->nvar + v];
+      const size_t off = ((size_t)b * s->nvar + v) * s->ncell;
+      if (cell)
+        KM_HIP(hipMemcpy(dev, cell + off, s->ncell * sizeof(double), hipMemcpyHostToDevice));
+      if (cell_out)
+        KM_HIP(hipMemcpy(cell_out + off, dev, s->ncell * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    if (s->cfg.mhd == KM_MHD_CT)
+      for (int d = 0; d < 3; ++d) {
+        double *dev = s->base.face_h[d][b];
+        const size_t off = (size_t)b * s->nstag;
+        if (face && face[d])
+          KM_HIP(hipMemcpy(dev, face[d] + off, s->nstag * sizeof(double), hipMemcpyHostToDevice));
+        if (face_out && face_out[d])
+          KM_HIP(hipMemcpy(face_out[d] + off, dev, s->nstag * sizeof(double),
+                           hipMemcpyDeviceToHost));
+      }
+  }
+  return KM_OK;
+}
+
+int km_sim_set_primitive(km_sim_t *s, const double *cell, const double *const face[3]) {
+  if (!s || !cell) return KM_EINVAL;
+  KM_HIP(hipStreamSynchronize(s->stream));
+  if (int rc = copy_state(s, cell, face, nullptr, nullptr)) return rc;
+  launch_prepare_conserved(s->cfg, s->base.pack, s->stream);
+  launch_ghost_fill(s->grid, s->base.pack, 0, KM_TEMP(s->cfg.mhd), s->nc,
+                    s->cfg.mhd == KM_MHD_CT, s->stream);
+  launch_estimate_dt(s->cfg, s->base.pack, s->est_d, s->stream);
+  hipMemcpyAsync(s->dt_d, s->est_d, sizeof(double), hipMemcpyDeviceToDevice, s->stream);
+  hipLaunchKernelGGL(k_set, dim3(1), dim3(64), 0, s->stream, s->time_d, 0.0);
+  if (int rc = check_launch()) return rc;
+  KM_HIP(hipStreamSynchronize(s->stream));
+  s->ncycle = 0;
+  s->prims_from_ic = true;
+  s->dt_keep = false;
+  if (s->graph_exec) {
+    (void)hipGraphExecDestroy(s->graph_exec);
+    (void)hipGraphDestroy(s->graph);
+    s->graph_exec = nullptr;
+    s->graph = nullptr;
+  }
+  return KM_OK;
+}
+
+int km_sim_get_state(km_sim_t *s, double *cell, double *const face[3]) {
+  if (!s || !cell) return KM_EINVAL;
+  KM_HIP(hipStreamSynchronize(s->stream));
+  if (s->path == KM_PATH_FUSED) {
+    // the fused path keeps only conserved variables current; derive the rest
+    launch_prepare_primitive(s->cfg, s->base.pack, s->stream);
+    KM_HIP(hipStreamSynchronize(s->stream));
+  }
+  return copy_state(s, nullptr, nullptr, cell, face);
+}
+
+int km_sim_set_dt(km_sim_t *s, double dt) {
+  if (!s) return KM_EINVAL;
+  KM_HIP(hipStreamSynchronize(s->stream));
+  KM_HIP(hipMemcpy(s->dt_d, &dt, sizeof(double), hipMemcpyHostToDevice));
+  s->dt_keep = true;
+  return KM_OK;
+}
+
+int km_sim_advance(km_sim_t *s, int32_t ncycles, double tlim) {
+  if (!s) return KM_EINVAL;
+  for (int n = 0; n < ncycles; ++n) {
+    const bool need_host =
+        s->path == KM_PATH_TASKS || s->prims_from_ic || s->dt_keep || tlim > 0.;
+    if (need_host) {
+      double h[3];
+      KM_HIP(hipMemcpyAsync(h, s->dt_d, 3 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+      KM_HIP(hipStreamSynchronize(s->stream));
+      double dt = h[0];
+      const double time = h[2];
+      if (tlim > 0.) {
+        if (time >= tlim) break;
+        if (time + dt > tlim) {
+          dt = tlim - time;
+          KM_HIP(hipMemcpy(s->dt_d, &dt, sizeof(double), hipMemcpyHostToDevice));
+        }
+      }
+      if (int rc = run_cycle(*s, dt)) return rc;
+      continue;
+    }
+    // fused path, no host decisions: capture one cycle in a graph and replay it
+    if (s->use_graph) {
+      if (!s->graph_exec) {
+        KM_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeRelaxed));
+        const int nc0 = s->ncycle;
+        int rc = run_cycle(*s, 0.0);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(s->stream, &g);
+        if (rc) return rc;
+        if (e != hipSuccess) {
+          set_error(std::string("graph capture: ") + hipGetErrorString(e));
+          return KM_EHIP;
+        }
+        s->graph = g;
+        KM_HIP(hipGraphInstantiate(&s->graph_exec, g, nullptr, nullptr, 0));
+        s->ncycle = nc0;  // captured, not yet run
+      }
+      KM_HIP(hipGraphLaunch(s->graph_exec, s->stream));
+      s->ncycle += 1;
+    } else {
+      if (int rc = run_cycle(*s, 0.0)) return rc;
+    }
+  }
+  KM_HIP(hipStreamSynchronize(s->stream));
+  return KM_OK;
+}
+
+int km_sim_info(const km_sim_t *s, double *time, double *dt, int32_t *ncycle) {
+  if (!s) return KM_EINVAL;
+  double h[3];
+  KM_HIP(hipStreamSynchronize(s->stream));
+  KM_HIP(hipMemcpy(h, s->dt_d, 3 * sizeof(double), hipMemcpyDeviceToHost));
+  if (time) *time = h[2];
+  if (dt) *dt = h[0];
+  if (ncycle) *ncycle = s->ncycle;
+  return KM_OK;
+}
+
+}  // extern "C"

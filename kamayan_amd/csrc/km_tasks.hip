@@ -1,0 +1,629 @@
+// km_tasks.hip -- one gfx950 kernel per task of the reference's RK stage
+// (BuildTaskListRKStage, kamayan_driver.cpp:186-248): fluxes per direction, edge EMFs,
+// flux divergence / Stokes, stage combine, cons->prim + EOS, CFL dt, divB and the
+// same-level ghost fill. These are the drop-in replacements for the individual hydro
+// and grid tasks; the fused stage path (km_fused.hip) reuses their arithmetic.
+#include <hip/hip_runtime.h>
+
+#include "km_device.hpp"
+#include "km_internal.hpp"
+
+namespace km {
+
+// ---------------------------------------------------------------------------------
+// hydro::CalculateFluxes (hydro_add_flux_tasks.cpp:36-264): one lane per pencil.
+// bx holds the (CT-widened, :59-70) interior bounds; cells bx.lo-1..bx.hi+1 along DIR are
+// reconstructed and faces bx.lo..bx.hi+1 solved, as in the reference's loops.
+// ---------------------------------------------------------------------------------
+template <int RECON, int RIEM, int MHD, int DIR>
+__global__ void __launch_bounds__(256) k_flux(km_pack_t u, int lim, Box bx) {
+  constexpr int NR = HV<MHD>::NR, NC = HV<MHD>::NC;
+  constexpr int DA = DIR == 0 ? 1 : 0;  // transverse dim mapped to consecutive lanes
+  constexpr int DB = DIR == 2 ? 1 : 2;
+  const Geo g(u);
+  const int na = bx.hi[DA] - bx.lo[DA] + 1;
+  const int nbb = bx.hi[DB] - bx.lo[DB] + 1;
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= (long)na * nbb * u.nblocks) return;
+  int idx[3];
+  idx[DA] = bx.lo[DA] + (int)(t % na);
+  t /= na;
+  idx[DB] = bx.lo[DB] + (int)(t % nbb);
+  const int b = (int)(t / nbb);
+  idx[DIR] = 0;
+  const long cb = g.c(idx[2], idx[1], idx[0]);
+  const long cs = g.sc[DIR];
+  const double *q[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) q[r] = cell_ptr(u, b, recon_slot<MHD>(r));
+  double *f[NC];
+#pragma unroll
+  for (int v = 0; v < NC; ++v) f[v] = flux_ptr(u, DIR, b, v);
+  const double *bface = MHD ? u.face[DIR][b] : nullptr;
+  const long sb = g.s(idx[2], idx[1], idx[0]);
+  const long ssd = g.ss[DIR];
+  march<double, RECON, NR>(
+      lim, bx.lo[DIR] - 1, bx.hi[DIR] + 1,
+      [&](int c, double *p) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) p[r] = q[r][cb + c * cs];
+      },
+      [&](int c, double *vL, double *vR) {
+        if constexpr (MHD) {  // :104-107,167-170,233-236
+          const double bf = bface[sb + c * ssd];
+          vL[R_MAGC + DIR] = bf;
+          vR[R_MAGC + DIR] = bf;
+        }
+        double F[NC];
+        riemann<RIEM, MHD, DIR>(vL, vR, F);
+#pragma unroll
+        for (int v = 0; v < NC; ++v) f[v][cb + c * cs] = F[v];
+      });
+}
+
+// ---------------------------------------------------------------------------------
+// hydro::CalculateEMF (hydro_add_flux_tasks.cpp:387-442), arithmetic averaging
+// E3(i-1/2,j-1/2) = 1/4 [Fy(Bx)(i-1,j) + Fy(Bx)(i,j) - Fx(By)(i,j-1) - Fx(By)(i,j)] and
+// cyclic; each edge family over its own interior range.
+// ---------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_emf(km_pack_t u) {
+  const Geo g(u);
+  const long n = g.nstag();
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= n * u.nblocks) return;
+  const int b = (int)(t / n);
+  long r = t % n;
+  const int i = (int)(r % g.S[0]);
+  r /= g.S[0];
+  const int j = (int)(r % g.S[1]);
+  const int k = (int)(r / g.S[1]);
+  const long s = g.s(k, j, i);
+  const long sj = g.sc[1], sk = g.sc[2];
+  const bool in_i = i >= g.lo[0] && i <= g.hi[0], in_i1 = i >= g.lo[0] && i <= g.hi[0] + 1;
+  const bool in_j = j >= g.lo[1] && j <= g.hi[1], in_j1 = j >= g.lo[1] && j <= g.hi[1] + 1;
+  const bool in_k = k >= g.lo[2] && k <= g.hi[2];
+  const bool in_k1 = k >= g.lo[2] && k <= g.hi[2] + (g.ndim > 2 ? 1 : 0);
+  if (i >= g.E[0] || j >= g.E[1] || k >= g.E[2]) return;
+  const long c = g.c(k, j, i);
+  if (in_i1 && in_j1 && in_k) {
+    const double *fyBx = flux_ptr(u, 1, b, C_MAGC + 0), *fxBy = flux_ptr(u, 0, b, C_MAGC + 1);
+    u.edge[2][b][s] = 1.0 * 0.25 * (fyBx[c - 1] + fyBx[c] - fxBy[c - sj] - fxBy[c]);
+  }
+  if (g.ndim > 2) {
+    if (in_i && in_j1 && in_k1) {
+      const double *fzBy = flux_ptr(u, 2, b, C_MAGC + 1), *fyBz = flux_ptr(u, 1, b, C_MAGC + 2);
+      u.edge[0][b][s] = 1.0 * 0.25 * (fzBy[c - sj] + fzBy[c] - fyBz[c - sk] - fyBz[c]);
+    }
+    if (in_i1 && in_j && in_k1) {
+      const double *fxBz = flux_ptr(u, 0, b, C_MAGC + 2), *fzBx = flux_ptr(u, 2, b, C_MAGC + 0);
+      u.edge[1][b][s] = 1.0 * 0.25 * (fxBz[c - sk] + fxBz[c] - fzBx[c - 1] - fzBx[c]);
+    }
+  }
+}
+
+// interior cell (b, k, j, i) of thread t; false past the end
+__device__ __forceinline__ bool interior_cell(const Geo &g, int nblocks, int &b, int &k, int &j,
+                                              int &i) {
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long n = (long)g.nx[0] * (g.ndim > 1 ? g.nx[1] : 1) * (g.ndim > 2 ? g.nx[2] : 1);
+  if (t >= n * nblocks) return false;
+  b = (int)(t / n);
+  long r = t % n;
+  i = g.lo[0] + (int)(r % g.nx[0]);
+  r /= g.nx[0];
+  const int ny = g.ndim > 1 ? g.nx[1] : 1;
+  j = g.lo[1] + (int)(r % ny);
+  k = g.lo[2] + (int)(r / ny);
+  return true;
+}
+
+// grid::FluxDivergence (grid_update.hpp:15-57)
+__global__ void __launch_bounds__(256) k_flux_div(km_pack_t u, km_pack_t dudt, int nc) {
+  const Geo g(u);
+  int b, k, j, i;
+  if (!interior_cell(g, u.nblocks, b, k, j, i)) return;
+  const km_block_t &blk = u.blocks[b];
+  const long c = g.c(k, j, i);
+  for (int var = 0; var < nc; ++var) {
+    double du = 0.;
+    for (int dir = 0; dir < g.ndim; ++dir) {
+      const double *F = flux_ptr(u, dir, b, var);
+      du -= (blk.area[dir] * F[c + g.sc[dir]] - blk.area[dir] * F[c]);
+    }
+    cell_ptr(dudt, b, var)[c] = du / blk.vol;
+  }
+}
+
+// grid::FluxStokes<geom, Face, edges...> (grid_update.hpp:80-137)
+template <int FACE>
+__global__ void __launch_bounds__(256) k_stokes(km_pack_t u, km_pack_t dudt) {
+  const Geo g(u);
+  const int di = FACE == 0, dj = FACE == 1, dk = FACE == 2;
+  const int ni = g.nx[0] + di, nj = (g.ndim > 1 ? g.nx[1] : 1) + dj,
+            nk = (g.ndim > 2 ? g.nx[2] : 1) + dk;
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long n = (long)ni * nj * nk;
+  if (t >= n * u.nblocks) return;
+  const int b = (int)(t / n);
+  long r = t % n;
+  const int i = g.lo[0] + (int)(r % ni);
+  r /= ni;
+  const int j = g.lo[1] + (int)(r % nj);
+  const int k = g.lo[2] + (int)(r / nj);
+  const km_block_t &blk = u.blocks[b];
+  const long s = g.s(k, j, i);
+  // (edge, axis of the upper edge, sign): AxisFromFaceEdge + permutation sign,
+  // grid_update.hpp:59-78,115-125; template edge order F1:E3,E2  F2:E3,E1  F3:E1,E2
+  constexpr int e0 = FACE == 0 ? 2 : FACE == 1 ? 2 : 0;
+  constexpr int a0 = FACE == 0 ? 1 : FACE == 1 ? 0 : 1;
+  constexpr double s0 = FACE == 0 ? 1.0 : -1.0;
+  constexpr int e1 = FACE == 0 ? 1 : FACE == 1 ? 0 : 1;
+  constexpr int a1 = FACE == 0 ? 2 : FACE == 1 ? 2 : 0;
+  constexpr double s1 = FACE == 0 ? -1.0 : 1.0;
+  double acc = 0.;
+  {
+    const double *E = u.edge[e0][b];
+    const double L = blk.dx[e0];
+    acc += s0 * (L * E[s] - L * E[s + g.ss[a0]]);
+  }
+  if (g.ndim > 2) {
+    const double *E = u.edge[e1][b];
+    const double L = blk.dx[e1];
+    acc += s1 * (L * E[s] - L * E[s + g.ss[a1]]);
+  }
+  acc *= 1. / (blk.area[FACE] + 1.e-36);
+  dudt.face[FACE][b][s] = acc;
+}
+
+// grid::ApplyDuDt_impl (grid.cpp:213-241), cell part
+__global__ void __launch_bounds__(256) k_apply_cell(km_pack_t base, km_pack_t u0, km_pack_t u1,
+                                                    km_pack_t dudt, int nc, double beta,
+                                                    double dt) {
+  const Geo g(u0);
+  int b, k, j, i;
+  if (!interior_cell(g, u0.nblocks, b, k, j, i)) return;
+  const long c = g.c(k, j, i);
+  for (int var = 0; var < nc; ++var) {
+    double *p0 = cell_ptr(u0, b, var);
+    const double v0 = beta * cell_ptr(base, b, var)[c] + (1.0 - beta) * p0[c];
+    p0[c] = v0;
+    cell_ptr(u1, b, var)[c] = v0 + beta * dt * cell_ptr(dudt, b, var)[c];
+  }
+}
+
+// grid::ApplyDuDt_impl (grid.cpp:262-282), face part
+template <int FACE>
+__global__ void __launch_bounds__(256) k_apply_face(km_pack_t base, km_pack_t u0, km_pack_t u1,
+                                                    km_pack_t dudt, double beta, double dt) {
+  const Geo g(u0);
+  const int di = FACE == 0, dj = FACE == 1, dk = FACE == 2;
+  const int ni = g.nx[0] + di, nj = (g.ndim > 1 ? g.nx[1] : 1) + dj,
+            nk = (g.ndim > 2 ? g.nx[2] : 1) + dk;
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long n = (long)ni * nj * nk;
+  if (t >= n * u0.nblocks) return;
+  const int b = (int)(t / n);
+  long r = t % n;
+  const int i = g.lo[0] + (int)(r % ni);
+  r /= ni;
+  const int j = g.lo[1] + (int)(r % nj);
+  const int k = g.lo[2] + (int)(r / nj);
+  const long s = g.s(k, j, i);
+  double *p0 = u0.face[FACE][b];
+  const double v0 = beta * base.face[FACE][b][s] + (1.0 - beta) * p0[s];
+  p0[s] = v0;
+  u1.face[FACE][b][s] = v0 + beta * dt * dudt.face[FACE][b][s];
+}
+
+// CT cell-centred B from face B (primconsflux.cpp:102-128)
+template <int MHD>
+__device__ __forceinline__ void face_average_b(const km_pack_t &u, const Geo &g, int b, int k,
+                                               int j, int i) {
+  if constexpr (MHD) {
+    const km_block_t &blk = u.blocks[b];
+    const long c = g.c(k, j, i), s = g.s(k, j, i);
+    if (g.ndim > 1) {
+      const double *B1 = u.face[0][b], *B2 = u.face[1][b];
+      cell_ptr(u, b, 5)[c] =
+          0.5 * blk.dx[0] * (blk.area[0] * B1[s + 1] + blk.area[0] * B1[s]) / blk.vol;
+      cell_ptr(u, b, 6)[c] =
+          0.5 * blk.dx[1] * (blk.area[1] * B2[s + g.ss[1]] + blk.area[1] * B2[s]) / blk.vol;
+    }
+    if (g.ndim > 2) {
+      const double *B3 = u.face[2][b];
+      cell_ptr(u, b, 7)[c] =
+          0.5 * blk.dx[2] * (blk.area[2] * B3[s + g.ss[2]] + blk.area[2] * B3[s]) / blk.vol;
+    }
+  }
+}
+
+// hydro PreparePrimitive_impl (primconsflux.cpp:82-142) + eos EosWrapped(ener)
+template <int MHD>
+__global__ void __launch_bounds__(256) k_prepare_primitive(km_pack_t u, Eos eos) {
+  constexpr int nc = HV<MHD>::NC;
+  const Geo g(u);
+  int b, k, j, i;
+  if (!interior_cell(g, u.nblocks, b, k, j, i)) return;
+  face_average_b<MHD>(u, g, b, k, j, i);
+  const long c = g.c(k, j, i);
+  const double dens = cell_ptr(u, b, 0)[c];
+  const double idens = 1.0 / dens;
+  double emag = 0., ekin = 0.;
+#pragma unroll
+  for (int dir = 0; dir < 3; dir++) {
+    const double v = idens * cell_ptr(u, b, 1 + dir)[c];
+    cell_ptr(u, b, nc + dir)[c] = v;
+    ekin += v * v;
+    if constexpr (MHD) {
+      const double B = cell_ptr(u, b, 5 + dir)[c];
+      emag += B * B;
+    }
+  }
+  ekin *= 0.5 * dens;
+  emag *= 0.5;
+  const double eint = cell_ptr(u, b, 4)[c] - ekin - emag;
+  const double sie = idens * eint;
+  cell_ptr(u, b, nc + 5)[c] = sie;
+  cell_ptr(u, b, nc + 3)[c] = eos_pres(eos, dens, sie);
+  cell_ptr(u, b, nc + 6)[c] = eos_temp(eos, dens, sie);
+  cell_ptr(u, b, nc + 4)[c] = eos_bmod(eos, dens, sie);
+}
+
+// eos PrepareConserved(dens_pres) then hydro ConvertToConserved_impl
+// (primconsflux.cpp:19-79)
+template <int MHD>
+__global__ void __launch_bounds__(256) k_prepare_conserved(km_pack_t u, Eos eos) {
+  constexpr int nc = HV<MHD>::NC, NR = HV<MHD>::NR;
+  const Geo g(u);
+  int b, k, j, i;
+  if (!interior_cell(g, u.nblocks, b, k, j, i)) return;
+  const long c = g.c(k, j, i);
+  const double dens = cell_ptr(u, b, 0)[c];
+  const double sie = cell_ptr(u, b, nc + 3)[c] / (eos.gm1 * dens);
+  cell_ptr(u, b, nc + 5)[c] = sie;
+  cell_ptr(u, b, nc + 6)[c] = eos_temp(eos, dens, sie);
+  cell_ptr(u, b, nc + 4)[c] = eos_bmod(eos, dens, sie);
+  face_average_b<MHD>(u, g, b, k, j, i);
+  double V[NR], U[nc];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) V[r] = cell_ptr(u, b, recon_slot<MHD>(r))[c];
+  prim2cons<MHD>(V, U);
+#pragma unroll
+  for (int v = 0; v < nc; ++v) cell_ptr(u, b, v)[c] = U[v];
+}
+
+__device__ __forceinline__ void atomic_min_pos(double *out, double v) {
+  // positive doubles order like their bit patterns
+  atomicMin(reinterpret_cast<unsigned long long *>(out),
+            static_cast<unsigned long long>(__double_as_longlong(v)));
+}
+
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = kmin(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+__global__ void k_fill_value(double *p, double v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+// hydro EstimateTimeStep (hydro_time_step.cpp:19-65): min over the interior of
+// dx/(|v| + cf); *out must hold DBL_MAX on entry
+template <int MHD>
+__global__ void __launch_bounds__(256) k_estimate_dt(km_pack_t u, double *out) {
+  constexpr int NR = HV<MHD>::NR;
+  const Geo g(u);
+  int b, k, j, i;
+  double dtl = DBL_MAX;
+  if (interior_cell(g, u.nblocks, b, k, j, i)) {
+    const km_block_t &blk = u.blocks[b];
+    const long c = g.c(k, j, i);
+    double V[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) V[r] = cell_ptr(u, b, recon_slot<MHD>(r))[c];
+    dtl = kmin(dtl, blk.dx[0] / (kabs(V[R_VEL + 0]) + fast_speed<MHD, 0>(V)));
+    if (g.ndim > 1) dtl = kmin(dtl, blk.dx[1] / (kabs(V[R_VEL + 1]) + fast_speed<MHD, 1>(V)));
+    if (g.ndim > 2) dtl = kmin(dtl, blk.dx[2] / (kabs(V[R_VEL + 2]) + fast_speed<MHD, 2>(V)));
+  }
+  dtl = wave_min(dtl);
+  if ((threadIdx.x & 63) == 0) atomic_min_pos(out, dtl);
+}
+
+__global__ void k_scale_value(double *p, double s) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = *p * s;
+}
+
+// hydro FillDerived_impl (hydro.cpp:150-194)
+__global__ void __launch_bounds__(256) k_fill_divb(km_pack_t u, int slot) {
+  const Geo g(u);
+  int b, k, j, i;
+  if (!interior_cell(g, u.nblocks, b, k, j, i)) return;
+  const km_block_t &blk = u.blocks[b];
+  const long s = g.s(k, j, i);
+  double divb = 0.;
+  for (int dir = 0; dir < g.ndim; dir++) {
+    const double *B = u.face[dir][b];
+    divb += blk.area[dir] * B[s + g.ss[dir]] - blk.area[dir] * B[s];
+  }
+  cell_ptr(u, b, slot)[g.c(k, j, i)] = divb / blk.vol;
+}
+
+// ---------------------------------------------------------------------------------
+// Same-level ghost fill of a uniform block grid (Parthenon boundary exchange plus
+// periodic / outflow / reflect boundaries): every ghost zone, including edges and
+// corners, maps per dimension to a source zone of the same or a neighbouring block.
+// ---------------------------------------------------------------------------------
+__device__ __forceinline__ int map_cell(int g, int N, int lo, int hi, bool &flip) {
+  if (g < 0) {
+    if (lo == KM_BC_PERIODIC) return ((g % N) + N) % N;
+    if (lo == KM_BC_OUTFLOW) return 0;
+    flip = !flip;
+    return -g - 1;
+  }
+  if (g >= N) {
+    if (hi == KM_BC_PERIODIC) return g % N;
+    if (hi == KM_BC_OUTFLOW) return N - 1;
+    flip = !flip;
+    return 2 * N - g - 1;
+  }
+  return g;
+}
+__device__ __forceinline__ int map_face(int g, int N, int lo, int hi, bool &flip) {
+  if (g < 0) {
+    if (lo == KM_BC_PERIODIC) return ((g % N) + N) % N;
+    if (lo == KM_BC_OUTFLOW) return 0;
+    flip = !flip;
+    return -g;
+  }
+  if (g > N) {
+    if (hi == KM_BC_PERIODIC) return g % N;
+    if (hi == KM_BC_OUTFLOW) return N;
+    flip = !flip;
+    return 2 * N - g;
+  }
+  return g;
+}
+
+__global__ void __launch_bounds__(256) k_ghost_cells(km_pack_t u, UniformGrid gr, int v0, int v1,
+                                                     int nc) {
+  const Geo g(u);
+  const long n = g.ncell();
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= n * u.nblocks) return;
+  const int b = (int)(t / n);
+  long r = t % n;
+  int idx[3];
+  idx[0] = (int)(r % g.E[0]);
+  r /= g.E[0];
+  idx[1] = (int)(r % g.E[1]);
+  idx[2] = (int)(r / g.E[1]);
+  bool ghost = false;
+  for (int d = 0; d < g.ndim; ++d)
+    if (idx[d] < g.lo[d] || idx[d] > g.hi[d]) ghost = true;
+  if (!ghost) return;
+  const int bl[3] = {b % gr.nb[0], (b / gr.nb[0]) % gr.nb[1], b / (gr.nb[0] * gr.nb[1])};
+  int src[3], sb[3];
+  bool flip[3] = {false, false, false};
+  for (int d = 0; d < 3; ++d) {
+    if (d >= g.ndim) {
+      sb[d] = 0;
+      src[d] = idx[d];
+      continue;
+    }
+    const int N = gr.nb[d] * g.nx[d];
+    const int m = map_cell(bl[d] * g.nx[d] + (idx[d] - g.lo[d]), N, gr.bc[2 * d], gr.bc[2 * d + 1],
+                           flip[d]);
+    sb[d] = m / g.nx[d];
+    src[d] = m % g.nx[d] + g.lo[d];
+  }
+  const int srcb = (sb[2] * gr.nb[1] + sb[1]) * gr.nb[0] + sb[0];
+  const long dc = g.c(idx[2], idx[1], idx[0]), scx = g.c(src[2], src[1], src[0]);
+  for (int v = v0; v < v1; ++v) {
+    double val = cell_ptr(u, srcb, v)[scx];
+    int ax = -1;
+    if (v >= 1 && v < 4) ax = v - 1;
+    else if (v >= nc && v < nc + 3) ax = v - nc;
+    else if (nc == 8 && v >= 5 && v < 8) ax = v - 5;
+    if (ax >= 0 && flip[ax]) val = -val;
+    cell_ptr(u, b, v)[dc] = val;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_ghost_faces(km_pack_t u, UniformGrid gr, int face) {
+  const Geo g(u);
+  const long n = g.nstag();
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= n * u.nblocks) return;
+  const int b = (int)(t / n);
+  long r = t % n;
+  int idx[3];
+  idx[0] = (int)(r % g.S[0]);
+  r /= g.S[0];
+  idx[1] = (int)(r % g.S[1]);
+  idx[2] = (int)(r / g.S[1]);
+  bool ghost = false;
+  for (int d = 0; d < 3; ++d) {
+    const int ext = d < g.ndim ? g.E[d] + (d == face ? 1 : 0) : 1;
+    if (idx[d] >= ext) return;
+    const int hi = g.hi[d] + (d == face ? 1 : 0);
+    if (d < g.ndim && (idx[d] < g.lo[d] || idx[d] > hi)) ghost = true;
+  }
+  if (!ghost) return;
+  const int bl[3] = {b % gr.nb[0], (b / gr.nb[0]) % gr.nb[1], b / (gr.nb[0] * gr.nb[1])};
+  int src[3], sb[3];
+  bool flip[3] = {false, false, false};
+  for (int d = 0; d < 3; ++d) {
+    if (d >= g.ndim) {
+      sb[d] = 0;
+      src[d] = idx[d];
+      continue;
+    }
+    const int N = gr.nb[d] * g.nx[d];
+    const int gi = bl[d] * g.nx[d] + (idx[d] - g.lo[d]);
+    if (d == face) {
+      const int m = map_face(gi, N, gr.bc[2 * d], gr.bc[2 * d + 1], flip[d]);
+      sb[d] = min(m / g.nx[d], gr.nb[d] - 1);
+      src[d] = m - sb[d] * g.nx[d] + g.lo[d];
+    } else {
+      const int m = map_cell(gi, N, gr.bc[2 * d], gr.bc[2 * d + 1], flip[d]);
+      sb[d] = m / g.nx[d];
+      src[d] = m % g.nx[d] + g.lo[d];
+    }
+  }
+  const int srcb = (sb[2] * gr.nb[1] + sb[1]) * gr.nb[0] + sb[0];
+  double val = u.face[face][srcb][g.s(src[2], src[1], src[0])];
+  if (flip[face]) val = -val;
+  u.face[face][b][g.s(idx[2], idx[1], idx[0])] = val;
+}
+
+// ---------------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------------
+static inline unsigned nblk(long n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+template <int RECON, int RIEM, int MHD>
+static void launch_flux_dirs(const km_pack_t &u, int lim, const Box &bx, hipStream_t st) {
+  const Geo g(u);
+  auto cnt = [&](int da, int db) {
+    return (long)(bx.hi[da] - bx.lo[da] + 1) * (bx.hi[db] - bx.lo[db] + 1) * u.nblocks;
+  };
+  hipLaunchKernelGGL((k_flux<RECON, RIEM, MHD, 0>), dim3(nblk(cnt(1, 2))), dim3(256), 0, st, u,
+                     lim, bx);
+  if (g.ndim > 1)
+    hipLaunchKernelGGL((k_flux<RECON, RIEM, MHD, 1>), dim3(nblk(cnt(0, 2))), dim3(256), 0, st, u,
+                       lim, bx);
+  if (g.ndim > 2)
+    hipLaunchKernelGGL((k_flux<RECON, RIEM, MHD, 2>), dim3(nblk(cnt(0, 1))), dim3(256), 0, st, u,
+                       lim, bx);
+}
+
+template <int RECON, int RIEM>
+static void launch_flux_mhd(const km_pack_t &u, int mhd, int lim, const Box &bx,
+                            hipStream_t st) {
+  if (mhd)
+    launch_flux_dirs<RECON, RIEM, 1>(u, lim, bx, st);
+  else
+    launch_flux_dirs<RECON, RIEM, 0>(u, lim, bx, st);
+}
+
+template <int RECON>
+static void launch_flux_riem(const km_pack_t &u, int riem, int mhd, int lim, const Box &bx,
+                             hipStream_t st) {
+  if (riem == RIEMANN_HLLC)
+    launch_flux_mhd<RECON, RIEMANN_HLLC>(u, mhd, lim, bx, st);
+  else
+    launch_flux_mhd<RECON, RIEMANN_HLL>(u, mhd, lim, bx, st);
+}
+
+void launch_calculate_fluxes(const km_cfg_t &cfg, const km_pack_t &u, const Box &bx,
+                             hipStream_t st) {
+  switch (cfg.recon) {
+    case KM_RECON_FOG: launch_flux_riem<RECON_FOG>(u, cfg.riemann, cfg.mhd, cfg.limiter, bx, st); break;
+    case KM_RECON_PLM: launch_flux_riem<RECON_PLM>(u, cfg.riemann, cfg.mhd, cfg.limiter, bx, st); break;
+    case KM_RECON_PPM: launch_flux_riem<RECON_PPM>(u, cfg.riemann, cfg.mhd, cfg.limiter, bx, st); break;
+    default: launch_flux_riem<RECON_WENOZ>(u, cfg.riemann, cfg.mhd, cfg.limiter, bx, st); break;
+  }
+}
+
+void launch_calculate_emf(const km_pack_t &u, hipStream_t st) {
+  const Geo g(u);
+  hipLaunchKernelGGL(k_emf, dim3(nblk(g.nstag() * u.nblocks)), dim3(256), 0, st, u);
+}
+
+static long interior_count(const Geo &g, int nblocks) {
+  return (long)g.nx[0] * (g.ndim > 1 ? g.nx[1] : 1) * (g.ndim > 2 ? g.nx[2] : 1) * nblocks;
+}
+
+static long face_count(const Geo &g, int face, int nblocks) {
+  return (long)(g.nx[0] + (face == 0)) * ((g.ndim > 1 ? g.nx[1] : 1) + (face == 1)) *
+         ((g.ndim > 2 ? g.nx[2] : 1) + (face == 2)) * nblocks;
+}
+
+void launch_fluxes_to_dudt(const km_cfg_t &cfg, const km_pack_t &u, const km_pack_t &dudt,
+                           hipStream_t st) {
+  const Geo g(u);
+  const int nc = cfg.mhd ? 8 : 5;
+  hipLaunchKernelGGL(k_flux_div, dim3(nblk(interior_count(g, u.nblocks))), dim3(256), 0, st, u,
+                     dudt, nc);
+  if (cfg.mhd && g.ndim >= 2) {
+    hipLaunchKernelGGL(k_stokes<0>, dim3(nblk(face_count(g, 0, u.nblocks))), dim3(256), 0, st,
+                       u, dudt);
+    hipLaunchKernelGGL(k_stokes<1>, dim3(nblk(face_count(g, 1, u.nblocks))), dim3(256), 0, st,
+                       u, dudt);
+    if (g.ndim > 2)
+      hipLaunchKernelGGL(k_stokes<2>, dim3(nblk(face_count(g, 2, u.nblocks))), dim3(256), 0, st,
+                         u, dudt);
+  }
+}
+
+void launch_apply_dudt(const km_cfg_t &cfg, const km_pack_t &base, const km_pack_t &u0,
+                       const km_pack_t &u1, const km_pack_t &dudt, double beta, double dt,
+                       hipStream_t st) {
+  const Geo g(u0);
+  const int nc = cfg.mhd ? 8 : 5;
+  hipLaunchKernelGGL(k_apply_cell, dim3(nblk(interior_count(g, u0.nblocks))), dim3(256), 0, st,
+                     base, u0, u1, dudt, nc, beta, dt);
+  if (cfg.mhd && g.ndim >= 2) {
+    hipLaunchKernelGGL(k_apply_face<0>, dim3(nblk(face_count(g, 0, u0.nblocks))), dim3(256), 0,
+                       st, base, u0, u1, dudt, beta, dt);
+    hipLaunchKernelGGL(k_apply_face<1>, dim3(nblk(face_count(g, 1, u0.nblocks))), dim3(256), 0,
+                       st, base, u0, u1, dudt, beta, dt);
+    if (g.ndim > 2)
+      hipLaunchKernelGGL(k_apply_face<2>, dim3(nblk(face_count(g, 2, u0.nblocks))), dim3(256),
+                         0, st, base, u0, u1, dudt, beta, dt);
+  }
+}
+
+void launch_prepare_primitive(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st) {
+  const Geo g(u);
+  const Eos eos = make_eos(cfg);
+  const unsigned grid = nblk(interior_count(g, u.nblocks));
+  if (cfg.mhd)
+    hipLaunchKernelGGL(k_prepare_primitive<1>, dim3(grid), dim3(256), 0, st, u, eos);
+  else
+    hipLaunchKernelGGL(k_prepare_primitive<0>, dim3(grid), dim3(256), 0, st, u, eos);
+}
+
+void launch_prepare_conserved(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st) {
+  const Geo g(u);
+  const Eos eos = make_eos(cfg);
+  const unsigned grid = nblk(interior_count(g, u.nblocks));
+  if (cfg.mhd)
+    hipLaunchKernelGGL(k_prepare_conserved<1>, dim3(grid), dim3(256), 0, st, u, eos);
+  else
+    hipLaunchKernelGGL(k_prepare_conserved<0>, dim3(grid), dim3(256), 0, st, u, eos);
+}
+
+void launch_estimate_dt(const km_cfg_t &cfg, const km_pack_t &u, double *dt_dev,
+                        hipStream_t st) {
+  const Geo g(u);
+  hipLaunchKernelGGL(k_fill_value, dim3(1), dim3(64), 0, st, dt_dev, DBL_MAX);
+  const unsigned grid = nblk(interior_count(g, u.nblocks));
+  if (cfg.mhd)
+    hipLaunchKernelGGL(k_estimate_dt<1>, dim3(grid), dim3(256), 0, st, u, dt_dev);
+  else
+    hipLaunchKernelGGL(k_estimate_dt<0>, dim3(grid), dim3(256), 0, st, u, dt_dev);
+  hipLaunchKernelGGL(k_scale_value, dim3(1), dim3(64), 0, st, dt_dev,
+                     cfg.cfl / static_cast<double>(g.ndim));
+}
+
+void launch_fill_divb(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st) {
+  const Geo g(u);
+  hipLaunchKernelGGL(k_fill_divb, dim3(nblk(interior_count(g, u.nblocks))), dim3(256), 0, st, u,
+                     KM_DIVB(cfg.mhd));
+}
+
+void launch_ghost_fill(const UniformGrid &gr, const km_pack_t &u, int v0, int v1, int nc,
+                       bool faces, hipStream_t st) {
+  const Geo g(u);
+  if (v1 > v0)
+    hipLaunchKernelGGL(k_ghost_cells, dim3(nblk(g.ncell() * u.nblocks)), dim3(256), 0, st, u,
+                       gr, v0, v1, nc);
+  if (faces)
+    for (int f = 0; f < (g.ndim > 2 ? 3 : 2); ++f)
+      hipLaunchKernelGGL(k_ghost_faces, dim3(nblk(g.nstag() * u.nblocks)), dim3(256), 0, st, u,
+                         gr, f);
+}
+
+}  // namespace km

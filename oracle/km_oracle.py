@@ -1,0 +1,225 @@
+"""Python access to the CPU FP64 oracle (libkm_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by the
+CPU-baseline leg of bench.py, as the checker. The product (kamayan_amd) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import pathlib
+import subprocess
+
+import numpy as np
+
+from kamayan_amd._abi import KmBlock, KmCfg, KmMeshDesc, KmPack, Layout, f64ptr
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = HERE / "libkm_oracle.so"
+
+_lib = None
+
+
+def build() -> pathlib.Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        build()
+    L = C.CDLL(str(LIB_PATH))
+    vp, i32, dbl = C.c_void_p, C.c_int32, C.c_double
+    P = C.POINTER
+    sig = {
+        "km_cpu_reconstruct": [i32, i32, P(dbl), P(dbl), P(dbl)],
+        "km_cpu_riemann": [i32, i32, i32, P(dbl), P(dbl), P(dbl)],
+        "km_cpu_eos": [i32, dbl, dbl, P(dbl)],
+        "km_cpu_fast_speed": [i32, i32, P(dbl), P(dbl)],
+        "km_cpu_prim2cons": [i32, P(dbl), P(dbl)],
+        "km_cpu_prim2flux": [i32, i32, P(dbl), P(dbl)],
+        "km_cpu_calculate_fluxes": [P(KmCfg), P(KmPack), vp],
+        "km_cpu_calculate_emf": [P(KmCfg), P(KmPack), vp],
+        "km_cpu_fluxes_to_dudt": [P(KmCfg), P(KmPack), P(KmPack), vp],
+        "km_cpu_apply_dudt": [P(KmCfg), P(KmPack), P(KmPack), P(KmPack), P(KmPack), dbl, dbl,
+                              vp],
+        "km_cpu_prepare_primitive": [P(KmCfg), P(KmPack), vp],
+        "km_cpu_prepare_conserved": [P(KmCfg), P(KmPack), vp],
+        "km_cpu_estimate_dt": [P(KmCfg), P(KmPack), P(dbl), vp],
+        "km_cpu_fill_divb": [P(KmCfg), P(KmPack), vp],
+        "km_cpu_ghost_fill_uniform": [P(KmCfg), P(KmPack), P(i32), P(i32), i32, i32, i32],
+        "km_cpu_mesh_blocks": [P(KmMeshDesc), vp],
+        "km_cpu_sim_create": [P(KmCfg), P(KmMeshDesc), P(vp)],
+        "km_cpu_sim_destroy": [vp],
+        "km_cpu_sim_set_primitive": [vp, vp, P(vp)],
+        "km_cpu_sim_get_state": [vp, vp, P(vp)],
+        "km_cpu_sim_advance": [vp, i32, dbl],
+        "km_cpu_sim_info": [vp, P(dbl), P(dbl), P(i32)],
+        "km_cpu_sim_set_dt": [vp, dbl],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    _lib = L
+    return L
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def reconstruct(recon: int, limiter: int, s5) -> tuple:
+    s = np.ascontiguousarray(s5, dtype=np.float64)
+    vm, vp = C.c_double(), C.c_double()
+    lib().km_cpu_reconstruct(recon, limiter, _dp(s), C.byref(vm), C.byref(vp))
+    return vm.value, vp.value
+
+
+def riemann(solver: int, mhd: int, d: int, vl, vr) -> np.ndarray:
+    a = np.zeros(10)
+    b = np.zeros(10)
+    a[: len(vl)] = vl
+    b[: len(vr)] = vr
+    f = np.zeros(8)
+    lib().km_cpu_riemann(solver, mhd, d, _dp(a), _dp(b), _dp(f))
+    return f[: (8 if mhd else 5)]
+
+
+def eos(mode: int, gamma: float, abar: float, state) -> np.ndarray:
+    s = np.ascontiguousarray(state, dtype=np.float64).copy()
+    lib().km_cpu_eos(mode, gamma, abar, _dp(s))
+    return s
+
+
+def fast_speed(mhd: int, d: int, V) -> float:
+    a = np.zeros(10)
+    a[: len(V)] = V
+    out = C.c_double()
+    lib().km_cpu_fast_speed(mhd, d, _dp(a), C.byref(out))
+    return out.value
+
+
+def prim2cons(mhd: int, V) -> np.ndarray:
+    a = np.zeros(10)
+    a[: len(V)] = V
+    u = np.zeros(8)
+    lib().km_cpu_prim2cons(mhd, _dp(a), _dp(u))
+    return u[: (8 if mhd else 5)]
+
+
+def prim2flux(mhd: int, d: int, V) -> np.ndarray:
+    a = np.zeros(10)
+    a[: len(V)] = V
+    f = np.zeros(8)
+    lib().km_cpu_prim2flux(mhd, d, _dp(a), _dp(f))
+    return f[: (8 if mhd else 5)]
+
+
+def block_table(desc: KmMeshDesc, nblocks: int):
+    arr = (KmBlock * nblocks)()
+    lib().km_cpu_mesh_blocks(C.byref(desc), C.cast(arr, C.c_void_p))
+    return arr
+
+
+class HostPack:
+    """numpy block-pack arrays and host pointer tables for a km_pack_t."""
+
+    def __init__(self, layout: Layout, nv: int, nflux: int = 0, faces: bool = False,
+                 edges: bool = False, blocks=None):
+        self.layout = layout
+        nb = layout.nblocks
+        self.cell = np.zeros(layout.cell_shape(nv))
+        self.flux = [np.zeros(layout.cell_shape(nflux)) if (nflux and d < layout.ndim) else None
+                     for d in range(3)]
+        self.face = [np.zeros(layout.stag_shape()) if faces else None for d in range(3)]
+        self.edge = [np.zeros(layout.stag_shape()) if edges else None for d in range(3)]
+        self._keep = []
+        p = self.pack = KmPack()
+        p.nblocks = nb
+        p.ndim = layout.ndim
+        p.ng = layout.ng
+        for d in range(3):
+            p.nx[d] = layout.nx[d] if d < layout.ndim else 1
+        p.nvar = nv
+        p.nflux = nflux
+        p.cell = self._table(self.cell.reshape(nb * nv, -1))
+        for d in range(3):
+            if self.flux[d] is not None:
+                p.flux[d] = self._table(self.flux[d].reshape(nb * nflux, -1))
+            if self.face[d] is not None:
+                p.face[d] = self._table(self.face[d].reshape(nb, -1))
+            if self.edge[d] is not None:
+                p.edge[d] = self._table(self.edge[d].reshape(nb, -1))
+        self.blocks = blocks
+        if blocks is not None:
+            p.blocks = C.cast(blocks, C.c_void_p)
+
+    def _table(self, rows: np.ndarray) -> int:
+        t = np.array([rows[r].ctypes.data for r in range(rows.shape[0])], dtype=np.uint64)
+        self._keep.append(t)
+        return t.ctypes.data
+
+
+def call(name: str, *args):
+    rc = getattr(lib(), name)(*args)
+    if rc != 0:
+        raise RuntimeError(f"{name} returned {rc}")
+
+
+class CpuSimulation:
+    """km_cpu_sim_*: the oracle's whole-mesh RK driver."""
+
+    def __init__(self, cfg: KmCfg, desc: KmMeshDesc, layout: Layout):
+        self.cfg, self.desc, self.layout = cfg, desc, layout
+        self.h = C.c_void_p()
+        call("km_cpu_sim_create", C.byref(cfg), C.byref(desc), C.byref(self.h))
+
+    def set_primitive(self, cell: np.ndarray, faces=None):
+        fp = (C.c_void_p * 3)()
+        if faces is not None:
+            for d in range(3):
+                fp[d] = faces[d].ctypes.data
+        call("km_cpu_sim_set_primitive", self.h, f64ptr(cell), fp)
+
+    def state(self):
+        mhd = self.cfg.mhd
+        nv = 16 if mhd == 1 else 12
+        cell = np.zeros(self.layout.cell_shape(nv))
+        fp = (C.c_void_p * 3)()
+        faces = None
+        if mhd == 1:
+            faces = [np.zeros(self.layout.stag_shape()) for _ in range(3)]
+            for d in range(3):
+                fp[d] = faces[d].ctypes.data
+        call("km_cpu_sim_get_state", self.h, f64ptr(cell), fp)
+        return cell, faces
+
+    def advance(self, ncycles: int, tlim: float = 0.0):
+        call("km_cpu_sim_advance", self.h, ncycles, tlim)
+
+    def set_dt(self, dt: float):
+        call("km_cpu_sim_set_dt", self.h, dt)
+
+    def info(self):
+        t, dt, n = C.c_double(), C.c_double(), C.c_int32()
+        call("km_cpu_sim_info", self.h, C.byref(t), C.byref(dt), C.byref(n))
+        return t.value, dt.value, n.value
+
+    def close(self):
+        if self.h:
+            lib().km_cpu_sim_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def set_threads(n: int) -> None:
+    os.environ["OMP_NUM_THREADS"] = str(n)

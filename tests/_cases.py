@@ -1,0 +1,72 @@
+"""Problem set-ups shared by the oracle and GPU tests (inputs only)."""
+from __future__ import annotations
+
+import numpy as np
+
+import kamayan_amd as km
+from kamayan_amd import Config, Layout, mesh_desc
+
+
+def vortex(n=64, nblock=32, ng=3, recon="wenoz", limiter="minmod", riemann="hll",
+           integrator="rk2", cfl=0.8):
+    """isentropic_vortex.in with the reconstruction suite's overrides
+    (reconstruction.py:152-171): [-5,5]^2 periodic, gamma=1.4."""
+    cfg = Config(**{"hydro/reconstruction": recon, "hydro/slope_limiter": limiter,
+                    "hydro/riemann": riemann, "parthenon/time/integrator": integrator,
+                    "hydro/cfl": cfl, "material/single/gamma": 1.4})
+    cfg.c.ndim = 2
+    lay = Layout(2, ng, (nblock, nblock, 1), (n // nblock, n // nblock, 1))
+    desc = mesh_desc(lay, xmin=(-5.0, -5.0, -5.0), xmax=(5.0, 5.0, 5.0))
+    cell, faces = km.problem_fill("isentropic_vortex", cfg, desc, lay)
+    return cfg, desc, lay, cell, faces
+
+
+def smooth3d(n=32, nblock=16, ng=4, recon="wenoz", riemann="hllc", integrator="rk3",
+             seed=20261204, precision="fp64"):
+    """3-D ideal hydro with the seeded smooth Fourier-mode state, gamma=5/3, periodic."""
+    cfg = Config(**{"hydro/reconstruction": recon, "hydro/riemann": riemann,
+                    "parthenon/time/integrator": integrator, "material/single/gamma": 5.0 / 3.0,
+                    "kamayan_amd/precision": precision})
+    cfg.c.ndim = 3
+    lay = Layout(3, ng, (nblock,) * 3, (n // nblock,) * 3)
+    desc = mesh_desc(lay)
+    cell, faces = km.problem_fill("smooth3d", cfg, desc, lay, seed=seed)
+    return cfg, desc, lay, cell, faces
+
+
+def mhd3d(n=16, nblock=8, ng=4, recon="ppm", limiter="mc", riemann="hll", integrator="rk2",
+          seed=20261205, ndim=3):
+    """CT MHD with face B from the curl of a seeded vector potential (mean Bx = 1)."""
+    cfg = Config(**{"hydro/reconstruction": recon, "hydro/slope_limiter": limiter,
+                    "hydro/riemann": riemann, "parthenon/time/integrator": integrator,
+                    "material/single/gamma": 5.0 / 3.0, "physics/MHD": "ct"})
+    cfg.c.ndim = ndim
+    nxb = (nblock, nblock, nblock if ndim > 2 else 1)
+    nbs = (n // nblock, n // nblock, n // nblock if ndim > 2 else 1)
+    lay = Layout(ndim, ng, nxb, nbs)
+    desc = mesh_desc(lay)
+    cell, faces = km.problem_fill("mhd_random", cfg, desc, lay, seed=seed)
+    return cfg, desc, lay, cell, faces
+
+
+def vortex_l1(cell, lay: Layout, exact, var=0):
+    """ErrorHistory (isentropic_vortex.hpp:150-200): sum |exact - q| V / (Lx Ly)."""
+    sl = lay.interior()
+    blocks = km.block_table(lay, (-5.0, -5.0, -5.0), (5.0, 5.0, 5.0))
+    err = 0.0
+    for b in range(lay.nblocks):
+        vol = blocks[b].vol
+        d = np.abs(exact[b, var][sl] - cell[b, var][sl])
+        err += float(np.sum(d * vol))
+    return err / (10.0 * 10.0)
+
+
+def rel_l1(a, b, lay: Layout, nvars):
+    """relative L1 per variable over block interiors: sum|a-b| / sum|b|."""
+    sl = lay.interior()
+    out = []
+    for v in range(nvars):
+        num = sum(float(np.sum(np.abs(a[bk, v][sl] - b[bk, v][sl]))) for bk in range(lay.nblocks))
+        den = sum(float(np.sum(np.abs(b[bk, v][sl]))) for bk in range(lay.nblocks))
+        out.append(num / den if den > 0 else num)
+    return np.array(out)

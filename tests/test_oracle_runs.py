@@ -1,0 +1,70 @@
+"""Whole-path pins of the oracle: the reference's isentropic-vortex regression
+thresholds (reconstruction.py:129-135, isentropic_vortex.py:13-60) and the
+conservation / divergence invariants of the update."""
+import math
+
+import numpy as np
+import pytest
+
+from tests import _cases
+
+
+def _run_vortex(oracle, **kw):
+    cfg, desc, lay, cell, _ = _cases.vortex(**kw)
+    sim = oracle.CpuSimulation(cfg.c, desc, lay)
+    sim.set_primitive(cell)
+    sim.advance(100000, tlim=10.0)
+    t, _, ncyc = sim.info()
+    assert t == pytest.approx(10.0, abs=1e-12)
+    out, _ = sim.state()
+    return _cases.vortex_l1(out, lay, cell), ncyc
+
+
+# reconstruction.py:129-135 (riemann = hll, the deck default)
+CEILINGS = [("fog", "minmod", 0.3), ("plm", "minmod", 0.08), ("plm", "mc", 0.02),
+            ("plm", "van_leer", 0.03), ("ppm", "minmod", 0.02), ("ppm", "mc", 0.015),
+            ("wenoz", "minmod", 0.005)]
+
+
+@pytest.mark.parametrize("recon,lim,ceiling", CEILINGS)
+def test_vortex_error_ceilings(oracle, recon, lim, ceiling):
+    err, ncyc = _run_vortex(oracle, recon=recon, limiter=lim)
+    assert not math.isnan(err)
+    assert err < ceiling, f"{recon}-{lim}: L1 {err} >= {ceiling} after {ncyc} cycles"
+
+
+def test_vortex_convergence_slope(oracle):
+    """isentropic_vortex.py:13-60: WENO-Z, RK2, nghost 3, 32^2 -> 64^2, slope > 1.5."""
+    e32, _ = _run_vortex(oracle, n=32, nblock=16)
+    e64, _ = _run_vortex(oracle, n=64, nblock=32)
+    slope = -math.log(e64 / e32) / math.log(2.0)
+    assert slope > 1.5, (e32, e64, slope)
+
+
+def test_conservation_periodic(oracle):
+    """Flux-divergence telescopes: sum V dU/dt = 0 on a periodic mesh (3-D, HLLC)."""
+    cfg, desc, lay, cell, _ = _cases.smooth3d(n=16, nblock=8)
+    sim = oracle.CpuSimulation(cfg.c, desc, lay)
+    sim.set_primitive(cell)
+    before, _ = sim.state()
+    sim.advance(3)
+    after, _ = sim.state()
+    sl = lay.interior()
+    for v in range(5):
+        tot0 = sum(float(np.sum(before[b, v][sl])) for b in range(lay.nblocks))
+        tot1 = sum(float(np.sum(after[b, v][sl])) for b in range(lay.nblocks))
+        scale = sum(float(np.sum(np.abs(before[b, v][sl]))) for b in range(lay.nblocks)) + 1.0
+        assert abs(tot1 - tot0) < 1e-13 * scale, (v, tot0, tot1)
+
+
+def test_ct_keeps_divb_at_roundoff(oracle):
+    """Constrained transport preserves the discrete divergence (mhd_blast.py:143-158)."""
+    cfg, desc, lay, cell, faces = _cases.mhd3d(n=16, nblock=8)
+    sim = oracle.CpuSimulation(cfg.c, desc, lay)
+    sim.set_primitive(cell, faces)
+    sim.advance(4)
+    out, f = sim.state()
+    divb = out[:, 15][(slice(None),) + lay.interior()]
+    bmax = max(float(np.max(np.abs(x))) for x in f)
+    assert float(np.sqrt(np.mean(divb**2))) < 1e-10
+    assert float(np.max(np.abs(divb))) < 1e-12 * bmax / (1.0 / 16)
