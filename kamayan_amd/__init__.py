@@ -65,6 +65,9 @@ def lib() -> C.CDLL:
         "km_sim_advance": [vp, i32, dbl],
         "km_sim_info": [vp, P(dbl), P(dbl), P(i32)],
         "km_sim_set_dt": [vp, dbl],
+        "km_sim_profile": [vp, i32],
+        "km_sim_kernel_times": [vp, P(dbl), P(i32), i32],
+        "km_sim_set_tile": [vp, i32],
         "km_problem_fill": [C.c_char_p, P(KmCfg), P(KmMeshDesc), u64, P(dbl), i32, vp, P(vp)],
         "km_mesh_blocks": [P(KmMeshDesc), vp],
         "km_device_count": [P(i32)],
@@ -195,6 +198,21 @@ class Simulation:
         t, dt, n = C.c_double(), C.c_double(), C.c_int32()
         check(lib().km_sim_info(self.h, C.byref(t), C.byref(dt), C.byref(n)), "info")
         return t.value, dt.value, n.value
+
+    def profile(self, on: bool = True):
+        check(lib().km_sim_profile(self.h, 1 if on else 0), "profile")
+
+    KERNEL_KINDS = ("sweep_x1", "sweep_x2", "sweep_x3_update", "ghost_fill", "stage")
+
+    def kernel_times(self):
+        """{kind: (total ms, launches)} of the kernels recorded since profile(True)."""
+        ms = (C.c_double * 5)()
+        cnt = (C.c_int32 * 5)()
+        check(lib().km_sim_kernel_times(self.h, ms, cnt, 5), "kernel_times")
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(self.KERNEL_KINDS)}
+
+    def set_tile(self, threads: int):
+        check(lib().km_sim_set_tile(self.h, threads), "set_tile")
 
 
 class DevicePack:

@@ -137,9 +137,17 @@ KM_HD void reconstruct(const T *w, T &vM, T &vP) {
     const T tau = kabs(b2 - b0);
     // tau / (eps + beta_k) is evaluated identically for both sides; computing it once
     // per k gives the same bits as the reference's two evaluations
-    const T r0 = kdiv(tau, eps + b0);
-    const T r1 = kdiv(tau, eps + b1);
-    const T r2 = kdiv(tau, eps + b2);
+    T r0 = kdiv(tau, eps + b0);
+    T r1 = kdiv(tau, eps + b1);
+    T r2 = kdiv(tau, eps + b2);
+    if constexpr (sizeof(T) == 4) {
+      // FP32: an indicator that rounds to 0 gives tau/eps ~ 1e22, whose square overflows
+      // float. Clamping at 1e15 keeps every weight finite; a ratio that large already
+      // makes its stencil's weight dominate by > 1e20.
+      r0 = kmin(r0, T(1.e15));
+      r1 = kmin(r1, T(1.e15));
+      r2 = kmin(r2, T(1.e15));
+    }
     const T q0 = T(1.0) + r0 * r0;
     const T q1 = T(1.0) + r1 * r1;
     const T q2 = T(1.0) + r2 * r2;

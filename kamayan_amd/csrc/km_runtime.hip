@@ -184,6 +184,25 @@ struct km_sim_t {
   bool use_graph = true;
   hipGraphExec_t graph_exec = nullptr;
   hipGraph_t graph = nullptr;
+  // optional HIP-event timing of the fused-stage kernels (kinds: sweep x1, x2, x3,
+  // ghost fill, whole stage)
+  bool profile = false;
+  std::vector<hipEvent_t> ev_free;
+  struct Rec {
+    int kind;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;
+  hipEvent_t ev() {
+    hipEvent_t e;
+    if (!ev_free.empty()) {
+      e = ev_free.back();
+      ev_free.pop_back();
+    } else {
+      (void)hipEventCreate(&e);
+    }
+    return e;
+  }
 
   template <class T>
   int alloc(T **out, size_t count) {
@@ -201,6 +220,11 @@ struct km_sim_t {
     return KM_OK;
   }
   ~km_sim_t() {
+    for (auto &r : recs) {
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+    for (auto e : ev_free) (void)hipEventDestroy(e);
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (stream) (void)hipStreamDestroy(stream);
@@ -322,19 +346,50 @@ int fused_stage(km_sim_t &s, int stage, double beta) {
   a.beta = beta;
   a.lim = cfg.limiter;
   a.eos = make_eos(cfg);
+  const int ndim = s.mesh.ndim;
+  hipEvent_t st0 = nullptr, st1 = nullptr;
+  if (s.profile) {
+    st0 = s.ev();
+    st1 = s.ev();
+    (void)hipEventRecord(st0, s.stream);
+  }
   if (last) hipLaunchKernelGGL(k_set, dim3(1), dim3(64), 0, s.stream, s.est_d, DBL_MAX);
-  int rc;
-  if (cfg.precision == KM_PREC_FP32_MIXED)
-    rc = fp32::launch_fused_stage(cfg, a, s.mesh.ndim, last, s.tile, s.stream);
-  else if (cfg.precision == KM_PREC_FP64_FMA)
-    rc = fma::launch_fused_stage(cfg, a, s.mesh.ndim, last, s.tile, s.stream);
-  else
-    rc = exact::launch_fused_stage(cfg, a, s.mesh.ndim, last, s.tile, s.stream);
-  if (rc) return rc;
+  for (int dir = 0; dir < ndim; ++dir) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (s.profile) {
+      e0 = s.ev();
+      e1 = s.ev();
+      (void)hipEventRecord(e0, s.stream);
+    }
+    int rc;
+    if (cfg.precision == KM_PREC_FP32_MIXED)
+      rc = fp32::launch_fused_sweep(cfg, a, ndim, dir, last, s.tile, s.stream);
+    else if (cfg.precision == KM_PREC_FP64_FMA)
+      rc = fma::launch_fused_sweep(cfg, a, ndim, dir, last, s.tile, s.stream);
+    else
+      rc = exact::launch_fused_sweep(cfg, a, ndim, dir, last, s.tile, s.stream);
+    if (rc) return rc;
+    if (s.profile) {
+      (void)hipEventRecord(e1, s.stream);
+      s.recs.push_back({dir, e0, e1});
+    }
+  }
   if (last)
     hipLaunchKernelGGL(k_scale, dim3(1), dim3(64), 0, s.stream, s.est_d,
-                       cfg.cfl / static_cast<double>(s.mesh.ndim));
+                       cfg.cfl / static_cast<double>(ndim));
+  hipEvent_t g0 = nullptr, g1 = nullptr;
+  if (s.profile) {
+    g0 = s.ev();
+    g1 = s.ev();
+    (void)hipEventRecord(g0, s.stream);
+  }
   launch_ghost_fill(s.grid, md1.pack, 0, s.nc, s.nc, false, s.stream);
+  if (s.profile) {
+    (void)hipEventRecord(g1, s.stream);
+    (void)hipEventRecord(st1, s.stream);
+    s.recs.push_back({3, g0, g1});
+    s.recs.push_back({4, st0, st1});
+  }
   return check_launch();
 }
 
@@ -663,7 +718,7 @@ int km_sim_advance(km_sim_t *s, int32_t ncycles, double tlim) {
       continue;
     }
     // fused path, no host decisions: capture one cycle in a graph and replay it
-    if (s->use_graph) {
+    if (s->use_graph && !s->profile) {
       if (!s->graph_exec) {
         KM_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeRelaxed));
         const int nc0 = s->ncycle;
@@ -686,6 +741,50 @@ int km_sim_advance(km_sim_t *s, int32_t ncycles, double tlim) {
     }
   }
   KM_HIP(hipStreamSynchronize(s->stream));
+  return KM_OK;
+}
+
+int km_sim_profile(km_sim_t *s, int32_t on) {
+  if (!s) return KM_EINVAL;
+  KM_HIP(hipStreamSynchronize(s->stream));
+  for (auto &r : s->recs) {
+    s->ev_free.push_back(r.a);
+    s->ev_free.push_back(r.b);
+  }
+  s->recs.clear();
+  s->profile = on != 0;
+  return KM_OK;
+}
+
+int km_sim_kernel_times(km_sim_t *s, double *ms, int32_t *count, int32_t nkinds) {
+  if (!s || !ms || !count) return KM_EINVAL;
+  KM_HIP(hipStreamSynchronize(s->stream));
+  for (int k = 0; k < nkinds; ++k) {
+    ms[k] = 0.0;
+    count[k] = 0;
+  }
+  for (auto &r : s->recs) {
+    if (r.kind >= nkinds) continue;
+    float t = 0.f;
+    KM_HIP(hipEventElapsedTime(&t, r.a, r.b));
+    ms[r.kind] += t;
+    count[r.kind] += 1;
+  }
+  return KM_OK;
+}
+
+int km_sim_set_tile(km_sim_t *s, int32_t threads) {
+  if (!s || threads < 64 || threads > 256 || threads % 64) {
+    set_error("tile must be 64, 128, 192 or 256 threads");
+    return KM_EINVAL;
+  }
+  s->tile = threads;
+  if (s->graph_exec) {
+    (void)hipGraphExecDestroy(s->graph_exec);
+    (void)hipGraphDestroy(s->graph);
+    s->graph_exec = nullptr;
+    s->graph = nullptr;
+  }
   return KM_OK;
 }
 

@@ -1,0 +1,240 @@
+"""GPU parity: every libkm_hydro.so task and whole-mesh path against the CPU oracle on
+identical seeded inputs. FP64 paths are compared bit for bit (both sides round each
+operation in the reference's order); the FMA form to 1e-12 relative L1 per conserved
+variable; the FP32-mixed form to the tolerance written beside it."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from tests import _cases
+
+pytestmark = pytest.mark.gpu
+
+
+def _random_state(lay, mhd, gamma, seed):
+    """Smooth-ish random primitives on every zone (ghosts included)."""
+    import kamayan_amd as km
+    rng = np.random.default_rng(seed)
+    nv = km.nvar(mhd)
+    nc = km.ncons(mhd)
+    cell = np.zeros(lay.cell_shape(nv))
+    shp = cell[:, 0].shape
+    rho = rng.uniform(0.6, 1.4, shp)
+    p = rng.uniform(0.6, 1.4, shp)
+    cell[:, 0] = rho
+    for d in range(3):
+        cell[:, nc + d] = rng.uniform(-0.6, 0.6, shp)
+    cell[:, nc + 3] = p
+    cell[:, nc + 4] = gamma * p
+    cell[:, nc + 5] = p / ((gamma - 1.0) * rho)
+    if mhd:
+        for d in range(3):
+            cell[:, 5 + d] = rng.uniform(-0.8, 0.8, shp)
+    # conserved slots consistent with the primitives
+    for d in range(3):
+        cell[:, 1 + d] = rho * cell[:, nc + d]
+    cell[:, 4] = rho * cell[:, nc + 5] + 0.5 * rho * sum(cell[:, nc + d] ** 2 for d in range(3))
+    faces = [rng.uniform(-0.8, 0.8, lay.stag_shape()) for _ in range(3)] if mhd else None
+    return cell, faces
+
+
+CASES = [
+    # (ndim, recon, limiter, riemann, mhd)
+    (3, "wenoz", "minmod", "hllc", 0),
+    (3, "ppm", "mc", "hll", 0),
+    (3, "plm", "van_leer", "hllc", 0),
+    (2, "fog", "minmod", "hll", 0),
+    (2, "ppm", "minmod", "hllc", 0),
+    (3, "ppm", "mc", "hll", 1),
+    (2, "plm", "mc", "hllc", 1),
+    (3, "wenoz", "minmod", "hllc", 1),
+]
+
+
+def _setup(gpu, oracle, ndim, recon, lim, riem, mhd, seed=5):
+    km = gpu
+    cfg = km.Config(**{"hydro/reconstruction": recon, "hydro/slope_limiter": lim,
+                       "hydro/riemann": riem, "physics/MHD": "ct" if mhd else "off",
+                       "material/single/gamma": 1.4})
+    cfg.c.ndim = ndim
+    nx = (8, 8, 8 if ndim > 2 else 1)
+    nb = (2, 2, 2 if ndim > 2 else 1)
+    lay = km.Layout(ndim, 4, nx, nb)
+    desc = km.mesh_desc(lay)
+    blocks_h = oracle.block_table(desc, lay.nblocks)
+    cell, faces = _random_state(lay, mhd, 1.4, seed)
+    nv, nc = km.nvar(mhd), km.ncons(mhd)
+    dev = km.DevicePack(lay, nv, nflux=nc, faces=bool(mhd), edges=bool(mhd), blocks=blocks_h)
+    host = oracle.HostPack(lay, nv, nflux=nc, faces=bool(mhd), edges=bool(mhd), blocks=blocks_h)
+    host.cell[...] = cell
+    if mhd:
+        for d in range(3):
+            host.face[d][...] = faces[d]
+    dev.upload(cell=cell, face=faces)
+    return km, cfg, lay, dev, host
+
+
+def _same(a, b, what):
+    if a is None and b is None:
+        return
+    diff = np.max(np.abs(a - b)) if a.size else 0.0
+    assert np.array_equal(a, b), f"{what}: max |gpu - oracle| = {diff:.3e}"
+
+
+@pytest.mark.parametrize("ndim,recon,lim,riem,mhd", CASES)
+def test_task_chain_bitwise(gpu, oracle, ndim, recon, lim, riem, mhd):
+    """fluxes -> EMF -> FluxesToDuDt -> ApplyDuDt -> PreparePrimitive -> dt -> divB."""
+    km, cfg, lay, dev, host = _setup(gpu, oracle, ndim, recon, lim, riem, mhd)
+    L, OL = km.lib(), oracle.lib()
+    nv, nc = km.nvar(mhd), km.ncons(mhd)
+    p = C.byref
+    km.check(L.km_calculate_fluxes(p(cfg.c), p(dev.pack), None), "fluxes")
+    oracle.call("km_cpu_calculate_fluxes", p(cfg.c), p(host.pack), None)
+    km.check(L.km_calculate_emf(p(cfg.c), p(dev.pack), None), "emf")
+    oracle.call("km_cpu_calculate_emf", p(cfg.c), p(host.pack), None)
+    cell, flux, face, edge = dev.download()
+    for d in range(ndim):
+        _same(flux[d], host.flux[d], f"flux[{d}]")
+    if mhd:
+        for d in range(3):
+            _same(edge[d], host.edge[d], f"edge[{d}]")
+    # dU/dt
+    blocks_h = host.blocks
+    ddev = km.DevicePack(lay, nv, faces=bool(mhd), blocks=blocks_h)
+    dhost = oracle.HostPack(lay, nv, faces=bool(mhd), blocks=blocks_h)
+    km.check(L.km_fluxes_to_dudt(p(cfg.c), p(dev.pack), p(ddev.pack), None), "dudt")
+    oracle.call("km_cpu_fluxes_to_dudt", p(cfg.c), p(host.pack), p(dhost.pack), None)
+    dcell, _, dface, _ = ddev.download()
+    _same(dcell, dhost.cell, "dudt cell")
+    if mhd:
+        for d in range(ndim):
+            _same(dface[d], dhost.face[d], f"dudt face[{d}]")
+    # stage combine into a second container, then cons -> prim + EOS
+    u1d = km.DevicePack(lay, nv, faces=bool(mhd), blocks=blocks_h)
+    u1h = oracle.HostPack(lay, nv, faces=bool(mhd), blocks=blocks_h)
+    beta, dt = 0.5, 1.0e-3
+    km.check(L.km_apply_dudt(p(cfg.c), p(dev.pack), p(dev.pack), p(u1d.pack), p(ddev.pack),
+                             beta, dt, None), "apply")
+    oracle.call("km_cpu_apply_dudt", p(cfg.c), p(host.pack), p(host.pack), p(u1h.pack),
+                p(dhost.pack), beta, dt, None)
+    km.check(L.km_prepare_primitive(p(cfg.c), p(u1d.pack), None), "prim")
+    oracle.call("km_cpu_prepare_primitive", p(cfg.c), p(u1h.pack), None)
+    c1, _, f1, _ = u1d.download()
+    _same(c1, u1h.cell, "u1 cell")
+    # dt estimate and divB on the new state
+    dtd = km.DevicePack(lay, 1)
+    km.check(L.km_estimate_dt(p(cfg.c), p(u1d.pack), C.c_void_p(dtd.cell), None), "dt")
+    got = dtd.download()[0].reshape(-1)[0]
+    want = C.c_double()
+    oracle.call("km_cpu_estimate_dt", p(cfg.c), p(u1h.pack), p(want), None)
+    assert got == want.value
+    if mhd:
+        km.check(L.km_fill_divb(p(cfg.c), p(u1d.pack), None), "divb")
+        oracle.call("km_cpu_fill_divb", p(cfg.c), p(u1h.pack), None)
+        _same(u1d.download()[0], u1h.cell, "divb")
+
+
+@pytest.mark.parametrize("ndim,mhd", [(3, 0), (2, 1), (3, 1)])
+def test_prepare_conserved_and_ghost_fill_bitwise(gpu, oracle, ndim, mhd):
+    km, cfg, lay, dev, host = _setup(gpu, oracle, ndim, "ppm", "mc", "hll", mhd, seed=9)
+    L = km.lib()
+    p = C.byref
+    km.check(L.km_prepare_conserved(p(cfg.c), p(dev.pack), None), "cons")
+    oracle.call("km_cpu_prepare_conserved", p(cfg.c), p(host.pack), None)
+    nb = (C.c_int32 * 3)(*[lay.nb[d] if d < ndim else 1 for d in range(3)])
+    for bcs in [(0,) * 6, (1,) * 6, (2, 2, 0, 0, 1, 1)]:
+        bc = (C.c_int32 * 6)(*bcs)
+        plan = C.c_void_p()
+        km.check(L.km_ghost_plan_uniform(p(dev.pack), nb, bc, p(plan)), "plan")
+        nv = km.nvar(mhd)
+        km.check(L.km_ghost_fill(plan, p(dev.pack), 0, nv, int(mhd), None), "ghost")
+        oracle.call("km_cpu_ghost_fill_uniform", p(cfg.c), p(host.pack), nb, bc, 0, nv, int(mhd))
+        L.km_ghost_plan_destroy(plan)
+        cell, _, face, _ = dev.download()
+        _same(cell, host.cell, f"cells bc={bcs}")
+        if mhd:
+            for d in range(ndim):
+                _same(face[d], host.face[d], f"face[{d}] bc={bcs}")
+
+
+def _run_both(km, oracle, case, ncycles, path, cfg_over=None, set_dt=None):
+    cfg, desc, lay, cell, faces = case
+    if cfg_over:
+        for k, v in cfg_over.items():
+            cfg.set(k, v)
+    sim = km.Simulation(cfg, desc, lay, path=path)
+    sim.set_primitive(cell, faces)
+    ref = oracle.CpuSimulation(cfg.c, desc, lay)
+    ref.set_primitive(cell, faces)
+    sim.advance(ncycles)
+    ref.advance(ncycles)
+    g, gf = sim.state()
+    r, rf = ref.state()
+    return lay, (g, gf, sim.info()), (r, rf, ref.info())
+
+
+WHOLE = [
+    ("vortex-plm-mc-hllc-rk2", lambda: _cases.vortex(n=128, nblock=32, ng=4, recon="plm",
+                                                     limiter="mc", riemann="hllc", cfl=0.4), 12),
+    ("smooth3d-wenoz-hllc-rk3", lambda: _cases.smooth3d(n=32, nblock=16), 3),
+    ("smooth3d-ppm-hll-rk2", lambda: _cases.smooth3d(n=32, nblock=16, recon="ppm", riemann="hll",
+                                                     integrator="rk2"), 3),
+]
+
+
+@pytest.mark.parametrize("name,mk,ncyc", WHOLE)
+def test_sim_tasks_bitwise(gpu, oracle, name, mk, ncyc):
+    lay, (g, _, gi), (r, _, ri) = _run_both(gpu, oracle, mk(), ncyc, gpu.PATH_TASKS)
+    assert gi == ri
+    sl = (slice(None), slice(None)) + lay.interior()
+    _same(g[sl], r[sl], name)
+
+
+@pytest.mark.parametrize("name,mk,ncyc", WHOLE)
+def test_sim_fused_fp64_bitwise(gpu, oracle, name, mk, ncyc):
+    lay, (g, _, gi), (r, _, ri) = _run_both(gpu, oracle, mk(), ncyc, gpu.PATH_FUSED)
+    assert gi == ri
+    sl = (slice(None), slice(5)) + lay.interior()
+    _same(g[sl], r[sl], name)
+
+
+@pytest.mark.parametrize("name,mk,ncyc", WHOLE)
+def test_sim_fused_fma_within_1e12(gpu, oracle, name, mk, ncyc):
+    lay, (g, _, gi), (r, _, ri) = _run_both(gpu, oracle, mk(), ncyc, gpu.PATH_FUSED,
+                                            {"kamayan_amd/precision": "fp64_fma"})
+    l1 = _cases.rel_l1(g, r, lay, 5)
+    assert np.all(l1 < 1e-12), l1
+
+
+@pytest.mark.parametrize("name,mk,ncyc", WHOLE)
+def test_sim_fused_fp32_mixed(gpu, oracle, name, mk, ncyc):
+    """FP32 reconstruction/Riemann: relative L1 vs the FP64 oracle below 1e-5."""
+    lay, (g, _, gi), (r, _, ri) = _run_both(gpu, oracle, mk(), ncyc, gpu.PATH_FUSED,
+                                            {"kamayan_amd/precision": "fp32_mixed"})
+    l1 = _cases.rel_l1(g, r, lay, 5)
+    assert np.all(l1 < 1e-5), l1
+
+
+def test_sim_mhd_tasks_bitwise(gpu, oracle):
+    lay, (g, gf, gi), (r, rf, ri) = _run_both(gpu, oracle, _cases.mhd3d(n=16, nblock=8), 3,
+                                              gpu.PATH_TASKS)
+    assert gi == ri
+    sl = (slice(None), slice(None)) + lay.interior()
+    _same(g[sl], r[sl], "mhd cells")
+    for d in range(3):
+        _same(gf[d], rf[d], f"mhd face {d}")
+    divb = g[:, 15][(slice(None),) + lay.interior()]
+    assert float(np.max(np.abs(divb))) < 1e-11
+
+
+def test_fp32_mixed_stays_finite_long_run(gpu, oracle):
+    """64^3 WENO-Z/HLLC RK3 for 12 cycles in FP32-mixed: finite, and relative L1 to
+    the FP64 oracle below 1e-5 per conserved variable."""
+    case = _cases.smooth3d(n=64, nblock=32)
+    lay, (g, _, _), (r, _, _) = _run_both(gpu, oracle, case, 12, gpu.PATH_FUSED,
+                                          {"kamayan_amd/precision": "fp32_mixed"})
+    sl = (slice(None), slice(0, 5)) + lay.interior()
+    assert np.all(np.isfinite(g[sl]))
+    l1 = _cases.rel_l1(g, r, lay, 5)
+    assert np.all(l1 < 1e-5), l1
