@@ -7,6 +7,7 @@ library has not been built; there is no CPU fallback.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import pathlib
 
 import numpy as np
@@ -16,7 +17,8 @@ from ._abi import (BC, KM_OK, LIMITER, MHD, PATH_FUSED, PATH_TASKS, PRECISION, R
                    ncons, nvar, slot)
 
 HERE = pathlib.Path(__file__).resolve().parent
-LIB_PATH = HERE / "libkm_hydro.so"
+# KAMAYAN_AMD_LIB selects another build of the library (e.g. a tuning variant)
+LIB_PATH = pathlib.Path(os.environ.get("KAMAYAN_AMD_LIB", HERE / "libkm_hydro.so"))
 
 __all__ = [
     "lib", "Config", "Simulation", "DevicePack", "problem_fill", "Layout", "mesh_desc",

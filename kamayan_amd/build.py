@@ -47,9 +47,10 @@ UNITS = [
 ]
 
 
-def _compile(src: str, contract: str, verbose: bool) -> pathlib.Path:
-    obj = OBJ / (pathlib.Path(src).stem + ".o")
-    cmd = [HIPCC, *COMMON, f"-ffp-contract={contract}", "-c", str(CSRC / src), "-o", str(obj)]
+def _compile(src: str, contract: str, verbose: bool, extra=(), objdir=OBJ) -> pathlib.Path:
+    obj = objdir / (pathlib.Path(src).stem + ".o")
+    cmd = [HIPCC, *COMMON, f"-ffp-contract={contract}", *extra, "-c", str(CSRC / src), "-o",
+           str(obj)]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
@@ -58,21 +59,23 @@ def _compile(src: str, contract: str, verbose: bool) -> pathlib.Path:
     return obj
 
 
-def build(verbose: bool = False, jobs: int | None = None) -> pathlib.Path:
-    """Compile all units and link libkm_hydro.so; returns its path."""
-    OBJ.mkdir(exist_ok=True)
+def build(verbose: bool = False, jobs: int | None = None, extra=(), out: pathlib.Path = OUT,
+          objdir: pathlib.Path = OBJ) -> pathlib.Path:
+    """Compile all units and link libkm_hydro.so (or `out`); returns its path.
+    `extra` adds compiler flags (e.g. -D tuning macros) to every unit."""
+    objdir.mkdir(parents=True, exist_ok=True)
     jobs = jobs or min(len(UNITS), os.cpu_count() or 4)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(lambda u: _compile(u[0], u[1], verbose), UNITS))
-    tmp = OUT.with_suffix(".so.tmp")
+        objs = list(ex.map(lambda u: _compile(u[0], u[1], verbose, extra, objdir), UNITS))
+    tmp = out.with_suffix(".so.tmp")
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stderr[-8000:]}")
-    os.replace(tmp, OUT)
-    return OUT
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

@@ -50,6 +50,21 @@ struct Geo {
 };
 
 KM_HD double *cell_ptr(const km_pack_t &p, int b, int v) { return p.cell[(long)b * p.nvar + v]; }
+
+// loads/stores through the global address space (the pack tables hold generic
+// pointers; telling the compiler they are global gives global_load/store instead of
+// flat ones)
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __attribute__((address_space(1))) const double gconst_double;
+typedef __attribute__((address_space(1))) double gdouble;
+__device__ __forceinline__ double ldg(const double *p, long i) {
+  return ((gconst_double *)p)[i];
+}
+__device__ __forceinline__ void stg(double *p, long i, double v) { ((gdouble *)p)[i] = v; }
+#else
+inline double ldg(const double *p, long i) { return p[i]; }
+inline void stg(double *p, long i, double v) { p[i] = v; }
+#endif
 KM_HD double *flux_ptr(const km_pack_t &p, int d, int b, int v) {
   return p.flux[d][(long)b * p.nflux + v];
 }
@@ -100,6 +115,47 @@ __device__ __forceinline__ void march(int lim, int c0, int c1, Load &&load, Face
     {
       T p[NR];
       load(c + W, p);
+#pragma unroll
+      for (int r = 0; r < NR; ++r) win[r][2 + W] = p[r];
+    }
+    T vM[NR], vP[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) reconstruct_rt<RECON>(lim, win[r], vM[r], vP[r]);
+    if (c > c0) face(c, vPprev, vM);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      vPprev[r] = vP[r];
+#pragma unroll
+      for (int o = 2 - W; o < 2 + W; ++o) win[r][o] = win[r][o + 1];
+    }
+  }
+}
+
+// March with the next cell's raw data fetched one step ahead, so the load latency
+// overlaps the current step's reconstruction and Riemann solve.
+// fetch(c) -> R (raw loads, no arithmetic); unpack(const R&, T p[NR]).
+// fetch0 serves the window prefill (cells c0-W .. c0+W), fetch the steps after it.
+template <class T, int RECON, int NR, class R, class Fetch0, class Fetch, class Unpack,
+          class Face>
+__device__ __forceinline__ void march_pf(int lim, int c0, int c1, Fetch0 &&fetch0, Fetch &&fetch,
+                                         Unpack &&unpack, Face &&face) {
+  constexpr int W = ReconWidth<RECON>::value;
+  T win[NR][5];
+#pragma unroll
+  for (int o = -W; o < W; ++o) {
+    T p[NR];
+    unpack(fetch0(c0 + o), p);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) win[r][2 + o] = p[r];
+  }
+  R nxt = fetch0(c0 + W);
+  T vPprev[NR];
+  for (int c = c0; c <= c1; ++c) {
+    const R cur = nxt;
+    if (c < c1) nxt = fetch(c + 1 + W);
+    {
+      T p[NR];
+      unpack(cur, p);
 #pragma unroll
       for (int r = 0; r < NR; ++r) win[r][2 + W] = p[r];
     }

@@ -39,10 +39,15 @@ struct FusedArgs {
   double *du;          // [nblocks][5][nx3][nx2][nx1] accumulator (interior only)
   const double *dt;    // device: step dt
   double *dt_est;      // device: CFL minimum of u1 (bit-ordered), or null
+  const int *nbr;      // [nblocks][3][2] face neighbours: block index, or
+                       // KM_NBR_OUTFLOW / KM_NBR_REFLECT at a physical boundary
   double beta;
   int lim;
   Eos eos;
 };
+
+constexpr int KM_NBR_OUTFLOW = -1;
+constexpr int KM_NBR_REFLECT = -2;
 
 // task launchers (km_tasks.hip)
 void launch_calculate_fluxes(const km_cfg_t &cfg, const km_pack_t &u, const Box &bx,

@@ -174,6 +174,7 @@ struct km_sim_t {
   km_block_t *blocks_d = nullptr;
   Cont base, stage[2], dudt, scratch;
   double *du = nullptr;
+  int *nbr_d = nullptr;  // [nb][3][2] face neighbours for the fused sweeps
   UniformGrid grid;
   hipStream_t stream = nullptr;
   double *dt_d = nullptr, *est_d = nullptr, *time_d = nullptr;
@@ -343,6 +344,7 @@ int fused_stage(km_sim_t &s, int stage, double beta) {
   a.du = s.du;
   a.dt = s.dt_d;
   a.dt_est = s.est_d;
+  a.nbr = s.nbr_d;
   a.beta = beta;
   a.lim = cfg.limiter;
   a.eos = make_eos(cfg);
@@ -377,17 +379,8 @@ int fused_stage(km_sim_t &s, int stage, double beta) {
   if (last)
     hipLaunchKernelGGL(k_scale, dim3(1), dim3(64), 0, s.stream, s.est_d,
                        cfg.cfl / static_cast<double>(ndim));
-  hipEvent_t g0 = nullptr, g1 = nullptr;
   if (s.profile) {
-    g0 = s.ev();
-    g1 = s.ev();
-    (void)hipEventRecord(g0, s.stream);
-  }
-  launch_ghost_fill(s.grid, md1.pack, 0, s.nc, s.nc, false, s.stream);
-  if (s.profile) {
-    (void)hipEventRecord(g1, s.stream);
     (void)hipEventRecord(st1, s.stream);
-    s.recs.push_back({3, g0, g1});
     s.recs.push_back({4, st0, st1});
   }
   return check_launch();
@@ -614,6 +607,35 @@ int km_sim_create(const km_cfg_t *cfg, const km_mesh_desc_t *mesh, int32_t path,
   if (path == KM_PATH_FUSED) {
     const long nint = (long)shape.nx[0] * shape.nx[1] * shape.nx[2];
     if ((rc = s->alloc(&s->du, (size_t)s->nb * 5 * nint))) return rc;
+    // face-neighbour table of the uniform block grid (periodic wraps; other
+    // boundaries are mapped inside the sweep)
+    std::vector<int> nbr((size_t)s->nb * 6);
+    for (int b = 0; b < s->nb; ++b) {
+      const int l[3] = {b % mesh->nb[0], (b / mesh->nb[0]) % mesh->nb[1],
+                        b / (mesh->nb[0] * mesh->nb[1])};
+      for (int d = 0; d < 3; ++d)
+        for (int side = 0; side < 2; ++side) {
+          int m[3] = {l[0], l[1], l[2]};
+          m[d] += side ? 1 : -1;
+          int val;
+          if (d >= mesh->ndim) {
+            val = KM_NBR_OUTFLOW;
+          } else if (m[d] < 0 || m[d] >= mesh->nb[d]) {
+            const int bc = mesh->bc[2 * d + side];
+            if (bc == KM_BC_PERIODIC) {
+              m[d] = (m[d] + mesh->nb[d]) % mesh->nb[d];
+              val = (m[2] * mesh->nb[1] + m[1]) * mesh->nb[0] + m[0];
+            } else {
+              val = bc == KM_BC_REFLECT ? KM_NBR_REFLECT : KM_NBR_OUTFLOW;
+            }
+          } else {
+            val = (m[2] * mesh->nb[1] + m[1]) * mesh->nb[0] + m[0];
+          }
+          nbr[(size_t)b * 6 + d * 2 + side] = val;
+        }
+    }
+    if ((rc = s->alloc(&s->nbr_d, nbr.size()))) return rc;
+    KM_HIP(hipMemcpy(s->nbr_d, nbr.data(), nbr.size() * sizeof(int), hipMemcpyHostToDevice));
   }
   if ((rc = s->alloc(&s->dt_d, 4))) return rc;
   s->est_d = s->dt_d + 1;
@@ -681,7 +703,9 @@ int km_sim_get_state(km_sim_t *s, double *cell, double *const face[3]) {
   if (!s || !cell) return KM_EINVAL;
   KM_HIP(hipStreamSynchronize(s->stream));
   if (s->path == KM_PATH_FUSED) {
-    // the fused path keeps only conserved variables current; derive the rest
+    // the fused path keeps only the conserved interior current; fill the ghost zones
+    // and derive the primitives
+    launch_ghost_fill(s->grid, s->base.pack, 0, s->nc, s->nc, false, s->stream);
     launch_prepare_primitive(s->cfg, s->base.pack, s->stream);
     KM_HIP(hipStreamSynchronize(s->stream));
   }

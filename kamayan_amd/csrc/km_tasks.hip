@@ -124,11 +124,15 @@ __global__ void __launch_bounds__(256) k_flux_div(km_pack_t u, km_pack_t dudt, i
   if (!interior_cell(g, u.nblocks, b, k, j, i)) return;
   const km_block_t &blk = u.blocks[b];
   const long c = g.c(k, j, i);
+  const long sc[3] = {g.sc[0], g.sc[1], g.sc[2]};
   for (int var = 0; var < nc; ++var) {
     double du = 0.;
-    for (int dir = 0; dir < g.ndim; ++dir) {
-      const double *F = flux_ptr(u, dir, b, var);
-      du -= (blk.area[dir] * F[c + g.sc[dir]] - blk.area[dir] * F[c]);
+#pragma unroll
+    for (int dir = 0; dir < 3; ++dir) {
+      if (dir < g.ndim) {
+        const double *F = flux_ptr(u, dir, b, var);
+        du -= (blk.area[dir] * F[c + sc[dir]] - blk.area[dir] * F[c]);
+      }
     }
     cell_ptr(dudt, b, var)[c] = du / blk.vol;
   }
@@ -327,7 +331,14 @@ __global__ void __launch_bounds__(256) k_estimate_dt(km_pack_t u, double *out) {
     if (g.ndim > 2) dtl = kmin(dtl, blk.dx[2] / (kabs(V[R_VEL + 2]) + fast_speed<MHD, 2>(V)));
   }
   dtl = wave_min(dtl);
-  if ((threadIdx.x & 63) == 0) atomic_min_pos(out, dtl);
+  __shared__ double part[4];
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = dtl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = part[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = kmin(m, part[w]);
+    atomic_min_pos(out, m);
+  }
 }
 
 __global__ void k_scale_value(double *p, double s) {
@@ -342,9 +353,13 @@ __global__ void __launch_bounds__(256) k_fill_divb(km_pack_t u, int slot) {
   const km_block_t &blk = u.blocks[b];
   const long s = g.s(k, j, i);
   double divb = 0.;
-  for (int dir = 0; dir < g.ndim; dir++) {
-    const double *B = u.face[dir][b];
-    divb += blk.area[dir] * B[s + g.ss[dir]] - blk.area[dir] * B[s];
+  const long ss[3] = {g.ss[0], g.ss[1], g.ss[2]};
+#pragma unroll
+  for (int dir = 0; dir < 3; dir++) {
+    if (dir < g.ndim) {
+      const double *B = u.face[dir][b];
+      divb += blk.area[dir] * B[s + ss[dir]] - blk.area[dir] * B[s];
+    }
   }
   cell_ptr(u, b, slot)[g.c(k, j, i)] = divb / blk.vol;
 }
@@ -399,12 +414,14 @@ __global__ void __launch_bounds__(256) k_ghost_cells(km_pack_t u, UniformGrid gr
   idx[1] = (int)(r % g.E[1]);
   idx[2] = (int)(r / g.E[1]);
   bool ghost = false;
-  for (int d = 0; d < g.ndim; ++d)
-    if (idx[d] < g.lo[d] || idx[d] > g.hi[d]) ghost = true;
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    if (d < g.ndim && (idx[d] < g.lo[d] || idx[d] > g.hi[d])) ghost = true;
   if (!ghost) return;
   const int bl[3] = {b % gr.nb[0], (b / gr.nb[0]) % gr.nb[1], b / (gr.nb[0] * gr.nb[1])};
   int src[3], sb[3];
   bool flip[3] = {false, false, false};
+#pragma unroll
   for (int d = 0; d < 3; ++d) {
     if (d >= g.ndim) {
       sb[d] = 0;
@@ -419,13 +436,14 @@ __global__ void __launch_bounds__(256) k_ghost_cells(km_pack_t u, UniformGrid gr
   }
   const int srcb = (sb[2] * gr.nb[1] + sb[1]) * gr.nb[0] + sb[0];
   const long dc = g.c(idx[2], idx[1], idx[0]), scx = g.c(src[2], src[1], src[0]);
+  const bool fl0 = flip[0], fl1 = flip[1], fl2 = flip[2];
   for (int v = v0; v < v1; ++v) {
     double val = cell_ptr(u, srcb, v)[scx];
     int ax = -1;
     if (v >= 1 && v < 4) ax = v - 1;
     else if (v >= nc && v < nc + 3) ax = v - nc;
     else if (nc == 8 && v >= 5 && v < 8) ax = v - 5;
-    if (ax >= 0 && flip[ax]) val = -val;
+    if ((ax == 0 && fl0) || (ax == 1 && fl1) || (ax == 2 && fl2)) val = -val;
     cell_ptr(u, b, v)[dc] = val;
   }
 }
@@ -443,6 +461,7 @@ __global__ void __launch_bounds__(256) k_ghost_faces(km_pack_t u, UniformGrid gr
   idx[1] = (int)(r % g.S[1]);
   idx[2] = (int)(r / g.S[1]);
   bool ghost = false;
+#pragma unroll
   for (int d = 0; d < 3; ++d) {
     const int ext = d < g.ndim ? g.E[d] + (d == face ? 1 : 0) : 1;
     if (idx[d] >= ext) return;
@@ -453,6 +472,7 @@ __global__ void __launch_bounds__(256) k_ghost_faces(km_pack_t u, UniformGrid gr
   const int bl[3] = {b % gr.nb[0], (b / gr.nb[0]) % gr.nb[1], b / (gr.nb[0] * gr.nb[1])};
   int src[3], sb[3];
   bool flip[3] = {false, false, false};
+#pragma unroll
   for (int d = 0; d < 3; ++d) {
     if (d >= g.ndim) {
       sb[d] = 0;
@@ -473,7 +493,7 @@ __global__ void __launch_bounds__(256) k_ghost_faces(km_pack_t u, UniformGrid gr
   }
   const int srcb = (sb[2] * gr.nb[1] + sb[1]) * gr.nb[0] + sb[0];
   double val = u.face[face][srcb][g.s(src[2], src[1], src[0])];
-  if (flip[face]) val = -val;
+  if ((face == 0 && flip[0]) || (face == 1 && flip[1]) || (face == 2 && flip[2])) val = -val;
   u.face[face][b][g.s(idx[2], idx[1], idx[0])] = val;
 }
 
