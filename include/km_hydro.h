@@ -170,18 +170,7 @@ int km_estimate_dt(const km_cfg_t *cfg, const km_pack_t *u, double *dt_dev, void
 int km_fill_divb(const km_cfg_t *cfg, const km_pack_t *u, void *stream);
 
 /* ---- ghost fill (Parthenon AddBoundaryExchangeTasks, kamayan_driver.cpp:150-151) ---- */
-typedef struct km_ghost_plan_t km_ghost_plan_t; /* opaque, device-resident */
-/* Same-level ghost fill plan for a uniform grid of nb[0] x nb[1] x nb[2] blocks stored
- * in x-fastest order; bc = {ix1, ox1, ix2, ox2, ix3, ox3} of KM_BC_*. */
-int km_ghost_plan_uniform(const km_pack_t *shape, const int32_t nb[3], const int32_t bc[6],
-                          km_ghost_plan_t **plan);
-int km_ghost_plan_destroy(km_ghost_plan_t *plan);
-/* Fill every ghost zone (faces, edges, corners) of cell vars [v0, v1) and, when
- * with_faces != 0, of the CT face arrays. */
-int km_ghost_fill(const km_ghost_plan_t *plan, const km_pack_t *u, int32_t v0, int32_t v1,
-                  int32_t with_faces, void *stream);
-
-/* ---- whole-mesh driver (KamayanDriver RK stages, kamayan_driver.cpp:121-248) ----- */
+/* Mesh description (parthenon/mesh and parthenon/meshblock deck blocks). */
 typedef struct km_mesh_desc_t {
   int32_t ndim;
   int32_t ng;
@@ -192,6 +181,34 @@ typedef struct km_mesh_desc_t {
   int32_t bc[6];    /* ix1, ox1, ix2, ox2, ix3, ox3 */
 } km_mesh_desc_t;
 
+typedef struct km_ghost_plan_t km_ghost_plan_t; /* opaque, device-resident */
+/* Same-level ghost fill plan for a uniform grid of nb[0] x nb[1] x nb[2] blocks stored
+ * in x-fastest order; bc = {ix1, ox1, ix2, ox2, ix3, ox3} of KM_BC_*. */
+int km_ghost_plan_uniform(const km_pack_t *shape, const int32_t nb[3], const int32_t bc[6],
+                          km_ghost_plan_t **plan);
+int km_ghost_plan_destroy(km_ghost_plan_t *plan);
+/* Ghost fill plan for a block-tree (AMR) mesh: the leaves of Parthenon's MeshBlockTree,
+ * leaves[4*b..4*b+3] = {level, lx1, lx2, lx3} of pack block b, over the root grid
+ * mesh->nb with the domain and boundaries of `mesh`. Leaves must tile the domain; leaves
+ * that touch (faces, edges, corners, across periodic boundaries) differ by at most one
+ * level; with more than one level every active block dimension holds an even number of
+ * at least 4*((ng+1)/2 + 1) cells, which keeps each prolongation stencil inside blocks
+ * adjacent to the block being filled. */
+int km_ghost_plan_amr(const km_pack_t *shape, const km_mesh_desc_t *mesh,
+                      const int32_t *leaves, int32_t nleaves, km_ghost_plan_t **plan);
+/* Fill every ghost zone (faces, edges, corners) of cell vars [v0, v1) and, when
+ * with_faces != 0, of the CT face arrays. With a block-tree plan a ghost zone is copied
+ * from a same-level neighbour, restricted from a finer one (RestrictAverage,
+ * refinement_operations.hpp:52-121) or prolongated from a coarser one
+ * (ProlongateSharedMinMod, :123-268); face arrays across levels are not supported. */
+int km_ghost_fill(const km_ghost_plan_t *plan, const km_pack_t *u, int32_t v0, int32_t v1,
+                  int32_t with_faces, void *stream);
+/* Flux correction (Parthenon AddFluxCorrectionTasks, kamayan_driver.cpp:198-207): every
+ * block face shared with finer leaves takes the area-weighted restriction of the fine
+ * fluxes, for the u->nflux conserved variables. No-op for a uniform plan. */
+int km_flux_correct(const km_ghost_plan_t *plan, const km_pack_t *u, void *stream);
+
+/* ---- whole-mesh driver (KamayanDriver RK stages, kamayan_driver.cpp:121-248) ----- */
 enum {
   KM_PATH_TASKS = 0, /* one kernel per reference task, same sequence as BuildTaskListRKStage */
   KM_PATH_FUSED = 1  /* fused directional sweeps per RK stage (hydro) */
@@ -200,6 +217,20 @@ enum {
 typedef struct km_sim_t km_sim_t; /* opaque: device mesh, stage containers, dt state */
 int km_sim_create(const km_cfg_t *cfg, const km_mesh_desc_t *mesh, int32_t path,
                   km_sim_t **sim);
+/* An ensemble of ninst independent runs advanced together: instance m is the uniform mesh
+ * meshes[m] (all instances share ndim, nghost and cells per block; block counts,
+ * extents and boundaries may differ) with its own dt and time. Its blocks follow those of
+ * instances 0..m-1 in the host buffers. Every kernel of a stage covers all instances in
+ * one launch. Ensembles advance by cycle count (tlim must be 0). */
+int km_sim_create_ensemble(const km_cfg_t *cfg, const km_mesh_desc_t *meshes, int32_t ninst,
+                           int32_t path, km_sim_t **sim);
+/* A block-tree (AMR) mesh with the leaves described at km_ghost_plan_amr, on the task
+ * path (hydro): each stage adds flux correction after the fluxes and fills ghost zones
+ * across levels. Host buffers hold the blocks in leaf order. */
+int km_sim_create_amr(const km_cfg_t *cfg, const km_mesh_desc_t *mesh, const int32_t *leaves,
+                      int32_t nleaves, int32_t path, km_sim_t **sim);
+/* time and dt of the first n instances (blocks the host) */
+int km_sim_ensemble_info(const km_sim_t *sim, double *times, double *dts, int32_t n);
 int km_sim_destroy(km_sim_t *sim);
 /* Host buffers in pack layout: cell = [nblocks][KM_NVAR][E3][E2][E1], face[d] =
  * [nblocks][S3][S2][S1] (NULL unless CT). km_sim_set_primitive takes DENS, VELOCITY,
@@ -228,8 +259,22 @@ int km_sim_set_tile(km_sim_t *sim, int32_t threads);
 int km_problem_fill(const char *name, const km_cfg_t *cfg, const km_mesh_desc_t *mesh,
                     uint64_t seed, const double *params, int32_t nparams, double *cell,
                     double *const face[3]);
+/* The same generators over an explicit block table (e.g. the leaves of a block tree). */
+int km_problem_fill_blocks(const char *name, const km_cfg_t *cfg, const km_mesh_desc_t *mesh,
+                           const km_block_t *blocks, int32_t nblocks, uint64_t seed,
+                           const double *params, int32_t nparams, double *cell,
+                           double *const face[3]);
 /* Block metric table for a uniform mesh (blocks x-fastest). */
 int km_mesh_blocks(const km_mesh_desc_t *mesh, km_block_t *blocks_host);
+/* Block metric table of block-tree leaves (checked as in km_ghost_plan_amr). */
+int km_amr_blocks(const km_mesh_desc_t *mesh, const int32_t *leaves, int32_t nleaves,
+                  km_block_t *blocks_host);
+/* Leaves of the root grid refined to level lmax wherever a block meets the ball
+ * |x - center| <= radius, then refined further until touching leaves differ by at most
+ * one level. *nleaves: capacity of `leaves` (entries of 4) in, leaf count out; a buffer
+ * too small returns KM_EINVAL with the count needed. */
+int km_amr_refine_ball(const km_mesh_desc_t *mesh, int32_t lmax, const double center[3],
+                       double radius, int32_t *leaves, int32_t *nleaves);
 
 /* ---- device memory helpers (so callers need no other runtime) --------------------- */
 int km_device_count(int32_t *count);

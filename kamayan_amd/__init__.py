@@ -61,6 +61,8 @@ def lib() -> C.CDLL:
         "km_ghost_plan_destroy": [vp],
         "km_ghost_fill": [vp, P(KmPack), i32, i32, i32, vp],
         "km_sim_create": [P(KmCfg), P(KmMeshDesc), i32, P(vp)],
+        "km_sim_create_ensemble": [P(KmCfg), P(KmMeshDesc), i32, i32, P(vp)],
+        "km_sim_ensemble_info": [vp, P(dbl), P(dbl), i32],
         "km_sim_destroy": [vp],
         "km_sim_set_primitive": [vp, vp, P(vp)],
         "km_sim_get_state": [vp, vp, P(vp)],
@@ -152,13 +154,24 @@ def problem_fill(name: str, cfg: Config, desc: KmMeshDesc, layout: Layout, seed:
 
 
 class Simulation:
-    """Whole-mesh driver on the GPU (km_sim_*)."""
+    """Whole-mesh driver on the GPU (km_sim_*). With `descs` (a list of mesh descriptors
+    sharing cells per block), an ensemble of independent runs advanced together; its
+    buffers hold the instances' blocks one after the other and `layout` must describe
+    the total block count."""
 
-    def __init__(self, cfg: Config, desc: KmMeshDesc, layout: Layout, path: int = PATH_TASKS):
+    def __init__(self, cfg: Config, desc: KmMeshDesc, layout: Layout, path: int = PATH_TASKS,
+                 descs=None):
         self.cfg, self.desc, self.layout = cfg, desc, layout
         self.h = C.c_void_p()
-        check(lib().km_sim_create(C.byref(cfg.c), C.byref(desc), path, C.byref(self.h)),
-              "km_sim_create")
+        self.ninst = 1
+        if descs is not None:
+            arr = (KmMeshDesc * len(descs))(*descs)
+            self.ninst = len(descs)
+            check(lib().km_sim_create_ensemble(C.byref(cfg.c), arr, len(descs), path,
+                                               C.byref(self.h)), "km_sim_create_ensemble")
+        else:
+            check(lib().km_sim_create(C.byref(cfg.c), C.byref(desc), path, C.byref(self.h)),
+                  "km_sim_create")
 
     def close(self):
         if self.h:
@@ -200,6 +213,13 @@ class Simulation:
         t, dt, n = C.c_double(), C.c_double(), C.c_int32()
         check(lib().km_sim_info(self.h, C.byref(t), C.byref(dt), C.byref(n)), "info")
         return t.value, dt.value, n.value
+
+    def ensemble_info(self):
+        """(times, dts) arrays of all instances."""
+        t = (C.c_double * self.ninst)()
+        d = (C.c_double * self.ninst)()
+        check(lib().km_sim_ensemble_info(self.h, t, d, self.ninst), "ensemble_info")
+        return np.array(t[:]), np.array(d[:])
 
     def profile(self, on: bool = True):
         check(lib().km_sim_profile(self.h, 1 if on else 0), "profile")

@@ -44,6 +44,8 @@ UNITS = [
     ("km_fused_fp32.hip", "fast"),
     ("km_config.cpp", "off"),
     ("km_problems.cpp", "off"),
+    ("km_amr.hip", "off"),
+    ("km_amr_tree.cpp", "off"),
 ]
 
 

@@ -1,0 +1,313 @@
+// km_amr.hip -- ghost fill across refinement levels and flux correction for block-tree
+// meshes: Parthenon's boundary exchange and flux correction (kamayan_driver.cpp:145-151,
+// 198-207) with kamayan's refinement operators (refinement_operations.hpp:20-268).
+//
+// Every ghost zone is computed from leaf interiors only, so one launch fills all blocks.
+// For a ghost zone of a level-L block, mapped through the physical boundaries:
+//   covered by a level-L leaf   -> copy
+//   covered by a level-L+1 leaf -> RestrictAverage of its 2^d children (:52-121)
+//   covered by a level-L-1 leaf -> ProlongateSharedMinMod (:123-268) of the coarse zone C
+//                                  holding it: minmod slopes from C's neighbours at level
+//                                  L-1, each a coarse leaf value or the restriction of a
+//                                  level-L leaf (what Parthenon's coarse buffer holds there)
+// Slopes use the spacings of the fine block's coarse buffer: 2 dx between coarse centres
+// and dx/2 from a coarse centre to a child centre (uniform Cartesian coordinates).
+// The mesh builder guarantees that every zone read here lies in a leaf that touches the
+// block being filled, so the levels met are L-1, L and L+1 only.
+#include <hip/hip_runtime.h>
+
+#include "km_amr.hpp"
+#include "km_device.hpp"
+#include "km_internal.hpp"
+
+namespace km {
+
+namespace {
+
+// a level-L zone (global index, in the domain) as stored in the leaf covering it
+struct Src {
+  int kind;  // 0: leaf at level L; 1: leaf at L+1 (restrict); 2: leaf at L-1
+  int leaf;
+  long idx;  // kind 0/2: the zone holding it; kind 1: lower corner of its children
+};
+
+__device__ __forceinline__ Src locate(const km_pack_t &u, const AmrGrid &t, const Geo &g, int L,
+                                      const int G[3]) {
+  Src s;
+  s.leaf = amr_leaf(t, g, L, G);
+  const km_block_t &nb = u.blocks[s.leaf];
+  int li[3];
+  if (nb.level == L) {
+    s.kind = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) li[d] = amr_local(nb, g, d, G[d]);
+  } else if (nb.level > L) {
+    s.kind = 1;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) li[d] = amr_local(nb, g, d, 2 * G[d]);
+  } else {
+    s.kind = 2;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) li[d] = amr_local(nb, g, d, G[d] >> 1);
+  }
+  s.idx = g.c(li[2], li[1], li[0]);
+  return s;
+}
+
+// map a level-L global index through the physical boundaries (flip[d] toggles when a
+// reflecting boundary mirrors along d)
+__device__ __forceinline__ void map_zone(const AmrGrid &t, const Geo &g, int L, const int Gin[3],
+                                         int G[3], bool flip[3]) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    flip[d] = false;
+    G[d] = d < g.ndim ? amr_map(t, d, Gin[d], amr_zones(t, g, d, L), flip[d]) : 0;
+  }
+}
+
+// RestrictAverage (refinement_operations.hpp:52-121) of the children of a zone: cell
+// volumes of a Cartesian block are all equal; the pairwise sum order is the reference's
+__device__ __forceinline__ double restrict_cell(const double *q, double vol, const Geo &g,
+                                                long idx) {
+  double V[2][2][2], Tm[2][2][2];
+#pragma unroll
+  for (int ok = 0; ok < 2; ++ok)
+#pragma unroll
+    for (int oj = 0; oj < 2; ++oj)
+#pragma unroll
+      for (int oi = 0; oi < 2; ++oi) {
+        V[ok][oj][oi] = 0.;
+        Tm[ok][oj][oi] = 0.;
+        if ((ok == 0 || g.ndim > 2) && (oj == 0 || g.ndim > 1)) {
+          V[ok][oj][oi] = vol;
+          Tm[ok][oj][oi] = vol * q[idx + ok * g.sc[2] + oj * g.sc[1] + oi];
+        }
+      }
+  const double tvol = ((V[0][0][0] + V[0][1][0]) + (V[0][0][1] + V[0][1][1])) +
+                      ((V[1][0][0] + V[1][1][0]) + (V[1][0][1] + V[1][1][1]));
+  return tvol > 0.0 ? (((Tm[0][0][0] + Tm[0][1][0]) + (Tm[0][0][1] + Tm[0][1][1])) +
+                       ((Tm[1][0][0] + Tm[1][1][0]) + (Tm[1][0][1] + Tm[1][1][1]))) /
+                          tvol
+                    : 0.0;
+}
+
+// value of a located zone (coarse leaf zone or restriction of finer children)
+__device__ __forceinline__ double value_at(const km_pack_t &u, const Geo &g, const Src &s, int v) {
+  const double *q = cell_ptr(u, s.leaf, v);
+  return s.kind == 1 ? restrict_cell(q, u.blocks[s.leaf].vol, g, s.idx) : q[s.idx];
+}
+
+// axis a vector component points along (momentum, velocity, cell-centred B), else -1
+__device__ __forceinline__ int vector_axis(int v, int nc) {
+  if (v >= KM_MOM0 && v < KM_MOM0 + 3) return v - KM_MOM0;
+  if (v >= nc && v < nc + 3) return v - nc;
+  if (nc == 8 && v >= KM_MAGC0 && v < KM_MAGC0 + 3) return v - KM_MAGC0;
+  return -1;
+}
+
+__device__ __forceinline__ double sgn(double x) { return __builtin_copysign(1.0, x); }
+
+// GradMinMod (refinement_operations.hpp:42-48)
+__device__ __forceinline__ double grad_minmod(double fc, double fm, double fp, double dxm,
+                                              double dxp) {
+  const double gxm = (fc - fm) / dxm;
+  const double gxp = (fp - fc) / dxp;
+  return 0.5 * (sgn(gxm) + sgn(gxp)) * kmin(kabs(gxm), kabs(gxp));
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(256) k_amr_ghost(km_pack_t u, AmrGrid t, int v0, int v1,
+                                                   int nc) {
+  const Geo g(u);
+  const long n = g.ncell();
+  const long tt = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (tt >= n * u.nblocks) return;
+  const int b = (int)(tt / n);
+  long r = tt % n;
+  int idx[3];
+  idx[0] = (int)(r % g.E[0]);
+  r /= g.E[0];
+  idx[1] = (int)(r % g.E[1]);
+  idx[2] = (int)(r / g.E[1]);
+  bool ghost = false;
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    if (d < g.ndim && (idx[d] < g.lo[d] || idx[d] > g.hi[d])) ghost = true;
+  if (!ghost) return;
+  const km_block_t &blk = u.blocks[b];
+  const int L = blk.level;
+  int G0[3], G[3];
+  bool fl[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) G0[d] = d < g.ndim ? blk.lx[d] * g.nx[d] + (idx[d] - g.lo[d]) : 0;
+  map_zone(t, g, L, G0, G, fl);
+  const Src s = locate(u, t, g, L, G);
+  const long dc = g.c(idx[2], idx[1], idx[0]);
+  if (s.kind != 2) {
+    for (int v = v0; v < v1; ++v) {
+      double x = value_at(u, g, s, v);
+      const int ax = vector_axis(v, nc);
+      if (ax >= 0 && fl[ax]) x = -x;
+      cell_ptr(u, b, v)[dc] = x;
+    }
+    return;
+  }
+  // coarser cover: the coarse zone C = G/2 and its neighbours at level L-1
+  int C[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) C[d] = d < g.ndim ? (G[d] >> 1) : 0;
+  Src nbm[3], nbp[3];
+  bool flm[3][3], flp[3][3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (d < g.ndim) {
+      int Cm[3] = {C[0], C[1], C[2]}, Cp[3] = {C[0], C[1], C[2]}, M[3];
+      Cm[d] -= 1;
+      Cp[d] += 1;
+      map_zone(t, g, L - 1, Cm, M, flm[d]);
+      nbm[d] = locate(u, t, g, L - 1, M);
+      map_zone(t, g, L - 1, Cp, M, flp[d]);
+      nbp[d] = locate(u, t, g, L - 1, M);
+    }
+  }
+  const int oi = G[0] & 1, oj = g.ndim > 1 ? (G[1] & 1) : 0, ok = g.ndim > 2 ? (G[2] & 1) : 0;
+  for (int v = v0; v < v1; ++v) {
+    const int ax = vector_axis(v, nc);
+    const double fc = cell_ptr(u, s.leaf, v)[s.idx];
+    double gr[3] = {0., 0., 0.}, df[3] = {0., 0., 0.};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if (d < g.ndim) {
+        double fm = value_at(u, g, nbm[d], v), fp = value_at(u, g, nbp[d], v);
+        if (ax >= 0 && flm[d][ax]) fm = -fm;
+        if (ax >= 0 && flp[d][ax]) fp = -fp;
+        const double dxc = 2.0 * blk.dx[d];
+        gr[d] = grad_minmod(fc, fm, fp, dxc, dxc);
+        df[d] = 0.5 * blk.dx[d];
+      }
+    }
+    // the child formulas of ProlongateSharedGeneral (refinement_operations.hpp:241-259)
+    // with equal minus/plus slopes and offsets
+    const double a = gr[0] * df[0], bb = gr[1] * df[1], c = gr[2] * df[2];
+    double x;
+    if (!ok && !oj && !oi) x = fc - (a + bb + c);
+    else if (!ok && !oj) x = fc + (a - bb - c);
+    else if (!ok && !oi) x = fc - (a - bb + c);
+    else if (!ok) x = fc + (a + bb - c);
+    else if (!oj && !oi) x = fc - (a + bb - c);
+    else if (!oj) x = fc + (a - bb + c);
+    else if (!oi) x = fc - (a - bb - c);
+    else x = fc + (a + bb + c);
+    if (ax >= 0 && fl[ax]) x = -x;
+    cell_ptr(u, b, v)[dc] = x;
+  }
+}
+
+// Flux correction: a face a block shares with finer leaves takes the area-weighted
+// RestrictAverage of the fine face fluxes (face elements average over their two
+// transverse axes).
+__global__ void __launch_bounds__(256) k_amr_flux_correct(km_pack_t u, AmrGrid t, int nc,
+                                                          int ntmax) {
+  const Geo g(u);
+  const long tt = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long per_block = (long)g.ndim * 2 * ntmax;
+  if (tt >= per_block * u.nblocks) return;
+  const int b = (int)(tt / per_block);
+  long r = tt % per_block;
+  const int dir = (int)(r / (2L * ntmax));
+  r %= 2L * ntmax;
+  const int side = (int)(r / ntmax);
+  const int f = (int)(r % ntmax);
+  // transverse extents of this face
+  int ta = -1, tb = -1;  // the (up to two) transverse active axes
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    if (d != dir && d < g.ndim) {
+      if (ta < 0) ta = d;
+      else tb = d;
+    }
+  const int na = ta >= 0 ? g.nx[ta] : 1, nbt = tb >= 0 ? g.nx[tb] : 1;
+  if (f >= na * nbt) return;
+  int li[3] = {0, 0, 0};  // interior offsets
+  if (ta >= 0) li[ta] = f % na;
+  if (tb >= 0) li[tb] = f / na;
+  li[dir] = side ? g.nx[dir] - 1 : 0;
+  const km_block_t &blk = u.blocks[b];
+  const int L = blk.level;
+  int Go[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) Go[d] = d < g.ndim ? blk.lx[d] * g.nx[d] + li[d] : 0;
+  Go[dir] += side ? 1 : -1;
+  const int N = amr_zones(t, g, dir, L);
+  if (Go[dir] < 0 || Go[dir] >= N) {
+    if (t.bc[2 * dir + side] != KM_BC_PERIODIC) return;
+    Go[dir] = (Go[dir] + N) % N;
+  }
+  const int fl = amr_leaf(t, g, L, Go);
+  const km_block_t &fb = u.blocks[fl];
+  if (fb.level <= L) return;
+  // the fine faces: children of the neighbour zone on the face this block touches
+  int fi[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) fi[d] = amr_local(fb, g, d, 2 * Go[d]);
+  fi[dir] = side ? g.lo[dir] : g.hi[dir] + 1;
+  const long fidx = g.c(fi[2], fi[1], fi[0]);
+  int ci[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) ci[d] = li[d] + g.lo[d];
+  ci[dir] = side ? g.hi[dir] + 1 : g.lo[dir];
+  const long cidx = g.c(ci[2], ci[1], ci[0]);
+  const bool inc[3] = {dir != 0, dir != 1 && g.ndim > 1, dir != 2 && g.ndim > 2};
+  const double A = fb.area[dir];
+  for (int v = 0; v < nc; ++v) {
+    const double *F = flux_ptr(u, dir, fl, v);
+    double V[2][2][2], Tm[2][2][2];
+#pragma unroll
+    for (int ok = 0; ok < 2; ++ok)
+#pragma unroll
+      for (int oj = 0; oj < 2; ++oj)
+#pragma unroll
+        for (int oi = 0; oi < 2; ++oi) {
+          V[ok][oj][oi] = 0.;
+          Tm[ok][oj][oi] = 0.;
+          if ((ok == 0 || inc[2]) && (oj == 0 || inc[1]) && (oi == 0 || inc[0])) {
+            V[ok][oj][oi] = A;
+            Tm[ok][oj][oi] = A * F[fidx + ok * g.sc[2] + oj * g.sc[1] + oi];
+          }
+        }
+    const double tvol = ((V[0][0][0] + V[0][1][0]) + (V[0][0][1] + V[0][1][1])) +
+                        ((V[1][0][0] + V[1][1][0]) + (V[1][0][1] + V[1][1][1]));
+    flux_ptr(u, dir, b, v)[cidx] =
+        tvol > 0.0 ? (((Tm[0][0][0] + Tm[0][1][0]) + (Tm[0][0][1] + Tm[0][1][1])) +
+                      ((Tm[1][0][0] + Tm[1][1][0]) + (Tm[1][0][1] + Tm[1][1][1]))) /
+                         tvol
+                   : 0.0;
+  }
+}
+
+void launch_amr_ghost_fill(const AmrGrid &t, const km_pack_t &u, int v0, int v1, int nc,
+                           hipStream_t st) {
+  const Geo g(u);
+  const long n = g.ncell() * u.nblocks;
+  if (n == 0 || v0 >= v1) return;
+  hipLaunchKernelGGL(k_amr_ghost, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, u, t, v0,
+                     v1, nc);
+}
+
+void launch_amr_flux_correct(const AmrGrid &t, const km_pack_t &u, int nc, hipStream_t st) {
+  const Geo g(u);
+  if (t.lmax == 0 || u.nblocks == 0) return;
+  int ntmax = 1;
+  for (int dir = 0; dir < g.ndim; ++dir) {
+    int nt = 1;
+    for (int d = 0; d < g.ndim; ++d)
+      if (d != dir) nt *= g.nx[d];
+    ntmax = nt > ntmax ? nt : ntmax;
+  }
+  const long n = (long)ntmax * g.ndim * 2 * u.nblocks;
+  hipLaunchKernelGGL(k_amr_flux_correct, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, u,
+                     t, nc, ntmax);
+}
+
+}  // namespace km

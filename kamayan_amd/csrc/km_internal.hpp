@@ -16,10 +16,19 @@ struct Box {
   int lo[3], hi[3];
 };
 
-// block grid of a single-level mesh, blocks stored x-fastest
+// block grid of a single-level mesh, blocks stored x-fastest starting at block `first`
 struct UniformGrid {
   int nb[3];
   int bc[6];
+  int first;
+};
+
+// Independent instances sharing one block pack (an ensemble): instance of each block and
+// the grid of each instance, both device arrays. A single mesh has inst == nullptr and
+// uses the grid passed by value.
+struct Instances {
+  const int *inst = nullptr;
+  const UniformGrid *grids = nullptr;
 };
 
 inline Eos make_eos(const km_cfg_t &cfg) {
@@ -37,8 +46,9 @@ struct FusedArgs {
   km_pack_t base;      // RK base state (conserved)
   km_pack_t u1;        // stage output (conserved, interior written)
   double *du;          // [nblocks][5][nx3][nx2][nx1] accumulator (interior only)
-  const double *dt;    // device: step dt
-  double *dt_est;      // device: CFL minimum of u1 (bit-ordered), or null
+  const double *dt;    // device: step dt per instance
+  double *dt_est;      // device: CFL minimum of u1 per instance (bit-ordered), or null
+  const int *inst;     // device: instance of each block, or null (one instance)
   const int *nbr;      // [nblocks][3][2] face neighbours: block index, or
                        // KM_NBR_OUTFLOW / KM_NBR_REFLECT at a physical boundary
   double beta;
@@ -57,13 +67,14 @@ void launch_fluxes_to_dudt(const km_cfg_t &cfg, const km_pack_t &u, const km_pac
                            hipStream_t st);
 void launch_apply_dudt(const km_cfg_t &cfg, const km_pack_t &base, const km_pack_t &u0,
                        const km_pack_t &u1, const km_pack_t &dudt, double beta, double dt,
-                       hipStream_t st);
+                       hipStream_t st, const double *dt_dev = nullptr, const int *inst = nullptr);
 void launch_prepare_primitive(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st);
 void launch_prepare_conserved(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st);
-void launch_estimate_dt(const km_cfg_t &cfg, const km_pack_t &u, double *dt_dev, hipStream_t st);
+void launch_estimate_dt(const km_cfg_t &cfg, const km_pack_t &u, double *dt_dev, hipStream_t st,
+                        const int *inst = nullptr, int ninst = 1);
 void launch_fill_divb(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st);
 void launch_ghost_fill(const UniformGrid &gr, const km_pack_t &u, int v0, int v1, int nc,
-                       bool faces, hipStream_t st);
+                       bool faces, hipStream_t st, Instances in = Instances());
 
 // fused stage sweeps: one translation unit per arithmetic form
 //   exact : FP64, -ffp-contract=off (bitwise the reference's operation order)

@@ -159,11 +159,21 @@ int km_problem_fill(const char *name, const km_cfg_t *cfg, const km_mesh_desc_t 
                     uint64_t seed, const double *params, int32_t nparams, double *cell,
                     double *const face[3]) {
   if (!name || !cfg || !mesh || !cell) return KM_EINVAL;
+  std::vector<km_block_t> blocks((size_t)mesh->nb[0] * mesh->nb[1] * mesh->nb[2]);
+  km::make_uniform_blocks(*mesh, blocks.data());
+  return km_problem_fill_blocks(name, cfg, mesh, blocks.data(), (int32_t)blocks.size(), seed,
+                                params, nparams, cell, face);
+}
+
+int km_problem_fill_blocks(const char *name, const km_cfg_t *cfg, const km_mesh_desc_t *mesh,
+                           const km_block_t *blocks, int32_t nblocks, uint64_t seed,
+                           const double *params, int32_t nparams, double *cell,
+                           double *const face[3]) {
+  if (!name || !cfg || !mesh || !blocks || !cell || nblocks < 0) return KM_EINVAL;
   const std::string pname(name);
   const int mhd = cfg->mhd;
-  const Layout L(*mesh, KM_NVAR(mhd));
-  std::vector<km_block_t> blocks(L.nblocks);
-  km::make_uniform_blocks(*mesh, blocks.data());
+  Layout L(*mesh, KM_NVAR(mhd));
+  L.nblocks = nblocks;
   const int nc = KM_NCONS(mhd);
   const int sV = nc, sP = nc + 3;
   auto P = [&](int n, double dflt) { return (params && n < nparams) ? params[n] : dflt; };

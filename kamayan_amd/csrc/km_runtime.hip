@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "../../include/km_hydro.h"
+#include "km_amr.hpp"
 #include "km_device.hpp"
 #include "km_internal.hpp"
 
@@ -120,20 +121,25 @@ static Box flux_box(const km_cfg_t &cfg, const km_pack_t &u) {
 
 static hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
-__global__ void k_set(double *p, double v) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+__global__ void k_set(double *p, double v, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = v;
 }
-__global__ void k_scale(double *p, double s) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *p = *p * s;
+__global__ void k_scale(double *p, double s, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = p[t] * s;
 }
-// Parthenon time-step rule: time += dt; dt = min(2 dt, estimate) unless overridden
-__global__ void k_dt_update(double *dt, double *time, const double *est, int keep) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    const double d = *dt;
-    *time = *time + d;
-    if (!keep) *dt = kmin(2.0 * d, *est);
+// Parthenon time-step rule per instance: time += dt; dt = min(2 dt, estimate) unless
+// overridden
+__global__ void k_dt_update(double *dt, double *time, const double *est, int keep, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) {
+    const double d = dt[t];
+    time[t] = time[t] + d;
+    if (!keep) dt[t] = kmin(2.0 * d, est[t]);
   }
 }
+static inline unsigned nb64(int n) { return (unsigned)((n + 63) / 64); }
 
 }  // namespace km
 
@@ -141,6 +147,12 @@ using namespace km;
 
 struct km_ghost_plan_t {
   UniformGrid grid;
+  bool amr = false;  // block-tree plan: agrid over a device lookup table
+  AmrGrid agrid;
+  int *lookup_d = nullptr;
+  ~km_ghost_plan_t() {
+    if (lookup_d) (void)hipFree(lookup_d);
+  }
 };
 
 // ---------------------------------------------------------------------------------
@@ -175,7 +187,14 @@ struct km_sim_t {
   Cont base, stage[2], dudt, scratch;
   double *du = nullptr;
   int *nbr_d = nullptr;  // [nb][3][2] face neighbours for the fused sweeps
-  UniformGrid grid;
+  UniformGrid grid;      // grid of instance 0
+  bool amr = false;      // block-tree mesh (task path): ghost fill across levels and
+  AmrGrid agrid;         // flux correction use agrid
+  int ninst = 1;         // independent instances sharing the pack (ensemble)
+  std::vector<km_mesh_desc_t> meshes;
+  std::vector<UniformGrid> grids_h;
+  int *inst_d = nullptr;           // instance of each block (null for one instance)
+  UniformGrid *grids_d = nullptr;  // grid of each instance
   hipStream_t stream = nullptr;
   double *dt_d = nullptr, *est_d = nullptr, *time_d = nullptr;
   int ncycle = 0;
@@ -309,6 +328,14 @@ Cont &container(km_sim_t &s, int idx) {
   return s.stage[idx - 1];
 }
 
+// boundary exchange of cell vars [v0, v1) (+ CT face B) on one container
+void ghost_fill(km_sim_t &s, const km_pack_t &p, int v0, int v1, bool faces) {
+  if (s.amr)
+    launch_amr_ghost_fill(s.agrid, p, v0, v1, s.nc, s.stream);
+  else
+    launch_ghost_fill(s.grid, p, v0, v1, s.nc, faces, s.stream, Instances{s.inst_d, s.grids_d});
+}
+
 // one RK stage as the reference's task list
 int task_stage(km_sim_t &s, int stage, double beta, double dt) {
   const km_cfg_t &cfg = s.cfg;
@@ -321,15 +348,19 @@ int task_stage(km_sim_t &s, int stage, double beta, double dt) {
   const bool ct = cfg.mhd == KM_MHD_CT;
   launch_calculate_fluxes(cfg, p0, flux_box(cfg, p0), s.stream);
   if (ct) launch_calculate_emf(p0, s.stream);
+  // flux correction at level jumps (AddFluxCorrectionTasks, kamayan_driver.cpp:198-207)
+  if (s.amr) launch_amr_flux_correct(s.agrid, p0, s.nc, s.stream);
   launch_fluxes_to_dudt(cfg, p0, s.dudt.pack, s.stream);
-  launch_apply_dudt(cfg, s.base.pack, p0, md1.pack, s.dudt.pack, beta, dt, s.stream);
+  (void)dt;
+  launch_apply_dudt(cfg, s.base.pack, p0, md1.pack, s.dudt.pack, beta, 0.0, s.stream, s.dt_d,
+                    s.inst_d);
   launch_prepare_primitive(cfg, md1.pack, s.stream);
   if (stage == cfg.nstages) {
-    launch_estimate_dt(cfg, s.base.pack, s.est_d, s.stream);
+    launch_estimate_dt(cfg, s.base.pack, s.est_d, s.stream, s.inst_d, s.ninst);
     if (ct) launch_fill_divb(cfg, s.base.pack, s.stream);
   }
   // FillGhost fields: conserved, VELOCITY, PRES, BMOD, EINT, face B (TEMP is not)
-  launch_ghost_fill(s.grid, md1.pack, 0, KM_TEMP(cfg.mhd), s.nc, ct, s.stream);
+  ghost_fill(s, md1.pack, 0, KM_TEMP(cfg.mhd), ct);
   return check_launch();
 }
 
@@ -344,6 +375,7 @@ int fused_stage(km_sim_t &s, int stage, double beta) {
   a.du = s.du;
   a.dt = s.dt_d;
   a.dt_est = s.est_d;
+  a.inst = s.inst_d;
   a.nbr = s.nbr_d;
   a.beta = beta;
   a.lim = cfg.limiter;
@@ -355,7 +387,9 @@ int fused_stage(km_sim_t &s, int stage, double beta) {
     st1 = s.ev();
     (void)hipEventRecord(st0, s.stream);
   }
-  if (last) hipLaunchKernelGGL(k_set, dim3(1), dim3(64), 0, s.stream, s.est_d, DBL_MAX);
+  if (last)
+    hipLaunchKernelGGL(k_set, dim3(nb64(s.ninst)), dim3(64), 0, s.stream, s.est_d, DBL_MAX,
+                       s.ninst);
   for (int dir = 0; dir < ndim; ++dir) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (s.profile) {
@@ -377,8 +411,8 @@ int fused_stage(km_sim_t &s, int stage, double beta) {
     }
   }
   if (last)
-    hipLaunchKernelGGL(k_scale, dim3(1), dim3(64), 0, s.stream, s.est_d,
-                       cfg.cfl / static_cast<double>(ndim));
+    hipLaunchKernelGGL(k_scale, dim3(nb64(s.ninst)), dim3(64), 0, s.stream, s.est_d,
+                       cfg.cfl / static_cast<double>(ndim), s.ninst);
   if (s.profile) {
     (void)hipEventRecord(st1, s.stream);
     s.recs.push_back({4, st0, st1});
@@ -399,8 +433,8 @@ int run_cycle(km_sim_t &s, double dt_host) {
     s.prims_from_ic = false;
     if (rc) return rc;
   }
-  hipLaunchKernelGGL(k_dt_update, dim3(1), dim3(64), 0, s.stream, s.dt_d, s.time_d, s.est_d,
-                     s.dt_keep ? 1 : 0);
+  hipLaunchKernelGGL(k_dt_update, dim3(nb64(s.ninst)), dim3(64), 0, s.stream, s.dt_d, s.time_d,
+                     s.est_d, s.dt_keep ? 1 : 0, s.ninst);
   s.dt_keep = false;
   s.ncycle += 1;
   return check_launch();
@@ -543,6 +577,34 @@ int km_ghost_plan_destroy(km_ghost_plan_t *plan) {
   return KM_OK;
 }
 
+int km_ghost_plan_amr(const km_pack_t *shape, const km_mesh_desc_t *mesh,
+                      const int32_t *leaves, int32_t nleaves, km_ghost_plan_t **plan) {
+  if (!shape || !mesh || !leaves || !plan) return KM_EINVAL;
+  if (nleaves != shape->nblocks || shape->ndim != mesh->ndim || shape->ng != mesh->ng) {
+    set_error("pack shape does not match the mesh and its leaves");
+    return KM_EINVAL;
+  }
+  for (int d = 0; d < shape->ndim; ++d)
+    if (shape->nx[d] != mesh->nx[d]) {
+      set_error("pack block size does not match the mesh");
+      return KM_EINVAL;
+    }
+  AmrTreeHost t;
+  const std::string err = build_amr_tree(*mesh, leaves, nleaves, t);
+  if (!err.empty()) {
+    set_error(err);
+    return KM_EINVAL;
+  }
+  auto p = std::make_unique<km_ghost_plan_t>();
+  p->amr = true;
+  KM_HIP(hipMalloc(&p->lookup_d, t.lookup.size() * sizeof(int)));
+  KM_HIP(hipMemcpy(p->lookup_d, t.lookup.data(), t.lookup.size() * sizeof(int),
+                   hipMemcpyHostToDevice));
+  p->agrid = t.grid(p->lookup_d);
+  *plan = p.release();
+  return KM_OK;
+}
+
 int km_ghost_fill(const km_ghost_plan_t *plan, const km_pack_t *u, int32_t v0, int32_t v1,
                   int32_t with_faces, void *stream) {
   if (!plan || !u || v0 < 0 || v1 > u->nvar || v0 > v1) return KM_EINVAL;
@@ -551,24 +613,56 @@ int km_ghost_fill(const km_ghost_plan_t *plan, const km_pack_t *u, int32_t v0, i
     set_error("face tables missing");
     return KM_EINVAL;
   }
-  launch_ghost_fill(plan->grid, *u, v0, v1, nc, with_faces != 0, as_stream(stream));
+  if (plan->amr) {
+    if (with_faces) {
+      set_error("face fields across refinement levels (Toth-Roe prolongation) are not built");
+      return KM_EINVAL;
+    }
+    launch_amr_ghost_fill(plan->agrid, *u, v0, v1, nc, as_stream(stream));
+  } else {
+    launch_ghost_fill(plan->grid, *u, v0, v1, nc, with_faces != 0, as_stream(stream));
+  }
+  return check_launch();
+}
+
+int km_flux_correct(const km_ghost_plan_t *plan, const km_pack_t *u, void *stream) {
+  if (!plan || !u) return KM_EINVAL;
+  if (!plan->amr) return KM_OK;  // one level: nothing to correct
+  for (int d = 0; d < u->ndim; ++d)
+    if (!u->flux[d]) {
+      set_error("flux tables missing");
+      return KM_EINVAL;
+    }
+  const int nc = u->nflux < (u->nvar >= 16 ? 8 : 5) ? u->nflux : (u->nvar >= 16 ? 8 : 5);
+  launch_amr_flux_correct(plan->agrid, *u, nc, as_stream(stream));
   return check_launch();
 }
 
 // ---- whole-mesh driver ----
-int km_sim_create(const km_cfg_t *cfg, const km_mesh_desc_t *mesh, int32_t path,
-                  km_sim_t **out) {
-  if (!cfg || !mesh || !out) return KM_EINVAL;
+static int sim_create(const km_cfg_t *cfg, const km_mesh_desc_t *meshes, int32_t ninst,
+                      int32_t path, const AmrTreeHost *tree, km_sim_t **out) {
+  if (!cfg || !meshes || !out || ninst < 1) return KM_EINVAL;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
     set_error("no GPU visible");
     return KM_ENODEV;
+  }
+  const km_mesh_desc_t *mesh = &meshes[0];
+  for (int m = 1; m < ninst; ++m) {
+    bool same = meshes[m].ndim == mesh->ndim && meshes[m].ng == mesh->ng;
+    for (int d = 0; d < mesh->ndim; ++d) same = same && meshes[m].nx[d] == mesh->nx[d];
+    if (!same) {
+      set_error("ensemble instances must share ndim, nghost and cells per block");
+      return KM_EINVAL;
+    }
   }
   auto s = std::make_unique<km_sim_t>();
   s->cfg = *cfg;
   s->cfg.ndim = mesh->ndim;
   s->mesh = *mesh;
   s->path = path;
+  s->ninst = ninst;
+  s->meshes.assign(meshes, meshes + ninst);
   if (path == KM_PATH_FUSED && cfg->mhd != KM_MHD_OFF) {
     set_error("the fused stage path is built for hydro (MHD off)");
     return KM_EINVAL;
@@ -577,7 +671,13 @@ int km_sim_create(const km_cfg_t *cfg, const km_mesh_desc_t *mesh, int32_t path,
     set_error("integrator must be rk1, rk2 or rk3");
     return KM_EINVAL;
   }
-  s->nb = mesh->nb[0] * mesh->nb[1] * mesh->nb[2];
+  s->nb = 0;
+  std::vector<int> first(ninst);
+  for (int m = 0; m < ninst; ++m) {
+    first[m] = s->nb;
+    s->nb += meshes[m].nb[0] * meshes[m].nb[1] * meshes[m].nb[2];
+  }
+  if (tree) s->nb = (int)tree->blocks.size();
   s->nvar = KM_NVAR(cfg->mhd);
   s->nc = KM_NCONS(cfg->mhd);
   km_pack_t shape;
@@ -591,14 +691,49 @@ int km_sim_create(const km_cfg_t *cfg, const km_mesh_desc_t *mesh, int32_t path,
   const Geo g(shape);
   s->ncell = g.ncell();
   s->nstag = g.nstag();
-  for (int d = 0; d < 3; ++d) s->grid.nb[d] = mesh->nb[d];
-  for (int d = 0; d < 6; ++d) s->grid.bc[d] = mesh->bc[d];
+  s->grids_h.resize(ninst);
+  for (int m = 0; m < ninst; ++m) {
+    for (int d = 0; d < 3; ++d) s->grids_h[m].nb[d] = meshes[m].nb[d];
+    for (int d = 0; d < 6; ++d) s->grids_h[m].bc[d] = meshes[m].bc[d];
+    s->grids_h[m].first = first[m];
+    for (int d = 0; d < meshes[m].ndim; ++d) {
+      const bool plo = meshes[m].bc[2 * d] == KM_BC_PERIODIC;
+      const bool phi = meshes[m].bc[2 * d + 1] == KM_BC_PERIODIC;
+      if (plo != phi) {
+        set_error("periodic boundaries must be paired");
+        return KM_EINVAL;
+      }
+    }
+  }
+  s->grid = s->grids_h[0];
   KM_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
   std::vector<km_block_t> blocks(s->nb);
-  km_mesh_blocks(mesh, blocks.data());
+  if (tree)
+    blocks = tree->blocks;
+  else
+    for (int m = 0; m < ninst; ++m) km_mesh_blocks(&meshes[m], blocks.data() + first[m]);
   int rc;
   if ((rc = s->alloc(&s->blocks_d, s->nb))) return rc;
   KM_HIP(hipMemcpy(s->blocks_d, blocks.data(), s->nb * sizeof(km_block_t), hipMemcpyHostToDevice));
+  if (tree) {
+    int *lookup_d = nullptr;
+    if ((rc = s->alloc(&lookup_d, tree->lookup.size()))) return rc;
+    KM_HIP(hipMemcpy(lookup_d, tree->lookup.data(), tree->lookup.size() * sizeof(int),
+                     hipMemcpyHostToDevice));
+    s->amr = true;
+    s->agrid = tree->grid(lookup_d);
+  }
+  if (ninst > 1) {
+    std::vector<int> inst(s->nb);
+    for (int m = 0; m < ninst; ++m)
+      for (int b = first[m]; b < first[m] + meshes[m].nb[0] * meshes[m].nb[1] * meshes[m].nb[2]; ++b)
+        inst[b] = m;
+    if ((rc = s->alloc(&s->inst_d, s->nb))) return rc;
+    KM_HIP(hipMemcpy(s->inst_d, inst.data(), inst.size() * sizeof(int), hipMemcpyHostToDevice));
+    if ((rc = s->alloc(&s->grids_d, ninst))) return rc;
+    KM_HIP(hipMemcpy(s->grids_d, s->grids_h.data(), ninst * sizeof(UniformGrid),
+                     hipMemcpyHostToDevice));
+  }
   if ((rc = make_cont(*s, s->base, true, nullptr, true, false))) return rc;
   for (int i = 0; i < 2; ++i)
     if ((rc = make_cont(*s, s->stage[i], false, &s->base, true, false))) return rc;
@@ -607,40 +742,85 @@ int km_sim_create(const km_cfg_t *cfg, const km_mesh_desc_t *mesh, int32_t path,
   if (path == KM_PATH_FUSED) {
     const long nint = (long)shape.nx[0] * shape.nx[1] * shape.nx[2];
     if ((rc = s->alloc(&s->du, (size_t)s->nb * 5 * nint))) return rc;
-    // face-neighbour table of the uniform block grid (periodic wraps; other
+    // face-neighbour table of every instance's uniform block grid (periodic wraps; other
     // boundaries are mapped inside the sweep)
     std::vector<int> nbr((size_t)s->nb * 6);
-    for (int b = 0; b < s->nb; ++b) {
-      const int l[3] = {b % mesh->nb[0], (b / mesh->nb[0]) % mesh->nb[1],
-                        b / (mesh->nb[0] * mesh->nb[1])};
-      for (int d = 0; d < 3; ++d)
-        for (int side = 0; side < 2; ++side) {
-          int m[3] = {l[0], l[1], l[2]};
-          m[d] += side ? 1 : -1;
-          int val;
-          if (d >= mesh->ndim) {
-            val = KM_NBR_OUTFLOW;
-          } else if (m[d] < 0 || m[d] >= mesh->nb[d]) {
-            const int bc = mesh->bc[2 * d + side];
-            if (bc == KM_BC_PERIODIC) {
-              m[d] = (m[d] + mesh->nb[d]) % mesh->nb[d];
-              val = (m[2] * mesh->nb[1] + m[1]) * mesh->nb[0] + m[0];
+    for (int m = 0; m < ninst; ++m) {
+      const km_mesh_desc_t &me = meshes[m];
+      const int nbm = me.nb[0] * me.nb[1] * me.nb[2];
+      for (int lb = 0; lb < nbm; ++lb) {
+        const int l[3] = {lb % me.nb[0], (lb / me.nb[0]) % me.nb[1], lb / (me.nb[0] * me.nb[1])};
+        for (int d = 0; d < 3; ++d)
+          for (int side = 0; side < 2; ++side) {
+            int mm[3] = {l[0], l[1], l[2]};
+            mm[d] += side ? 1 : -1;
+            int val;
+            if (d >= me.ndim) {
+              val = KM_NBR_OUTFLOW;
+            } else if (mm[d] < 0 || mm[d] >= me.nb[d]) {
+              const int bc = me.bc[2 * d + side];
+              if (bc == KM_BC_PERIODIC) {
+                mm[d] = (mm[d] + me.nb[d]) % me.nb[d];
+                val = first[m] + (mm[2] * me.nb[1] + mm[1]) * me.nb[0] + mm[0];
+              } else {
+                val = bc == KM_BC_REFLECT ? KM_NBR_REFLECT : KM_NBR_OUTFLOW;
+              }
             } else {
-              val = bc == KM_BC_REFLECT ? KM_NBR_REFLECT : KM_NBR_OUTFLOW;
+              val = first[m] + (mm[2] * me.nb[1] + mm[1]) * me.nb[0] + mm[0];
             }
-          } else {
-            val = (m[2] * mesh->nb[1] + m[1]) * mesh->nb[0] + m[0];
+            nbr[(size_t)(first[m] + lb) * 6 + d * 2 + side] = val;
           }
-          nbr[(size_t)b * 6 + d * 2 + side] = val;
-        }
+      }
     }
     if ((rc = s->alloc(&s->nbr_d, nbr.size()))) return rc;
     KM_HIP(hipMemcpy(s->nbr_d, nbr.data(), nbr.size() * sizeof(int), hipMemcpyHostToDevice));
   }
-  if ((rc = s->alloc(&s->dt_d, 4))) return rc;
-  s->est_d = s->dt_d + 1;
-  s->time_d = s->dt_d + 2;
+  if ((rc = s->alloc(&s->dt_d, 3 * (size_t)ninst))) return rc;
+  s->est_d = s->dt_d + ninst;
+  s->time_d = s->dt_d + 2 * ninst;
   *out = s.release();
+  return KM_OK;
+}
+
+int km_sim_create_ensemble(const km_cfg_t *cfg, const km_mesh_desc_t *meshes, int32_t ninst,
+                           int32_t path, km_sim_t **out) {
+  return sim_create(cfg, meshes, ninst, path, nullptr, out);
+}
+
+int km_sim_create(const km_cfg_t *cfg, const km_mesh_desc_t *mesh, int32_t path,
+                  km_sim_t **out) {
+  return sim_create(cfg, mesh, 1, path, nullptr, out);
+}
+
+int km_sim_create_amr(const km_cfg_t *cfg, const km_mesh_desc_t *mesh, const int32_t *leaves,
+                      int32_t nleaves, int32_t path, km_sim_t **out) {
+  if (!cfg || !mesh || !leaves || !out) return KM_EINVAL;
+  if (path != KM_PATH_TASKS) {
+    set_error("block-tree meshes run on the task path");
+    return KM_EINVAL;
+  }
+  if (cfg->mhd != KM_MHD_OFF) {
+    set_error("face fields across refinement levels (Toth-Roe prolongation) are not built");
+    return KM_EINVAL;
+  }
+  AmrTreeHost t;
+  const std::string err = build_amr_tree(*mesh, leaves, nleaves, t);
+  if (!err.empty()) {
+    set_error(err);
+    return KM_EINVAL;
+  }
+  return sim_create(cfg, mesh, 1, path, &t, out);
+}
+
+int km_sim_ensemble_info(const km_sim_t *s, double *times, double *dts, int32_t n) {
+  if (!s || n < 1 || n > s->ninst) return KM_EINVAL;
+  std::vector<double> h(3 * (size_t)s->ninst);
+  KM_HIP(hipStreamSynchronize(s->stream));
+  KM_HIP(hipMemcpy(h.data(), s->dt_d, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int m = 0; m < n; ++m) {
+    if (dts) dts[m] = h[m];
+    if (times) times[m] = h[2 * s->ninst + m];
+  }
   return KM_OK;
 }
 
@@ -680,11 +860,12 @@ int km_sim_set_primitive(km_sim_t *s, const double *cell, const double *const fa
   KM_HIP(hipStreamSynchronize(s->stream));
   if (int rc = copy_state(s, cell, face, nullptr, nullptr)) return rc;
   launch_prepare_conserved(s->cfg, s->base.pack, s->stream);
-  launch_ghost_fill(s->grid, s->base.pack, 0, KM_TEMP(s->cfg.mhd), s->nc,
-                    s->cfg.mhd == KM_MHD_CT, s->stream);
-  launch_estimate_dt(s->cfg, s->base.pack, s->est_d, s->stream);
-  hipMemcpyAsync(s->dt_d, s->est_d, sizeof(double), hipMemcpyDeviceToDevice, s->stream);
-  hipLaunchKernelGGL(k_set, dim3(1), dim3(64), 0, s->stream, s->time_d, 0.0);
+  ghost_fill(*s, s->base.pack, 0, KM_TEMP(s->cfg.mhd), s->cfg.mhd == KM_MHD_CT);
+  launch_estimate_dt(s->cfg, s->base.pack, s->est_d, s->stream, s->inst_d, s->ninst);
+  hipMemcpyAsync(s->dt_d, s->est_d, s->ninst * sizeof(double), hipMemcpyDeviceToDevice,
+                 s->stream);
+  hipLaunchKernelGGL(k_set, dim3(nb64(s->ninst)), dim3(64), 0, s->stream, s->time_d, 0.0,
+                     s->ninst);
   if (int rc = check_launch()) return rc;
   KM_HIP(hipStreamSynchronize(s->stream));
   s->ncycle = 0;
@@ -705,7 +886,8 @@ int km_sim_get_state(km_sim_t *s, double *cell, double *const face[3]) {
   if (s->path == KM_PATH_FUSED) {
     // the fused path keeps only the conserved interior current; fill the ghost zones
     // and derive the primitives
-    launch_ghost_fill(s->grid, s->base.pack, 0, s->nc, s->nc, false, s->stream);
+    launch_ghost_fill(s->grid, s->base.pack, 0, s->nc, s->nc, false, s->stream,
+                      Instances{s->inst_d, s->grids_d});
     launch_prepare_primitive(s->cfg, s->base.pack, s->stream);
     KM_HIP(hipStreamSynchronize(s->stream));
   }
@@ -726,11 +908,14 @@ int km_sim_advance(km_sim_t *s, int32_t ncycles, double tlim) {
     const bool need_host =
         s->path == KM_PATH_TASKS || s->prims_from_ic || s->dt_keep || tlim > 0.;
     if (need_host) {
-      double h[3];
-      KM_HIP(hipMemcpyAsync(h, s->dt_d, 3 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+      double dt = 0., time = 0.;
+      KM_HIP(hipMemcpyAsync(&dt, s->dt_d, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+      KM_HIP(hipMemcpyAsync(&time, s->time_d, sizeof(double), hipMemcpyDeviceToHost, s->stream));
       KM_HIP(hipStreamSynchronize(s->stream));
-      double dt = h[0];
-      const double time = h[2];
+      if (tlim > 0. && s->ninst > 1) {
+        set_error("tlim is not supported for ensembles; advance by cycles");
+        return KM_EINVAL;
+      }
       if (tlim > 0.) {
         if (time >= tlim) break;
         if (time + dt > tlim) {
@@ -814,11 +999,12 @@ int km_sim_set_tile(km_sim_t *s, int32_t threads) {
 
 int km_sim_info(const km_sim_t *s, double *time, double *dt, int32_t *ncycle) {
   if (!s) return KM_EINVAL;
-  double h[3];
+  double hd = 0., ht = 0.;
   KM_HIP(hipStreamSynchronize(s->stream));
-  KM_HIP(hipMemcpy(h, s->dt_d, 3 * sizeof(double), hipMemcpyDeviceToHost));
-  if (time) *time = h[2];
-  if (dt) *dt = h[0];
+  KM_HIP(hipMemcpy(&hd, s->dt_d, sizeof(double), hipMemcpyDeviceToHost));
+  KM_HIP(hipMemcpy(&ht, s->time_d, sizeof(double), hipMemcpyDeviceToHost));
+  if (time) *time = ht;
+  if (dt) *dt = hd;
   if (ncycle) *ncycle = s->ncycle;
   return KM_OK;
 }

@@ -15,7 +15,7 @@ namespace km {
 // bx holds the (CT-widened, :59-70) interior bounds; cells bx.lo-1..bx.hi+1 along DIR are
 // reconstructed and faces bx.lo..bx.hi+1 solved, as in the reference's loops.
 // ---------------------------------------------------------------------------------
-template <int RECON, int RIEM, int MHD, int DIR>
+template <class T, int RECON, int RIEM, int MHD, int DIR>
 __global__ void __launch_bounds__(256) k_flux(km_pack_t u, int lim, Box bx) {
   constexpr int NR = HV<MHD>::NR, NC = HV<MHD>::NC;
   constexpr int DA = DIR == 0 ? 1 : 0;  // transverse dim mapped to consecutive lanes
@@ -42,22 +42,24 @@ __global__ void __launch_bounds__(256) k_flux(km_pack_t u, int lim, Box bx) {
   const double *bface = MHD ? u.face[DIR][b] : nullptr;
   const long sb = g.s(idx[2], idx[1], idx[0]);
   const long ssd = g.ss[DIR];
-  march<double, RECON, NR>(
+  // T = double: the reference's arithmetic; T = float: the FP32-mixed form (states and
+  // Riemann solve in FP32, fluxes stored in FP64)
+  march<T, RECON, NR>(
       lim, bx.lo[DIR] - 1, bx.hi[DIR] + 1,
-      [&](int c, double *p) {
+      [&](int c, T *p) {
 #pragma unroll
-        for (int r = 0; r < NR; ++r) p[r] = q[r][cb + c * cs];
+        for (int r = 0; r < NR; ++r) p[r] = T(ldg(q[r], cb + c * cs));
       },
-      [&](int c, double *vL, double *vR) {
+      [&](int c, T *vL, T *vR) {
         if constexpr (MHD) {  // :104-107,167-170,233-236
-          const double bf = bface[sb + c * ssd];
+          const T bf = T(ldg(bface, sb + c * ssd));
           vL[R_MAGC + DIR] = bf;
           vR[R_MAGC + DIR] = bf;
         }
-        double F[NC];
+        T F[NC];
         riemann<RIEM, MHD, DIR>(vL, vR, F);
 #pragma unroll
-        for (int v = 0; v < NC; ++v) f[v][cb + c * cs] = F[v];
+        for (int v = 0; v < NC; ++v) stg(f[v], cb + c * cs, double(F[v]));
       });
 }
 
@@ -182,10 +184,12 @@ __global__ void __launch_bounds__(256) k_stokes(km_pack_t u, km_pack_t dudt) {
 // grid::ApplyDuDt_impl (grid.cpp:213-241), cell part
 __global__ void __launch_bounds__(256) k_apply_cell(km_pack_t base, km_pack_t u0, km_pack_t u1,
                                                     km_pack_t dudt, int nc, double beta,
-                                                    double dt) {
+                                                    double dt, const double *dtp,
+                                                    const int *inst) {
   const Geo g(u0);
   int b, k, j, i;
   if (!interior_cell(g, u0.nblocks, b, k, j, i)) return;
+  if (dtp) dt = dtp[inst ? inst[b] : 0];
   const long c = g.c(k, j, i);
   for (int var = 0; var < nc; ++var) {
     double *p0 = cell_ptr(u0, b, var);
@@ -198,7 +202,8 @@ __global__ void __launch_bounds__(256) k_apply_cell(km_pack_t base, km_pack_t u0
 // grid::ApplyDuDt_impl (grid.cpp:262-282), face part
 template <int FACE>
 __global__ void __launch_bounds__(256) k_apply_face(km_pack_t base, km_pack_t u0, km_pack_t u1,
-                                                    km_pack_t dudt, double beta, double dt) {
+                                                    km_pack_t dudt, double beta, double dt,
+                                                    const double *dtp, const int *inst) {
   const Geo g(u0);
   const int di = FACE == 0, dj = FACE == 1, dk = FACE == 2;
   const int ni = g.nx[0] + di, nj = (g.ndim > 1 ? g.nx[1] : 1) + dj,
@@ -213,6 +218,7 @@ __global__ void __launch_bounds__(256) k_apply_face(km_pack_t base, km_pack_t u0
   const int j = g.lo[1] + (int)(r % nj);
   const int k = g.lo[2] + (int)(r / nj);
   const long s = g.s(k, j, i);
+  if (dtp) dt = dtp[inst ? inst[b] : 0];
   double *p0 = u0.face[FACE][b];
   const double v0 = beta * base.face[FACE][b][s] + (1.0 - beta) * p0[s];
   p0[s] = v0;
@@ -308,19 +314,21 @@ __device__ __forceinline__ double wave_min(double v) {
   return v;
 }
 
-__global__ void k_fill_value(double *p, double v) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+__global__ void k_fill_value(double *p, double v, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = v;
 }
 
 // hydro EstimateTimeStep (hydro_time_step.cpp:19-65): min over the interior of
 // dx/(|v| + cf); *out must hold DBL_MAX on entry
 template <int MHD>
-__global__ void __launch_bounds__(256) k_estimate_dt(km_pack_t u, double *out) {
+__global__ void __launch_bounds__(256) k_estimate_dt(km_pack_t u, double *out, const int *inst) {
   constexpr int NR = HV<MHD>::NR;
   const Geo g(u);
-  int b, k, j, i;
+  int b = 0, k, j, i;
   double dtl = DBL_MAX;
-  if (interior_cell(g, u.nblocks, b, k, j, i)) {
+  const bool valid = interior_cell(g, u.nblocks, b, k, j, i);
+  if (valid) {
     const km_block_t &blk = u.blocks[b];
     const long c = g.c(k, j, i);
     double V[NR];
@@ -329,6 +337,19 @@ __global__ void __launch_bounds__(256) k_estimate_dt(km_pack_t u, double *out) {
     dtl = kmin(dtl, blk.dx[0] / (kabs(V[R_VEL + 0]) + fast_speed<MHD, 0>(V)));
     if (g.ndim > 1) dtl = kmin(dtl, blk.dx[1] / (kabs(V[R_VEL + 1]) + fast_speed<MHD, 1>(V)));
     if (g.ndim > 2) dtl = kmin(dtl, blk.dx[2] / (kabs(V[R_VEL + 2]) + fast_speed<MHD, 2>(V)));
+  }
+  if (inst) {
+    // instances may change inside a workgroup: reduce per wavefront when it holds one
+    // instance, else per lane
+    const int my = valid ? inst[b] : -1;
+    const int first = __shfl(my, 0, 64);
+    if (__all(my == first)) {
+      dtl = wave_min(dtl);
+      if ((threadIdx.x & 63) == 0 && first >= 0) atomic_min_pos(out + first, dtl);
+    } else if (valid) {
+      atomic_min_pos(out + my, dtl);
+    }
+    return;
   }
   dtl = wave_min(dtl);
   __shared__ double part[4];
@@ -341,8 +362,9 @@ __global__ void __launch_bounds__(256) k_estimate_dt(km_pack_t u, double *out) {
   }
 }
 
-__global__ void k_scale_value(double *p, double s) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *p = *p * s;
+__global__ void k_scale_value(double *p, double s, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = p[t] * s;
 }
 
 // hydro FillDerived_impl (hydro.cpp:150-194)
@@ -400,8 +422,8 @@ __device__ __forceinline__ int map_face(int g, int N, int lo, int hi, bool &flip
   return g;
 }
 
-__global__ void __launch_bounds__(256) k_ghost_cells(km_pack_t u, UniformGrid gr, int v0, int v1,
-                                                     int nc) {
+__global__ void __launch_bounds__(256) k_ghost_cells(km_pack_t u, UniformGrid gr0, int v0, int v1,
+                                                     int nc, Instances in) {
   const Geo g(u);
   const long n = g.ncell();
   long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -418,7 +440,9 @@ __global__ void __launch_bounds__(256) k_ghost_cells(km_pack_t u, UniformGrid gr
   for (int d = 0; d < 3; ++d)
     if (d < g.ndim && (idx[d] < g.lo[d] || idx[d] > g.hi[d])) ghost = true;
   if (!ghost) return;
-  const int bl[3] = {b % gr.nb[0], (b / gr.nb[0]) % gr.nb[1], b / (gr.nb[0] * gr.nb[1])};
+  const UniformGrid gr = in.inst ? in.grids[in.inst[b]] : gr0;
+  const int lb = b - gr.first;
+  const int bl[3] = {lb % gr.nb[0], (lb / gr.nb[0]) % gr.nb[1], lb / (gr.nb[0] * gr.nb[1])};
   int src[3], sb[3];
   bool flip[3] = {false, false, false};
 #pragma unroll
@@ -434,7 +458,7 @@ __global__ void __launch_bounds__(256) k_ghost_cells(km_pack_t u, UniformGrid gr
     sb[d] = m / g.nx[d];
     src[d] = m % g.nx[d] + g.lo[d];
   }
-  const int srcb = (sb[2] * gr.nb[1] + sb[1]) * gr.nb[0] + sb[0];
+  const int srcb = gr.first + (sb[2] * gr.nb[1] + sb[1]) * gr.nb[0] + sb[0];
   const long dc = g.c(idx[2], idx[1], idx[0]), scx = g.c(src[2], src[1], src[0]);
   const bool fl0 = flip[0], fl1 = flip[1], fl2 = flip[2];
   for (int v = v0; v < v1; ++v) {
@@ -448,7 +472,8 @@ __global__ void __launch_bounds__(256) k_ghost_cells(km_pack_t u, UniformGrid gr
   }
 }
 
-__global__ void __launch_bounds__(256) k_ghost_faces(km_pack_t u, UniformGrid gr, int face) {
+__global__ void __launch_bounds__(256) k_ghost_faces(km_pack_t u, UniformGrid gr0, int face,
+                                                     Instances in) {
   const Geo g(u);
   const long n = g.nstag();
   long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -469,7 +494,9 @@ __global__ void __launch_bounds__(256) k_ghost_faces(km_pack_t u, UniformGrid gr
     if (d < g.ndim && (idx[d] < g.lo[d] || idx[d] > hi)) ghost = true;
   }
   if (!ghost) return;
-  const int bl[3] = {b % gr.nb[0], (b / gr.nb[0]) % gr.nb[1], b / (gr.nb[0] * gr.nb[1])};
+  const UniformGrid gr = in.inst ? in.grids[in.inst[b]] : gr0;
+  const int lb = b - gr.first;
+  const int bl[3] = {lb % gr.nb[0], (lb / gr.nb[0]) % gr.nb[1], lb / (gr.nb[0] * gr.nb[1])};
   int src[3], sb[3];
   bool flip[3] = {false, false, false};
 #pragma unroll
@@ -491,7 +518,7 @@ __global__ void __launch_bounds__(256) k_ghost_faces(km_pack_t u, UniformGrid gr
       src[d] = m % g.nx[d] + g.lo[d];
     }
   }
-  const int srcb = (sb[2] * gr.nb[1] + sb[1]) * gr.nb[0] + sb[0];
+  const int srcb = gr.first + (sb[2] * gr.nb[1] + sb[1]) * gr.nb[0] + sb[0];
   double val = u.face[face][srcb][g.s(src[2], src[1], src[0])];
   if ((face == 0 && flip[0]) || (face == 1 && flip[1]) || (face == 2 && flip[2])) val = -val;
   u.face[face][b][g.s(idx[2], idx[1], idx[0])] = val;
@@ -502,47 +529,58 @@ __global__ void __launch_bounds__(256) k_ghost_faces(km_pack_t u, UniformGrid gr
 // ---------------------------------------------------------------------------------
 static inline unsigned nblk(long n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
 
-template <int RECON, int RIEM, int MHD>
+template <class T, int RECON, int RIEM, int MHD>
 static void launch_flux_dirs(const km_pack_t &u, int lim, const Box &bx, hipStream_t st) {
   const Geo g(u);
   auto cnt = [&](int da, int db) {
     return (long)(bx.hi[da] - bx.lo[da] + 1) * (bx.hi[db] - bx.lo[db] + 1) * u.nblocks;
   };
-  hipLaunchKernelGGL((k_flux<RECON, RIEM, MHD, 0>), dim3(nblk(cnt(1, 2))), dim3(256), 0, st, u,
+  hipLaunchKernelGGL((k_flux<T, RECON, RIEM, MHD, 0>), dim3(nblk(cnt(1, 2))), dim3(256), 0, st, u,
                      lim, bx);
   if (g.ndim > 1)
-    hipLaunchKernelGGL((k_flux<RECON, RIEM, MHD, 1>), dim3(nblk(cnt(0, 2))), dim3(256), 0, st, u,
+    hipLaunchKernelGGL((k_flux<T, RECON, RIEM, MHD, 1>), dim3(nblk(cnt(0, 2))), dim3(256), 0, st, u,
                        lim, bx);
   if (g.ndim > 2)
-    hipLaunchKernelGGL((k_flux<RECON, RIEM, MHD, 2>), dim3(nblk(cnt(0, 1))), dim3(256), 0, st, u,
+    hipLaunchKernelGGL((k_flux<T, RECON, RIEM, MHD, 2>), dim3(nblk(cnt(0, 1))), dim3(256), 0, st, u,
                        lim, bx);
 }
 
 template <int RECON, int RIEM>
-static void launch_flux_mhd(const km_pack_t &u, int mhd, int lim, const Box &bx,
+static void launch_flux_mhd(const km_pack_t &u, int mhd, bool fp32, int lim, const Box &bx,
                             hipStream_t st) {
-  if (mhd)
-    launch_flux_dirs<RECON, RIEM, 1>(u, lim, bx, st);
-  else
-    launch_flux_dirs<RECON, RIEM, 0>(u, lim, bx, st);
+  if (fp32) {
+    if (mhd)
+      launch_flux_dirs<float, RECON, RIEM, 1>(u, lim, bx, st);
+    else
+      launch_flux_dirs<float, RECON, RIEM, 0>(u, lim, bx, st);
+  } else {
+    if (mhd)
+      launch_flux_dirs<double, RECON, RIEM, 1>(u, lim, bx, st);
+    else
+      launch_flux_dirs<double, RECON, RIEM, 0>(u, lim, bx, st);
+  }
 }
 
 template <int RECON>
-static void launch_flux_riem(const km_pack_t &u, int riem, int mhd, int lim, const Box &bx,
-                             hipStream_t st) {
+static void launch_flux_riem(const km_pack_t &u, int riem, int mhd, bool fp32, int lim,
+                             const Box &bx, hipStream_t st) {
   if (riem == RIEMANN_HLLC)
-    launch_flux_mhd<RECON, RIEMANN_HLLC>(u, mhd, lim, bx, st);
+    launch_flux_mhd<RECON, RIEMANN_HLLC>(u, mhd, fp32, lim, bx, st);
   else
-    launch_flux_mhd<RECON, RIEMANN_HLL>(u, mhd, lim, bx, st);
+    launch_flux_mhd<RECON, RIEMANN_HLL>(u, mhd, fp32, lim, bx, st);
 }
 
 void launch_calculate_fluxes(const km_cfg_t &cfg, const km_pack_t &u, const Box &bx,
                              hipStream_t st) {
+  // the per-task path has two arithmetic forms: FP64 (also used for FP64_FMA, whose
+  // contraction only exists in the fused sweeps) and FP32-mixed
+  const bool fp32 = cfg.precision == KM_PREC_FP32_MIXED;
+  const int r = cfg.riemann, m = cfg.mhd, l = cfg.limiter;
   switch (cfg.recon) {
-    case KM_RECON_FOG: launch_flux_riem<RECON_FOG>(u, cfg.riemann, cfg.mhd, cfg.limiter, bx, st); break;
-    case KM_RECON_PLM: launch_flux_riem<RECON_PLM>(u, cfg.riemann, cfg.mhd, cfg.limiter, bx, st); break;
-    case KM_RECON_PPM: launch_flux_riem<RECON_PPM>(u, cfg.riemann, cfg.mhd, cfg.limiter, bx, st); break;
-    default: launch_flux_riem<RECON_WENOZ>(u, cfg.riemann, cfg.mhd, cfg.limiter, bx, st); break;
+    case KM_RECON_FOG: launch_flux_riem<RECON_FOG>(u, r, m, fp32, l, bx, st); break;
+    case KM_RECON_PLM: launch_flux_riem<RECON_PLM>(u, r, m, fp32, l, bx, st); break;
+    case KM_RECON_PPM: launch_flux_riem<RECON_PPM>(u, r, m, fp32, l, bx, st); break;
+    default: launch_flux_riem<RECON_WENOZ>(u, r, m, fp32, l, bx, st); break;
   }
 }
 
@@ -579,19 +617,19 @@ void launch_fluxes_to_dudt(const km_cfg_t &cfg, const km_pack_t &u, const km_pac
 
 void launch_apply_dudt(const km_cfg_t &cfg, const km_pack_t &base, const km_pack_t &u0,
                        const km_pack_t &u1, const km_pack_t &dudt, double beta, double dt,
-                       hipStream_t st) {
+                       hipStream_t st, const double *dt_dev, const int *inst) {
   const Geo g(u0);
   const int nc = cfg.mhd ? 8 : 5;
   hipLaunchKernelGGL(k_apply_cell, dim3(nblk(interior_count(g, u0.nblocks))), dim3(256), 0, st,
-                     base, u0, u1, dudt, nc, beta, dt);
+                     base, u0, u1, dudt, nc, beta, dt, dt_dev, inst);
   if (cfg.mhd && g.ndim >= 2) {
     hipLaunchKernelGGL(k_apply_face<0>, dim3(nblk(face_count(g, 0, u0.nblocks))), dim3(256), 0,
-                       st, base, u0, u1, dudt, beta, dt);
+                       st, base, u0, u1, dudt, beta, dt, dt_dev, inst);
     hipLaunchKernelGGL(k_apply_face<1>, dim3(nblk(face_count(g, 1, u0.nblocks))), dim3(256), 0,
-                       st, base, u0, u1, dudt, beta, dt);
+                       st, base, u0, u1, dudt, beta, dt, dt_dev, inst);
     if (g.ndim > 2)
       hipLaunchKernelGGL(k_apply_face<2>, dim3(nblk(face_count(g, 2, u0.nblocks))), dim3(256),
-                         0, st, base, u0, u1, dudt, beta, dt);
+                         0, st, base, u0, u1, dudt, beta, dt, dt_dev, inst);
   }
 }
 
@@ -616,16 +654,16 @@ void launch_prepare_conserved(const km_cfg_t &cfg, const km_pack_t &u, hipStream
 }
 
 void launch_estimate_dt(const km_cfg_t &cfg, const km_pack_t &u, double *dt_dev,
-                        hipStream_t st) {
+                        hipStream_t st, const int *inst, int ninst) {
   const Geo g(u);
-  hipLaunchKernelGGL(k_fill_value, dim3(1), dim3(64), 0, st, dt_dev, DBL_MAX);
+  hipLaunchKernelGGL(k_fill_value, dim3(nblk(ninst, 64)), dim3(64), 0, st, dt_dev, DBL_MAX, ninst);
   const unsigned grid = nblk(interior_count(g, u.nblocks));
   if (cfg.mhd)
-    hipLaunchKernelGGL(k_estimate_dt<1>, dim3(grid), dim3(256), 0, st, u, dt_dev);
+    hipLaunchKernelGGL(k_estimate_dt<1>, dim3(grid), dim3(256), 0, st, u, dt_dev, inst);
   else
-    hipLaunchKernelGGL(k_estimate_dt<0>, dim3(grid), dim3(256), 0, st, u, dt_dev);
-  hipLaunchKernelGGL(k_scale_value, dim3(1), dim3(64), 0, st, dt_dev,
-                     cfg.cfl / static_cast<double>(g.ndim));
+    hipLaunchKernelGGL(k_estimate_dt<0>, dim3(grid), dim3(256), 0, st, u, dt_dev, inst);
+  hipLaunchKernelGGL(k_scale_value, dim3(nblk(ninst, 64)), dim3(64), 0, st, dt_dev,
+                     cfg.cfl / static_cast<double>(g.ndim), ninst);
 }
 
 void launch_fill_divb(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st) {
@@ -635,15 +673,15 @@ void launch_fill_divb(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st) {
 }
 
 void launch_ghost_fill(const UniformGrid &gr, const km_pack_t &u, int v0, int v1, int nc,
-                       bool faces, hipStream_t st) {
+                       bool faces, hipStream_t st, Instances in) {
   const Geo g(u);
   if (v1 > v0)
     hipLaunchKernelGGL(k_ghost_cells, dim3(nblk(g.ncell() * u.nblocks)), dim3(256), 0, st, u,
-                       gr, v0, v1, nc);
+                       gr, v0, v1, nc, in);
   if (faces)
     for (int f = 0; f < (g.ndim > 2 ? 3 : 2); ++f)
       hipLaunchKernelGGL(k_ghost_faces, dim3(nblk(g.nstag() * u.nblocks)), dim3(256), 0, st, u,
-                         gr, f);
+                         gr, f, in);
 }
 
 }  // namespace km

@@ -70,3 +70,28 @@ def rel_l1(a, b, lay: Layout, nvars):
         den = sum(float(np.sum(np.abs(b[bk, v][sl]))) for bk in range(lay.nblocks))
         out.append(num / den if den > 0 else num)
     return np.array(out)
+
+
+def kh_ensemble(resolutions, nblock=32, ng=3, recon="wenoz", riemann="hllc", integrator="rk3",
+                seed=20261207, precision="fp64"):
+    """Independent 2-D Kelvin-Helmholtz runs of uneven resolution on [0,1]^2 (periodic):
+    shear +-0.5, density ratio 2, P = 2.5, gamma = 1.4, seeded vy perturbation with
+    amplitude in [0.001, 0.05] and mode number 1-4. Returns the ensemble set-up and the
+    per-instance (desc, layout, params)."""
+    cfg = Config(**{"hydro/reconstruction": recon, "hydro/riemann": riemann,
+                    "parthenon/time/integrator": integrator, "material/single/gamma": 1.4,
+                    "kamayan_amd/precision": precision})
+    cfg.c.ndim = 2
+    rng = np.random.default_rng(seed)
+    members = []
+    cells = []
+    for n in resolutions:
+        lay = Layout(2, ng, (nblock, nblock, 1), (n // nblock, n // nblock, 1))
+        desc = mesh_desc(lay)
+        par = [float(rng.uniform(0.001, 0.05)), float(rng.integers(1, 5)), 2.5]
+        cell, _ = km.problem_fill("kelvin_helmholtz", cfg, desc, lay, params=par)
+        members.append((desc, lay, par))
+        cells.append(cell)
+    total = sum(m[1].nblocks for m in members)
+    lay_all = Layout(2, ng, (nblock, nblock, 1), (total, 1, 1))
+    return cfg, members, lay_all, np.concatenate(cells, axis=0)

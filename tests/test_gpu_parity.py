@@ -238,3 +238,28 @@ def test_fp32_mixed_stays_finite_long_run(gpu, oracle):
     assert np.all(np.isfinite(g[sl]))
     l1 = _cases.rel_l1(g, r, lay, 5)
     assert np.all(l1 < 1e-5), l1
+
+
+@pytest.mark.parametrize("path", ["fused", "tasks"])
+def test_ensemble_members_bitwise(gpu, oracle, path):
+    """A batched ensemble of uneven-resolution KH runs: every member equals its own
+    independent oracle run bit for bit, time and dt included."""
+    cfg, members, lay_all, cell = _cases.kh_ensemble([32, 64, 128, 64])
+    descs = [m[0] for m in members]
+    sim = gpu.Simulation(cfg, descs[0], lay_all, descs=descs,
+                         path=gpu.PATH_FUSED if path == "fused" else gpu.PATH_TASKS)
+    sim.set_primitive(cell)
+    sim.advance(4)
+    g, _ = sim.state()
+    times, dts = sim.ensemble_info()
+    off = 0
+    for m, (desc, lay, _) in enumerate(members):
+        ref = oracle.CpuSimulation(cfg.c, desc, lay)
+        ref.set_primitive(cell[off:off + lay.nblocks].copy())
+        ref.advance(4)
+        r, _ = ref.state()
+        t, dt, _ = ref.info()
+        assert times[m] == t and dts[m] == dt, (m, times[m], t, dts[m], dt)
+        sl = (slice(None), slice(0, 5)) + lay.interior()
+        _same(g[off:off + lay.nblocks][sl], r[sl], f"member {m}")
+        off += lay.nblocks
