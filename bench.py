@@ -83,6 +83,165 @@ def _cpu_baseline(km, cell, cycles):
                       f"restatement (OpenMP, {threads} threads), {t1 - t0:.1f} s"}
 
 
+# ---------------------------------------------------------------------------------
+# Secondary workloads (--workload): whole-cycle throughput of the other benchmark
+# configurations. Their roofline figure is the algorithmic stage bytes over the
+# measured cycle time (every kernel of the cycle included).
+# ---------------------------------------------------------------------------------
+def _mhd_setup(km, precision):
+    cfg = km.Config(**{"hydro/reconstruction": "ppm", "hydro/slope_limiter": "mc",
+                       "hydro/riemann": "hll", "parthenon/time/integrator": "rk2",
+                       "material/single/gamma": 5.0 / 3.0, "physics/MHD": "ct",
+                       "kamayan_amd/precision": precision})
+    cfg.c.ndim = 3
+    lay = km.Layout(3, 4, (32, 32, 32), (6, 6, 6))
+    desc = km.mesh_desc(lay)
+    cell, faces = km.problem_fill("mhd_random", cfg, desc, lay, seed=20261205)
+    return cfg, desc, lay, cell, faces
+
+
+def _sedov_setup(km):
+    cfg = km.Config(**{"hydro/reconstruction": "plm", "hydro/slope_limiter": "mc",
+                       "hydro/riemann": "hllc", "parthenon/time/integrator": "rk2",
+                       "material/single/gamma": 1.4})
+    cfg.c.ndim = 3
+    root = km.Layout(3, 4, (16, 16, 16), (4, 4, 4))
+    desc = km.mesh_desc(root, xmin=(-0.5,) * 3, xmax=(0.5,) * 3, bc=("outflow",) * 6)
+    leaves = km.refine_ball(desc, 2, (0.0, 0.0, 0.0), 0.4)
+    lay = km.Layout(3, 4, (16, 16, 16), (4, 4, 4), nleaves=len(leaves))
+    cell, _ = km.problem_fill("sedov", cfg, desc, lay, params=[1.0, 2.0 / 256.0, 1.0, 1e-5],
+                              leaves=leaves)
+    return cfg, desc, lay, leaves, cell
+
+
+def _timed(km, sim, steps, warmup):
+    sim.advance(warmup)
+    km.check(km.lib().km_synchronize(), "sync")
+    t0 = time.perf_counter()
+    sim.advance(steps)
+    km.check(km.lib().km_synchronize(), "sync")
+    return time.perf_counter() - t0
+
+
+def _divb_max(cell, lay, mhd_slot):
+    sl = (slice(None), mhd_slot) + lay.interior()
+    return float(np.abs(cell[sl]).max())
+
+
+def _secondary(km, args):
+    from oracle import km_oracle  # CPU baseline leg only
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    out = {"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "unit": "zone-cycles/s",
+           "metric": "zone-cycles/s on 1 MI355X, FP64 vs FP32-mixed at equal L1 error; % HBM peak"}
+    if args.workload == "mhd3d":
+        cfg, desc, lay, cell, faces = _mhd_setup(km, "fp64")
+        zones = 192 ** 3
+        stage_bytes = 8 * 3 * (5 + 3)
+        nst = 2
+        sim = km.Simulation(cfg, desc, lay, path=km.PATH_TASKS)
+        sim.set_primitive(cell, faces)
+        dt = _timed(km, sim, args.steps, args.warmup)
+        s64, f64 = sim.state()
+        sim.close()
+        cfg32, _, _, _, _ = _mhd_setup(km, "fp32_mixed")
+        sim = km.Simulation(cfg32, desc, lay, path=km.PATH_TASKS)
+        sim.set_primitive(cell, faces)
+        dt32 = _timed(km, sim, args.steps, args.warmup)
+        s32, _ = sim.state()
+        sim.close()
+        out["variants"] = {"fp32_mixed": {"value": zones * args.steps / dt32,
+                                          "speedup_vs_fp64": dt / dt32,
+                                          "rel_l1_vs_fp64_per_var": _rel_l1(s32, s64, lay),
+                                          "max_abs_divb": _divb_max(s32, lay, 15)}}
+        out["max_abs_divb_fp64"] = _divb_max(s64, lay, 15)
+        out["config"] = {"workload": "mhd3d_192c_32b_ppm_mc_hll_rk2_ct", "zones": zones,
+                         "block": [32] * 3, "nghost": 4, "path": "tasks",
+                         "parallelism": "single GPU"}
+        ref = km_oracle.CpuSimulation(cfg.c, desc, lay)
+        ref.set_primitive(cell, faces)
+        sample_cycles = 1
+    elif args.workload == "sedov_amr":
+        cfg, desc, lay, leaves, cell = _sedov_setup(km)
+        zones = len(leaves) * 16 ** 3
+        stage_bytes = 8 * 3 * 5
+        nst = 2
+        sim = km.Simulation(cfg, desc, lay, path=km.PATH_TASKS, leaves=leaves)
+        sim.set_primitive(cell)
+        dt = _timed(km, sim, args.steps, args.warmup)
+        sim.close()
+        lv = np.bincount(leaves[:, 0], minlength=3)
+        out["config"] = {"workload": "sedov3d_amr_root64c_16b_3lev_plm_mc_hllc_rk2",
+                         "zones": zones, "leaves": int(len(leaves)),
+                         "leaves_per_level": [int(x) for x in lv], "block": [16] * 3,
+                         "nghost": 4, "path": "tasks", "parallelism": "single GPU"}
+        ref = km_oracle.CpuSimulation(cfg.c, desc, lay, leaves=leaves)
+        ref.set_primitive(cell)
+        sample_cycles = 1
+    elif args.workload == "kh_ensemble":
+        sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+        rng = np.random.default_rng(20261207)
+        res = [int(x) for x in rng.choice([64, 128, 256, 512], size=256)]
+        cfg = km.Config(**{"hydro/reconstruction": "wenoz", "hydro/riemann": "hllc",
+                           "parthenon/time/integrator": "rk3", "material/single/gamma": 1.4,
+                           "hydro/cfl": 0.8})
+        cfg.c.ndim = 2
+        descs, cells = [], []
+        for n in res:
+            lay_m = km.Layout(2, 4, (32, 32, 1), (n // 32, n // 32, 1))
+            d = km.mesh_desc(lay_m)
+            par = [float(rng.uniform(0.001, 0.05)), float(rng.integers(1, 5)), 2.5]
+            c, _ = km.problem_fill("kelvin_helmholtz", cfg, d, lay_m, params=par)
+            descs.append(d)
+            cells.append(c)
+        total = sum(x.nb[0] * x.nb[1] for x in descs)
+        lay = km.Layout(2, 4, (32, 32, 1), (total, 1, 1))
+        cell = np.concatenate(cells, axis=0)
+        zones = sum(n * n for n in res)
+        stage_bytes = 8 * 3 * 5
+        nst = 3
+        sim = km.Simulation(cfg, descs[0], lay, path=km.PATH_FUSED, descs=descs)
+        sim.set_primitive(cell)
+        dt = _timed(km, sim, args.steps, args.warmup)
+        sim.close()
+        out["config"] = {"workload": "kh2d_ensemble_256runs_64to512_32b_wenoz_hllc_rk3",
+                         "zones": zones, "runs": len(res), "block": [32, 32], "nghost": 4,
+                         "path": "fused", "parallelism": "single GPU"}
+        # CPU sample: the largest member alone
+        big = int(np.argmax(res))
+        lay1 = km.Layout(2, 4, (32, 32, 1), (res[big] // 32, res[big] // 32, 1))
+        ref = km_oracle.CpuSimulation(cfg.c, descs[big], lay1)
+        ref.set_primitive(cells[big])
+        sample_cycles = 20
+        cpu_zones = res[big] ** 2
+    else:
+        raise SystemExit(f"unknown workload {args.workload}")
+    value = zones * args.steps / dt
+    achieved = stage_bytes * nst * value / 1e9
+    out.update({"value": value, "ms_per_step": dt * 1e3 / args.steps,
+                "data": "synthetic (in-repo seeded generator)",
+                "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                             "kernel": "whole cycle (all kernels)",
+                             "algorithmic_bytes_per_zone_cycle": stage_bytes * nst}})
+    if not args.no_cpu:
+        if args.workload != "kh_ensemble":
+            cpu_zones = zones
+        t0 = time.perf_counter()
+        ref.advance(sample_cycles)
+        t1 = time.perf_counter()
+        ref.close()
+        out["cpu_baseline"] = {"value": cpu_zones * sample_cycles / (t1 - t0),
+                               "unit": "zone-cycles/s", "cores": threads, "kind": "port",
+                               "sample": f"{sample_cycles} cycle(s), {cpu_zones} zones, "
+                                         f"{t1 - t0:.1f} s on the FP64 CPU restatement"}
+        out["gpu_vs_cpu"] = value / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +250,11 @@ def main() -> int:
     ap.add_argument("--cpu-cycles", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--workload", default="hydro3d",
+                    choices=["hydro3d", "mhd3d", "sedov_amr", "kh_ensemble"])
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="after the run, copy BYTES between two device buffers with a plain "
+                         "copy kernel (the reference for scaling HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the "
@@ -101,6 +265,8 @@ def main() -> int:
     if km.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         return 1
+    if args.workload != "hydro3d":
+        return _secondary(km, args)
 
     _, _, lay, cell = _setup(km)
     zones = N ** 3
@@ -148,7 +314,28 @@ def main() -> int:
         out["cpu_baseline"] = cpu
         out["gpu_vs_cpu"] = value / cpu["value"]
     print(json.dumps(out), flush=True)
+    if args.calibrate_copy:
+        _copy_kernel(args.calibrate_copy)
     return 0
+
+
+def _copy_kernel(nbytes: int) -> None:
+    """One device-to-device copy of nbytes by the library's plain copy kernel (k_copy16)."""
+    import ctypes as C
+
+    import kamayan_amd as km
+
+    nbytes -= nbytes % 16
+    a, b = C.c_void_p(), C.c_void_p()
+    km.check(km.lib().km_malloc(C.byref(a), nbytes), "malloc")
+    km.check(km.lib().km_malloc(C.byref(b), nbytes), "malloc")
+    km.check(km.lib().km_synchronize(), "sync")
+    km.check(km.lib().km_copy_device(b, a, nbytes, None), "copy")
+    km.check(km.lib().km_synchronize(), "sync")
+    km.lib().km_free(a)
+    km.lib().km_free(b)
+    print(json.dumps({"copy_kernel": "k_copy16", "bytes_read": nbytes,
+                      "bytes_written": nbytes}), flush=True)
 
 
 if __name__ == "__main__":

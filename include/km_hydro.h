@@ -282,6 +282,9 @@ int km_malloc(void **ptr, size_t bytes);
 int km_free(void *ptr);
 int km_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int km_memcpy_d2h(void *dst, const void *src, size_t bytes);
+/* Device-to-device copy by a plain streaming kernel (16-byte aligned, multiple of 16 B);
+ * also the reference kernel for calibrating HBM byte counters. */
+int km_copy_device(void *dst, const void *src, size_t bytes, void *stream);
 int km_synchronize(void);
 
 const char *km_last_error(void);

@@ -73,13 +73,21 @@ def lib() -> C.CDLL:
         "km_sim_kernel_times": [vp, P(dbl), P(i32), i32],
         "km_sim_set_tile": [vp, i32],
         "km_problem_fill": [C.c_char_p, P(KmCfg), P(KmMeshDesc), u64, P(dbl), i32, vp, P(vp)],
+        "km_problem_fill_blocks": [C.c_char_p, P(KmCfg), P(KmMeshDesc), vp, i32, u64, P(dbl),
+                                   i32, vp, P(vp)],
         "km_mesh_blocks": [P(KmMeshDesc), vp],
+        "km_amr_blocks": [P(KmMeshDesc), P(i32), i32, vp],
+        "km_amr_refine_ball": [P(KmMeshDesc), i32, P(dbl), dbl, P(i32), P(i32)],
+        "km_ghost_plan_amr": [P(KmPack), P(KmMeshDesc), P(i32), i32, P(vp)],
+        "km_flux_correct": [vp, P(KmPack), vp],
+        "km_sim_create_amr": [P(KmCfg), P(KmMeshDesc), P(i32), i32, i32, P(vp)],
         "km_device_count": [P(i32)],
         "km_malloc": [P(vp), C.c_size_t],
         "km_free": [vp],
         "km_memcpy_h2d": [vp, vp, C.c_size_t],
         "km_memcpy_d2h": [vp, vp, C.c_size_t],
         "km_synchronize": [],
+        "km_copy_device": [vp, vp, C.c_size_t, vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -131,9 +139,36 @@ def block_table(layout: Layout, xmin, xmax) -> np.ndarray:
     return arr
 
 
+def _i32(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def refine_ball(desc: KmMeshDesc, lmax: int, center, radius: float) -> np.ndarray:
+    """Leaves ([n, 4] int32: level, lx1, lx2, lx3) of the root grid refined to lmax
+    where blocks meet the ball, 2:1 balanced."""
+    c = np.ascontiguousarray(center, dtype=np.float64)
+    n = C.c_int32(0)
+    lib().km_amr_refine_ball(C.byref(desc), lmax, c.ctypes.data_as(C.POINTER(C.c_double)),
+                             radius, None, C.byref(n))
+    out = np.zeros((n.value, 4), dtype=np.int32)
+    n2 = C.c_int32(n.value)
+    check(lib().km_amr_refine_ball(C.byref(desc), lmax, c.ctypes.data_as(C.POINTER(C.c_double)),
+                                   radius, _i32(out), C.byref(n2)), "km_amr_refine_ball")
+    return out
+
+
+def amr_blocks(desc: KmMeshDesc, leaves: np.ndarray):
+    leaves = np.ascontiguousarray(leaves, dtype=np.int32)
+    arr = (KmBlock * len(leaves))()
+    check(lib().km_amr_blocks(C.byref(desc), _i32(leaves), len(leaves), C.cast(arr, C.c_void_p)),
+          "km_amr_blocks")
+    return arr
+
+
 def problem_fill(name: str, cfg: Config, desc: KmMeshDesc, layout: Layout, seed: int = 0,
-                 params=None):
-    """Primitive initial state (cell array, face arrays or None) of a problem generator."""
+                 params=None, leaves=None):
+    """Primitive initial state (cell array, face arrays or None) of a problem generator;
+    with `leaves`, over the leaf blocks of a block tree."""
     mhd = cfg.c.mhd
     cell = np.zeros(layout.cell_shape(nvar(mhd)))
     faces = None
@@ -148,8 +183,14 @@ def problem_fill(name: str, cfg: Config, desc: KmMeshDesc, layout: Layout, seed:
         params = np.ascontiguousarray(params, dtype=np.float64)
         par = params.ctypes.data_as(C.POINTER(C.c_double))
         npar = len(params)
-    check(lib().km_problem_fill(name.encode(), C.byref(cfg.c), C.byref(desc), seed, par, npar,
-                                f64ptr(cell), face_ptrs), f"problem {name}")
+    if leaves is not None:
+        blocks = amr_blocks(desc, leaves)
+        check(lib().km_problem_fill_blocks(name.encode(), C.byref(cfg.c), C.byref(desc),
+                                           C.cast(blocks, C.c_void_p), len(leaves), seed, par,
+                                           npar, f64ptr(cell), face_ptrs), f"problem {name}")
+    else:
+        check(lib().km_problem_fill(name.encode(), C.byref(cfg.c), C.byref(desc), seed, par,
+                                    npar, f64ptr(cell), face_ptrs), f"problem {name}")
     return cell, faces
 
 
@@ -160,11 +201,16 @@ class Simulation:
     the total block count."""
 
     def __init__(self, cfg: Config, desc: KmMeshDesc, layout: Layout, path: int = PATH_TASKS,
-                 descs=None):
+                 descs=None, leaves=None):
         self.cfg, self.desc, self.layout = cfg, desc, layout
         self.h = C.c_void_p()
         self.ninst = 1
-        if descs is not None:
+        if leaves is not None:
+            self._leaves = np.ascontiguousarray(leaves, dtype=np.int32)
+            check(lib().km_sim_create_amr(C.byref(cfg.c), C.byref(desc), _i32(self._leaves),
+                                          len(self._leaves), path, C.byref(self.h)),
+                  "km_sim_create_amr")
+        elif descs is not None:
             arr = (KmMeshDesc * len(descs))(*descs)
             self.ninst = len(descs)
             check(lib().km_sim_create_ensemble(C.byref(cfg.c), arr, len(descs), path,

@@ -78,16 +78,18 @@ class KmMeshDesc(C.Structure):
 
 @dataclass
 class Layout:
-    """Shapes of the block-pack arrays of a uniform mesh."""
+    """Shapes of the block-pack arrays of a mesh: nb is the (root) block grid; a block-tree
+    mesh sets nleaves to its leaf count."""
 
     ndim: int
     ng: int
     nx: tuple
     nb: tuple
+    nleaves: int = 0
 
     @property
     def nblocks(self) -> int:
-        return self.nb[0] * self.nb[1] * self.nb[2]
+        return self.nleaves if self.nleaves else self.nb[0] * self.nb[1] * self.nb[2]
 
     @property
     def E(self) -> tuple:

@@ -56,9 +56,13 @@ KM_HD int amr_leaf(const AmrGrid &t, const Geo &g, int L, const int G[3]) {
   return t.lookup[((long)f[2] * t.nbf[1] + f[1]) * t.nbf[0] + f[0]];
 }
 
-// local (ghost-inclusive) index in leaf `blk` of its own-level global zone G along d
+// local (ghost-inclusive) index in leaf `blk` of its own-level global zone G along d,
+// clamped to the interior (the mesh checks make the clamp a no-op; it keeps every read
+// inside the block whatever the leaves)
 KM_HD int amr_local(const km_block_t &blk, const Geo &g, int d, int G) {
-  return d < g.ndim ? G - blk.lx[d] * g.nx[d] + g.lo[d] : 0;
+  if (d >= g.ndim) return 0;
+  const int i = G - blk.lx[d] * g.nx[d] + g.lo[d];
+  return i < g.lo[d] ? g.lo[d] : (i > g.hi[d] ? g.hi[d] : i);
 }
 
 // host-side tree: leaf metrics in the caller's leaf order and the finest-level lookup

@@ -18,9 +18,9 @@ struct Box {
 
 // block grid of a single-level mesh, blocks stored x-fastest starting at block `first`
 struct UniformGrid {
-  int nb[3];
-  int bc[6];
-  int first;
+  int nb[3] = {1, 1, 1};
+  int bc[6] = {0, 0, 0, 0, 0, 0};
+  int first = 0;
 };
 
 // Independent instances sharing one block pack (an ensemble): instance of each block and

@@ -140,6 +140,12 @@ __global__ void k_dt_update(double *dt, double *time, const double *est, int kee
   }
 }
 static inline unsigned nb64(int n) { return (unsigned)((n + 63) / 64); }
+// plain device copy, 16 B per lane per iteration, grid-stride
+__global__ void __launch_bounds__(256) k_copy16(double2 *dst, const double2 *src, long n) {
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n;
+       t += (long)gridDim.x * blockDim.x)
+    dst[t] = src[t];
+}
 
 }  // namespace km
 
@@ -148,7 +154,7 @@ using namespace km;
 struct km_ghost_plan_t {
   UniformGrid grid;
   bool amr = false;  // block-tree plan: agrid over a device lookup table
-  AmrGrid agrid;
+  AmrGrid agrid{};
   int *lookup_d = nullptr;
   ~km_ghost_plan_t() {
     if (lookup_d) (void)hipFree(lookup_d);
@@ -189,7 +195,7 @@ struct km_sim_t {
   int *nbr_d = nullptr;  // [nb][3][2] face neighbours for the fused sweeps
   UniformGrid grid;      // grid of instance 0
   bool amr = false;      // block-tree mesh (task path): ghost fill across levels and
-  AmrGrid agrid;         // flux correction use agrid
+  AmrGrid agrid{};       // flux correction use agrid
   int ninst = 1;         // independent instances sharing the pack (ensemble)
   std::vector<km_mesh_desc_t> meshes;
   std::vector<UniformGrid> grids_h;
@@ -470,6 +476,20 @@ int km_memcpy_d2h(void *dst, const void *src, size_t bytes) {
   KM_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
   return KM_OK;
 }
+int km_copy_device(void *dst, const void *src, size_t bytes, void *stream) {
+  if (!dst || !src || bytes % 16 != 0 || (reinterpret_cast<uintptr_t>(dst) % 16) ||
+      (reinterpret_cast<uintptr_t>(src) % 16)) {
+    set_error("km_copy_device needs 16-byte aligned pointers and a multiple of 16 bytes");
+    return KM_EINVAL;
+  }
+  const long n = (long)(bytes / 16);
+  if (n == 0) return KM_OK;
+  const long want = (n + 255) / 256;
+  const unsigned grid = (unsigned)(want < 8192 ? want : 8192);
+  hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, as_stream(stream),
+                     static_cast<double2 *>(dst), static_cast<const double2 *>(src), n);
+  return check_launch();
+}
 int km_synchronize(void) {
   KM_HIP(hipDeviceSynchronize());
   return KM_OK;
@@ -565,9 +585,10 @@ int km_ghost_plan_uniform(const km_pack_t *shape, const int32_t nb[3], const int
       return KM_EINVAL;
     }
   }
-  auto *p = new km_ghost_plan_t;
+  auto *p = new km_ghost_plan_t();
   for (int d = 0; d < 3; ++d) p->grid.nb[d] = nb[d];
   for (int d = 0; d < 6; ++d) p->grid.bc[d] = bc[d];
+  p->grid.first = 0;  // the pack holds this grid's blocks from block 0
   *plan = p;
   return KM_OK;
 }

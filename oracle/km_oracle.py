@@ -59,6 +59,10 @@ def lib() -> C.CDLL:
         "km_cpu_sim_advance": [vp, i32, dbl],
         "km_cpu_sim_info": [vp, P(dbl), P(dbl), P(i32)],
         "km_cpu_sim_set_dt": [vp, dbl],
+        "km_cpu_amr_blocks": [P(KmMeshDesc), P(i32), i32, vp],
+        "km_cpu_amr_ghost_fill": [P(KmCfg), P(KmPack), P(KmMeshDesc), P(i32), i32, i32, i32],
+        "km_cpu_flux_correct": [P(KmCfg), P(KmPack), P(KmMeshDesc), P(i32), i32],
+        "km_cpu_sim_create_amr": [P(KmCfg), P(KmMeshDesc), P(i32), i32, P(vp)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -125,6 +129,17 @@ def block_table(desc: KmMeshDesc, nblocks: int):
     return arr
 
 
+def _i32(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def amr_blocks(desc: KmMeshDesc, leaves: np.ndarray):
+    leaves = np.ascontiguousarray(leaves, dtype=np.int32)
+    arr = (KmBlock * len(leaves))()
+    call("km_cpu_amr_blocks", C.byref(desc), _i32(leaves), len(leaves), C.cast(arr, C.c_void_p))
+    return arr
+
+
 class HostPack:
     """numpy block-pack arrays and host pointer tables for a km_pack_t."""
 
@@ -173,10 +188,15 @@ def call(name: str, *args):
 class CpuSimulation:
     """km_cpu_sim_*: the oracle's whole-mesh RK driver."""
 
-    def __init__(self, cfg: KmCfg, desc: KmMeshDesc, layout: Layout):
+    def __init__(self, cfg: KmCfg, desc: KmMeshDesc, layout: Layout, leaves=None):
         self.cfg, self.desc, self.layout = cfg, desc, layout
         self.h = C.c_void_p()
-        call("km_cpu_sim_create", C.byref(cfg), C.byref(desc), C.byref(self.h))
+        if leaves is not None:
+            self._leaves = np.ascontiguousarray(leaves, dtype=np.int32)
+            call("km_cpu_sim_create_amr", C.byref(cfg), C.byref(desc), _i32(self._leaves),
+                 len(self._leaves), C.byref(self.h))
+        else:
+            call("km_cpu_sim_create", C.byref(cfg), C.byref(desc), C.byref(self.h))
 
     def set_primitive(self, cell: np.ndarray, faces=None):
         fp = (C.c_void_p * 3)()

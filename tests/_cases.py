@@ -95,3 +95,31 @@ def kh_ensemble(resolutions, nblock=32, ng=3, recon="wenoz", riemann="hllc", int
     total = sum(m[1].nblocks for m in members)
     lay_all = Layout(2, ng, (nblock, nblock, 1), (total, 1, 1))
     return cfg, members, lay_all, np.concatenate(cells, axis=0)
+
+
+def amr_case(problem="smooth3d", ndim=3, nroot=2, nblock=8, ng=2, lmax=1, center=(0.5,) * 3,
+             radius=0.3, bc="periodic", recon="plm", limiter="mc", riemann="hllc",
+             integrator="rk2", seed=20261206, params=None, xmin=(0.0,) * 3, xmax=(1.0,) * 3,
+             gamma=5.0 / 3.0):
+    """A block-tree mesh: a root grid of nroot^ndim blocks of nblock cells refined to lmax
+    where blocks meet a ball, 2:1 balanced; the problem filled on every leaf at its own
+    resolution. Returns (cfg, desc, layout, leaves, cell)."""
+    cfg = Config(**{"hydro/reconstruction": recon, "hydro/slope_limiter": limiter,
+                    "hydro/riemann": riemann, "parthenon/time/integrator": integrator,
+                    "material/single/gamma": gamma})
+    cfg.c.ndim = ndim
+    nxb = tuple(nblock if d < ndim else 1 for d in range(3))
+    nbs = tuple(nroot if d < ndim else 1 for d in range(3))
+    root = Layout(ndim, ng, nxb, nbs)
+    desc = mesh_desc(root, xmin=xmin, xmax=xmax, bc=(bc,) * 6)
+    leaves = km.refine_ball(desc, lmax, center, radius)
+    lay = Layout(ndim, ng, nxb, nbs, nleaves=len(leaves))
+    cell, _ = km.problem_fill(problem, cfg, desc, lay, seed=seed, params=params, leaves=leaves)
+    return cfg, desc, lay, leaves, cell
+
+
+def amr_totals(cell, lay: Layout, blocks, nvars=5):
+    """sum over leaves of V * u per variable (interior zones)."""
+    sl = lay.interior()
+    return np.array([sum(float(np.sum(cell[b, v][sl])) * blocks[b].vol
+                         for b in range(lay.nblocks)) for v in range(nvars)])
