@@ -25,6 +25,10 @@ N, NB, NG = 256, 64, 4
 SEED = 20261204
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_ZONE_STAGE = 8 * 3 * 5  # stage input + RK base read, stage output written
+# HBM bytes per RK stage of this workload measured with rocprofv3 FETCH_SIZE / WRITE_SIZE,
+# calibrated against the library's 1 GiB copy kernel (bench.py --calibrate-copy); the
+# summary is profiles/fused_stage_traffic_hydro3d_256c_wenoz_hllc_rk3.md
+MEASURED_TRAFFIC_PER_STAGE = 1.154e10
 
 
 def _setup(km, precision="fp64", n=N, nb=NB):
@@ -295,7 +299,7 @@ def main() -> int:
                    "cfl": 0.8, "gamma": 5.0 / 3.0, "seed": SEED, "path": "fused",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC_PER_STAGE,
                      "kernel": "one RK stage: sweep_x1 + sweep_x2 + sweep_x3_update + ghost_fill",
                      "algorithmic_bytes_per_launch": BYTES_PER_ZONE_STAGE * zones,
                      "avg_launch_ms": stage_ms},

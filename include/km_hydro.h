@@ -192,20 +192,24 @@ int km_ghost_plan_destroy(km_ghost_plan_t *plan);
  * mesh->nb with the domain and boundaries of `mesh`. Leaves must tile the domain; leaves
  * that touch (faces, edges, corners, across periodic boundaries) differ by at most one
  * level; with more than one level every active block dimension holds an even number of
- * at least 4*((ng+1)/2 + 1) cells, which keeps each prolongation stencil inside blocks
- * adjacent to the block being filled. */
+ * at least max(2 ng, 2 ((ng+1)/2 + 1)) cells (8 for nghost 4), which keeps ghost zones
+ * and prolongation stencils inside blocks adjacent to the block being filled. */
 int km_ghost_plan_amr(const km_pack_t *shape, const km_mesh_desc_t *mesh,
                       const int32_t *leaves, int32_t nleaves, km_ghost_plan_t **plan);
 /* Fill every ghost zone (faces, edges, corners) of cell vars [v0, v1) and, when
  * with_faces != 0, of the CT face arrays. With a block-tree plan a ghost zone is copied
  * from a same-level neighbour, restricted from a finer one (RestrictAverage,
  * refinement_operations.hpp:52-121) or prolongated from a coarser one
- * (ProlongateSharedMinMod, :123-268); face arrays across levels are not supported. */
+ * (ProlongateSharedMinMod, :123-268). CT face arrays across levels (even nghost) take
+ * RestrictAverage / ProlongateSharedMinMod on faces and ProlongateInternalTothAndRoe
+ * (:270-398) for fine faces inside coarse cells; a block's own faces are never written. */
 int km_ghost_fill(const km_ghost_plan_t *plan, const km_pack_t *u, int32_t v0, int32_t v1,
                   int32_t with_faces, void *stream);
 /* Flux correction (Parthenon AddFluxCorrectionTasks, kamayan_driver.cpp:198-207): every
  * block face shared with finer leaves takes the area-weighted restriction of the fine
- * fluxes, for the u->nflux conserved variables. No-op for a uniform plan. */
+ * fluxes, for the u->nflux conserved variables, and, when u->edge tables are present
+ * (CT), each edge EMF lying in such a face takes the length-weighted restriction of the
+ * fine edges on it. No-op for a uniform plan. */
 int km_flux_correct(const km_ghost_plan_t *plan, const km_pack_t *u, void *stream);
 
 /* ---- whole-mesh driver (KamayanDriver RK stages, kamayan_driver.cpp:121-248) ----- */
@@ -225,8 +229,9 @@ int km_sim_create(const km_cfg_t *cfg, const km_mesh_desc_t *mesh, int32_t path,
 int km_sim_create_ensemble(const km_cfg_t *cfg, const km_mesh_desc_t *meshes, int32_t ninst,
                            int32_t path, km_sim_t **sim);
 /* A block-tree (AMR) mesh with the leaves described at km_ghost_plan_amr, on the task
- * path (hydro): each stage adds flux correction after the fluxes and fills ghost zones
- * across levels. Host buffers hold the blocks in leaf order. */
+ * path (hydro or CT MHD; CT needs an even nghost): each stage adds flux (and EMF)
+ * correction after the fluxes and fills ghost zones across levels. Host buffers hold the
+ * blocks in leaf order. */
 int km_sim_create_amr(const km_cfg_t *cfg, const km_mesh_desc_t *mesh, const int32_t *leaves,
                       int32_t nleaves, int32_t path, km_sim_t **sim);
 /* time and dt of the first n instances (blocks the host) */
@@ -254,7 +259,8 @@ int km_sim_set_tile(km_sim_t *sim, int32_t threads);
 
 /* ---- problem generators (host-side; fill primitives on the interior) -------------- */
 /* names: "isentropic_vortex" (isentropic_vortex.cpp:105-175), "smooth3d", "mhd_random",
- * "kelvin_helmholtz", "sedov" (sedov.cpp:75-143, 3-D form). Buffers as for
+ * "kelvin_helmholtz", "sedov" (sedov.cpp:75-143, 3-D form), "mhd_blast"
+ * (mhd_blast.cpp:47-160, Cartesian). Buffers as for
  * km_sim_set_primitive; `seed` feeds the random modes where a problem has them. */
 int km_problem_fill(const char *name, const km_cfg_t *cfg, const km_mesh_desc_t *mesh,
                     uint64_t seed, const double *params, int32_t nparams, double *cell,

@@ -286,6 +286,307 @@ __global__ void __launch_bounds__(256) k_amr_flux_correct(km_pack_t u, AmrGrid t
   }
 }
 
+// ---------------------------------------------------------------------------------
+// Face-centred B (CT) across levels. A ghost face of a level-L block takes its value from
+// the leaf holding the cell beside it on the block's side (the cell above a face below
+// the block, the cell below a face above it): a same-level copy, RestrictAverage of the
+// finer leaf's faces over the two transverse axes (refinement_operations.hpp:52-121 on
+// face elements), or, under a coarser leaf, ProlongateSharedMinMod (:123-268) for the
+// fine faces lying on coarse faces and ProlongateInternalTothAndRoe (:270-398) for the
+// fine faces inside coarse cells. Shared faces are filled first (all components), then
+// the internal ones from them and from the block's own faces. A block's own faces are
+// never written.
+// ---------------------------------------------------------------------------------
+namespace {
+
+struct Face {
+  int kind;  // 0 copy, 1 restrict, 2 coarser leaf
+  int leaf;
+  long idx;  // staggered index (kind 1: lower corner of the fine faces)
+};
+
+// ghost face of block `blk` at local staggered index idx along face dim d: mapped level-L
+// global indices G (face index along d, cell indices across) and flips
+__device__ __forceinline__ void face_global(const AmrGrid &t, const Geo &g, const km_block_t &blk,
+                                            int d, const int idx[3], int G[3], bool fl[3]) {
+  const int L = blk.level;
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    fl[e] = false;
+    if (e >= g.ndim) {
+      G[e] = 0;
+      continue;
+    }
+    const int G0 = blk.lx[e] * g.nx[e] + (idx[e] - g.lo[e]);
+    if (e == d)
+      G[e] = amr_map_face(t, e, G0, amr_zones(t, g, e, L), fl[e]);
+    else
+      G[e] = amr_map(t, e, G0, amr_zones(t, g, e, L), fl[e]);
+  }
+}
+
+// The leaf holding face G (level L) seen from side `up` along d (up: the cell above the
+// face, else the cell below), and where the face sits in it.
+__device__ __forceinline__ Face locate_face(const km_pack_t &u, const AmrGrid &t, const Geo &g,
+                                            int L, int d, const int G[3], bool up) {
+  Face f;
+  int Gc[3] = {G[0], G[1], G[2]};
+  if (d < g.ndim) {
+    const int N = amr_zones(t, g, d, L);
+    int c = up ? G[d] : G[d] - 1;
+    c = c < 0 ? 0 : (c > N - 1 ? N - 1 : c);
+    Gc[d] = c;
+  }
+  f.leaf = amr_leaf(t, g, L, Gc);
+  const km_block_t &nb = u.blocks[f.leaf];
+  int li[3];
+  if (nb.level == L) {
+    f.kind = 0;
+#pragma unroll
+    for (int e = 0; e < 3; ++e)
+      li[e] = e == d ? amr_local_face(nb, g, e, G[e]) : amr_local(nb, g, e, G[e]);
+  } else if (nb.level > L) {
+    f.kind = 1;
+#pragma unroll
+    for (int e = 0; e < 3; ++e)
+      li[e] = e == d ? amr_local_face(nb, g, e, 2 * G[e]) : amr_local(nb, g, e, 2 * G[e]);
+  } else {
+    f.kind = 2;
+#pragma unroll
+    for (int e = 0; e < 3; ++e)
+      li[e] = e == d ? amr_local_face(nb, g, e, G[e] >> 1) : amr_local(nb, g, e, G[e] >> 1);
+  }
+  f.idx = g.s(li[2], li[1], li[0]);
+  return f;
+}
+
+// RestrictAverage of the fine faces of component d behind a located face (kind 1): the
+// transverse active axes are averaged with the fine face areas as weights
+__device__ __forceinline__ double face_value(const km_pack_t &u, const Geo &g, int d,
+                                             const Face &f) {
+  const double *q = u.face[d][f.leaf];
+  if (f.kind != 1) return q[f.idx];
+  const bool inc[3] = {d != 0, d != 1 && g.ndim > 1, d != 2 && g.ndim > 2};
+  const double A = u.blocks[f.leaf].area[d];
+  double V[2][2][2], Tm[2][2][2];
+#pragma unroll
+  for (int ok = 0; ok < 2; ++ok)
+#pragma unroll
+    for (int oj = 0; oj < 2; ++oj)
+#pragma unroll
+      for (int oi = 0; oi < 2; ++oi) {
+        V[ok][oj][oi] = 0.;
+        Tm[ok][oj][oi] = 0.;
+        if ((ok == 0 || inc[2]) && (oj == 0 || inc[1]) && (oi == 0 || inc[0])) {
+          V[ok][oj][oi] = A;
+          Tm[ok][oj][oi] = A * q[f.idx + ok * g.ss[2] + oj * g.ss[1] + oi];
+        }
+      }
+  const double tvol = ((V[0][0][0] + V[0][1][0]) + (V[0][0][1] + V[0][1][1])) +
+                      ((V[1][0][0] + V[1][1][0]) + (V[1][0][1] + V[1][1][1]));
+  return tvol > 0.0 ? (((Tm[0][0][0] + Tm[0][1][0]) + (Tm[0][0][1] + Tm[0][1][1])) +
+                       ((Tm[1][0][0] + Tm[1][1][0]) + (Tm[1][0][1] + Tm[1][1][1]))) /
+                          tvol
+                    : 0.0;
+}
+
+// a ghost face element of face dim d (false for own faces and for padding)
+__device__ __forceinline__ bool ghost_face(const Geo &g, int d, const int idx[3]) {
+  bool ghost = false;
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    const int ext = e < g.ndim ? g.E[e] + (e == d ? 1 : 0) : 1;
+    if (idx[e] >= ext) return false;
+    const int hi = g.hi[e] + (e == d ? 1 : 0);
+    if (e < g.ndim && (idx[e] < g.lo[e] || idx[e] > hi)) ghost = true;
+  }
+  return ghost;
+}
+
+// which side of a ghost face faces the block: the cell above for faces below the block
+__device__ __forceinline__ bool face_up(const Geo &g, int d, const int idx[3]) {
+  return d >= g.ndim || idx[d] <= g.hi[d];
+}
+
+__device__ __forceinline__ bool decode_stag(const Geo &g, int nblocks, int &b, int idx[3]) {
+  const long n = g.nstag();
+  const long tt = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (tt >= n * nblocks) return false;
+  b = (int)(tt / n);
+  long r = tt % n;
+  idx[0] = (int)(r % g.S[0]);
+  r /= g.S[0];
+  idx[1] = (int)(r % g.S[1]);
+  idx[2] = (int)(r / g.S[1]);
+  return true;
+}
+
+}  // namespace
+
+// phase 1: copies, restrictions and shared-face prolongation
+__global__ void __launch_bounds__(256) k_amr_faces_shared(km_pack_t u, AmrGrid t, int d) {
+  const Geo g(u);
+  int b, idx[3];
+  if (!decode_stag(g, u.nblocks, b, idx)) return;
+  if (!ghost_face(g, d, idx)) return;
+  const km_block_t &blk = u.blocks[b];
+  const int L = blk.level;
+  int G[3];
+  bool fl[3];
+  face_global(t, g, blk, d, idx, G, fl);
+  const bool up = face_up(g, d, idx);
+  const Face f = locate_face(u, t, g, L, d, G, up);
+  const long dst = g.s(idx[2], idx[1], idx[0]);
+  if (f.kind != 2) {
+    double x = face_value(u, g, d, f);
+    if (d < g.ndim && fl[d]) x = -x;
+    u.face[d][b][dst] = x;
+    return;
+  }
+  if (d < g.ndim && (G[d] & 1)) return;  // inside a coarse cell: phase 2
+  // shared fine face on coarse face (G[d]/2, C): transverse minmod slopes at level L-1
+  const double fc = u.face[d][f.leaf][f.idx];
+  double a[3] = {0., 0., 0.};
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    if (e == d || e >= g.ndim) continue;
+    double fv[2];
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      int C[3], M[3];
+      bool nf[3] = {false, false, false};
+#pragma unroll
+      for (int q = 0; q < 3; ++q) C[q] = q < g.ndim ? G[q] >> 1 : 0;
+      C[e] += side ? 1 : -1;
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+        M[q] = (q == e) ? amr_map(t, q, C[q], amr_zones(t, g, q, L - 1), nf[q]) : C[q];
+      const Face s = locate_face(u, t, g, L - 1, d, M, up);
+      fv[side] = face_value(u, g, d, s);
+    }
+    const double dxc = 2.0 * blk.dx[e];
+    a[e] = grad_minmod(fc, fv[0], fv[1], dxc, dxc) * (0.5 * blk.dx[e]);
+  }
+  const int oi = d != 0 ? (G[0] & 1) : 0;
+  const int oj = (d != 1 && g.ndim > 1) ? (G[1] & 1) : 0;
+  const int ok = (d != 2 && g.ndim > 2) ? (G[2] & 1) : 0;
+  double x;
+  if (!ok && !oj && !oi) x = fc - (a[0] + a[1] + a[2]);
+  else if (!ok && !oj) x = fc + (a[0] - a[1] - a[2]);
+  else if (!ok && !oi) x = fc - (a[0] - a[1] + a[2]);
+  else if (!ok) x = fc + (a[0] + a[1] - a[2]);
+  else if (!oj && !oi) x = fc - (a[0] + a[1] - a[2]);
+  else if (!oj) x = fc + (a[0] - a[1] + a[2]);
+  else if (!oi) x = fc - (a[0] - a[1] - a[2]);
+  else x = fc + (a[0] + a[1] + a[2]);
+  if (d < g.ndim && fl[d]) x = -x;
+  u.face[d][b][dst] = x;
+}
+
+// phase 2: ProlongateInternalTothAndRoe (refinement_operations.hpp:278-398), Cartesian, for
+// the fine faces of component d inside a coarse cell under a coarser leaf. The fine
+// element (kk, jj, ii) of the reference's loop is reached through its permuted offsets;
+// the coarse cell spacings enter unpermuted, as the reference writes them.
+__global__ void __launch_bounds__(256) k_amr_faces_internal(km_pack_t u, AmrGrid t, int d) {
+  const Geo g(u);
+  int b, idx[3];
+  if (!decode_stag(g, u.nblocks, b, idx)) return;
+  if (!ghost_face(g, d, idx)) return;
+  if (d < g.ndim && !((idx[d] - g.lo[d]) & 1)) return;  // shared position
+  const km_block_t &blk = u.blocks[b];
+  int G[3];
+  bool fl[3];
+  face_global(t, g, blk, d, idx, G, fl);
+  const Face f = locate_face(u, t, g, blk.level, d, G, face_up(g, d, idx));
+  if (f.kind != 2) return;
+  const int g2 = g.ndim > 1, g3 = g.ndim > 2;
+  // fine lower corner of the coarse cell
+  int fc[3];
+#pragma unroll
+  for (int e = 0; e < 3; ++e) fc[e] = e < g.ndim ? g.lo[e] + ((idx[e] - g.lo[e]) & ~1) : 0;
+  const int fi = fc[0], fj = fc[1], fk = fc[2];
+  // permuted element index of (ok, oj, oi) for component d
+  auto kji = [&](int ok, int oj, int oi, int out[3]) {
+    if (d == 0) {
+      out[0] = fk + ok * g3, out[1] = fj + oj * g2, out[2] = fi + oi;
+    } else if (d == 1) {
+      out[0] = fk + oj * g3, out[1] = fj + oi * g2, out[2] = fi + ok;
+    } else {
+      out[0] = fk + oi * g3, out[1] = fj + ok * g2, out[2] = fi + oj;
+    }
+  };
+  auto fine = [&](int eidx, int ok, int oj, int oi) {
+    int p[3];
+    kji(ok, oj, oi, p);
+    return u.face[(d + eidx) % 3][b][g.s(p[0], p[1], p[2])];
+  };
+  // this element's (ok, oj) in the reference's write loop
+  int ok = 0, oj = 0;
+  {
+    const int o[3] = {idx[0] - fi, idx[1] - fj, idx[2] - fk};
+    if (d == 0) {
+      ok = o[2], oj = o[1];
+    } else if (d == 1) {
+      oj = o[2], ok = o[0];
+    } else {
+      ok = o[1], oj = o[0];
+    }
+  }
+  auto sg = [](int off) { return off == 0 ? -1.0 : 1.0; };
+  double Uxx = 0.0, Vxyz = 0.0, Wxyz = 0.0;
+  for (int v = 0; v < 2; ++v)
+    for (int uu = 0; uu <= 2; uu += 2)
+      for (int tt = 0; tt < 2; ++tt) {
+        const double fine2 = fine(1, v, uu, tt) * 1.0;
+        const double fine3 = fine(2, uu, v, tt) * 1.0;
+        Uxx += sg(tt) * sg(uu) * (fine2 + fine3);
+        Vxyz += sg(tt) * sg(uu) * sg(v) * fine2;
+        Wxyz += sg(tt) * sg(uu) * sg(v) * fine3;
+      }
+  Uxx *= 0.125;
+  const double cdx = 2.0 * blk.dx[0], cdy = 2.0 * blk.dx[1], cdz = 2.0 * blk.dx[2];
+  const double dx2 = cdx * cdx, dy2 = cdy * cdy, dz2 = cdz * cdz;
+  Vxyz *= 0.125 * dz2 / (dx2 + dz2);
+  Wxyz *= 0.125 * dy2 / (dx2 + dy2);
+  double x = 0.5 * (fine(0, ok, oj, 0) * 1.0 + fine(0, ok, oj, 2) * 1.0) + Uxx + sg(ok) * Vxyz +
+             sg(oj) * Wxyz;
+  x *= 1.0;
+  u.face[d][b][g.s(idx[2], idx[1], idx[0])] = x;
+}
+
+// EMF correction at level jumps: every edge on a block's boundary that a finer leaf also
+// has on its boundary takes the length-weighted RestrictAverage of the fine edges on it
+// (edge elements average along their own axis). The finer leaf may sit across a face, an
+// edge or a corner of the block; blocks that share the edge then all carry the same value,
+// which keeps their shared faces equal. 2:1 balance keeps the fine edges read here out of
+// the fine leaves' own corrections.
+__global__ void __launch_bounds__(256) k_amr_emf_correct(km_pack_t u, AmrGrid t, int te) {
+  const Geo g(u);
+  int b, idx[3];
+  if (!decode_stag(g, u.nblocks, b, idx)) return;
+  double val;
+  if (amr_emf_edge(u, t, te, b, idx, val)) u.edge[te][b][g.s(idx[2], idx[1], idx[0])] = val;
+}
+
+void launch_amr_ghost_faces(const AmrGrid &t, const km_pack_t &u, hipStream_t st) {
+  const Geo g(u);
+  const long n = g.nstag() * u.nblocks;
+  if (n == 0) return;
+  const dim3 grid((unsigned)((n + 255) / 256));
+  for (int d = 0; d < 3; ++d) hipLaunchKernelGGL(k_amr_faces_shared, grid, dim3(256), 0, st, u, t, d);
+  for (int d = 0; d < 3; ++d)
+    hipLaunchKernelGGL(k_amr_faces_internal, grid, dim3(256), 0, st, u, t, d);
+}
+
+void launch_amr_emf_correct(const AmrGrid &t, const km_pack_t &u, hipStream_t st) {
+  const Geo g(u);
+  if (t.lmax == 0 || u.nblocks == 0 || g.ndim < 2) return;
+  const long n = g.nstag() * u.nblocks;
+  const dim3 grid((unsigned)((n + 255) / 256));
+  for (int te = g.ndim < 3 ? 2 : 0; te < 3; ++te)
+    hipLaunchKernelGGL(k_amr_emf_correct, grid, dim3(256), 0, st, u, t, te);
+}
+
 void launch_amr_ghost_fill(const AmrGrid &t, const km_pack_t &u, int v0, int v1, int nc,
                            hipStream_t st) {
   const Geo g(u);

@@ -48,6 +48,25 @@ KM_HD int amr_map(const AmrGrid &t, int d, int G, int N, bool &flip) {
   return G;
 }
 
+// the same map for a face index along d (N + 1 faces; 0 and N coincide when periodic)
+KM_HD int amr_map_face(const AmrGrid &t, int d, int G, int N, bool &flip) {
+  if (G < 0) {
+    const int bc = t.bc[2 * d];
+    if (bc == KM_BC_PERIODIC) return ((G % N) + N) % N;
+    if (bc == KM_BC_OUTFLOW) return 0;
+    flip = !flip;
+    return -G;
+  }
+  if (G > N) {
+    const int bc = t.bc[2 * d + 1];
+    if (bc == KM_BC_PERIODIC) return G % N;
+    if (bc == KM_BC_OUTFLOW) return N;
+    flip = !flip;
+    return 2 * N - G;
+  }
+  return G;
+}
+
 // leaf containing the (in-domain) level-L zone G[3]
 KM_HD int amr_leaf(const AmrGrid &t, const Geo &g, int L, const int G[3]) {
   int f[3];
@@ -63,6 +82,115 @@ KM_HD int amr_local(const km_block_t &blk, const Geo &g, int d, int G) {
   if (d >= g.ndim) return 0;
   const int i = G - blk.lx[d] * g.nx[d] + g.lo[d];
   return i < g.lo[d] ? g.lo[d] : (i > g.hi[d] ? g.hi[d] : i);
+}
+
+// local face index in leaf `blk` of its own-level global face G along d (clamped to the
+// block's own faces lo .. hi+1)
+KM_HD int amr_local_face(const km_block_t &blk, const Geo &g, int d, int G) {
+  if (d >= g.ndim) return 0;
+  const int i = G - blk.lx[d] * g.nx[d] + g.lo[d];
+  return i < g.lo[d] ? g.lo[d] : (i > g.hi[d] + 1 ? g.hi[d] + 1 : i);
+}
+
+// the corrected value of edge idx (family te) of block b, if a finer leaf shares it
+KM_HD bool amr_emf_edge(const km_pack_t &u, const AmrGrid &t, int te, int b, const int idx[3],
+                        double &val) {
+  const Geo g(u);
+  // edge family te: cell index along te, node indices across
+  int ax[2], n = 0;
+#pragma unroll
+  for (int e = 0; e < 3; ++e)
+    if (e != te) ax[n++] = e;
+  bool onb = false;
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    if (e >= g.ndim) {
+      if (idx[e] != 0) return false;
+      continue;
+    }
+    const int hi = g.hi[e] + (e == te ? 0 : 1);
+    if (idx[e] < g.lo[e] || idx[e] > hi) return false;
+    if (e != te && (idx[e] == g.lo[e] || idx[e] == g.hi[e] + 1)) onb = true;
+  }
+  if (!onb) return false;
+  const km_block_t &blk = u.blocks[b];
+  const int L = blk.level;
+  if (L >= t.lmax) return false;
+  int node[3];  // level-L global node (across axes) / cell (along te)
+#pragma unroll
+  for (int e = 0; e < 3; ++e) node[e] = e < g.ndim ? blk.lx[e] * g.nx[e] + (idx[e] - g.lo[e]) : 0;
+  // the four cells around the edge: look for a finer leaf
+  for (int q = 0; q < 4; ++q) {
+    int c[3] = {node[0], node[1], node[2]};
+    int fnode[3];
+    bool skip = false;
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+      const int e = ax[m];
+      if (e >= g.ndim) {
+        c[e] = 0;
+        fnode[e] = 0;
+        continue;
+      }
+      const int below = (q >> m) & 1;  // cell below the node along e
+      const int N = amr_zones(t, g, e, L);
+      int ce = node[e] - below;
+      if (ce < 0 || ce >= N) {
+        if (t.bc[2 * e + (ce < 0 ? 0 : 1)] != KM_BC_PERIODIC) {
+          skip = true;
+          continue;
+        }
+        ce = (ce + N) % N;
+      }
+      c[e] = ce;
+      fnode[e] = 2 * (ce + below);  // the node in the wrapped frame, at level L+1
+    }
+    if (skip) continue;
+    int cf[3];
+#pragma unroll
+    for (int e = 0; e < 3; ++e) cf[e] = e < g.ndim ? 2 * c[e] : 0;
+    const int fl = amr_leaf(t, g, L + 1, cf);
+    const km_block_t &fb = u.blocks[fl];
+    if (fb.level != L + 1) continue;
+    int fe[3];
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+      if (e >= g.ndim) {
+        fe[e] = 0;
+      } else if (e == te) {
+        fe[e] = amr_local(fb, g, e, 2 * node[e]);
+      } else {
+        int li = fnode[e] - fb.lx[e] * g.nx[e] + g.lo[e];
+        fe[e] = li < g.lo[e] ? g.lo[e] : (li > g.hi[e] + 1 ? g.hi[e] + 1 : li);
+      }
+    }
+    const bool inc[3] = {te == 0, te == 1 && g.ndim > 1, te == 2 && g.ndim > 2};
+    const double len = fb.dx[te];
+    const double *E = u.edge[te][fl];
+    const long fidx = g.s(fe[2], fe[1], fe[0]);
+    double V[2][2][2], Tm[2][2][2];
+#pragma unroll
+    for (int ok = 0; ok < 2; ++ok)
+#pragma unroll
+      for (int oj = 0; oj < 2; ++oj)
+#pragma unroll
+        for (int oi = 0; oi < 2; ++oi) {
+          V[ok][oj][oi] = 0.;
+          Tm[ok][oj][oi] = 0.;
+          if ((ok == 0 || inc[2]) && (oj == 0 || inc[1]) && (oi == 0 || inc[0])) {
+            V[ok][oj][oi] = len;
+            Tm[ok][oj][oi] = len * E[fidx + ok * g.ss[2] + oj * g.ss[1] + oi];
+          }
+        }
+    const double tvol = ((V[0][0][0] + V[0][1][0]) + (V[0][0][1] + V[0][1][1])) +
+                        ((V[1][0][0] + V[1][1][0]) + (V[1][0][1] + V[1][1][1]));
+    val = tvol > 0.0 ? (((Tm[0][0][0] + Tm[0][1][0]) + (Tm[0][0][1] + Tm[0][1][1])) +
+                        ((Tm[1][0][0] + Tm[1][1][0]) + (Tm[1][0][1] + Tm[1][1][1]))) /
+                           tvol
+                     : 0.0;
+    return true;
+  }
+  return false;
 }
 
 // host-side tree: leaf metrics in the caller's leaf order and the finest-level lookup
@@ -86,5 +214,8 @@ int amr_min_cells(int ng);
 void launch_amr_ghost_fill(const AmrGrid &t, const km_pack_t &u, int v0, int v1, int nc,
                            hipStream_t st);
 void launch_amr_flux_correct(const AmrGrid &t, const km_pack_t &u, int nc, hipStream_t st);
+// CT: face B ghost fill across levels and EMF correction at level jumps
+void launch_amr_ghost_faces(const AmrGrid &t, const km_pack_t &u, hipStream_t st);
+void launch_amr_emf_correct(const AmrGrid &t, const km_pack_t &u, hipStream_t st);
 
 }  // namespace km

@@ -56,7 +56,13 @@ void amr_block_metric(const km_mesh_desc_t &m, int L, const int l[3], km_block_t
   blk.level = L;
 }
 
-int amr_min_cells(int ng) { return 4 * ((ng + 1) / 2 + 1); }
+// Ghost zones reach ng cells past a block and a prolongation stencil 2*(ceil(ng/2) + 1);
+// with blocks of at least that many cells (and 2 ng), every leaf those reads land in
+// touches the block being filled, so 2:1 balance bounds its level to L-1 .. L+1.
+int amr_min_cells(int ng) {
+  const int stencil = 2 * ((ng + 1) / 2 + 1);
+  return stencil > 2 * ng ? stencil : 2 * ng;
+}
 
 std::string build_amr_tree(const km_mesh_desc_t &m, const int32_t *leaves, int n,
                            AmrTreeHost &t) {

@@ -273,6 +273,48 @@ int km_problem_fill_blocks(const char *name, const km_cfg_t *cfg, const km_mesh_
     return KM_OK;
   }
 
+  if (pname == "mhd_blast") {
+    // mhd_blast.cpp:47-160, Cartesian: rho = 1, v = 0, P = p_explosion inside
+    // r = sqrt(x^2 + y^2) <= radius else p_ambient, cell B = (bx, 0, 0), face B from
+    // Az = bx y: B1 = (Az(y_{j+1}) - Az(y_j)) / dy, B2 = 0 (and B3 = 0 in 3-D)
+    if (mhd != KM_MHD_CT || !face || !face[0] || !face[1] || !face[2] || L.ndim < 2)
+      return KM_EINVAL;
+    const double rho = P(0, 1.0), pamb = P(1, 1.0e-1), pexp = P(2, 1.0e1), bx = P(3, 1.0),
+                 radius = P(4, 0.1);
+    for (int b = 0; b < L.nblocks; ++b) {
+      const km_block_t &bk = blocks[b];
+      double *B1 = face[0] + (long)b * L.nstag(), *B2 = face[1] + (long)b * L.nstag();
+      const int kx = L.ndim > 2 ? 1 : 0;
+      for (int k = L.lo(2); k <= L.hi(2) + kx; ++k)
+        for (int j = L.lo(1); j <= L.hi(1) + 1; ++j)
+          for (int i = L.lo(0); i <= L.hi(0) + 1; ++i) {
+            const double x = xc(bk, 0, i, L.lo(0)), y = xc(bk, 1, j, L.lo(1));
+            const bool cell = i <= L.hi(0) && j <= L.hi(1) && k <= L.hi(2);
+            if (cell) {
+              const long c = L.c(k, j, i);
+              const double r = std::sqrt(x * x + y * y);
+              at(b, KM_DENS)[c] = rho;
+              at(b, sV + 0)[c] = 0.;
+              at(b, sV + 1)[c] = 0.;
+              at(b, sV + 2)[c] = 0.;
+              at(b, sP)[c] = r <= radius ? pexp : pamb;
+              at(b, KM_MAGC0 + 0)[c] = bx;
+              at(b, KM_MAGC0 + 1)[c] = 0.;
+              at(b, KM_MAGC0 + 2)[c] = 0.;
+            }
+            const long s = L.s(k, j, i);
+            if (j <= L.hi(1) && k <= L.hi(2)) {
+              const double yp = xf(bk, 1, j + 1, L.lo(1)), ym = xf(bk, 1, j, L.lo(1));
+              B1[s] = (bx * yp - bx * ym) / bk.dx[1];
+            }
+            if (i <= L.hi(0) && k <= L.hi(2)) B2[s] = (bx * xf(bk, 1, j, L.lo(1)) -
+                                                       bx * xf(bk, 1, j, L.lo(1))) /
+                                                      bk.dx[0];
+          }
+    }
+    return KM_OK;
+  }
+
   if (pname == "mhd_random") {
     // face B = (B0, 0, 0) + discrete curl of an edge-centred vector potential
     // A = sum_m amp sin(2 pi k.x + phi) e_m, e_m a random unit vector normal to k;

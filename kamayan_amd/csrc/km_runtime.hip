@@ -336,9 +336,10 @@ Cont &container(km_sim_t &s, int idx) {
 
 // boundary exchange of cell vars [v0, v1) (+ CT face B) on one container
 void ghost_fill(km_sim_t &s, const km_pack_t &p, int v0, int v1, bool faces) {
-  if (s.amr)
+  if (s.amr) {
     launch_amr_ghost_fill(s.agrid, p, v0, v1, s.nc, s.stream);
-  else
+    if (faces) launch_amr_ghost_faces(s.agrid, p, s.stream);
+  } else
     launch_ghost_fill(s.grid, p, v0, v1, s.nc, faces, s.stream, Instances{s.inst_d, s.grids_d});
 }
 
@@ -354,8 +355,12 @@ int task_stage(km_sim_t &s, int stage, double beta, double dt) {
   const bool ct = cfg.mhd == KM_MHD_CT;
   launch_calculate_fluxes(cfg, p0, flux_box(cfg, p0), s.stream);
   if (ct) launch_calculate_emf(p0, s.stream);
-  // flux correction at level jumps (AddFluxCorrectionTasks, kamayan_driver.cpp:198-207)
-  if (s.amr) launch_amr_flux_correct(s.agrid, p0, s.nc, s.stream);
+  // flux correction at level jumps (AddFluxCorrectionTasks, kamayan_driver.cpp:198-207):
+  // cell fluxes and, for CT, the edge EMFs
+  if (s.amr) {
+    launch_amr_flux_correct(s.agrid, p0, s.nc, s.stream);
+    if (ct) launch_amr_emf_correct(s.agrid, p0, s.stream);
+  }
   launch_fluxes_to_dudt(cfg, p0, s.dudt.pack, s.stream);
   (void)dt;
   launch_apply_dudt(cfg, s.base.pack, p0, md1.pack, s.dudt.pack, beta, 0.0, s.stream, s.dt_d,
@@ -635,11 +640,12 @@ int km_ghost_fill(const km_ghost_plan_t *plan, const km_pack_t *u, int32_t v0, i
     return KM_EINVAL;
   }
   if (plan->amr) {
-    if (with_faces) {
-      set_error("face fields across refinement levels (Toth-Roe prolongation) are not built");
+    if (with_faces && (u->ng % 2 || !u->face[2])) {
+      set_error("face fields across levels need an even nghost and all three face tables");
       return KM_EINVAL;
     }
     launch_amr_ghost_fill(plan->agrid, *u, v0, v1, nc, as_stream(stream));
+    if (with_faces) launch_amr_ghost_faces(plan->agrid, *u, as_stream(stream));
   } else {
     launch_ghost_fill(plan->grid, *u, v0, v1, nc, with_faces != 0, as_stream(stream));
   }
@@ -656,6 +662,9 @@ int km_flux_correct(const km_ghost_plan_t *plan, const km_pack_t *u, void *strea
     }
   const int nc = u->nflux < (u->nvar >= 16 ? 8 : 5) ? u->nflux : (u->nvar >= 16 ? 8 : 5);
   launch_amr_flux_correct(plan->agrid, *u, nc, as_stream(stream));
+  // CT: the edge EMFs too (E3 always, E1 and E2 in 3-D)
+  if (u->edge[2] && (u->ndim < 3 || (u->edge[0] && u->edge[1])))
+    launch_amr_emf_correct(plan->agrid, *u, as_stream(stream));
   return check_launch();
 }
 
@@ -820,14 +829,14 @@ int km_sim_create_amr(const km_cfg_t *cfg, const km_mesh_desc_t *mesh, const int
     set_error("block-tree meshes run on the task path");
     return KM_EINVAL;
   }
-  if (cfg->mhd != KM_MHD_OFF) {
-    set_error("face fields across refinement levels (Toth-Roe prolongation) are not built");
-    return KM_EINVAL;
-  }
   AmrTreeHost t;
   const std::string err = build_amr_tree(*mesh, leaves, nleaves, t);
   if (!err.empty()) {
     set_error(err);
+    return KM_EINVAL;
+  }
+  if (cfg->mhd == KM_MHD_CT && t.lmax > 0 && mesh->ng % 2) {
+    set_error("constrained transport across refinement levels needs an even nghost");
     return KM_EINVAL;
   }
   return sim_create(cfg, mesh, 1, path, &t, out);

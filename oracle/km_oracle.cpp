@@ -1136,6 +1136,7 @@ void AmrGhostFill(const km_cfg_t &cfg, const Tree &t, const km_pack_t &u, int v0
   auto vec_axis = [&](int v) -> int {
     if (v >= KM_MOM0 && v < KM_MOM0 + 3) return v - KM_MOM0;
     if (v >= KM_VEL0(mhd) && v < KM_VEL0(mhd) + 3) return v - KM_VEL0(mhd);
+    if (mhd == KM_MHD_CT && v >= KM_MAGC0 && v < KM_MAGC0 + 3) return v - KM_MAGC0;
     return -1;
   };
   Composite c;
@@ -1216,6 +1217,358 @@ void AmrGhostFill(const km_cfg_t &cfg, const Tree &t, const km_pack_t &u, int v0
     }
   }
   if (bad) throw std::runtime_error("block-tree ghost fill met an unsupported level jump");
+}
+
+// ---- CT face fields on block-tree meshes ------------------------------------------
+// A ghost face of a level-L block is taken from the leaf holding the cell beside it on
+// the block's side (above a face below the block, below a face above it): a copy, the
+// RestrictAverage of the finer leaf's faces (face elements average over their two
+// transverse axes, weighted by face area), or, under a coarser leaf,
+// ProlongateSharedMinMod for fine faces on coarse faces (transverse minmod slopes from the
+// neighbouring coarse faces) and, once every shared face is set,
+// ProlongateInternalTothAndRoe (refinement_operations.hpp:270-398) for the fine faces
+// inside coarse cells, components in x1, x2, x3 order. A block's own faces are never
+// written. The leaf holding a cell is found through the tree's per-level tables.
+
+// leaf holding level-L cell z (in the domain), searching levels L+1, L, L-1
+int LeafOfCell(const Tree &t, int L, const int z[3], int *lev) {
+  for (int Lq = std::min(L + 1, t.lmax); Lq >= std::max(L - 1, 0); --Lq) {
+    int l[3] = {0, 0, 0};
+    for (int d = 0; d < t.ndim; ++d) {
+      const int pos = Lq >= L ? (z[d] << (Lq - L)) : (z[d] >> (L - Lq));
+      l[d] = pos / t.nx[d];
+    }
+    const int b = t.at[Lq][t.flat(Lq, l)];
+    if (b >= 0) {
+      *lev = Lq;
+      return b;
+    }
+  }
+  return -1;
+}
+
+struct FaceRef {
+  int lev, leaf;
+  int li[3];  // staggered element in the leaf (finer: lower corner of the fine faces)
+};
+
+// face d at level-L global index G (face index along d, cells across), seen from side `up`
+bool FindFace(const Tree &t, int L, int d, const int G[3], bool up, FaceRef *f) {
+  int z[3] = {G[0], G[1], G[2]};
+  if (d < t.ndim) {
+    const int N = t.zones(L, d);
+    z[d] = up ? G[d] : G[d] - 1;
+    z[d] = std::max(0, std::min(N - 1, z[d]));
+  }
+  f->leaf = LeafOfCell(t, L, z, &f->lev);
+  if (f->leaf < 0) return false;
+  const int sh = f->lev - L;  // +1 finer, 0, -1 coarser
+  for (int e = 0; e < 3; ++e) {
+    if (e >= t.ndim) {
+      f->li[e] = 0;
+      continue;
+    }
+    const int Ge = sh > 0 ? 2 * G[e] : (sh < 0 ? G[e] / 2 : G[e]);
+    f->li[e] = Ge - t.loc[3 * f->leaf + e] * t.nx[e] + t.nx[e] * 0;
+  }
+  return true;
+}
+
+double FaceAt(const Shape &sh, const km_pack_t &u, int d, int leaf, const int li[3]) {
+  const int lo[3] = {sh.is, sh.js, sh.ks};
+  return facep(u, d, leaf)[sh.s(li[2] + lo[2], li[1] + lo[1], li[0] + lo[0])];
+}
+
+// value of a face found at level L: copy, or RestrictAverage of the finer faces
+double FaceValue(const Tree &t, const Shape &sh, const km_pack_t &u, int L, int d,
+                 const FaceRef &f) {
+  if (f.lev == L) return FaceAt(sh, u, d, f.leaf, f.li);
+  const bool inc[3] = {d != 0, d != 1 && t.ndim > 1, d != 2 && t.ndim > 2};
+  double vol[2][2][2], terms[2][2][2];
+  for (int ok = 0; ok < 2; ++ok)
+    for (int oj = 0; oj < 2; ++oj)
+      for (int oi = 0; oi < 2; ++oi) vol[ok][oj][oi] = terms[ok][oj][oi] = 0;
+  for (int ok = 0; ok < 1 + inc[2]; ++ok)
+    for (int oj = 0; oj < 1 + inc[1]; ++oj)
+      for (int oi = 0; oi < 1 + inc[0]; ++oi) {
+        const int li[3] = {f.li[0] + oi, f.li[1] + oj, f.li[2] + ok};
+        vol[ok][oj][oi] = u.blocks[f.leaf].area[d];
+        terms[ok][oj][oi] = vol[ok][oj][oi] * FaceAt(sh, u, d, f.leaf, li);
+      }
+  const double tvol = ((vol[0][0][0] + vol[0][1][0]) + (vol[0][0][1] + vol[0][1][1])) +
+                      ((vol[1][0][0] + vol[1][1][0]) + (vol[1][0][1] + vol[1][1][1]));
+  return tvol > 0.0
+             ? (((terms[0][0][0] + terms[0][1][0]) + (terms[0][0][1] + terms[0][1][1])) +
+                ((terms[1][0][0] + terms[1][1][0]) + (terms[1][0][1] + terms[1][1][1]))) /
+                   tvol
+             : 0.0;
+}
+
+// is (k, j, i) of face dim d a ghost element of the block (not an own face, not padding)
+bool GhostFaceElem(const Tree &t, const Shape &sh, int d, const int idx[3]) {
+  const int lo[3] = {sh.is, sh.js, sh.ks};
+  bool ghost = false;
+  for (int e = 0; e < 3; ++e) {
+    const int ext = e < t.ndim ? sh.E[e] + (e == d) : 1;
+    if (idx[e] >= ext) return false;
+    const int hi = lo[e] + sh.nx[e] - 1 + (e == d);
+    if (e < t.ndim && (idx[e] < lo[e] || idx[e] > hi)) ghost = true;
+  }
+  return ghost;
+}
+
+void AmrGhostFaces(const km_cfg_t &cfg, const Tree &t, const km_pack_t &u) {
+  (void)cfg;
+  const Shape sh(u);
+  const int lo[3] = {sh.is, sh.js, sh.ks};
+  const int nleaf = (int)t.level.size();
+  int bad = 0;
+  // phase 1: copies, restrictions, shared prolongation
+  for (int d = 0; d < 3; ++d) {
+#pragma omp parallel for schedule(dynamic, 4) reduction(| : bad)
+    for (int b = 0; b < nleaf; ++b) {
+      const int L = t.level[b];
+      const km_block_t &blk = u.blocks[b];
+      for (int k = 0; k < sh.S[2]; ++k)
+        for (int j = 0; j < sh.S[1]; ++j)
+          for (int i = 0; i < sh.S[0]; ++i) {
+            const int idx[3] = {i, j, k};
+            if (!GhostFaceElem(t, sh, d, idx)) continue;
+            int G[3] = {0, 0, 0};
+            bool fl[3] = {false, false, false};
+            for (int e = 0; e < t.ndim; ++e) {
+              const int G0 = t.loc[3 * b + e] * t.nx[e] + (idx[e] - lo[e]);
+              G[e] = e == d ? map_face(G0, t.zones(L, e), t.bc[2 * e], t.bc[2 * e + 1], &fl[e])
+                            : map_cell(G0, t.zones(L, e), t.bc[2 * e], t.bc[2 * e + 1], &fl[e]);
+            }
+            const bool up = d >= t.ndim || idx[d] <= lo[d] + sh.nx[d] - 1;
+            FaceRef f;
+            if (!FindFace(t, L, d, G, up, &f)) {
+              bad |= 1;
+              continue;
+            }
+            double x;
+            if (f.lev >= L) {
+              x = FaceValue(t, sh, u, L, d, f);
+            } else {
+              if (d < t.ndim && (G[d] & 1)) continue;  // inside a coarse cell
+              const double fc = FaceAt(sh, u, d, f.leaf, f.li);
+              double a[3] = {0., 0., 0.};
+              for (int e = 0; e < t.ndim; ++e) {
+                if (e == d) continue;
+                double fv[2];
+                for (int side = 0; side < 2; ++side) {
+                  int C[3] = {0, 0, 0};
+                  bool nf[3] = {false, false, false};
+                  for (int q = 0; q < t.ndim; ++q) C[q] = G[q] / 2;
+                  C[e] += side ? 1 : -1;
+                  C[e] = map_cell(C[e], t.zones(L - 1, e), t.bc[2 * e], t.bc[2 * e + 1], &nf[e]);
+                  FaceRef s2;
+                  if (!FindFace(t, L - 1, d, C, up, &s2) || s2.lev < L - 1) {
+                    bad |= 1;
+                    fv[side] = 0.;
+                    continue;
+                  }
+                  fv[side] = FaceValue(t, sh, u, L - 1, d, s2);
+                }
+                const double dxc = 2.0 * blk.dx[e];
+                a[e] = GradMinMod(fc, fv[0], fv[1], dxc, dxc) * (0.5 * blk.dx[e]);
+              }
+              const bool oi = d != 0 && (G[0] & 1), oj = d != 1 && t.ndim > 1 && (G[1] & 1),
+                         ok = d != 2 && t.ndim > 2 && (G[2] & 1);
+              if (!oi && !oj && !ok) x = fc - (a[0] + a[1] + a[2]);
+              else if (oi && !oj && !ok) x = fc + (a[0] - a[1] - a[2]);
+              else if (!oi && oj && !ok) x = fc - (a[0] - a[1] + a[2]);
+              else if (oi && oj && !ok) x = fc + (a[0] + a[1] - a[2]);
+              else if (!oi && !oj && ok) x = fc - (a[0] + a[1] - a[2]);
+              else if (oi && !oj && ok) x = fc + (a[0] - a[1] + a[2]);
+              else if (!oi && oj && ok) x = fc - (a[0] - a[1] - a[2]);
+              else x = fc + (a[0] + a[1] + a[2]);
+            }
+            if (d < t.ndim && fl[d]) x = -x;
+            facep(u, d, b)[sh.s(k, j, i)] = x;
+          }
+    }
+  }
+  // phase 2: Toth & Roe for fine faces inside coarse cells
+  const int g2 = t.ndim > 1, g3 = t.ndim > 2;
+  for (int d = 0; d < 3; ++d) {
+#pragma omp parallel for schedule(dynamic, 4) reduction(| : bad)
+    for (int b = 0; b < nleaf; ++b) {
+      const int L = t.level[b];
+      const km_block_t &blk = u.blocks[b];
+      for (int k = 0; k < sh.S[2]; ++k)
+        for (int j = 0; j < sh.S[1]; ++j)
+          for (int i = 0; i < sh.S[0]; ++i) {
+            const int idx[3] = {i, j, k};
+            if (!GhostFaceElem(t, sh, d, idx)) continue;
+            if (d < t.ndim && ((idx[d] - lo[d]) % 2 == 0)) continue;
+            int G[3] = {0, 0, 0};
+            bool fl[3] = {false, false, false};
+            for (int e = 0; e < t.ndim; ++e) {
+              const int G0 = t.loc[3 * b + e] * t.nx[e] + (idx[e] - lo[e]);
+              G[e] = e == d ? map_face(G0, t.zones(L, e), t.bc[2 * e], t.bc[2 * e + 1], &fl[e])
+                            : map_cell(G0, t.zones(L, e), t.bc[2 * e], t.bc[2 * e + 1], &fl[e]);
+            }
+            const bool up = d >= t.ndim || idx[d] <= lo[d] + sh.nx[d] - 1;
+            FaceRef f;
+            if (!FindFace(t, L, d, G, up, &f)) {
+              bad |= 1;
+              continue;
+            }
+            if (f.lev >= L) continue;
+            // lower corner (fine) of the coarse cell around this element
+            int c[3] = {0, 0, 0};
+            for (int e = 0; e < t.ndim; ++e) {
+              const int off = idx[e] - lo[e];
+              c[e] = lo[e] + off - (((off % 2) + 2) % 2);
+            }
+            const int fi = c[0], fj = c[1], fk = c[2];
+            auto elem = [&](int ok, int oj, int oi, int out[3]) {  // {k, j, i}
+              if (d == 0) {
+                out[0] = fk + ok * g3, out[1] = fj + oj * g2, out[2] = fi + oi;
+              } else if (d == 1) {
+                out[0] = fk + oj * g3, out[1] = fj + oi * g2, out[2] = fi + ok;
+              } else {
+                out[0] = fk + oi * g3, out[1] = fj + ok * g2, out[2] = fi + oj;
+              }
+            };
+            auto fine = [&](int eidx, int ok, int oj, int oi) {
+              int q[3];
+              elem(ok, oj, oi, q);
+              return facep(u, (d + eidx) % 3, b)[sh.s(q[0], q[1], q[2])];
+            };
+            int ok = 0, oj = 0;
+            const int o[3] = {i - fi, j - fj, k - fk};
+            if (d == 0) {
+              ok = o[2];
+              oj = o[1];
+            } else if (d == 1) {
+              ok = o[0];
+              oj = o[2];
+            } else {
+              ok = o[1];
+              oj = o[0];
+            }
+            auto sg = [](int off) -> double { return off == 0 ? -1.0 : 1.0; };
+            double Uxx = 0.0, Vxyz = 0.0, Wxyz = 0.0;
+            for (const int v : {0, 1})
+              for (const int uu : {0, 2})
+                for (const int tt : {0, 1}) {
+                  const double fine2 = fine(1, v, uu, tt) * 1.0;
+                  const double fine3 = fine(2, uu, v, tt) * 1.0;
+                  Uxx += sg(tt) * sg(uu) * (fine2 + fine3);
+                  Vxyz += sg(tt) * sg(uu) * sg(v) * fine2;
+                  Wxyz += sg(tt) * sg(uu) * sg(v) * fine3;
+                }
+            Uxx *= 0.125;
+            const double cdx = 2.0 * blk.dx[0], cdy = 2.0 * blk.dx[1], cdz = 2.0 * blk.dx[2];
+            const double dx2 = cdx * cdx, dy2 = cdy * cdy, dz2 = cdz * cdz;
+            Vxyz *= 0.125 * dz2 / (dx2 + dz2);
+            Wxyz *= 0.125 * dy2 / (dx2 + dy2);
+            double x = 0.5 * (fine(0, ok, oj, 0) * 1.0 + fine(0, ok, oj, 2) * 1.0) + Uxx +
+                       sg(ok) * Vxyz + sg(oj) * Wxyz;
+            x *= 1.0;
+            facep(u, d, b)[sh.s(k, j, i)] = x;
+          }
+    }
+  }
+  if (bad) throw std::runtime_error("block-tree face fill met an unsupported level jump");
+}
+
+// EMF correction: an edge on a block's boundary that a finer leaf also has on its
+// boundary (across a face, an edge or a corner of the block) takes the length-weighted
+// RestrictAverage of the fine edges on it, so every block sharing the edge agrees.
+void AmrEmfCorrect(const Tree &t, const km_pack_t &u) {
+  const Shape sh(u);
+  const int lo[3] = {sh.is, sh.js, sh.ks};
+  const int nleaf = (int)t.level.size();
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int b = 0; b < nleaf; ++b) {
+    const int L = t.level[b];
+    if (L >= t.lmax) continue;
+    for (int te = (t.ndim < 3 ? 2 : 0); te < 3; ++te) {
+      if (!u.edge[te]) continue;
+      const int a0 = te == 0 ? 1 : 0, a1 = te == 2 ? 1 : 2;  // the two axes across
+      const int ncell_t = te < t.ndim ? sh.nx[te] : 1;
+      const int n0 = sh.nx[a0] + 1, n1 = a1 < t.ndim ? sh.nx[a1] + 1 : 1;
+      for (int m1 = 0; m1 < n1; ++m1)
+        for (int m0 = 0; m0 < n0; ++m0) {
+          const bool onb = m0 == 0 || m0 == sh.nx[a0] ||
+                           (a1 < t.ndim && (m1 == 0 || m1 == sh.nx[a1]));
+          if (!onb) continue;
+          for (int ct = 0; ct < ncell_t; ++ct) {
+            int gn[3] = {0, 0, 0};  // level-L global node across, cell along te
+            gn[a0] = t.loc[3 * b + a0] * t.nx[a0] + m0;
+            if (a1 < t.ndim) gn[a1] = t.loc[3 * b + a1] * t.nx[a1] + m1;
+            if (te < t.ndim) gn[te] = t.loc[3 * b + te] * t.nx[te] + ct;
+            for (int q = 0; q < 4; ++q) {
+              int c[3] = {gn[0], gn[1], gn[2]}, fnode[3] = {0, 0, 0};
+              bool skip = false;
+              const int axes[2] = {a0, a1};
+              for (int m = 0; m < 2; ++m) {
+                const int e = axes[m];
+                if (e >= t.ndim) continue;
+                const int below = (q >> m) & 1;
+                const int N = t.zones(L, e);
+                int ce = gn[e] - below;
+                if (ce < 0 || ce >= N) {
+                  if (t.bc[2 * e + (ce < 0 ? 0 : 1)] != KM_BC_PERIODIC) {
+                    skip = true;
+                    break;
+                  }
+                  ce = (ce + N) % N;
+                }
+                c[e] = ce;
+                fnode[e] = 2 * (ce + below);
+              }
+              if (skip) continue;
+              int zf[3] = {0, 0, 0};
+              for (int e = 0; e < t.ndim; ++e) zf[e] = 2 * c[e];
+              int lev = -1;
+              const int fb = LeafOfCell(t, L + 1, zf, &lev);
+              if (fb < 0 || lev != L + 1) continue;
+              int fe[3] = {0, 0, 0};
+              for (int e = 0; e < t.ndim; ++e) {
+                if (e == te)
+                  fe[e] = 2 * gn[e] - t.loc[3 * fb + e] * t.nx[e] + lo[e];
+                else
+                  fe[e] = fnode[e] - t.loc[3 * fb + e] * t.nx[e] + lo[e];
+              }
+              const bool inc[3] = {te == 0, te == 1 && t.ndim > 1, te == 2 && t.ndim > 2};
+              const double len = u.blocks[fb].dx[te];
+              const double *E = edgep(u, te, fb);
+              double vol[2][2][2], terms[2][2][2];
+              for (int ok = 0; ok < 2; ++ok)
+                for (int oj = 0; oj < 2; ++oj)
+                  for (int oi = 0; oi < 2; ++oi) vol[ok][oj][oi] = terms[ok][oj][oi] = 0;
+              for (int ok = 0; ok < 1 + inc[2]; ++ok)
+                for (int oj = 0; oj < 1 + inc[1]; ++oj)
+                  for (int oi = 0; oi < 1 + inc[0]; ++oi) {
+                    vol[ok][oj][oi] = len;
+                    terms[ok][oj][oi] =
+                        vol[ok][oj][oi] * E[sh.s(fe[2] + ok, fe[1] + oj, fe[0] + oi)];
+                  }
+              const double tvol =
+                  ((vol[0][0][0] + vol[0][1][0]) + (vol[0][0][1] + vol[0][1][1])) +
+                  ((vol[1][0][0] + vol[1][1][0]) + (vol[1][0][1] + vol[1][1][1]));
+              int ce3[3] = {0, 0, 0};
+              ce3[a0] = lo[a0] + m0;
+              if (a1 < t.ndim) ce3[a1] = lo[a1] + m1;
+              if (te < t.ndim) ce3[te] = lo[te] + ct;
+              edgep(u, te, b)[sh.s(ce3[2], ce3[1], ce3[0])] =
+                  tvol > 0.0 ? (((terms[0][0][0] + terms[0][1][0]) +
+                                 (terms[0][0][1] + terms[0][1][1])) +
+                                ((terms[1][0][0] + terms[1][1][0]) +
+                                 (terms[1][0][1] + terms[1][1][1]))) /
+                                   tvol
+                             : 0.0;
+              break;
+            }
+          }
+        }
+    }
+  }
 }
 
 void AmrFluxCorrect(const km_cfg_t &cfg, const Tree &t, const km_pack_t &u) {
@@ -1415,9 +1768,10 @@ void MakeContainer(km_cpu_sim_t &s, Container &c, bool full, Container *share_pr
 // hydro.cpp:102-105): conserved, VELOCITY, PRES, BMOD, EINT, and face B; TEMP is not
 // FillGhost (eos.cpp:67).
 void FillGhosts(km_cpu_sim_t &s, const km_pack_t &p) {
-  if (s.amr)
+  if (s.amr) {
     AmrGhostFill(s.cfg, s.tree, p, 0, KM_TEMP(s.cfg.mhd));
-  else
+    if (s.cfg.mhd == KM_MHD_CT) AmrGhostFaces(s.cfg, s.tree, p);
+  } else
     GhostFill(s.cfg, s.grid, p, 0, KM_TEMP(s.cfg.mhd), true);
 }
 
@@ -1444,7 +1798,10 @@ void Cycle(km_cpu_sim_t &s) {
     }
     CalculateFluxes(s.cfg, p0);
     CalculateEMF(s.cfg, p0);
-    if (s.amr) AmrFluxCorrect(s.cfg, s.tree, p0);
+    if (s.amr) {
+      AmrFluxCorrect(s.cfg, s.tree, p0);
+      if (s.cfg.mhd == KM_MHD_CT) AmrEmfCorrect(s.tree, p0);
+    }
     FluxesToDuDt(s.cfg, p0, s.dudt.pack);
     ApplyDuDt(s.cfg, s.base.pack, p0, md1.pack, s.dudt.pack, betas[stage - 1], dt);
     PreparePrimitive(s.cfg, md1.pack);
@@ -1608,6 +1965,21 @@ int km_cpu_flux_correct(const km_cfg_t *cfg, const km_pack_t *u, const km_mesh_d
   const std::string err = BuildTree(*mesh, leaves, n, t);
   if (!err.empty() || n != u->nblocks) return KM_EINVAL;
   AmrFluxCorrect(*cfg, t, *u);
+  if (u->edge[2] && (u->ndim < 3 || (u->edge[0] && u->edge[1]))) AmrEmfCorrect(t, *u);
+  return KM_OK;
+}
+
+int km_cpu_amr_ghost_faces(const km_cfg_t *cfg, const km_pack_t *u, const km_mesh_desc_t *mesh,
+                           const int32_t *leaves, int32_t n) {
+  Tree t;
+  const std::string err = BuildTree(*mesh, leaves, n, t);
+  if (!err.empty() || n != u->nblocks || !u->face[0] || !u->face[1] || !u->face[2]) return KM_EINVAL;
+  try {
+    AmrGhostFaces(*cfg, t, *u);
+  } catch (const std::exception &e) {
+    std::fprintf(stderr, "km_cpu_amr_ghost_faces: %s\n", e.what());
+    return KM_EINVAL;
+  }
   return KM_OK;
 }
 
@@ -1621,8 +1993,8 @@ int km_cpu_sim_create_amr(const km_cfg_t *cfg, const km_mesh_desc_t *mesh,
   s->nc = KM_NCONS(cfg->mhd);
   s->nstages = cfg->nstages;
   const std::string err = BuildTree(*mesh, leaves, n, s->tree);
-  if (!err.empty() || cfg->mhd != KM_MHD_OFF) {
-    std::fprintf(stderr, "km_cpu_sim_create_amr: %s\n", err.empty() ? "hydro only" : err.c_str());
+  if (!err.empty() || (cfg->mhd == KM_MHD_CT && mesh->ng % 2)) {
+    std::fprintf(stderr, "km_cpu_sim_create_amr: %s\n", err.empty() ? "CT needs even nghost" : err.c_str());
     delete s;
     return KM_EINVAL;
   }

@@ -62,6 +62,7 @@ def lib() -> C.CDLL:
         "km_cpu_amr_blocks": [P(KmMeshDesc), P(i32), i32, vp],
         "km_cpu_amr_ghost_fill": [P(KmCfg), P(KmPack), P(KmMeshDesc), P(i32), i32, i32, i32],
         "km_cpu_flux_correct": [P(KmCfg), P(KmPack), P(KmMeshDesc), P(i32), i32],
+        "km_cpu_amr_ghost_faces": [P(KmCfg), P(KmPack), P(KmMeshDesc), P(i32), i32],
         "km_cpu_sim_create_amr": [P(KmCfg), P(KmMeshDesc), P(i32), i32, P(vp)],
     }
     for name, args in sig.items():

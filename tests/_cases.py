@@ -123,3 +123,40 @@ def amr_totals(cell, lay: Layout, blocks, nvars=5):
     sl = lay.interior()
     return np.array([sum(float(np.sum(cell[b, v][sl])) * blocks[b].vol
                          for b in range(lay.nblocks)) for v in range(nvars)])
+
+
+def amr_mhd_case(ndim=2, nroot=4, nblock=8, ng=4, lmax=2, center=(0.0, 0.0, 0.0), radius=0.12,
+                 bc="outflow", recon="plm", limiter="mc", riemann="hll", integrator="rk2",
+                 params=None):
+    """CT MHD blast (mhd_blast.cpp, Cartesian) on a block-tree mesh over [-0.5, 0.5]^d:
+    the shape of the reference's multilevel blast test (32^2 root in 8^2 blocks, nghost 4,
+    three levels). Returns (cfg, desc, layout, leaves, cell, faces)."""
+    cfg = Config(**{"hydro/reconstruction": recon, "hydro/slope_limiter": limiter,
+                    "hydro/riemann": riemann, "parthenon/time/integrator": integrator,
+                    "material/single/gamma": 5.0 / 3.0, "physics/MHD": "ct"})
+    cfg.c.ndim = ndim
+    nxb = tuple(nblock if d < ndim else 1 for d in range(3))
+    nbs = tuple(nroot if d < ndim else 1 for d in range(3))
+    root = Layout(ndim, ng, nxb, nbs)
+    desc = mesh_desc(root, xmin=(-0.5,) * 3, xmax=(0.5,) * 3, bc=(bc,) * 6)
+    leaves = km.refine_ball(desc, lmax, center, radius)
+    lay = Layout(ndim, ng, nxb, nbs, nleaves=len(leaves))
+    cell, faces = km.problem_fill("mhd_blast", cfg, desc, lay, params=params, leaves=leaves)
+    return cfg, desc, lay, leaves, cell, faces
+
+
+def divb_cells(faces, lay, blocks):
+    """(A+ B+ - A- B-) / V of every cell whose six faces lie in the face arrays."""
+    out = []
+    nd = lay.ndim
+    for b in range(lay.nblocks):
+        bk = blocks[b]
+        f0, f1, f2 = faces[0][b], faces[1][b], faces[2][b]
+        e1, e2, e3 = lay.E
+        d = (bk.area[0] * (f0[:e3, :e2, 1:e1 + 1] - f0[:e3, :e2, :e1]))
+        if nd > 1:
+            d = d + bk.area[1] * (f1[:e3, 1:e2 + 1, :e1] - f1[:e3, :e2, :e1])
+        if nd > 2:
+            d = d + bk.area[2] * (f2[1:e3 + 1, :e2, :e1] - f2[:e3, :e2, :e1])
+        out.append(d / bk.vol)
+    return out
