@@ -28,7 +28,7 @@ BYTES_PER_ZONE_STAGE = 8 * 3 * 5  # stage input + RK base read, stage output wri
 # HBM bytes per RK stage of this workload measured with rocprofv3 FETCH_SIZE / WRITE_SIZE,
 # calibrated against the library's 1 GiB copy kernel (bench.py --calibrate-copy); the
 # summary is profiles/fused_stage_traffic_hydro3d_256c_wenoz_hllc_rk3.md
-MEASURED_TRAFFIC_PER_STAGE = 1.154e10
+MEASURED_TRAFFIC_PER_STAGE = 9.10e9
 
 
 def _setup(km, precision="fp64", n=N, nb=NB):
