@@ -29,6 +29,11 @@ BYTES_PER_ZONE_STAGE = 8 * 3 * 5  # stage input + RK base read, stage output wri
 # calibrated against the library's 1 GiB copy kernel (bench.py --calibrate-copy); the
 # summary is profiles/fused_stage_traffic_hydro3d_256c_wenoz_hllc_rk3.md
 MEASURED_TRAFFIC_PER_STAGE = 9.10e9
+# VALU instructions per zone per stage (SQ_INSTS_VALU x 64 / zones, same profile): x1 + x2
+# + the RK3 average of the two x3 forms; peak issue rate of FP64 VALU work on MI355X:
+# 256 CUs x 64 lanes x 2.4 GHz (78.6 TFLOP/s FP64 vector = one FMA per lane per clock)
+VALU_INSTR_PER_ZONE_STAGE = 1497 + 1433 + (2 * 1745 + 1866) / 3
+VALU_PEAK_LANE_INSTR_PER_S = 256 * 64 * 2.4e9
 
 
 def _setup(km, precision="fp64", n=N, nb=NB):
@@ -304,6 +309,12 @@ def main() -> int:
                      "algorithmic_bytes_per_launch": BYTES_PER_ZONE_STAGE * zones,
                      "avg_launch_ms": stage_ms},
         "kernels": kernels,
+        "valu": {"bound": "fp64 valu issue",
+                 "achieved": VALU_INSTR_PER_ZONE_STAGE * zones / (stage_ms * 1e-3),
+                 "peak": VALU_PEAK_LANE_INSTR_PER_S, "unit": "lane-instr/s",
+                 "frac": VALU_INSTR_PER_ZONE_STAGE * zones / (stage_ms * 1e-3)
+                 / VALU_PEAK_LANE_INSTR_PER_S,
+                 "instr_per_zone_stage": VALU_INSTR_PER_ZONE_STAGE},
     }
     if not args.no_variants:
         var = {}

@@ -120,21 +120,12 @@ __device__ __forceinline__ double grad_minmod(double fc, double fm, double fp, d
 __global__ void __launch_bounds__(256) k_amr_ghost(km_pack_t u, AmrGrid t, int v0, int v1,
                                                    int nc) {
   const Geo g(u);
-  const long n = g.ncell();
+  const long n = ghost_count(g);
   const long tt = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (tt >= n * u.nblocks) return;
   const int b = (int)(tt / n);
-  long r = tt % n;
   int idx[3];
-  idx[0] = (int)(r % g.E[0]);
-  r /= g.E[0];
-  idx[1] = (int)(r % g.E[1]);
-  idx[2] = (int)(r / g.E[1]);
-  bool ghost = false;
-#pragma unroll
-  for (int d = 0; d < 3; ++d)
-    if (d < g.ndim && (idx[d] < g.lo[d] || idx[d] > g.hi[d])) ghost = true;
-  if (!ghost) return;
+  ghost_cell(g, tt % n, idx);
   const km_block_t &blk = u.blocks[b];
   const int L = blk.level;
   int G0[3], G[3];
@@ -590,7 +581,7 @@ void launch_amr_emf_correct(const AmrGrid &t, const km_pack_t &u, hipStream_t st
 void launch_amr_ghost_fill(const AmrGrid &t, const km_pack_t &u, int v0, int v1, int nc,
                            hipStream_t st) {
   const Geo g(u);
-  const long n = g.ncell() * u.nblocks;
+  const long n = ghost_count(g) * u.nblocks;
   if (n == 0 || v0 >= v1) return;
   hipLaunchKernelGGL(k_amr_ghost, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, u, t, v0,
                      v1, nc);

@@ -51,6 +51,51 @@ struct Geo {
 
 KM_HD double *cell_ptr(const km_pack_t &p, int b, int v) { return p.cell[(long)b * p.nvar + v]; }
 
+// ghost zones of one block, enumerated without the interior: the two x3 slabs (full
+// x1-x2 planes), then the two x2 slabs (interior x3 rows), then the two x1 slabs
+// (interior x2-x3 rows, ng cells at each end); x1 runs fastest within each piece
+KM_HD long ghost_count(const Geo &g) {
+  return g.ncell() - (long)g.nx[0] * (g.ndim > 1 ? g.nx[1] : 1) * (g.ndim > 2 ? g.nx[2] : 1);
+}
+KM_HD void ghost_cell(const Geo &g, long r, int idx[3]) {
+  const int ng = g.ng;
+  if (g.ndim > 2) {
+    const long zs = (long)ng * g.E[1] * g.E[0];
+    if (r < 2 * zs) {
+      const int side = r >= zs;
+      long q = r - (side ? zs : 0);
+      idx[0] = (int)(q % g.E[0]);
+      q /= g.E[0];
+      idx[1] = (int)(q % g.E[1]);
+      idx[2] = (int)(q / g.E[1]) + (side ? g.hi[2] + 1 : 0);
+      return;
+    }
+    r -= 2 * zs;
+  }
+  const int nk = g.ndim > 2 ? g.nx[2] : 1;
+  if (g.ndim > 1) {
+    const long ys = (long)ng * g.E[0] * nk;
+    if (r < 2 * ys) {
+      const int side = r >= ys;
+      long q = r - (side ? ys : 0);
+      idx[0] = (int)(q % g.E[0]);
+      q /= g.E[0];
+      idx[1] = (int)(q % ng) + (side ? g.hi[1] + 1 : 0);
+      idx[2] = g.lo[2] + (int)(q / ng);
+      return;
+    }
+    r -= 2 * ys;
+  }
+  const int nj = g.ndim > 1 ? g.nx[1] : 1;
+  const int i = (int)(r % ng);
+  long q = r / ng;
+  const int side = (int)(q % 2);
+  q /= 2;
+  idx[0] = i + (side ? g.hi[0] + 1 : 0);
+  idx[1] = g.lo[1] + (int)(q % nj);
+  idx[2] = g.lo[2] + (int)(q / nj);
+}
+
 // loads/stores through the global address space (the pack tables hold generic
 // pointers; telling the compiler they are global gives global_load/store instead of
 // flat ones)
@@ -62,8 +107,8 @@ __device__ __forceinline__ double ldg(const double *p, long i) {
 }
 __device__ __forceinline__ void stg(double *p, long i, double v) { ((gdouble *)p)[i] = v; }
 #else
-inline double ldg(const double *p, long i) { return p[i]; }
-inline void stg(double *p, long i, double v) { p[i] = v; }
+__host__ __device__ inline double ldg(const double *p, long i) { return p[i]; }
+__host__ __device__ inline void stg(double *p, long i, double v) { p[i] = v; }
 #endif
 KM_HD double *flux_ptr(const km_pack_t &p, int d, int b, int v) {
   return p.flux[d][(long)b * p.nflux + v];

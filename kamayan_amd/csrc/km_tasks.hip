@@ -64,6 +64,59 @@ __global__ void __launch_bounds__(256) k_flux(km_pack_t u, int lim, Box bx) {
 }
 
 // ---------------------------------------------------------------------------------
+// hydro::CalculateFluxes along x1 with one lane per face. The march above gives every lane
+// its own row; along x1 that makes each load and store touch 64 separate cache lines.
+// Here the lanes of a wavefront cover consecutive faces of a row, so every access is
+// contiguous. Each lane reconstructs the two cells beside its face from the same
+// stencil values the march reads, so the faces are bitwise the march's.
+// ---------------------------------------------------------------------------------
+template <class T, int RECON, int RIEM, int MHD>
+__global__ void __launch_bounds__(256) k_flux_x1(km_pack_t u, int lim, Box bx) {
+  constexpr int NR = HV<MHD>::NR, NC = HV<MHD>::NC;
+  constexpr int W = ReconWidth<RECON>::value;
+  const Geo g(u);
+  const int nf = bx.hi[0] - bx.lo[0] + 2;  // faces bx.lo .. bx.hi + 1
+  const int n1 = bx.hi[1] - bx.lo[1] + 1, n2 = bx.hi[2] - bx.lo[2] + 1;
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= (long)nf * n1 * n2 * u.nblocks) return;
+  const int c = bx.lo[0] + (int)(t % nf);
+  t /= nf;
+  const int j = bx.lo[1] + (int)(t % n1);
+  t /= n1;
+  const int k = bx.lo[2] + (int)(t % n2);
+  const int b = (int)(t / n2);
+  const long row = g.c(k, j, 0);
+  T vL[NR], vR[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    const double *q = cell_ptr(u, b, recon_slot<MHD>(r));
+    T s[2 * W + 2];  // cells c-1-W .. c+W
+#pragma unroll
+    for (int o = 0; o < 2 * W + 2; ++o) s[o] = T(ldg(q, row + c - 1 - W + o));
+    T wl[5], wr[5];
+#pragma unroll
+    for (int o = -W; o <= W; ++o) {
+      wl[2 + o] = s[W + o];
+      wr[2 + o] = s[W + 1 + o];
+    }
+    T m, p;
+    reconstruct_rt<RECON>(lim, wl, m, p);
+    vL[r] = p;
+    reconstruct_rt<RECON>(lim, wr, m, p);
+    vR[r] = m;
+  }
+  if constexpr (MHD) {  // :104-107
+    const T bf = T(ldg(u.face[0][b], g.s(k, j, c)));
+    vL[R_MAGC + 0] = bf;
+    vR[R_MAGC + 0] = bf;
+  }
+  T F[NC];
+  riemann<RIEM, MHD, 0>(vL, vR, F);
+#pragma unroll
+  for (int v = 0; v < NC; ++v) stg(flux_ptr(u, 0, b, v), row + c, double(F[v]));
+}
+
+// ---------------------------------------------------------------------------------
 // hydro::CalculateEMF (hydro_add_flux_tasks.cpp:387-442), arithmetic averaging
 // E3(i-1/2,j-1/2) = 1/4 [Fy(Bx)(i-1,j) + Fy(Bx)(i,j) - Fx(By)(i,j-1) - Fx(By)(i,j)] and
 // cyclic; each edge family over its own interior range.
@@ -425,21 +478,12 @@ __device__ __forceinline__ int map_face(int g, int N, int lo, int hi, bool &flip
 __global__ void __launch_bounds__(256) k_ghost_cells(km_pack_t u, UniformGrid gr0, int v0, int v1,
                                                      int nc, Instances in) {
   const Geo g(u);
-  const long n = g.ncell();
-  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long n = ghost_count(g);
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (t >= n * u.nblocks) return;
   const int b = (int)(t / n);
-  long r = t % n;
   int idx[3];
-  idx[0] = (int)(r % g.E[0]);
-  r /= g.E[0];
-  idx[1] = (int)(r % g.E[1]);
-  idx[2] = (int)(r / g.E[1]);
-  bool ghost = false;
-#pragma unroll
-  for (int d = 0; d < 3; ++d)
-    if (d < g.ndim && (idx[d] < g.lo[d] || idx[d] > g.hi[d])) ghost = true;
-  if (!ghost) return;
+  ghost_cell(g, t % n, idx);
   const UniformGrid gr = in.inst ? in.grids[in.inst[b]] : gr0;
   const int lb = b - gr.first;
   const int bl[3] = {lb % gr.nb[0], (lb / gr.nb[0]) % gr.nb[1], lb / (gr.nb[0] * gr.nb[1])};
@@ -535,7 +579,8 @@ static void launch_flux_dirs(const km_pack_t &u, int lim, const Box &bx, hipStre
   auto cnt = [&](int da, int db) {
     return (long)(bx.hi[da] - bx.lo[da] + 1) * (bx.hi[db] - bx.lo[db] + 1) * u.nblocks;
   };
-  hipLaunchKernelGGL((k_flux<T, RECON, RIEM, MHD, 0>), dim3(nblk(cnt(1, 2))), dim3(256), 0, st, u,
+  const long nface_x1 = (long)(bx.hi[0] - bx.lo[0] + 2) * cnt(1, 2);
+  hipLaunchKernelGGL((k_flux_x1<T, RECON, RIEM, MHD>), dim3(nblk(nface_x1)), dim3(256), 0, st, u,
                      lim, bx);
   if (g.ndim > 1)
     hipLaunchKernelGGL((k_flux<T, RECON, RIEM, MHD, 1>), dim3(nblk(cnt(0, 2))), dim3(256), 0, st, u,
@@ -676,8 +721,8 @@ void launch_ghost_fill(const UniformGrid &gr, const km_pack_t &u, int v0, int v1
                        bool faces, hipStream_t st, Instances in) {
   const Geo g(u);
   if (v1 > v0)
-    hipLaunchKernelGGL(k_ghost_cells, dim3(nblk(g.ncell() * u.nblocks)), dim3(256), 0, st, u,
-                       gr, v0, v1, nc, in);
+    hipLaunchKernelGGL(k_ghost_cells, dim3(nblk(ghost_count(g) * u.nblocks)), dim3(256), 0, st,
+                       u, gr, v0, v1, nc, in);
   if (faces)
     for (int f = 0; f < (g.ndim > 2 ? 3 : 2); ++f)
       hipLaunchKernelGGL(k_ghost_faces, dim3(nblk(g.nstag() * u.nblocks)), dim3(256), 0, st, u,
