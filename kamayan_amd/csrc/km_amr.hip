@@ -117,6 +117,8 @@ __device__ __forceinline__ double grad_minmod(double fc, double fm, double fp, d
 
 }  // namespace
 
+// Zones a level-L or a level-L+1 leaf covers: copy or RestrictAverage. Zones a coarser
+// leaf covers are left to k_amr_prolong.
 __global__ void __launch_bounds__(256) k_amr_ghost(km_pack_t u, AmrGrid t, int v0, int v1,
                                                    int nc) {
   const Geo g(u);
@@ -134,17 +136,94 @@ __global__ void __launch_bounds__(256) k_amr_ghost(km_pack_t u, AmrGrid t, int v
   for (int d = 0; d < 3; ++d) G0[d] = d < g.ndim ? blk.lx[d] * g.nx[d] + (idx[d] - g.lo[d]) : 0;
   map_zone(t, g, L, G0, G, fl);
   const Src s = locate(u, t, g, L, G);
+  if (s.kind == 2) return;
   const long dc = g.c(idx[2], idx[1], idx[0]);
-  if (s.kind != 2) {
-    for (int v = v0; v < v1; ++v) {
-      double x = value_at(u, g, s, v);
-      const int ax = vector_axis(v, nc);
-      if (ax >= 0 && fl[ax]) x = -x;
-      cell_ptr(u, b, v)[dc] = x;
+  // variables go in chunks: every load of a chunk is issued before its stores, so the
+  // loads of one chunk overlap instead of waiting on the previous variable's store
+  constexpr int kChunk = 4;
+  const double vol = s.kind == 1 ? u.blocks[s.leaf].vol : 0.0;
+  for (int v = v0; v < v1; v += kChunk) {
+    double x[kChunk];
+#pragma unroll
+    for (int q = 0; q < kChunk; ++q) {
+      x[q] = 0.0;
+      if (v + q < v1) {
+        const double *src = cell_ptr(u, s.leaf, v + q);
+        x[q] = s.kind == 1 ? restrict_cell(src, vol, g, s.idx) : src[s.idx];
+      }
     }
-    return;
+#pragma unroll
+    for (int q = 0; q < kChunk; ++q) {
+      if (v + q < v1) {
+        const int ax = vector_axis(v + q, nc);
+        cell_ptr(u, b, v + q)[dc] = (ax >= 0 && fl[ax]) ? -x[q] : x[q];
+      }
+    }
   }
-  // coarser cover: the coarse zone C = G/2 and its neighbours at level L-1
+}
+
+// Zones a level-(L-1) leaf covers: ProlongateSharedMinMod. One thread takes one coarse
+// ghost zone of a block, i.e. the up to 2^d fine ghost zones that share a coarse parent
+// C: C's value and limited slopes are formed once and each child applies its own
+// offsets. The block's ghost layer seen at half resolution has nx/2 interior zones and
+// ceil(ng/2) ghosts per side; the physical boundary maps (periodic shift by an even
+// count, outflow clamp, reflect mirror) send both children of an unmapped coarse zone
+// to children of one mapped C, and a level-(L-1) leaf covers either both or neither.
+__global__ void __launch_bounds__(256) k_amr_prolong(km_pack_t u, AmrGrid t, int v0, int v1,
+                                                     int nc) {
+  const Geo g(u);
+  km_pack_t half = u;
+  half.ng = (u.ng + 1) / 2;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) half.nx[d] = d < u.ndim ? u.nx[d] / 2 : 1;
+  const Geo gc(half);
+  const long n = ghost_count(gc);
+  const long tt = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (tt >= n * u.nblocks) return;
+  const int b = (int)(tt / n);
+  int cz[3];
+  ghost_cell(gc, tt % n, cz);
+  const km_block_t &blk = u.blocks[b];
+  const int L = blk.level;
+  if (L == 0) return;
+  const int nch = 1 << g.ndim;
+  // children: padded fine index, their mapped parities and flips
+  long dc[8];
+  int par[8];
+  bool valid[8];
+  bool fl[8][3];
+  int first = -1;
+  int G[3];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    valid[c] = false;
+    dc[c] = 0;
+    par[c] = 0;
+    fl[c][0] = fl[c][1] = fl[c][2] = false;
+    if (c >= nch) continue;
+    int fi[3], G0[3], Gc[3];
+    bool in = true;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      fi[d] = d < g.ndim ? g.ng + 2 * (cz[d] - gc.ng) + ((c >> d) & 1) : 0;
+      if (fi[d] < 0 || fi[d] >= g.E[d]) in = false;
+      G0[d] = d < g.ndim ? blk.lx[d] * g.nx[d] + (fi[d] - g.lo[d]) : 0;
+    }
+    if (!in) continue;
+    valid[c] = true;
+    dc[c] = g.c(fi[2], fi[1], fi[0]);
+    map_zone(t, g, L, G0, Gc, fl[c]);
+    par[c] = (Gc[0] & 1) | (g.ndim > 1 ? (Gc[1] & 1) << 1 : 0) | (g.ndim > 2 ? (Gc[2] & 1) << 2 : 0);
+    if (first < 0) {
+      first = c;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) G[d] = Gc[d];
+    }
+  }
+  if (first < 0) return;
+  const Src s = locate(u, t, g, L, G);
+  if (s.kind != 2) return;
+  // the coarse zone C = G/2 and its neighbours at level L-1
   int C[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) C[d] = d < g.ndim ? (G[d] >> 1) : 0;
@@ -162,11 +241,14 @@ __global__ void __launch_bounds__(256) k_amr_ghost(km_pack_t u, AmrGrid t, int v
       nbp[d] = locate(u, t, g, L - 1, M);
     }
   }
-  const int oi = G[0] & 1, oj = g.ndim > 1 ? (G[1] & 1) : 0, ok = g.ndim > 2 ? (G[2] & 1) : 0;
+  double df[3] = {0., 0., 0.};
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    if (d < g.ndim) df[d] = 0.5 * blk.dx[d];
   for (int v = v0; v < v1; ++v) {
     const int ax = vector_axis(v, nc);
     const double fc = cell_ptr(u, s.leaf, v)[s.idx];
-    double gr[3] = {0., 0., 0.}, df[3] = {0., 0., 0.};
+    double gr[3] = {0., 0., 0.};
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       if (d < g.ndim) {
@@ -175,23 +257,27 @@ __global__ void __launch_bounds__(256) k_amr_ghost(km_pack_t u, AmrGrid t, int v
         if (ax >= 0 && flp[d][ax]) fp = -fp;
         const double dxc = 2.0 * blk.dx[d];
         gr[d] = grad_minmod(fc, fm, fp, dxc, dxc);
-        df[d] = 0.5 * blk.dx[d];
       }
     }
     // the child formulas of ProlongateSharedGeneral (refinement_operations.hpp:241-259)
     // with equal minus/plus slopes and offsets
-    const double a = gr[0] * df[0], bb = gr[1] * df[1], c = gr[2] * df[2];
-    double x;
-    if (!ok && !oj && !oi) x = fc - (a + bb + c);
-    else if (!ok && !oj) x = fc + (a - bb - c);
-    else if (!ok && !oi) x = fc - (a - bb + c);
-    else if (!ok) x = fc + (a + bb - c);
-    else if (!oj && !oi) x = fc - (a + bb - c);
-    else if (!oj) x = fc + (a - bb + c);
-    else if (!oi) x = fc - (a - bb - c);
-    else x = fc + (a + bb + c);
-    if (ax >= 0 && fl[ax]) x = -x;
-    cell_ptr(u, b, v)[dc] = x;
+    const double a = gr[0] * df[0], bb = gr[1] * df[1], cc = gr[2] * df[2];
+    double *dst = cell_ptr(u, b, v);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      if (!valid[c]) continue;
+      const int oi = par[c] & 1, oj = (par[c] >> 1) & 1, ok = (par[c] >> 2) & 1;
+      double y;
+      if (!ok && !oj && !oi) y = fc - (a + bb + cc);
+      else if (!ok && !oj) y = fc + (a - bb - cc);
+      else if (!ok && !oi) y = fc - (a - bb + cc);
+      else if (!ok) y = fc + (a + bb - cc);
+      else if (!oj && !oi) y = fc - (a + bb - cc);
+      else if (!oj) y = fc + (a - bb + cc);
+      else if (!oi) y = fc - (a - bb - cc);
+      else y = fc + (a + bb + cc);
+      dst[dc[c]] = (ax >= 0 && fl[c][ax]) ? -y : y;
+    }
   }
 }
 
@@ -585,6 +671,14 @@ void launch_amr_ghost_fill(const AmrGrid &t, const km_pack_t &u, int v0, int v1,
   if (n == 0 || v0 >= v1) return;
   hipLaunchKernelGGL(k_amr_ghost, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, u, t, v0,
                      v1, nc);
+  if (t.lmax > 0) {
+    km_pack_t half = u;
+    half.ng = (u.ng + 1) / 2;
+    for (int d = 0; d < 3; ++d) half.nx[d] = d < u.ndim ? u.nx[d] / 2 : 1;
+    const long nc2 = ghost_count(Geo(half)) * u.nblocks;
+    hipLaunchKernelGGL(k_amr_prolong, dim3((unsigned)((nc2 + 255) / 256)), dim3(256), 0, st, u, t,
+                       v0, v1, nc);
+  }
 }
 
 void launch_amr_flux_correct(const AmrGrid &t, const km_pack_t &u, int nc, hipStream_t st) {

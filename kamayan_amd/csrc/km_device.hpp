@@ -53,47 +53,59 @@ KM_HD double *cell_ptr(const km_pack_t &p, int b, int v) { return p.cell[(long)b
 
 // ghost zones of one block, enumerated without the interior: the two x3 slabs (full
 // x1-x2 planes), then the two x2 slabs (interior x3 rows), then the two x1 slabs
-// (interior x2-x3 rows, ng cells at each end); x1 runs fastest within each piece
-KM_HD long ghost_count(const Geo &g) {
-  return g.ncell() - (long)g.nx[0] * (g.ndim > 1 ? g.nx[1] : 1) * (g.ndim > 2 ? g.nx[2] : 1);
+// (interior x2-x3 rows, ng cells at each end); x1 runs fastest within each piece.
+// stag = d enumerates the ghost faces of the d-staggered array instead (one more
+// interior face and one more element along d); stag = -1 the cells.
+KM_HD long ghost_count(const Geo &g, int stag = -1) {
+  long all = 1, in = 1;
+  for (int d = 0; d < 3; ++d) {
+    const int s = d == stag ? 1 : 0;
+    all *= g.E[d] + s;
+    in *= d < g.ndim ? g.nx[d] + s : 1;
+  }
+  return all - in;
 }
-KM_HD void ghost_cell(const Geo &g, long r, int idx[3]) {
+KM_HD void ghost_cell(const Geo &g, long r, int idx[3], int stag = -1) {
   const int ng = g.ng;
+  int E[3], n[3];
+  for (int d = 0; d < 3; ++d) {
+    const int s = d == stag ? 1 : 0;
+    E[d] = g.E[d] + s;
+    n[d] = d < g.ndim ? g.nx[d] + s : 1;
+  }
   if (g.ndim > 2) {
-    const long zs = (long)ng * g.E[1] * g.E[0];
+    const long zs = (long)ng * E[1] * E[0];
     if (r < 2 * zs) {
       const int side = r >= zs;
       long q = r - (side ? zs : 0);
-      idx[0] = (int)(q % g.E[0]);
-      q /= g.E[0];
-      idx[1] = (int)(q % g.E[1]);
-      idx[2] = (int)(q / g.E[1]) + (side ? g.hi[2] + 1 : 0);
+      idx[0] = (int)(q % E[0]);
+      q /= E[0];
+      idx[1] = (int)(q % E[1]);
+      idx[2] = (int)(q / E[1]) + (side ? ng + n[2] : 0);
       return;
     }
     r -= 2 * zs;
   }
-  const int nk = g.ndim > 2 ? g.nx[2] : 1;
   if (g.ndim > 1) {
-    const long ys = (long)ng * g.E[0] * nk;
+    const long ys = (long)ng * E[0] * n[2];
     if (r < 2 * ys) {
       const int side = r >= ys;
       long q = r - (side ? ys : 0);
-      idx[0] = (int)(q % g.E[0]);
-      q /= g.E[0];
-      idx[1] = (int)(q % ng) + (side ? g.hi[1] + 1 : 0);
+      idx[0] = (int)(q % E[0]);
+      q /= E[0];
+      idx[1] = (int)(q % ng) + (side ? ng + n[1] : 0);
       idx[2] = g.lo[2] + (int)(q / ng);
       return;
     }
     r -= 2 * ys;
   }
-  const int nj = g.ndim > 1 ? g.nx[1] : 1;
   const int i = (int)(r % ng);
   long q = r / ng;
   const int side = (int)(q % 2);
   q /= 2;
-  idx[0] = i + (side ? g.hi[0] + 1 : 0);
-  idx[1] = g.lo[1] + (int)(q % nj);
-  idx[2] = g.lo[2] + (int)(q / nj);
+  idx[0] = i + (side ? ng + n[0] : 0);
+  idx[1] = g.lo[1] + (int)(q % n[1]);
+  idx[2] = g.lo[2] + (int)(q / n[1]);
 }
 
 // loads/stores through the global address space (the pack tables hold generic

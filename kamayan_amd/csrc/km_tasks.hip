@@ -505,39 +505,35 @@ __global__ void __launch_bounds__(256) k_ghost_cells(km_pack_t u, UniformGrid gr
   const int srcb = gr.first + (sb[2] * gr.nb[1] + sb[1]) * gr.nb[0] + sb[0];
   const long dc = g.c(idx[2], idx[1], idx[0]), scx = g.c(src[2], src[1], src[0]);
   const bool fl0 = flip[0], fl1 = flip[1], fl2 = flip[2];
-  for (int v = v0; v < v1; ++v) {
-    double val = cell_ptr(u, srcb, v)[scx];
-    int ax = -1;
-    if (v >= 1 && v < 4) ax = v - 1;
-    else if (v >= nc && v < nc + 3) ax = v - nc;
-    else if (nc == 8 && v >= 5 && v < 8) ax = v - 5;
-    if ((ax == 0 && fl0) || (ax == 1 && fl1) || (ax == 2 && fl2)) val = -val;
-    cell_ptr(u, b, v)[dc] = val;
+  // variables go in chunks whose loads are all issued before their stores
+  constexpr int kChunk = 4;
+  for (int v = v0; v < v1; v += kChunk) {
+    double x[kChunk];
+#pragma unroll
+    for (int q = 0; q < kChunk; ++q) x[q] = v + q < v1 ? cell_ptr(u, srcb, v + q)[scx] : 0.0;
+#pragma unroll
+    for (int q = 0; q < kChunk; ++q) {
+      const int w = v + q;
+      if (w >= v1) continue;
+      int ax = -1;
+      if (w >= 1 && w < 4) ax = w - 1;
+      else if (w >= nc && w < nc + 3) ax = w - nc;
+      else if (nc == 8 && w >= 5 && w < 8) ax = w - 5;
+      const bool f = (ax == 0 && fl0) || (ax == 1 && fl1) || (ax == 2 && fl2);
+      cell_ptr(u, b, w)[dc] = f ? -x[q] : x[q];
+    }
   }
 }
 
 __global__ void __launch_bounds__(256) k_ghost_faces(km_pack_t u, UniformGrid gr0, int face,
                                                      Instances in) {
   const Geo g(u);
-  const long n = g.nstag();
-  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long n = ghost_count(g, face);
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (t >= n * u.nblocks) return;
   const int b = (int)(t / n);
-  long r = t % n;
   int idx[3];
-  idx[0] = (int)(r % g.S[0]);
-  r /= g.S[0];
-  idx[1] = (int)(r % g.S[1]);
-  idx[2] = (int)(r / g.S[1]);
-  bool ghost = false;
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const int ext = d < g.ndim ? g.E[d] + (d == face ? 1 : 0) : 1;
-    if (idx[d] >= ext) return;
-    const int hi = g.hi[d] + (d == face ? 1 : 0);
-    if (d < g.ndim && (idx[d] < g.lo[d] || idx[d] > hi)) ghost = true;
-  }
-  if (!ghost) return;
+  ghost_cell(g, t % n, idx, face);
   const UniformGrid gr = in.inst ? in.grids[in.inst[b]] : gr0;
   const int lb = b - gr.first;
   const int bl[3] = {lb % gr.nb[0], (lb / gr.nb[0]) % gr.nb[1], lb / (gr.nb[0] * gr.nb[1])};
@@ -725,8 +721,8 @@ void launch_ghost_fill(const UniformGrid &gr, const km_pack_t &u, int v0, int v1
                        u, gr, v0, v1, nc, in);
   if (faces)
     for (int f = 0; f < (g.ndim > 2 ? 3 : 2); ++f)
-      hipLaunchKernelGGL(k_ghost_faces, dim3(nblk(g.nstag() * u.nblocks)), dim3(256), 0, st, u,
-                         gr, f, in);
+      hipLaunchKernelGGL(k_ghost_faces, dim3(nblk(ghost_count(g, f) * u.nblocks)), dim3(256), 0,
+                         st, u, gr, f, in);
 }
 
 }  // namespace km

@@ -20,6 +20,9 @@ MESHES = [
     # the smallest blocks the reference's own multilevel tests use: 8 cells, nghost 4
     (2, 8, 4, 3, (0.4, 0.55, 0.5), 0.1, "outflow"),
     (3, 8, 4, 2, (0.3, 0.3, 0.6), 0.12, "periodic"),
+    # odd ghost widths: coarse parents with one child inside the ghost layer
+    (2, 8, 3, 2, (0.1, 0.45, 0.5), 0.2, "reflect"),
+    (3, 8, 3, 1, (0.15, 0.3, 0.8), 0.15, "outflow"),
 ]
 
 
