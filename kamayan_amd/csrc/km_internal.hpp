@@ -65,6 +65,11 @@ void launch_calculate_fluxes(const km_cfg_t &cfg, const km_pack_t &u, const Box 
 void launch_calculate_emf(const km_pack_t &u, hipStream_t st);
 void launch_fluxes_to_dudt(const km_cfg_t &cfg, const km_pack_t &u, const km_pack_t &dudt,
                            hipStream_t st);
+// FluxesToDuDt, ApplyDuDt and PreparePrimitive of one stage, with the stage's dt on the
+// device (dt_dev, indexed by inst when given)
+void launch_update(const km_cfg_t &cfg, const km_pack_t &base, const km_pack_t &u0,
+                   const km_pack_t &u1, const km_pack_t &dudt, double beta, hipStream_t st,
+                   const double *dt_dev, const int *inst);
 void launch_apply_dudt(const km_cfg_t &cfg, const km_pack_t &base, const km_pack_t &u0,
                        const km_pack_t &u1, const km_pack_t &dudt, double beta, double dt,
                        hipStream_t st, const double *dt_dev = nullptr, const int *inst = nullptr);

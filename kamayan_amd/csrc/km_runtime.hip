@@ -361,11 +361,9 @@ int task_stage(km_sim_t &s, int stage, double beta, double dt) {
     launch_amr_flux_correct(s.agrid, p0, s.nc, s.stream);
     if (ct) launch_amr_emf_correct(s.agrid, p0, s.stream);
   }
-  launch_fluxes_to_dudt(cfg, p0, s.dudt.pack, s.stream);
+  // FluxesToDuDt, ApplyDuDt, PreparePrimitive
   (void)dt;
-  launch_apply_dudt(cfg, s.base.pack, p0, md1.pack, s.dudt.pack, beta, 0.0, s.stream, s.dt_d,
-                    s.inst_d);
-  launch_prepare_primitive(cfg, md1.pack, s.stream);
+  launch_update(cfg, s.base.pack, p0, md1.pack, s.dudt.pack, beta, s.stream, s.dt_d, s.inst_d);
   if (stage == cfg.nstages) {
     launch_estimate_dt(cfg, s.base.pack, s.est_d, s.stream, s.inst_d, s.ninst);
     if (ct) launch_fill_divb(cfg, s.base.pack, s.stream);

@@ -300,6 +300,34 @@ __device__ __forceinline__ void face_average_b(const km_pack_t &u, const Geo &g,
   }
 }
 
+// primitives of one zone from its conserved values U (cell-centred B in U[5..7] for MHD)
+template <int MHD>
+__device__ __forceinline__ void primitive_zone(const km_pack_t &u, int b, long c, const Eos &eos,
+                                               const double *U) {
+  constexpr int nc = HV<MHD>::NC;
+  const double dens = U[0];
+  const double idens = 1.0 / dens;
+  double emag = 0., ekin = 0.;
+#pragma unroll
+  for (int dir = 0; dir < 3; dir++) {
+    const double v = idens * U[1 + dir];
+    cell_ptr(u, b, nc + dir)[c] = v;
+    ekin += v * v;
+    if constexpr (MHD) {
+      const double B = U[5 + dir];
+      emag += B * B;
+    }
+  }
+  ekin *= 0.5 * dens;
+  emag *= 0.5;
+  const double eint = U[4] - ekin - emag;
+  const double sie = idens * eint;
+  cell_ptr(u, b, nc + 5)[c] = sie;
+  cell_ptr(u, b, nc + 3)[c] = eos_pres(eos, dens, sie);
+  cell_ptr(u, b, nc + 6)[c] = eos_temp(eos, dens, sie);
+  cell_ptr(u, b, nc + 4)[c] = eos_bmod(eos, dens, sie);
+}
+
 // hydro PreparePrimitive_impl (primconsflux.cpp:82-142) + eos EosWrapped(ener)
 template <int MHD>
 __global__ void __launch_bounds__(256) k_prepare_primitive(km_pack_t u, Eos eos) {
@@ -309,27 +337,52 @@ __global__ void __launch_bounds__(256) k_prepare_primitive(km_pack_t u, Eos eos)
   if (!interior_cell(g, u.nblocks, b, k, j, i)) return;
   face_average_b<MHD>(u, g, b, k, j, i);
   const long c = g.c(k, j, i);
-  const double dens = cell_ptr(u, b, 0)[c];
-  const double idens = 1.0 / dens;
-  double emag = 0., ekin = 0.;
+  double U[nc];
 #pragma unroll
-  for (int dir = 0; dir < 3; dir++) {
-    const double v = idens * cell_ptr(u, b, 1 + dir)[c];
-    cell_ptr(u, b, nc + dir)[c] = v;
-    ekin += v * v;
-    if constexpr (MHD) {
-      const double B = cell_ptr(u, b, 5 + dir)[c];
-      emag += B * B;
+  for (int v = 0; v < nc; ++v) U[v] = cell_ptr(u, b, v)[c];
+  primitive_zone<MHD>(u, b, c, eos, U);
+}
+
+// FluxesToDuDt (grid_update.hpp:20-57), the cell part of ApplyDuDt (grid.cpp:213-241)
+// and, for hydro, PreparePrimitive of one interior zone in a single pass: the zone's
+// dU/dt and new conserved state stay in registers, with the arithmetic of k_flux_div,
+// k_apply_cell and k_prepare_primitive unchanged. MHD keeps the primitive pass apart,
+// since cell-centred B comes from face B that k_apply_face updates afterwards.
+template <int MHD>
+__global__ void __launch_bounds__(256) k_update_cell(km_pack_t base, km_pack_t u0, km_pack_t u1,
+                                                     double beta, double dt, const double *dtp,
+                                                     const int *inst, Eos eos) {
+  constexpr int nc = HV<MHD>::NC;
+  const Geo g(u0);
+  int b, k, j, i;
+  if (!interior_cell(g, u0.nblocks, b, k, j, i)) return;
+  if (dtp) dt = dtp[inst ? inst[b] : 0];
+  const km_block_t &blk = u0.blocks[b];
+  const long c = g.c(k, j, i);
+  const long sc[3] = {g.sc[0], g.sc[1], g.sc[2]};
+  double du[nc], U[nc];
+#pragma unroll
+  for (int var = 0; var < nc; ++var) {
+    double d = 0.;
+#pragma unroll
+    for (int dir = 0; dir < 3; ++dir) {
+      if (dir < g.ndim) {
+        const double *F = flux_ptr(u0, dir, b, var);
+        d -= (blk.area[dir] * F[c + sc[dir]] - blk.area[dir] * F[c]);
+      }
     }
+    du[var] = d / blk.vol;
   }
-  ekin *= 0.5 * dens;
-  emag *= 0.5;
-  const double eint = cell_ptr(u, b, 4)[c] - ekin - emag;
-  const double sie = idens * eint;
-  cell_ptr(u, b, nc + 5)[c] = sie;
-  cell_ptr(u, b, nc + 3)[c] = eos_pres(eos, dens, sie);
-  cell_ptr(u, b, nc + 6)[c] = eos_temp(eos, dens, sie);
-  cell_ptr(u, b, nc + 4)[c] = eos_bmod(eos, dens, sie);
+#pragma unroll
+  for (int var = 0; var < nc; ++var)
+    U[var] = beta * cell_ptr(base, b, var)[c] + (1.0 - beta) * cell_ptr(u0, b, var)[c];
+#pragma unroll
+  for (int var = 0; var < nc; ++var) {
+    cell_ptr(u0, b, var)[c] = U[var];
+    U[var] = U[var] + beta * dt * du[var];
+    cell_ptr(u1, b, var)[c] = U[var];
+  }
+  if constexpr (!MHD) primitive_zone<0>(u1, b, c, eos, U);
 }
 
 // eos PrepareConserved(dens_pres) then hydro ConvertToConserved_impl
@@ -672,6 +725,38 @@ void launch_apply_dudt(const km_cfg_t &cfg, const km_pack_t &base, const km_pack
       hipLaunchKernelGGL(k_apply_face<2>, dim3(nblk(face_count(g, 2, u0.nblocks))), dim3(256),
                          0, st, base, u0, u1, dudt, beta, dt, dt_dev, inst);
   }
+}
+
+void launch_update(const km_cfg_t &cfg, const km_pack_t &base, const km_pack_t &u0,
+                   const km_pack_t &u1, const km_pack_t &dudt, double beta, hipStream_t st,
+                   const double *dt_dev, const int *inst) {
+  const Geo g(u0);
+  const Eos eos = make_eos(cfg);
+  const unsigned grid = nblk(interior_count(g, u0.nblocks));
+  if (!cfg.mhd) {
+    hipLaunchKernelGGL(k_update_cell<0>, dim3(grid), dim3(256), 0, st, base, u0, u1, beta, 0.0,
+                       dt_dev, inst, eos);
+    return;
+  }
+  hipLaunchKernelGGL(k_update_cell<1>, dim3(grid), dim3(256), 0, st, base, u0, u1, beta, 0.0,
+                     dt_dev, inst, eos);
+  if (g.ndim >= 2) {
+    hipLaunchKernelGGL(k_stokes<0>, dim3(nblk(face_count(g, 0, u0.nblocks))), dim3(256), 0, st,
+                       u0, dudt);
+    hipLaunchKernelGGL(k_stokes<1>, dim3(nblk(face_count(g, 1, u0.nblocks))), dim3(256), 0, st,
+                       u0, dudt);
+    if (g.ndim > 2)
+      hipLaunchKernelGGL(k_stokes<2>, dim3(nblk(face_count(g, 2, u0.nblocks))), dim3(256), 0, st,
+                         u0, dudt);
+    hipLaunchKernelGGL(k_apply_face<0>, dim3(nblk(face_count(g, 0, u0.nblocks))), dim3(256), 0,
+                       st, base, u0, u1, dudt, beta, 0.0, dt_dev, inst);
+    hipLaunchKernelGGL(k_apply_face<1>, dim3(nblk(face_count(g, 1, u0.nblocks))), dim3(256), 0,
+                       st, base, u0, u1, dudt, beta, 0.0, dt_dev, inst);
+    if (g.ndim > 2)
+      hipLaunchKernelGGL(k_apply_face<2>, dim3(nblk(face_count(g, 2, u0.nblocks))), dim3(256),
+                         0, st, base, u0, u1, dudt, beta, 0.0, dt_dev, inst);
+  }
+  hipLaunchKernelGGL(k_prepare_primitive<1>, dim3(grid), dim3(256), 0, st, u1, eos);
 }
 
 void launch_prepare_primitive(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st) {
