@@ -189,6 +189,40 @@ def _secondary(km, args):
         ref = km_oracle.CpuSimulation(cfg.c, desc, lay, leaves=leaves)
         ref.set_primitive(cell)
         sample_cycles = 1
+    elif args.workload == "sedov_amr_adaptive":
+        # the same start, then tagging on pres (Loehner, kamayan/refinement0) and remesh
+        # after every cycle, as the reference's adaptive Sedov runs (sedov.py:38-44)
+        from kamayan_amd._abi import amr_adapt, amr_criterion
+        cfg, desc, lay, leaves, cell = _sedov_setup(km)
+        adapt = amr_adapt(3, [amr_criterion(km.slot("PRES"), 3)], derefine_count=10)
+        stage_bytes = 8 * 3 * 5
+        nst = 2
+        sim = km.Simulation(cfg, desc, lay, path=km.PATH_TASKS, leaves=leaves)
+        sim.set_primitive(cell)
+        sim.set_adaptive(adapt)
+        sim.advance(args.warmup)
+        counts = []
+        km.check(km.lib().km_synchronize(), "sync")
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            counts.append(len(sim.leaves()))
+            sim.advance(1)
+        km.check(km.lib().km_synchronize(), "sync")
+        dt = time.perf_counter() - t0
+        final = sim.leaves()
+        sim.close()
+        zones = sum(counts) * 16 ** 3 / args.steps  # mean zones per timed cycle
+        lv = np.bincount(final[:, 0], minlength=3)
+        out["config"] = {"workload": "sedov3d_amr_adaptive_root64c_16b_numlevel3_plm_mc_hllc_rk2",
+                         "zones": zones, "leaves_first": counts[0], "leaves_last": int(len(final)),
+                         "leaves_per_level_last": [int(x) for x in lv], "block": [16] * 3,
+                         "nghost": 4, "refinement": "loehner on pres, derefine_count 10",
+                         "path": "tasks", "parallelism": "single GPU"}
+        ref = km_oracle.CpuSimulation(cfg.c, desc, lay, leaves=leaves)
+        ref.set_primitive(cell)
+        ref.set_adaptive(adapt)
+        sample_cycles = 1
+        cpu_zones = len(leaves) * 16 ** 3
     elif args.workload == "kh_ensemble":
         sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
         rng = np.random.default_rng(20261207)
@@ -236,7 +270,7 @@ def _secondary(km, args):
                              "kernel": "whole cycle (all kernels)",
                              "algorithmic_bytes_per_zone_cycle": stage_bytes * nst}})
     if not args.no_cpu:
-        if args.workload != "kh_ensemble":
+        if args.workload not in ("kh_ensemble", "sedov_amr_adaptive"):
             cpu_zones = zones
         t0 = time.perf_counter()
         ref.advance(sample_cycles)
@@ -260,7 +294,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--workload", default="hydro3d",
-                    choices=["hydro3d", "mhd3d", "sedov_amr", "kh_ensemble"])
+                    choices=["hydro3d", "mhd3d", "sedov_amr", "sedov_amr_adaptive",
+                             "kh_ensemble"])
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="after the run, copy BYTES between two device buffers with a plain "
                          "copy kernel (the reference for scaling HBM byte counters)")

@@ -256,7 +256,6 @@ int km_sim_profile(km_sim_t *sim, int32_t on);
 int km_sim_kernel_times(km_sim_t *sim, double *ms, int32_t *count, int32_t nkinds);
 /* Threads per workgroup of the fused sweeps (64..256, multiple of 64). */
 int km_sim_set_tile(km_sim_t *sim, int32_t threads);
-
 /* ---- problem generators (host-side; fill primitives on the interior) -------------- */
 /* names: "isentropic_vortex" (isentropic_vortex.cpp:105-175), "smooth3d", "mhd_random",
  * "kelvin_helmholtz", "sedov" (sedov.cpp:75-143, 3-D form), "mhd_blast"
@@ -281,6 +280,74 @@ int km_amr_blocks(const km_mesh_desc_t *mesh, const int32_t *leaves, int32_t nle
  * too small returns KM_EINVAL with the count needed. */
 int km_amr_refine_ball(const km_mesh_desc_t *mesh, int32_t lmax, const double center[3],
                        double radius, int32_t *leaves, int32_t *nleaves);
+
+/* ---- adaptive refinement: tagging task (kamayan_driver.cpp:178-181) and remesh ------ */
+enum { KM_AMR_DEREFINE = -1, KM_AMR_SAME = 0, KM_AMR_REFINE = 1 };
+/* One <kamayan/refinementN> block (grid.cpp:96-124) with method = loehner. */
+typedef struct km_amr_criterion_t {
+  int32_t var;         /* cell slot of field (+ comp4), e.g. KM_PRES(mhd) for field = pres */
+  int32_t max_level;   /* max_level, default numlevel (grid.cpp:112-113) */
+  double refine_tol;   /* refine_tol, default 0.8 */
+  double derefine_tol; /* derefine_tol, default 0.2 */
+  double filter;       /* filter, default 0.01 */
+} km_amr_criterion_t;
+#define KM_AMR_MAX_CRIT 4
+typedef struct km_amr_adapt_t {
+  int32_t numlevel;       /* parthenon/mesh/numlevel, default 1 */
+  int32_t derefine_count; /* parthenon/mesh/derefine_count, default 10 */
+  int32_t ncrit;          /* criteria in use */
+  km_amr_criterion_t crit[KM_AMR_MAX_CRIT];
+} km_amr_adapt_t;
+/* numlevel 1, derefine_count 10, no criteria. */
+int km_amr_adapt_default(km_amr_adapt_t *adapt);
+/* Set one deck key for physics/MHD = mhd: "parthenon/mesh/numlevel",
+ * "parthenon/mesh/derefine_count", "kamayan/refinementN/<key>" with key field (dens,
+ * momentum, ener, magc, velocity, pres, "bulk modulus", eint, temp), comp4, method
+ * (loehner; Parthenon's derivative_order_1/2 are not provided: KM_EINVAL), refine_tol,
+ * derefine_tol, filter, max_level. Criterion N exists once its field is set. */
+int km_amr_adapt_set(km_amr_adapt_t *adapt, int32_t mhd, const char *block_key,
+                     const char *value);
+/* grid::AMRLoehner::operator() (grid_refinement.cpp:30-152): per block, the largest
+ * Loehner error over the interior of cell slot crit->var gives flag = refine (error >
+ * refine_tol and level < max_level), derefine (error < derefine_tol) or same, and
+ * tags_dev[b] = max(tags_dev[b], flag) (Kokkos ScatterMax). reset != 0 first sets every
+ * tag to KM_AMR_DEREFINE, Parthenon's starting value (CheckAllRefinement). Reads two ghost
+ * layers, so ghost zones must be filled. Blocks the host until the tags are written. */
+int km_amr_tag_loehner(const km_pack_t *u, const km_amr_criterion_t *crit, int32_t *tags_dev,
+                       int32_t reset, void *stream);
+/* Parthenon's MeshRefinement::SetRefinement for every leaf (refine below level
+ * numlevel-1; derefine after derefine_count consecutive derefine tags and only without a
+ * finer neighbour), then Mesh::UpdateMeshBlockTree: leaves whose 2^d siblings are all
+ * flagged merge into their parent, flagged leaves split (refining coarser neighbours first
+ * to keep touching leaves within one level); refinements go in increasing level order,
+ * then merges in decreasing level order, each merge refused when a leaf two levels finer
+ * would touch the parent. Host arrays. In: leaves[4*nleaves], tags[nleaves];
+ * deref_count[nleaves] is updated in place. Out: new_leaves (capacity *nnew in, count
+ * out) in Morton order over the finest level, new_deref per new leaf (kept leaves keep
+ * theirs, new ones start at 0), origin[9*i] = {0, old leaf} kept, {1, old parent} split,
+ * {2, old children x1-fastest} merged; *changed = 0 when nothing changes. */
+int km_amr_update_tree(const km_mesh_desc_t *mesh, int32_t numlevel, int32_t derefine_count,
+                       const int32_t *leaves, int32_t nleaves, const int32_t *tags,
+                       int32_t *deref_count, int32_t *new_leaves, int32_t *new_deref,
+                       int32_t *origin, int32_t *nnew, int32_t *changed);
+/* Block data of the new mesh from the old one (Parthenon RedistributeAndRefineMeshBlocks)
+ * for cell slots [v0, v1): a kept leaf copies every zone, a split leaf's children take
+ * ProlongateSharedMinMod (refinement_operations.hpp:123-268) of the parent's zones (its
+ * ghost zones give the slopes at its edges) over their interior, a merged parent the
+ * RestrictAverage (:52-121) of its children over its interior. origin_dev: the origin
+ * table of km_amr_update_tree in device memory. Both packs share ndim, ng and nx. */
+int km_amr_remesh(const km_pack_t *old_u, const km_pack_t *new_u, const int32_t *origin_dev,
+                  int32_t v0, int32_t v1, void *stream);
+/* Adaptive refinement of a block-tree simulation (hydro): after every cycle each leaf is
+ * tagged by the criteria (km_amr_tag_loehner on the filled base state), the tree is
+ * updated (km_amr_update_tree) and, when it changed, every cell field moves to the new
+ * leaves (km_amr_remesh), ghost zones are refilled and dt is re-estimated on the new
+ * mesh before the time-step rule (EvolutionDriver::Execute and
+ * InitializeBlockTimeStepsAfterAMR in Parthenon). Resets every derefine count. */
+int km_sim_set_adaptive(km_sim_t *sim, const km_amr_adapt_t *adapt);
+/* Current leaves of a block-tree simulation in block order: *nleaves capacity in (leaf
+ * entries of 4), count out; host buffers of get/set_state follow this count. */
+int km_sim_amr_leaves(const km_sim_t *sim, int32_t *leaves, int32_t *nleaves);
 
 /* ---- device memory helpers (so callers need no other runtime) --------------------- */
 int km_device_count(int32_t *count);

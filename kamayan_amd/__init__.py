@@ -13,8 +13,8 @@ import pathlib
 import numpy as np
 
 from ._abi import (BC, KM_OK, LIMITER, MHD, PATH_FUSED, PATH_TASKS, PRECISION, RECON,
-                   RIEMANN, KmBlock, KmCfg, KmMeshDesc, KmPack, Layout, f64ptr, mesh_desc,
-                   ncons, nvar, slot)
+                   RIEMANN, KmAmrAdapt, KmAmrCriterion, KmBlock, KmCfg, KmMeshDesc, KmPack,
+                   Layout, amr_adapt, amr_criterion, f64ptr, mesh_desc, ncons, nvar, slot)
 
 HERE = pathlib.Path(__file__).resolve().parent
 # KAMAYAN_AMD_LIB selects another build of the library (e.g. a tuning variant)
@@ -23,7 +23,8 @@ LIB_PATH = pathlib.Path(os.environ.get("KAMAYAN_AMD_LIB", HERE / "libkm_hydro.so
 __all__ = [
     "lib", "Config", "Simulation", "DevicePack", "problem_fill", "Layout", "mesh_desc",
     "KmCfg", "KmPack", "KmBlock", "KmMeshDesc", "ncons", "nvar", "slot", "PATH_TASKS",
-    "PATH_FUSED", "LIB_PATH", "device_count", "block_table",
+    "PATH_FUSED", "LIB_PATH", "device_count", "block_table", "AmrAdapt", "amr_adapt",
+    "amr_criterion", "amr_update_tree", "amr_tag_loehner", "amr_remesh",
 ]
 
 _lib = None
@@ -88,6 +89,14 @@ def lib() -> C.CDLL:
         "km_memcpy_d2h": [vp, vp, C.c_size_t],
         "km_synchronize": [],
         "km_copy_device": [vp, vp, C.c_size_t, vp],
+        "km_amr_adapt_default": [P(KmAmrAdapt)],
+        "km_amr_adapt_set": [P(KmAmrAdapt), i32, C.c_char_p, C.c_char_p],
+        "km_amr_tag_loehner": [P(KmPack), P(KmAmrCriterion), vp, i32, vp],
+        "km_amr_update_tree": [P(KmMeshDesc), i32, i32, P(i32), i32, P(i32), P(i32), P(i32),
+                               P(i32), P(i32), P(i32), P(i32)],
+        "km_amr_remesh": [P(KmPack), P(KmPack), vp, i32, i32, vp],
+        "km_sim_set_adaptive": [vp, P(KmAmrAdapt)],
+        "km_sim_amr_leaves": [vp, P(i32), P(i32)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -165,6 +174,52 @@ def amr_blocks(desc: KmMeshDesc, leaves: np.ndarray):
     return arr
 
 
+class AmrAdapt:
+    """km_amr_adapt_t built from the reference's deck keys (parthenon/mesh/numlevel,
+    parthenon/mesh/derefine_count, kamayan/refinementN/...)."""
+
+    def __init__(self, mhd: int = 0, deck_keys=None):
+        self.mhd = mhd
+        self.c = KmAmrAdapt()
+        check(lib().km_amr_adapt_default(C.byref(self.c)), "km_amr_adapt_default")
+        for k, v in (deck_keys or {}).items():
+            self.set(k, v)
+
+    def set(self, key: str, value) -> "AmrAdapt":
+        check(lib().km_amr_adapt_set(C.byref(self.c), self.mhd, key.encode(),
+                                     str(value).encode()), key)
+        return self
+
+
+def amr_update_tree(desc: KmMeshDesc, numlevel: int, derefine_count: int, leaves, tags,
+                    deref: np.ndarray):
+    """Refinement flags + tree update on the host (km_amr_update_tree); deref (int32) is
+    updated in place. Returns (new_leaves, new_deref, origin[n, 9], changed)."""
+    leaves = np.ascontiguousarray(leaves, dtype=np.int32)
+    tags = np.ascontiguousarray(tags, dtype=np.int32)
+    assert deref.dtype == np.int32 and deref.flags["C_CONTIGUOUS"]
+    cap = 8 * len(leaves) + 64
+    nl = np.zeros((cap, 4), np.int32)
+    nd = np.zeros(cap, np.int32)
+    org = np.zeros((cap, 9), np.int32)
+    n, ch = C.c_int32(cap), C.c_int32(0)
+    check(lib().km_amr_update_tree(C.byref(desc), numlevel, derefine_count, _i32(leaves),
+                                   len(leaves), _i32(tags), _i32(deref), _i32(nl), _i32(nd),
+                                   _i32(org), C.byref(n), C.byref(ch)), "km_amr_update_tree")
+    k = n.value
+    return nl[:k].copy(), nd[:k].copy(), org[:k].copy(), bool(ch.value)
+
+
+def amr_tag_loehner(pack: "DevicePack", crit: KmAmrCriterion, tags_dev: int, reset: bool):
+    check(lib().km_amr_tag_loehner(C.byref(pack.pack), C.byref(crit), C.c_void_p(tags_dev),
+                                   1 if reset else 0, None), "km_amr_tag_loehner")
+
+
+def amr_remesh(old: "DevicePack", new: "DevicePack", origin_dev: int, v0: int, v1: int):
+    check(lib().km_amr_remesh(C.byref(old.pack), C.byref(new.pack), C.c_void_p(origin_dev), v0,
+                              v1, None), "km_amr_remesh")
+
+
 def problem_fill(name: str, cfg: Config, desc: KmMeshDesc, layout: Layout, seed: int = 0,
                  params=None, leaves=None):
     """Primitive initial state (cell array, face arrays or None) of a problem generator;
@@ -237,8 +292,25 @@ class Simulation:
                 fp[d] = faces[d].ctypes.data
         check(lib().km_sim_set_primitive(self.h, f64ptr(cell), fp), "set_primitive")
 
+    def set_adaptive(self, adapt):
+        """Adaptive refinement (AmrAdapt or KmAmrAdapt) of a block-tree simulation."""
+        c = adapt.c if isinstance(adapt, AmrAdapt) else adapt
+        check(lib().km_sim_set_adaptive(self.h, C.byref(c)), "set_adaptive")
+
+    def leaves(self) -> np.ndarray:
+        """Current leaves; also resizes `layout` to the current leaf count."""
+        n = C.c_int32(0)
+        lib().km_sim_amr_leaves(self.h, None, C.byref(n))
+        out = np.zeros((max(n.value, 1), 4), np.int32)
+        check(lib().km_sim_amr_leaves(self.h, _i32(out), C.byref(n)), "amr_leaves")
+        L = self.layout
+        self.layout = Layout(L.ndim, L.ng, L.nx, L.nb, nleaves=n.value)
+        return out[: n.value]
+
     def state(self):
         mhd = self.cfg.c.mhd
+        if getattr(self, "_leaves", None) is not None:
+            self.leaves()
         cell = np.zeros(self.layout.cell_shape(nvar(mhd)))
         fp = (C.c_void_p * 3)()
         faces = None

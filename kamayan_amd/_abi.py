@@ -76,6 +76,41 @@ class KmMeshDesc(C.Structure):
     ]
 
 
+KM_AMR_DEREFINE, KM_AMR_SAME, KM_AMR_REFINE = -1, 0, 1
+KM_AMR_MAX_CRIT = 4
+
+
+class KmAmrCriterion(C.Structure):
+    _fields_ = [
+        ("var", C.c_int32), ("max_level", C.c_int32), ("refine_tol", C.c_double),
+        ("derefine_tol", C.c_double), ("filter", C.c_double),
+    ]
+
+
+class KmAmrAdapt(C.Structure):
+    _fields_ = [
+        ("numlevel", C.c_int32), ("derefine_count", C.c_int32), ("ncrit", C.c_int32),
+        ("crit", KmAmrCriterion * KM_AMR_MAX_CRIT),
+    ]
+
+
+def amr_criterion(var: int, max_level: int, refine_tol: float = 0.8,
+                  derefine_tol: float = 0.2, filter: float = 0.01) -> KmAmrCriterion:
+    """One <kamayan/refinementN> Loehner criterion (defaults of grid.cpp:107-113)."""
+    c = KmAmrCriterion()
+    c.var, c.max_level = var, max_level
+    c.refine_tol, c.derefine_tol, c.filter = refine_tol, derefine_tol, filter
+    return c
+
+
+def amr_adapt(numlevel: int, criteria, derefine_count: int = 10) -> KmAmrAdapt:
+    a = KmAmrAdapt()
+    a.numlevel, a.derefine_count, a.ncrit = numlevel, derefine_count, len(criteria)
+    for i, c in enumerate(criteria):
+        a.crit[i] = c
+    return a
+
+
 @dataclass
 class Layout:
     """Shapes of the block-pack arrays of a mesh: nb is the (root) block grid; a block-tree

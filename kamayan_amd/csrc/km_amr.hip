@@ -696,4 +696,172 @@ void launch_amr_flux_correct(const AmrGrid &t, const km_pack_t &u, int nc, hipSt
                      t, nc, ntmax);
 }
 
+// ---- adaptive refinement ----------------------------------------------------------
+// grid::AMRLoehner::operator() (grid_refinement.cpp:61-151), one lane per interior zone:
+// first derivatives d_p = 0.5 (u(+e_p) - u(-e_p)) / dx_p are formed where the second
+// differences need them (the same values the reference stores in FirstDer), the zone's
+// num/den is folded into its block's maximum with a 64-bit atomicMax on the bit pattern
+// (non-negative doubles order like their patterns; a NaN ratio is skipped, as Kokkos::max
+// skips it), and k_amr_tag turns each block maximum into its tag.
+__global__ void __launch_bounds__(256) k_amr_loehner(km_pack_t u, int var, double filter,
+                                                     unsigned long long *err2) {
+  const Geo g(u);
+  const long per = (long)g.nx[0] * (g.ndim > 1 ? g.nx[1] : 1) * (g.ndim > 2 ? g.nx[2] : 1);
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= per * u.nblocks) return;
+  const int b = (int)(t / per);
+  long r = t % per;
+  const int i = g.lo[0] + (int)(r % g.nx[0]);
+  r /= g.nx[0];
+  const int j = g.lo[1] + (int)(r % (g.ndim > 1 ? g.nx[1] : 1));
+  const int k = g.lo[2] + (int)(r / (g.ndim > 1 ? g.nx[1] : 1));
+  const km_block_t &blk = u.blocks[b];
+  const double *q = cell_ptr(u, b, var);
+  const long c = g.c(k, j, i);
+  const int nd = g.ndim;
+  auto der = [&](int p, long z) { return 0.5 * (q[z + g.sc[p]] - q[z - g.sc[p]]) / blk.dx[p]; };
+  double num = 0.0, den = 1.e-12;
+  for (int qd = 0; qd < nd; ++qd)
+    for (int p = 0; p < nd; ++p) {
+      const double dp = der(p, c + g.sc[qd]), dm = der(p, c - g.sc[qd]);
+      const double a = 0.5 * (dp - dm) / blk.dx[qd];
+      num += a * a;
+      const double e = 0.5 * (kabs(dp) + kabs(dm)) / blk.dx[p] +
+                       filter *
+                           (kabs(q[c + g.sc[qd]]) + kabs(q[c - g.sc[qd]]) +
+                            kabs(q[c - g.sc[p]]) + kabs(q[c + g.sc[p]])) /
+                           (blk.dx[qd] * blk.dx[p]);
+      den += e * e;
+    }
+  if (den == 0.0) return;
+  const double v = num / den;
+  if (!(v > 0.0)) return;  // zero adds nothing to a maximum that starts at zero; NaN skipped
+  atomicMax(err2 + b, (unsigned long long)__double_as_longlong(v));
+}
+
+__global__ void k_amr_tag(const km_block_t *blocks, int nblocks, const unsigned long long *err2,
+                          km_amr_criterion_t cr, int *tags, int reset) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nblocks) return;
+  const double err = sqrt(__longlong_as_double((long long)err2[b]));
+  int flag = KM_AMR_SAME;
+  if (err > cr.refine_tol && blocks[b].level < cr.max_level) flag = KM_AMR_REFINE;
+  if (err < cr.derefine_tol) flag = KM_AMR_DEREFINE;
+  const int cur = reset ? KM_AMR_DEREFINE : tags[b];
+  tags[b] = cur > flag ? cur : flag;
+}
+
+__global__ void k_zero_u64(unsigned long long *p, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = 0ull;
+}
+
+void launch_amr_tag_loehner(const km_pack_t &u, const km_amr_criterion_t &crit, int *tags,
+                            bool reset, double *err2_scratch, hipStream_t st) {
+  const Geo g(u);
+  auto *e2 = reinterpret_cast<unsigned long long *>(err2_scratch);
+  const unsigned nb = (unsigned)((u.nblocks + 255) / 256);
+  hipLaunchKernelGGL(k_zero_u64, dim3(nb), dim3(256), 0, st, e2, u.nblocks);
+  const long n = (long)g.nx[0] * (g.ndim > 1 ? g.nx[1] : 1) * (g.ndim > 2 ? g.nx[2] : 1) *
+                 u.nblocks;
+  hipLaunchKernelGGL(k_amr_loehner, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, u,
+                     crit.var, crit.filter, e2);
+  hipLaunchKernelGGL(k_amr_tag, dim3(nb), dim3(256), 0, st, u.blocks, u.nblocks, e2, crit, tags,
+                     reset ? 1 : 0);
+}
+
+// RedistributeAndRefineMeshBlocks data movement: one lane per zone of a new block. A
+// kept leaf copies the zone; a split leaf's child takes ProlongateSharedMinMod of its
+// parent's zone C = G/2 with the parent's C -+ 1 along each axis (interior or ghost
+// zones of the parent); a merged parent takes the RestrictAverage of the 2^d zones of
+// the child holding 2C. Ghost zones of new leaves are left for the ghost fill.
+__global__ void __launch_bounds__(256) k_amr_remesh(km_pack_t o, km_pack_t nw,
+                                                    const int32_t *origin, int v0, int v1) {
+  const Geo g(nw);
+  const long n = g.ncell();
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= n * nw.nblocks) return;
+  const int b = (int)(t / n);
+  const long z = t % n;
+  int idx[3];
+  idx[0] = (int)(z % g.E[0]);
+  idx[1] = (int)((z / g.E[0]) % g.E[1]);
+  idx[2] = (int)(z / ((long)g.E[0] * g.E[1]));
+  const int32_t *org = origin + 9 * (long)b;
+  const int kind = org[0];
+  if (kind == 0) {
+    const int ob = org[1];
+    for (int v = v0; v < v1; ++v) cell_ptr(nw, b, v)[z] = cell_ptr(o, ob, v)[z];
+    return;
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    if (d < g.ndim && (idx[d] < g.lo[d] || idx[d] > g.hi[d])) return;
+  const km_block_t &blk = nw.blocks[b];
+  if (kind == 1) {
+    const int pb = org[1];
+    const km_block_t &par = o.blocks[pb];
+    int pc[3] = {0, 0, 0}, bit[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if (d >= g.ndim) continue;
+      const int G = blk.lx[d] * g.nx[d] + (idx[d] - g.lo[d]);
+      bit[d] = G & 1;
+      pc[d] = (G >> 1) - par.lx[d] * g.nx[d] + g.lo[d];
+    }
+    const long cc = g.c(pc[2], pc[1], pc[0]);
+    double df[3] = {0., 0., 0.};
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      if (d < g.ndim) df[d] = 0.5 * blk.dx[d];
+    for (int v = v0; v < v1; ++v) {
+      const double *q = cell_ptr(o, pb, v);
+      const double fc = q[cc];
+      double gr[3] = {0., 0., 0.};
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        if (d >= g.ndim) continue;
+        const double dxc = 2.0 * blk.dx[d];
+        gr[d] = grad_minmod(fc, q[cc - g.sc[d]], q[cc + g.sc[d]], dxc, dxc);
+      }
+      // the child formulas of ProlongateSharedGeneral (refinement_operations.hpp:241-259)
+      const double a = gr[0] * df[0], bb = gr[1] * df[1], c3 = gr[2] * df[2];
+      const int oi = bit[0], oj = bit[1], ok = bit[2];
+      double x;
+      if (!ok && !oj && !oi) x = fc - (a + bb + c3);
+      else if (!ok && !oj) x = fc + (a - bb - c3);
+      else if (!ok && !oi) x = fc - (a - bb + c3);
+      else if (!ok) x = fc + (a + bb - c3);
+      else if (!oj && !oi) x = fc - (a + bb - c3);
+      else if (!oj) x = fc + (a - bb + c3);
+      else if (!oi) x = fc - (a - bb - c3);
+      else x = fc + (a + bb + c3);
+      cell_ptr(nw, b, v)[z] = x;
+    }
+    return;
+  }
+  int ch = 0, f[3] = {0, 0, 0};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (d >= g.ndim) continue;
+    const int lc = idx[d] - g.lo[d];
+    const int half = (2 * lc) / g.nx[d];
+    ch |= half << d;
+    f[d] = 2 * lc - half * g.nx[d] + g.lo[d];
+  }
+  const int cb = org[1 + ch];
+  const double vol = o.blocks[cb].vol;
+  const long fz = g.c(f[2], f[1], f[0]);
+  for (int v = v0; v < v1; ++v) cell_ptr(nw, b, v)[z] = restrict_cell(cell_ptr(o, cb, v), vol, g, fz);
+}
+
+void launch_amr_remesh(const km_pack_t &old_u, const km_pack_t &new_u, const int32_t *origin,
+                       int v0, int v1, hipStream_t st) {
+  const Geo g(new_u);
+  const long n = g.ncell() * new_u.nblocks;
+  if (n == 0 || v0 >= v1) return;
+  hipLaunchKernelGGL(k_amr_remesh, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, old_u,
+                     new_u, origin, v0, v1);
+}
+
 }  // namespace km

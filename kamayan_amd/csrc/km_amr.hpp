@@ -210,6 +210,17 @@ std::string refine_ball(const km_mesh_desc_t &m, int lmax, const double c[3], do
                         std::vector<int32_t> &leaves);
 // smallest block size (cells per active dimension) a refined mesh accepts
 int amr_min_cells(int ng);
+// refinement flags and tree update of an adaptive mesh (km_amr_update_tree); deref is
+// updated in place; origin rows of 9 as documented at km_amr_update_tree
+std::string update_amr_tree(const km_mesh_desc_t &mesh, int numlevel, int threshold,
+                            const int32_t *leaves, int n, const int32_t *tags,
+                            int32_t *deref, std::vector<int32_t> &out_leaves,
+                            std::vector<int32_t> &out_deref, std::vector<int32_t> &origin,
+                            bool &changed);
+void launch_amr_tag_loehner(const km_pack_t &u, const km_amr_criterion_t &crit, int *tags,
+                            bool reset, double *err2_scratch, hipStream_t st);
+void launch_amr_remesh(const km_pack_t &old_u, const km_pack_t &new_u, const int32_t *origin,
+                       int v0, int v1, hipStream_t st);
 
 void launch_amr_ghost_fill(const AmrGrid &t, const km_pack_t &u, int v0, int v1, int nc,
                            hipStream_t st);

@@ -202,6 +202,113 @@ int km_cfg_parse_deck(km_cfg_t *cfg, const char *deck_text) {
   return KM_OK;
 }
 
+// Adaptive-refinement keys: parthenon/mesh/{numlevel,derefine_count} and the
+// <kamayan/refinementN> blocks of grid::SetupParams (grid.cpp:96-124); a criterion's
+// max_level defaults to numlevel, also when numlevel is set after the field.
+int km_amr_adapt_default(km_amr_adapt_t *a) {
+  if (!a) return KM_EINVAL;
+  std::memset(a, 0, sizeof(*a));
+  a->numlevel = 1;         // grid.cpp:48
+  a->derefine_count = 10;  // Parthenon's parthenon/mesh/derefine_count default
+  return KM_OK;
+}
+
+int km_amr_adapt_set(km_amr_adapt_t *a, int32_t mhd, const char *block_key, const char *value) {
+  if (!a || !block_key || !value) return KM_EINVAL;
+  const std::string key = trim(block_key), val = trim(value);
+  long iv = 0;
+  double rv = 0.;
+  if (key == "parthenon/mesh/numlevel" || key == "parthenon/mesh/derefine_count") {
+    if (!parse_int(val, &iv) || iv < (key.back() == 'l' ? 1 : 0) || iv > 1000) {
+      km::set_error("invalid value '" + val + "' for " + key);
+      return KM_EINVAL;
+    }
+    if (key.back() == 'l') {
+      for (int c = 0; c < a->ncrit; ++c)
+        if (a->crit[c].max_level == a->numlevel) a->crit[c].max_level = (int32_t)iv;
+      a->numlevel = (int32_t)iv;
+    } else {
+      a->derefine_count = (int32_t)iv;
+    }
+    return KM_OK;
+  }
+  const std::string pre = "kamayan/refinement";
+  if (key.compare(0, pre.size(), pre) != 0) return KM_OK;  // unrelated key
+  const size_t slash = key.find('/', pre.size());
+  long n = -1;
+  if (slash == std::string::npos || !parse_int(key.substr(pre.size(), slash - pre.size()), &n) ||
+      n < 0 || n >= KM_AMR_MAX_CRIT) {
+    km::set_error("unsupported refinement block in " + key);
+    return KM_EINVAL;
+  }
+  const std::string sub = key.substr(slash + 1);
+  km_amr_criterion_t &c = a->crit[n];
+  if (n >= a->ncrit) {
+    if (sub != "field") return KM_OK;  // the criterion starts with its field
+    for (int k = a->ncrit; k <= n; ++k) {
+      a->crit[k].var = -1;
+      a->crit[k].max_level = a->numlevel;  // grid.cpp:112-113
+      a->crit[k].refine_tol = 0.8;         // grid.cpp:107
+      a->crit[k].derefine_tol = 0.2;       // grid.cpp:108-109
+      a->crit[k].filter = 0.01;            // grid.cpp:110-111
+    }
+    a->ncrit = (int32_t)n + 1;
+  }
+  if (sub == "field") {
+    const int nc = mhd == KM_MHD_CT ? 8 : 5;
+    int base = -1;
+    if (val == "dens") base = KM_DENS;
+    else if (val == "momentum") base = KM_MOM0;
+    else if (val == "ener") base = KM_ENER;
+    else if (val == "magc" && mhd == KM_MHD_CT) base = KM_MAGC0;
+    else if (val == "velocity") base = nc + 0;
+    else if (val == "pres") base = nc + 3;
+    else if (val == "bulk modulus") base = nc + 4;
+    else if (val == "eint") base = nc + 5;
+    else if (val == "temp") base = nc + 6;
+    if (base < 0) {
+      km::set_error("unknown refinement field '" + val + "'");
+      return KM_EINVAL;
+    }
+    c.var = base;
+    return KM_OK;
+  }
+  if (sub == "comp4") {
+    if (!parse_int(val, &iv) || iv < 0 || iv > 2 || c.var < 0) {
+      km::set_error("invalid value '" + val + "' for " + key);
+      return KM_EINVAL;
+    }
+    c.var += (int32_t)iv;
+    return KM_OK;
+  }
+  if (sub == "method") {
+    if (val != "loehner") {
+      km::set_error("refinement method '" + val +
+                    "' is not provided (Parthenon's built-in criteria are not part of this "
+                    "library)");
+      return KM_EINVAL;
+    }
+    return KM_OK;
+  }
+  if (sub == "max_level") {
+    if (!parse_int(val, &iv) || iv < 0) {
+      km::set_error("invalid value '" + val + "' for " + key);
+      return KM_EINVAL;
+    }
+    c.max_level = (int32_t)iv;
+    return KM_OK;
+  }
+  if (sub == "refine_tol" || sub == "derefine_tol" || sub == "filter") {
+    if (!parse_real(val, &rv)) {
+      km::set_error("invalid value '" + val + "' for " + key);
+      return KM_EINVAL;
+    }
+    (sub == "refine_tol" ? c.refine_tol : sub == "derefine_tol" ? c.derefine_tol : c.filter) = rv;
+    return KM_OK;
+  }
+  return KM_OK;
+}
+
 int km_cfg_min_ghost(const km_cfg_t *cfg) {
   if (!cfg) return -1;
   static const int ng[] = {1, 2, 3, 3};  // Hydro.py:14 {fog, plm, ppm, wenoz}

@@ -230,13 +230,24 @@ struct km_sim_t {
     return e;
   }
 
+  // block-tree meshes keep every buffer sized by the leaf count in mesh_bufs, which a
+  // remesh replaces as a whole; alloc() files new buffers under *target
+  std::vector<std::unique_ptr<DevBuf>> mesh_bufs;
+  std::vector<std::unique_ptr<DevBuf>> *target = &bufs;
+  // adaptive refinement
+  bool adaptive = false;
+  km_amr_adapt_t adapt{};
+  std::vector<int32_t> leaves, deref;
+  int *tags_d = nullptr;
+  double *err2_d = nullptr;
+
   template <class T>
   int alloc(T **out, size_t count) {
     auto b = std::make_unique<DevBuf>();
     KM_HIP(hipMalloc(&b->p, count * sizeof(T) + 256));
     KM_HIP(hipMemset(b->p, 0, count * sizeof(T) + 256));
     *out = static_cast<T *>(b->p);
-    bufs.push_back(std::move(b));
+    target->push_back(std::move(b));
     return KM_OK;
   }
   int upload_table(double ***dst, const std::vector<double *> &h) {
@@ -429,6 +440,80 @@ int fused_stage(km_sim_t &s, int stage, double beta) {
   return check_launch();
 }
 
+// Replace the mesh of a block-tree simulation by new leaves: every leaf-sized buffer is
+// allocated anew, base data moves by km_amr_remesh, then the old buffers are released.
+int amr_remesh(km_sim_t &s, const std::vector<int32_t> &nl, const std::vector<int32_t> &origin) {
+  const int n = (int)nl.size() / 4;
+  AmrTreeHost t;
+  const std::string err = build_amr_tree(s.mesh, nl.data(), n, t);
+  if (!err.empty()) {
+    set_error("remesh: " + err);
+    return KM_EINVAL;
+  }
+  std::vector<std::unique_ptr<DevBuf>> fresh;
+  s.target = &fresh;
+  const km_pack_t old_pack = s.base.pack;
+  s.nb = n;
+  int rc;
+  if ((rc = s.alloc(&s.blocks_d, n))) return rc;
+  KM_HIP(hipMemcpy(s.blocks_d, t.blocks.data(), n * sizeof(km_block_t), hipMemcpyHostToDevice));
+  int *lookup_d = nullptr;
+  if ((rc = s.alloc(&lookup_d, t.lookup.size()))) return rc;
+  KM_HIP(hipMemcpy(lookup_d, t.lookup.data(), t.lookup.size() * sizeof(int),
+                   hipMemcpyHostToDevice));
+  s.agrid = t.grid(lookup_d);
+  if ((rc = make_cont(s, s.base, true, nullptr, true, false))) return rc;
+  for (int i = 0; i < 2; ++i)
+    if ((rc = make_cont(s, s.stage[i], false, &s.base, true, false))) return rc;
+  if ((rc = make_cont(s, s.dudt, false, nullptr, true, false))) return rc;
+  if ((rc = make_cont(s, s.scratch, false, nullptr, false, true))) return rc;
+  if ((rc = s.alloc(&s.tags_d, n))) return rc;
+  if ((rc = s.alloc(&s.err2_d, n))) return rc;
+  int32_t *origin_d = nullptr;
+  if ((rc = s.alloc(&origin_d, origin.size()))) return rc;
+  KM_HIP(hipMemcpy(origin_d, origin.data(), origin.size() * sizeof(int32_t),
+                   hipMemcpyHostToDevice));
+  launch_amr_remesh(old_pack, s.base.pack, origin_d, 0, s.nvar, s.stream);
+  if ((rc = check_launch())) return rc;
+  KM_HIP(hipStreamSynchronize(s.stream));
+  s.mesh_bufs.swap(fresh);  // the old mesh's buffers go with `fresh`
+  s.target = &s.bufs;
+  return KM_OK;
+}
+
+// After a cycle: tag the filled base state, update the tree and, when it changed,
+// remesh, refill ghost zones and re-estimate dt on the new mesh (Parthenon's
+// EvolutionDriver::Execute: Step, LoadBalancingAndAdaptiveRefinement,
+// InitializeBlockTimeStepsAfterAMR, then the time-step rule)
+int adapt_step(km_sim_t &s) {
+  const km_amr_adapt_t &a = s.adapt;
+  std::vector<int32_t> tags(s.nb, KM_AMR_DEREFINE);
+  if (a.ncrit > 0) {
+    for (int c = 0; c < a.ncrit; ++c)
+      launch_amr_tag_loehner(s.base.pack, a.crit[c], s.tags_d, c == 0, s.err2_d, s.stream);
+    if (int rc = check_launch()) return rc;
+    KM_HIP(hipMemcpyAsync(tags.data(), s.tags_d, s.nb * sizeof(int32_t), hipMemcpyDeviceToHost,
+                          s.stream));
+    KM_HIP(hipStreamSynchronize(s.stream));
+  }
+  std::vector<int32_t> nl, nd, org;
+  bool changed = false;
+  const std::string err = update_amr_tree(s.mesh, a.numlevel, a.derefine_count,
+                                          s.leaves.data(), s.nb, tags.data(), s.deref.data(),
+                                          nl, nd, org, changed);
+  if (!err.empty()) {
+    set_error(err);
+    return KM_EINVAL;
+  }
+  if (!changed) return KM_OK;
+  if (int rc = amr_remesh(s, nl, org)) return rc;
+  s.leaves = std::move(nl);
+  s.deref = std::move(nd);
+  ghost_fill(s, s.base.pack, 0, KM_TEMP(s.cfg.mhd), false);
+  launch_estimate_dt(s.cfg, s.base.pack, s.est_d, s.stream, nullptr, 1);
+  return check_launch();
+}
+
 // one cycle; dt_host is the step dt when the task path needs it by value
 int run_cycle(km_sim_t &s, double dt_host) {
   const int n = s.cfg.nstages;
@@ -442,6 +527,8 @@ int run_cycle(km_sim_t &s, double dt_host) {
     s.prims_from_ic = false;
     if (rc) return rc;
   }
+  if (s.adaptive)
+    if (int rc = adapt_step(s)) return rc;
   hipLaunchKernelGGL(k_dt_update, dim3(nb64(s.ninst)), dim3(64), 0, s.stream, s.dt_d, s.time_d,
                      s.est_d, s.dt_keep ? 1 : 0, s.ninst);
   s.dt_keep = false;
@@ -741,6 +828,7 @@ static int sim_create(const km_cfg_t *cfg, const km_mesh_desc_t *meshes, int32_t
   else
     for (int m = 0; m < ninst; ++m) km_mesh_blocks(&meshes[m], blocks.data() + first[m]);
   int rc;
+  if (tree) s->target = &s->mesh_bufs;
   if ((rc = s->alloc(&s->blocks_d, s->nb))) return rc;
   KM_HIP(hipMemcpy(s->blocks_d, blocks.data(), s->nb * sizeof(km_block_t), hipMemcpyHostToDevice));
   if (tree) {
@@ -803,6 +891,7 @@ static int sim_create(const km_cfg_t *cfg, const km_mesh_desc_t *meshes, int32_t
     if ((rc = s->alloc(&s->nbr_d, nbr.size()))) return rc;
     KM_HIP(hipMemcpy(s->nbr_d, nbr.data(), nbr.size() * sizeof(int), hipMemcpyHostToDevice));
   }
+  s->target = &s->bufs;
   if ((rc = s->alloc(&s->dt_d, 3 * (size_t)ninst))) return rc;
   s->est_d = s->dt_d + ninst;
   s->time_d = s->dt_d + 2 * ninst;
@@ -837,7 +926,12 @@ int km_sim_create_amr(const km_cfg_t *cfg, const km_mesh_desc_t *mesh, const int
     set_error("constrained transport across refinement levels needs an even nghost");
     return KM_EINVAL;
   }
-  return sim_create(cfg, mesh, 1, path, &t, out);
+  const int rc = sim_create(cfg, mesh, 1, path, &t, out);
+  if (rc == KM_OK) {
+    (*out)->leaves.assign(leaves, leaves + 4 * (size_t)nleaves);
+    (*out)->deref.assign(nleaves, 0);
+  }
+  return rc;
 }
 
 int km_sim_ensemble_info(const km_sim_t *s, double *times, double *dts, int32_t n) {
@@ -1023,6 +1117,94 @@ int km_sim_set_tile(km_sim_t *s, int32_t threads) {
     s->graph = nullptr;
   }
   return KM_OK;
+}
+
+int km_sim_set_adaptive(km_sim_t *s, const km_amr_adapt_t *a) {
+  if (!s || !a) return KM_EINVAL;
+  if (!s->amr) {
+    set_error("adaptive refinement needs a block-tree simulation (km_sim_create_amr)");
+    return KM_EINVAL;
+  }
+  if (s->cfg.mhd == KM_MHD_CT) {
+    set_error("adaptive refinement is provided for hydro; CT meshes keep their leaves");
+    return KM_EINVAL;
+  }
+  if (s->mesh.ng < 2) {
+    set_error("the Loehner estimator reads two ghost layers: nghost must be at least 2");
+    return KM_EINVAL;
+  }
+  if (a->ncrit < 0 || a->ncrit > KM_AMR_MAX_CRIT || a->numlevel < 1 || a->numlevel > 16 ||
+      a->derefine_count < 0) {
+    set_error("bad adaptive settings");
+    return KM_EINVAL;
+  }
+  for (int c = 0; c < a->ncrit; ++c)
+    if (a->crit[c].var < 0 || a->crit[c].var >= s->nvar) {
+      set_error("refinement field slot out of range");
+      return KM_EINVAL;
+    }
+  for (int b = 0; b < s->nb; ++b)
+    if (s->leaves[4 * (size_t)b] > a->numlevel - 1) {
+      set_error("a leaf is finer than numlevel allows");
+      return KM_EINVAL;
+    }
+  KM_HIP(hipStreamSynchronize(s->stream));
+  if (!s->tags_d) {
+    s->target = &s->mesh_bufs;
+    int rc = s->alloc(&s->tags_d, s->nb);
+    if (!rc) rc = s->alloc(&s->err2_d, s->nb);
+    s->target = &s->bufs;
+    if (rc) return rc;
+  }
+  s->adaptive = true;
+  s->adapt = *a;
+  s->deref.assign(s->nb, 0);
+  return KM_OK;
+}
+
+int km_sim_amr_leaves(const km_sim_t *s, int32_t *leaves, int32_t *nleaves) {
+  if (!s || !nleaves || !s->amr) return KM_EINVAL;
+  const int n = s->nb;
+  const int cap = *nleaves;
+  *nleaves = n;
+  if (!leaves || cap < n) {
+    set_error("leaf buffer holds " + std::to_string(cap) + " entries, " + std::to_string(n) +
+              " needed");
+    return KM_EINVAL;
+  }
+  std::memcpy(leaves, s->leaves.data(), s->leaves.size() * sizeof(int32_t));
+  return KM_OK;
+}
+
+int km_amr_tag_loehner(const km_pack_t *u, const km_amr_criterion_t *crit, int32_t *tags_dev,
+                       int32_t reset, void *stream) {
+  if (!u || !crit || !tags_dev || crit->var < 0 || crit->var >= u->nvar || u->ng < 2) {
+    set_error("km_amr_tag_loehner: bad pack, field slot or ghost width (needs 2)");
+    return KM_EINVAL;
+  }
+  DevBuf err2;  // per-block maxima, released after the stream drains
+  KM_HIP(hipMalloc(&err2.p, u->nblocks * sizeof(double) + 256));
+  launch_amr_tag_loehner(*u, *crit, tags_dev, reset != 0, static_cast<double *>(err2.p),
+                         as_stream(stream));
+  if (int rc = check_launch()) return rc;
+  KM_HIP(hipStreamSynchronize(as_stream(stream)));
+  return KM_OK;
+}
+
+int km_amr_remesh(const km_pack_t *old_u, const km_pack_t *new_u, const int32_t *origin_dev,
+                  int32_t v0, int32_t v1, void *stream) {
+  if (!old_u || !new_u || !origin_dev || old_u->ndim != new_u->ndim || old_u->ng != new_u->ng ||
+      v0 < 0 || v1 > old_u->nvar || v1 > new_u->nvar) {
+    set_error("km_amr_remesh: packs must share ndim, ng, nx and hold the slots");
+    return KM_EINVAL;
+  }
+  for (int d = 0; d < 3; ++d)
+    if (old_u->nx[d] != new_u->nx[d]) {
+      set_error("km_amr_remesh: packs must share nx");
+      return KM_EINVAL;
+    }
+  launch_amr_remesh(*old_u, *new_u, origin_dev, v0, v1, as_stream(stream));
+  return check_launch();
 }
 
 int km_sim_info(const km_sim_t *s, double *time, double *dt, int32_t *ncycle) {

@@ -22,6 +22,8 @@
 // vendored: parity unpinned there beyond the invariants listed in the tests.
 
 #include <algorithm>
+#include <array>
+#include <functional>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -1684,6 +1686,439 @@ void MakeBlocks(const km_mesh_desc_t &m, std::vector<km_block_t> &blocks) {
 }
 
 // ---------------------------------------------------------------------------------
+// Adaptive refinement. Tagging is kamayan's grid::AMRLoehner::operator()
+// (grid_refinement.cpp:30-152), run by the RefinementTagging task of the last stage
+// (kamayan_driver.cpp:178-181). The tag bookkeeping, the tree update and the data
+// movement of a remesh belong to Parthenon (not vendored); they are restated here from
+// Parthenon's published algorithm (MeshRefinement::SetRefinement,
+// Mesh::UpdateMeshBlockTree with MeshBlockTree::Refine/Derefine,
+// RedistributeAndRefineMeshBlocks, EvolutionDriver::InitializeBlockTimeStepsAfterAMR):
+// parity unpinned against Parthenon itself; pinned by the invariants and hand-worked
+// cases in tests/test_oracle_adapt.py.
+// ---------------------------------------------------------------------------------
+
+// largest num/den of the Loehner estimator over the interior of block b (:61-140):
+//   d_p(z) = 0.5 (u(z+e_p) - u(z-e_p)) / dx_p   on the interior and one ghost layer
+//   num    = sum_q sum_p [0.5 (d_p(z+e_q) - d_p(z-e_q)) / dx_q]^2
+//   den    = 1e-12 + sum_q sum_p [0.5 (|d_p(z+e_q)| + |d_p(z-e_q)|) / dx_p
+//            + filter (|u(z+e_q)| + |u(z-e_q)| + |u(z-e_p)| + |u(z+e_p)|) / (dx_q dx_p)]^2
+// squares are x*x (std::pow(x, 2) rounds the same product once)
+double LoehnerMaxErr2(const km_pack_t &u, int b, int var, double filter) {
+  const Shape sh(u);
+  const km_block_t &blk = u.blocks[b];
+  const double *q = cellp(u, b, var);
+  const long so[3] = {1, sh.E[0], (long)sh.E[0] * sh.E[1]};
+  const int nd = sh.ndim;
+  // first derivatives over interior +- 1 (the scratch field FirstDer)
+  const long nc = sh.ncell();
+  std::vector<double> der(3 * (size_t)nc, 0.0);
+  const int k3 = nd > 2, k2 = nd > 1;
+  for (int k = sh.ks - k3; k <= sh.ke + k3; ++k)
+    for (int j = sh.js - k2; j <= sh.je + k2; ++j)
+      for (int i = sh.is - 1; i <= sh.ie + 1; ++i) {
+        const long c = sh.c(k, j, i);
+        for (int p = 0; p < nd; ++p)
+          der[p * nc + c] = 0.5 * (q[c + so[p]] - q[c - so[p]]) / blk.dx[p];
+      }
+  double mx = 0.0;
+  for (int k = sh.ks; k <= sh.ke; ++k)
+    for (int j = sh.js; j <= sh.je; ++j) {
+      double row = 0.0;
+      for (int i = sh.is; i <= sh.ie; ++i) {
+        const long c = sh.c(k, j, i);
+        double num = 0.0, den = 1.e-12;
+        for (int qd = 0; qd < nd; ++qd)
+          for (int p = 0; p < nd; ++p) {
+            const double dp = der[p * nc + c + so[qd]], dm = der[p * nc + c - so[qd]];
+            const double a = 0.5 * (dp - dm) / blk.dx[qd];
+            num += a * a;
+            const double e =
+                0.5 * (std::fabs(dp) + std::fabs(dm)) / blk.dx[p] +
+                filter *
+                    (std::fabs(q[c + so[qd]]) + std::fabs(q[c - so[qd]]) +
+                     std::fabs(q[c - so[p]]) + std::fabs(q[c + so[p]])) /
+                    (blk.dx[qd] * blk.dx[p]);
+            den += e * e;
+          }
+        row = den == 0.0 ? row : std::max(row, num / den);
+      }
+      mx = std::max(mx, row);
+    }
+  return mx;
+}
+
+void TagLoehner(const km_pack_t &u, const km_amr_criterion_t &cr, int32_t *tags, bool reset) {
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int b = 0; b < u.nblocks; ++b) {
+    if (reset) tags[b] = KM_AMR_DEREFINE;
+    const double err = std::sqrt(LoehnerMaxErr2(u, b, cr.var, cr.filter));
+    int flag = KM_AMR_SAME;
+    if (err > cr.refine_tol && u.blocks[b].level < cr.max_level) flag = KM_AMR_REFINE;
+    if (err < cr.derefine_tol) flag = KM_AMR_DEREFINE;
+    tags[b] = std::max(tags[b], flag);
+  }
+}
+
+// ---- the leaf set ----
+struct LeafSet {
+  int ndim, nb0[3], bc[6];
+  std::vector<std::vector<char>> leaf;  // leaf[L][flat] at each level 0..Lcap
+  int nbl(int L, int d) const { return d < ndim ? (nb0[d] << L) : 1; }
+  long flat(int L, const int l[3]) const {
+    return ((long)l[2] * nbl(L, 1) + l[1]) * nbl(L, 0) + l[0];
+  }
+  bool is_leaf(int L, const int l[3]) const { return leaf[L][flat(L, l)] != 0; }
+  void set(int L, const int l[3], bool on) { leaf[L][flat(L, l)] = on ? 1 : 0; }
+  // position l + o at level L through periodic boundaries; false outside the domain
+  bool shift(int L, const int l[3], const int o[3], int out[3]) const {
+    for (int d = 0; d < 3; ++d) {
+      out[d] = l[d] + o[d];
+      if (d >= ndim) {
+        out[d] = 0;
+        continue;
+      }
+      const int n = nbl(L, d);
+      if (out[d] < 0 || out[d] >= n) {
+        if (bc[2 * d] != KM_BC_PERIODIC) return false;
+        out[d] = (out[d] + n) % n;
+      }
+    }
+    return true;
+  }
+  // level of the leaf holding node (L, l), or L + 1 if the node is split
+  int cover(int L, const int l[3]) const {
+    for (int Lq = L; Lq >= 0; --Lq) {
+      int a[3];
+      for (int d = 0; d < 3; ++d) a[d] = l[d] >> (L - Lq);
+      if (is_leaf(Lq, a)) return Lq;
+    }
+    return L + 1;
+  }
+};
+
+void ForEachDir(int ndim, const std::function<void(const int o[3])> &f) {
+  for (int oz = (ndim > 2 ? -1 : 0); oz <= (ndim > 2 ? 1 : 0); ++oz)
+    for (int oy = (ndim > 1 ? -1 : 0); oy <= (ndim > 1 ? 1 : 0); ++oy)
+      for (int ox = -1; ox <= 1; ++ox) {
+        if (!ox && !oy && !oz) continue;
+        const int o[3] = {ox, oy, oz};
+        f(o);
+      }
+}
+
+// MeshBlockTree::Refine: coarser leaves touching (L, l) split first
+void RefineLeaf(LeafSet &s, int L, const int l[3], int &applied) {
+  if (!s.is_leaf(L, l)) return;
+  ForEachDir(s.ndim, [&](const int o[3]) {
+    int n[3];
+    if (!s.shift(L, l, o, n)) return;
+    const int cl = s.cover(L, n);
+    if (cl < L) {
+      int a[3];
+      for (int d = 0; d < 3; ++d) a[d] = n[d] >> (L - cl);
+      RefineLeaf(s, cl, a, applied);
+    }
+  });
+  ++applied;
+  s.set(L, l, false);
+  for (int c = 0; c < (1 << s.ndim); ++c) {
+    int ch[3];
+    for (int d = 0; d < 3; ++d) ch[d] = d < s.ndim ? 2 * l[d] + ((c >> d) & 1) : 0;
+    s.set(L + 1, ch, true);
+  }
+}
+
+// MeshBlockTree::Derefine of parent (L, l): its children must all be leaves and no
+// split child of a neighbouring node may touch it
+void DerefineParent(LeafSet &s, int L, const int l[3], int &applied) {
+  const int nch = 1 << s.ndim;
+  for (int c = 0; c < nch; ++c) {
+    int ch[3];
+    for (int d = 0; d < 3; ++d) ch[d] = d < s.ndim ? 2 * l[d] + ((c >> d) & 1) : 0;
+    if (!s.is_leaf(L + 1, ch)) return;
+  }
+  bool ok = true;
+  ForEachDir(s.ndim, [&](const int o[3]) {
+    int n[3];
+    if (!ok || !s.shift(L, l, o, n)) return;
+    if (s.cover(L, n) <= L) return;  // a leaf at level <= L
+    // node n is split: its children on the side facing (L, l)
+    for (int c = 0; c < nch; ++c) {
+      int ch[3];
+      bool touch = true;
+      for (int d = 0; d < 3; ++d) {
+        const int bit = (c >> d) & 1;
+        if (d >= s.ndim) {
+          ch[d] = 0;
+          continue;
+        }
+        if (o[d] == -1 && bit != 1) touch = false;
+        if (o[d] == 1 && bit != 0) touch = false;
+        ch[d] = 2 * n[d] + bit;
+      }
+      if (!touch) continue;
+      if (s.cover(L + 1, ch) > L + 1) ok = false;
+    }
+  });
+  if (!ok) return;
+  for (int c = 0; c < nch; ++c) {
+    int ch[3];
+    for (int d = 0; d < 3; ++d) ch[d] = d < s.ndim ? 2 * l[d] + ((c >> d) & 1) : 0;
+    s.set(L + 1, ch, false);
+  }
+  s.set(L, l, true);
+  ++applied;
+}
+
+// Morton key of a leaf's lower corner on level `lev` (x in the lowest bit)
+uint64_t MortonKey(int ndim, int L, const int l[3], int lev) {
+  uint64_t key = 0;
+  int c[3];
+  for (int d = 0; d < 3; ++d) c[d] = d < ndim ? (l[d] << (lev - L)) : 0;
+  for (int bit = 20; bit >= 0; --bit)
+    for (int d = ndim - 1; d >= 0; --d) key = (key << 1) | ((c[d] >> bit) & 1);
+  return key;
+}
+
+// SetRefinement + UpdateMeshBlockTree; returns true when the tree changed
+bool UpdateTree(const km_mesh_desc_t &m, int numlevel, int threshold,
+                const std::vector<int32_t> &leaves, const int32_t *tags,
+                std::vector<int32_t> &deref, std::vector<int32_t> &out_leaves,
+                std::vector<int32_t> &out_deref, std::vector<int32_t> &origin) {
+  const int n = (int)leaves.size() / 4;
+  const int maxlev = numlevel - 1;
+  LeafSet s;
+  s.ndim = m.ndim;
+  for (int d = 0; d < 3; ++d) s.nb0[d] = d < m.ndim ? m.nb[d] : 1;
+  for (int d = 0; d < 6; ++d) s.bc[d] = m.bc[d];
+  s.leaf.assign(maxlev + 2, {});
+  for (int L = 0; L <= maxlev + 1; ++L)
+    s.leaf[L].assign((size_t)s.nbl(L, 0) * s.nbl(L, 1) * s.nbl(L, 2), 0);
+  for (int b = 0; b < n; ++b) {
+    const int l[3] = {leaves[4 * b + 1], leaves[4 * b + 2], leaves[4 * b + 3]};
+    s.set(leaves[4 * b], l, true);
+  }
+  // MeshRefinement::SetRefinement per leaf
+  std::vector<int> flag(n, 0);
+  for (int b = 0; b < n; ++b) {
+    const int L = leaves[4 * b];
+    const int l[3] = {leaves[4 * b + 1], leaves[4 * b + 2], leaves[4 * b + 3]};
+    const int aret = std::max(-1, (int)tags[b]);
+    if (aret >= 0) deref[b] = 0;
+    if (aret > 0) {
+      flag[b] = L == maxlev ? 0 : 1;
+    } else if (aret < 0) {
+      if (L == 0) {
+        deref[b] = 0;
+      } else {
+        deref[b] += 1;
+        bool finer = false;
+        ForEachDir(s.ndim, [&](const int o[3]) {
+          int nn[3];
+          if (s.shift(L, l, o, nn) && s.cover(L, nn) > L) finer = true;
+        });
+        flag[b] = (!finer && deref[b] >= threshold) ? -1 : 0;
+      }
+    }
+  }
+  // UpdateMeshBlockTree
+  const int nleaf = 1 << m.ndim;
+  int nref = 0, nderef = 0;
+  for (int b = 0; b < n; ++b) {
+    nref += flag[b] == 1;
+    nderef += flag[b] == -1;
+  }
+  std::vector<std::array<int, 4>> ref, par;
+  for (int b = 0; b < n; ++b)
+    if (flag[b] == 1) ref.push_back({leaves[4 * b], leaves[4 * b + 1], leaves[4 * b + 2], leaves[4 * b + 3]});
+  if (nderef >= nleaf) {
+    // parents all of whose children are leaves flagged -1
+    std::vector<std::vector<char>> fl(maxlev + 2);
+    for (int L = 0; L <= maxlev + 1; ++L) fl[L].assign(s.leaf[L].size(), 0);
+    for (int b = 0; b < n; ++b)
+      if (flag[b] == -1) {
+        const int l[3] = {leaves[4 * b + 1], leaves[4 * b + 2], leaves[4 * b + 3]};
+        fl[leaves[4 * b]][s.flat(leaves[4 * b], l)] = 1;
+      }
+    for (int b = 0; b < n; ++b) {
+      if (flag[b] != -1) continue;
+      const int L = leaves[4 * b];
+      const int l[3] = {leaves[4 * b + 1], leaves[4 * b + 2], leaves[4 * b + 3]};
+      bool first = true;
+      for (int d = 0; d < m.ndim; ++d)
+        if (l[d] & 1) first = false;
+      if (!first) continue;
+      bool all = true;
+      for (int c = 0; c < nleaf; ++c) {
+        int ch[3];
+        for (int d = 0; d < 3; ++d) ch[d] = d < m.ndim ? l[d] + ((c >> d) & 1) : 0;
+        if (!fl[L][s.flat(L, ch)]) all = false;
+      }
+      if (all) par.push_back({L - 1, l[0] >> 1, m.ndim > 1 ? l[1] >> 1 : 0, m.ndim > 2 ? l[2] >> 1 : 0});
+    }
+  }
+  int applied = 0;
+  std::stable_sort(ref.begin(), ref.end(), [](const auto &a, const auto &b) { return a[0] < b[0]; });
+  std::stable_sort(par.begin(), par.end(), [](const auto &a, const auto &b) { return a[0] > b[0]; });
+  for (const auto &r : ref) {
+    const int l[3] = {r[1], r[2], r[3]};
+    RefineLeaf(s, r[0], l, applied);
+  }
+  for (const auto &p : par) {
+    const int l[3] = {p[1], p[2], p[3]};
+    DerefineParent(s, p[0], l, applied);
+  }
+  if (applied == 0) {
+    out_leaves = leaves;
+    out_deref = deref;
+    origin.assign(9 * (size_t)n, -1);
+    for (int b = 0; b < n; ++b) {
+      origin[9 * b] = 0;
+      origin[9 * b + 1] = b;
+    }
+    return false;
+  }
+  // new leaves in Morton order; origins against the old leaves
+  struct NL {
+    uint64_t key;
+    int L, l[3];
+  };
+  std::vector<NL> nl;
+  for (int L = 0; L <= maxlev + 1; ++L)
+    for (int k = 0; k < s.nbl(L, 2); ++k)
+      for (int j = 0; j < s.nbl(L, 1); ++j)
+        for (int i = 0; i < s.nbl(L, 0); ++i) {
+          const int l[3] = {i, j, k};
+          if (s.is_leaf(L, l)) nl.push_back({MortonKey(m.ndim, L, l, maxlev + 1), L, {i, j, k}});
+        }
+  std::sort(nl.begin(), nl.end(), [](const NL &a, const NL &b) { return a.key < b.key; });
+  // old leaf index by (level, position)
+  std::vector<std::vector<int>> old(maxlev + 2);
+  for (int L = 0; L <= maxlev + 1; ++L) old[L].assign(s.leaf[L].size(), -1);
+  for (int b = 0; b < n; ++b) {
+    const int l[3] = {leaves[4 * b + 1], leaves[4 * b + 2], leaves[4 * b + 3]};
+    old[leaves[4 * b]][s.flat(leaves[4 * b], l)] = b;
+  }
+  out_leaves.clear();
+  out_deref.clear();
+  origin.assign(9 * nl.size(), -1);
+  for (size_t i = 0; i < nl.size(); ++i) {
+    const NL &e = nl[i];
+    out_leaves.insert(out_leaves.end(), {e.L, e.l[0], e.l[1], e.l[2]});
+    const int ob = old[e.L][s.flat(e.L, e.l)];
+    int *o = &origin[9 * i];
+    if (ob >= 0) {
+      o[0] = 0;
+      o[1] = ob;
+      out_deref.push_back(deref[ob]);
+      continue;
+    }
+    out_deref.push_back(0);
+    if (e.L > 0) {
+      const int pl[3] = {e.l[0] >> 1, m.ndim > 1 ? e.l[1] >> 1 : 0, m.ndim > 2 ? e.l[2] >> 1 : 0};
+      const int pb = old[e.L - 1][s.flat(e.L - 1, pl)];
+      if (pb >= 0) {
+        o[0] = 1;
+        o[1] = pb;
+        continue;
+      }
+    }
+    o[0] = 2;
+    for (int c = 0; c < nleaf; ++c) {
+      int ch[3];
+      for (int d = 0; d < 3; ++d) ch[d] = d < m.ndim ? 2 * e.l[d] + ((c >> d) & 1) : 0;
+      o[1 + c] = old[e.L + 1][s.flat(e.L + 1, ch)];
+      if (o[1 + c] < 0) throw std::runtime_error("remesh: new leaf with no source");
+    }
+  }
+  return true;
+}
+
+// RedistributeAndRefineMeshBlocks data movement for cell slots [v0, v1)
+void Remesh(const km_pack_t &o, const km_pack_t &nw, const int32_t *origin, int v0, int v1) {
+  const Shape sh(nw);
+  const int st[3] = {sh.is, sh.js, sh.ks};
+  const int nd = sh.ndim;
+  const long so[3] = {1, sh.E[0], (long)sh.E[0] * sh.E[1]};
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int b = 0; b < nw.nblocks; ++b) {
+    const int32_t *org = origin + 9 * b;
+    const km_block_t &blk = nw.blocks[b];
+    for (int v = v0; v < v1; ++v) {
+      double *dst = cellp(nw, b, v);
+      if (org[0] == 0) {
+        std::memcpy(dst, cellp(o, org[1], v), sh.ncell() * sizeof(double));
+        continue;
+      }
+      for (int k = sh.ks; k <= sh.ke; ++k)
+        for (int j = sh.js; j <= sh.je; ++j)
+          for (int i = sh.is; i <= sh.ie; ++i) {
+            const int idx[3] = {i, j, k};
+            double x;
+            if (org[0] == 1) {
+              // ProlongateSharedMinMod from the parent's zones
+              const km_block_t &pb = o.blocks[org[1]];
+              const double *q = cellp(o, org[1], v);
+              int pc[3] = {0, 0, 0}, par[3] = {0, 0, 0};
+              for (int d = 0; d < nd; ++d) {
+                const int G = blk.lx[d] * sh.nx[d] + (idx[d] - st[d]);
+                par[d] = G & 1;
+                pc[d] = (G >> 1) - pb.lx[d] * sh.nx[d] + st[d];
+              }
+              const long cc = sh.c(pc[2], pc[1], pc[0]);
+              const double fc = q[cc];
+              double g[3] = {0., 0., 0.}, dxf[3] = {0., 0., 0.};
+              for (int d = 0; d < nd; ++d) {
+                const double dxc = 2.0 * blk.dx[d];
+                g[d] = GradMinMod(fc, q[cc - so[d]], q[cc + so[d]], dxc, dxc);
+                dxf[d] = 0.5 * blk.dx[d];
+              }
+              const bool oi = par[0], oj = par[1], ok = par[2];
+              const double a = g[0] * dxf[0], bb = g[1] * dxf[1], c3 = g[2] * dxf[2];
+              if (!oi && !oj && !ok) x = fc - (a + bb + c3);
+              else if (oi && !oj && !ok) x = fc + (a - bb - c3);
+              else if (!oi && oj && !ok) x = fc - (a - bb + c3);
+              else if (oi && oj && !ok) x = fc + (a + bb - c3);
+              else if (!oi && !oj && ok) x = fc - (a + bb - c3);
+              else if (oi && !oj && ok) x = fc + (a - bb + c3);
+              else if (!oi && oj && ok) x = fc - (a - bb - c3);
+              else x = fc + (a + bb + c3);
+            } else {
+              // RestrictAverage of the child holding the 2^d fine zones
+              int c = 0, f[3] = {0, 0, 0};
+              for (int d = 0; d < nd; ++d) {
+                const int lc = idx[d] - st[d];
+                const int half = (2 * lc) / sh.nx[d];
+                c |= half << d;
+                f[d] = 2 * lc - half * sh.nx[d] + st[d];
+              }
+              const int cb = org[1 + c];
+              const double *q = cellp(o, cb, v);
+              const double vol = o.blocks[cb].vol;
+              double V[2][2][2], T[2][2][2];
+              for (int ok = 0; ok < 2; ++ok)
+                for (int oj = 0; oj < 2; ++oj)
+                  for (int oi = 0; oi < 2; ++oi) {
+                    V[ok][oj][oi] = T[ok][oj][oi] = 0.0;
+                    if ((ok == 0 || nd > 2) && (oj == 0 || nd > 1)) {
+                      V[ok][oj][oi] = vol;
+                      T[ok][oj][oi] = vol * q[sh.c(f[2] + ok, f[1] + oj, f[0] + oi)];
+                    }
+                  }
+              const double tvol = ((V[0][0][0] + V[0][1][0]) + (V[0][0][1] + V[0][1][1])) +
+                                  ((V[1][0][0] + V[1][1][0]) + (V[1][0][1] + V[1][1][1]));
+              x = tvol > 0.0 ? (((T[0][0][0] + T[0][1][0]) + (T[0][0][1] + T[0][1][1])) +
+                                ((T[1][0][0] + T[1][1][0]) + (T[1][0][1] + T[1][1][1]))) /
+                                   tvol
+                             : 0.0;
+            }
+            dst[sh.c(k, j, i)] = x;
+          }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------
 // Whole-mesh driver: KamayanDriver::MakeTaskCollection / BuildTaskListRKStage
 // (kamayan_driver.cpp:121-248) over a single MeshData partition.
 // ---------------------------------------------------------------------------------
@@ -1709,6 +2144,10 @@ struct km_cpu_sim_t {
   double time = 0., dt = 0., dt_est = 0.;
   int ncycle = 0;
   bool dt_override = false;
+  // adaptive refinement (block-tree meshes)
+  bool adaptive = false;
+  km_amr_adapt_t adapt{};
+  std::vector<int32_t> leaves, deref;
 };
 
 namespace {
@@ -1813,6 +2252,37 @@ void Cycle(km_cpu_sim_t &s) {
   }
   s.time += dt;
   s.ncycle += 1;
+}
+
+// Tagging of the filled base state, tree update and, when the tree changed, the remesh,
+// ghost fill and dt estimate on the new mesh (EvolutionDriver::Execute order: Step,
+// LoadBalancingAndAdaptiveRefinement, InitializeBlockTimeStepsAfterAMR, SetGlobalTimeStep)
+void AdaptStep(km_cpu_sim_t &s) {
+  std::vector<int32_t> tags(s.nblocks, KM_AMR_DEREFINE);
+  for (int c = 0; c < s.adapt.ncrit; ++c) TagLoehner(s.base.pack, s.adapt.crit[c], tags.data(), false);
+  std::vector<int32_t> nl, nd, org;
+  if (!UpdateTree(s.mesh, s.adapt.numlevel, s.adapt.derefine_count, s.leaves, tags.data(),
+                  s.deref, nl, nd, org))
+    return;
+  const int n = (int)nl.size() / 4;
+  Tree t;
+  const std::string err = BuildTree(s.mesh, nl.data(), n, t);
+  if (!err.empty()) throw std::runtime_error("remesh produced an invalid tree: " + err);
+  std::vector<km_block_t> old_blocks = std::move(s.blocks);
+  Container old_base = std::move(s.base);
+  s.blocks.assign(n, km_block_t{});
+  for (int b = 0; b < n; ++b) LeafMetric(s.mesh, t.level[b], &t.loc[3 * b], s.blocks[b]);
+  s.tree = std::move(t);
+  s.nblocks = n;
+  MakeContainer(s, s.base, true, nullptr, false);
+  Remesh(old_base.pack, s.base.pack, org.data(), 0, s.nvar);
+  for (int i = 0; i < 2; ++i) MakeContainer(s, s.stage[i], false, &s.base, false);
+  MakeContainer(s, s.dudt, false, nullptr, true);
+  s.leaves = std::move(nl);
+  s.deref = std::move(nd);
+  FillGhosts(s, s.base.pack);
+  FillDivB(s.cfg, s.base.pack);
+  s.dt_est = EstimateDt(s.cfg, s.base.pack);
 }
 
 }  // namespace
@@ -1999,6 +2469,8 @@ int km_cpu_sim_create_amr(const km_cfg_t *cfg, const km_mesh_desc_t *mesh,
     return KM_EINVAL;
   }
   s->amr = true;
+  s->leaves.assign(leaves, leaves + 4 * (size_t)n);
+  s->deref.assign(n, 0);
   s->blocks.resize(n);
   for (int b = 0; b < n; ++b) LeafMetric(*mesh, s->tree.level[b], &s->tree.loc[3 * b], s->blocks[b]);
   s->nblocks = n;
@@ -2074,6 +2546,7 @@ int km_cpu_sim_advance(km_cpu_sim_t *s, int32_t ncycles, double tlim) {
     }
     try {
       Cycle(*s);
+      if (s->adaptive) AdaptStep(*s);
     } catch (const std::exception &e) {
       std::fprintf(stderr, "km_cpu_sim_advance: %s\n", e.what());
       return KM_EINVAL;
@@ -2083,6 +2556,68 @@ int km_cpu_sim_advance(km_cpu_sim_t *s, int32_t ncycles, double tlim) {
     }
     s->dt_override = false;
   }
+  return KM_OK;
+}
+
+// ---- adaptive refinement twins (host pointers) ----
+int km_cpu_amr_tag_loehner(const km_pack_t *u, const km_amr_criterion_t *crit, int32_t *tags,
+                           int32_t reset) {
+  TagLoehner(*u, *crit, tags, reset != 0);
+  return KM_OK;
+}
+
+int km_cpu_amr_update_tree(const km_mesh_desc_t *mesh, int32_t numlevel,
+                           int32_t derefine_count, const int32_t *leaves, int32_t nleaves,
+                           const int32_t *tags, int32_t *deref_count, int32_t *new_leaves,
+                           int32_t *new_deref, int32_t *origin, int32_t *nnew,
+                           int32_t *changed) {
+  std::vector<int32_t> lv(leaves, leaves + 4 * (size_t)nleaves);
+  std::vector<int32_t> dc(deref_count, deref_count + nleaves), nl, nd, org;
+  bool ch;
+  try {
+    ch = UpdateTree(*mesh, numlevel, derefine_count, lv, tags, dc, nl, nd, org);
+  } catch (const std::exception &e) {
+    std::fprintf(stderr, "km_cpu_amr_update_tree: %s\n", e.what());
+    return KM_EINVAL;
+  }
+  const int n = (int)nl.size() / 4;
+  if (n > *nnew) {
+    *nnew = n;
+    return KM_EINVAL;
+  }
+  std::copy(dc.begin(), dc.end(), deref_count);
+  std::copy(nl.begin(), nl.end(), new_leaves);
+  std::copy(nd.begin(), nd.end(), new_deref);
+  std::copy(org.begin(), org.end(), origin);
+  *nnew = n;
+  *changed = ch ? 1 : 0;
+  return KM_OK;
+}
+
+int km_cpu_amr_remesh(const km_pack_t *old_u, const km_pack_t *new_u, const int32_t *origin,
+                      int32_t v0, int32_t v1) {
+  Remesh(*old_u, *new_u, origin, v0, v1);
+  return KM_OK;
+}
+
+int km_cpu_sim_set_adaptive(km_cpu_sim_t *s, const km_amr_adapt_t *a) {
+  if (!s->amr || s->cfg.mhd == KM_MHD_CT || a->ncrit < 0 || a->ncrit > KM_AMR_MAX_CRIT ||
+      a->numlevel < 1)
+    return KM_EINVAL;
+  s->adaptive = true;
+  s->adapt = *a;
+  s->deref.assign(s->nblocks, 0);
+  return KM_OK;
+}
+
+int km_cpu_sim_amr_leaves(const km_cpu_sim_t *s, int32_t *leaves, int32_t *nleaves) {
+  const int n = (int)s->leaves.size() / 4;
+  if (*nleaves < n) {
+    *nleaves = n;
+    return KM_EINVAL;
+  }
+  std::copy(s->leaves.begin(), s->leaves.end(), leaves);
+  *nleaves = n;
   return KM_OK;
 }
 

@@ -12,7 +12,8 @@ import subprocess
 
 import numpy as np
 
-from kamayan_amd._abi import KmBlock, KmCfg, KmMeshDesc, KmPack, Layout, f64ptr
+from kamayan_amd._abi import (KmAmrAdapt, KmAmrCriterion, KmBlock, KmCfg, KmMeshDesc, KmPack,
+                              Layout, f64ptr)
 
 HERE = pathlib.Path(__file__).resolve().parent
 LIB_PATH = HERE / "libkm_oracle.so"
@@ -64,6 +65,12 @@ def lib() -> C.CDLL:
         "km_cpu_flux_correct": [P(KmCfg), P(KmPack), P(KmMeshDesc), P(i32), i32],
         "km_cpu_amr_ghost_faces": [P(KmCfg), P(KmPack), P(KmMeshDesc), P(i32), i32],
         "km_cpu_sim_create_amr": [P(KmCfg), P(KmMeshDesc), P(i32), i32, P(vp)],
+        "km_cpu_amr_tag_loehner": [P(KmPack), P(KmAmrCriterion), P(i32), i32],
+        "km_cpu_amr_update_tree": [P(KmMeshDesc), i32, i32, P(i32), i32, P(i32), P(i32),
+                                   P(i32), P(i32), P(i32), P(i32), P(i32)],
+        "km_cpu_amr_remesh": [P(KmPack), P(KmPack), P(i32), i32, i32],
+        "km_cpu_sim_set_adaptive": [vp, P(KmAmrAdapt)],
+        "km_cpu_sim_amr_leaves": [vp, P(i32), P(i32)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -141,6 +148,35 @@ def amr_blocks(desc: KmMeshDesc, leaves: np.ndarray):
     return arr
 
 
+def amr_tag_loehner(pack: KmPack, crit: KmAmrCriterion, tags: np.ndarray, reset: bool):
+    """AMRLoehner tags (in place) of a host pack."""
+    assert tags.dtype == np.int32
+    call("km_cpu_amr_tag_loehner", C.byref(pack), C.byref(crit), _i32(tags), 1 if reset else 0)
+
+
+def amr_update_tree(desc: KmMeshDesc, numlevel: int, derefine_count: int, leaves, tags,
+                    deref: np.ndarray):
+    """SetRefinement + UpdateMeshBlockTree; deref (int32) is updated in place. Returns
+    (new_leaves, new_deref, origin[n, 9], changed)."""
+    leaves = np.ascontiguousarray(leaves, dtype=np.int32)
+    tags = np.ascontiguousarray(tags, dtype=np.int32)
+    cap = 8 * len(leaves) + 64
+    nl = np.zeros((cap, 4), np.int32)
+    nd = np.zeros(cap, np.int32)
+    org = np.zeros((cap, 9), np.int32)
+    n, ch = C.c_int32(cap), C.c_int32(0)
+    call("km_cpu_amr_update_tree", C.byref(desc), numlevel, derefine_count, _i32(leaves),
+         len(leaves), _i32(tags), _i32(deref), _i32(nl), _i32(nd), _i32(org), C.byref(n),
+         C.byref(ch))
+    k = n.value
+    return nl[:k].copy(), nd[:k].copy(), org[:k].copy(), bool(ch.value)
+
+
+def amr_remesh(old: KmPack, new: KmPack, origin: np.ndarray, v0: int, v1: int):
+    origin = np.ascontiguousarray(origin, dtype=np.int32)
+    call("km_cpu_amr_remesh", C.byref(old), C.byref(new), _i32(origin), v0, v1)
+
+
 class HostPack:
     """numpy block-pack arrays and host pointer tables for a km_pack_t."""
 
@@ -206,9 +242,23 @@ class CpuSimulation:
                 fp[d] = faces[d].ctypes.data
         call("km_cpu_sim_set_primitive", self.h, f64ptr(cell), fp)
 
+    def set_adaptive(self, adapt: KmAmrAdapt):
+        call("km_cpu_sim_set_adaptive", self.h, C.byref(adapt))
+
+    def leaves(self) -> np.ndarray:
+        n = C.c_int32(0)
+        lib().km_cpu_sim_amr_leaves(self.h, _i32(np.zeros(4, np.int32)), C.byref(n))
+        out = np.zeros((max(n.value, 1), 4), np.int32)
+        call("km_cpu_sim_amr_leaves", self.h, _i32(out), C.byref(n))
+        self.layout = Layout(self.layout.ndim, self.layout.ng, self.layout.nx, self.layout.nb,
+                             nleaves=n.value)
+        return out[: n.value]
+
     def state(self):
         mhd = self.cfg.mhd
         nv = 16 if mhd == 1 else 12
+        if getattr(self, "_leaves", None) is not None:
+            self.leaves()
         cell = np.zeros(self.layout.cell_shape(nv))
         fp = (C.c_void_p * 3)()
         faces = None
