@@ -54,7 +54,9 @@ enum { KM_GEOM_CARTESIAN = 0, KM_GEOM_CYLINDRICAL = 1 };                        
  * FP32_MIXED: reconstruction + Riemann in FP32, everything else FP64;
  * FP64_FMA: FP64 with fused multiply-adds (agreement with FP64 to ~1e-14 relative). */
 enum { KM_PREC_FP64 = 0, KM_PREC_FP32_MIXED = 1, KM_PREC_FP64_FMA = 2 };
-enum { KM_BC_PERIODIC = 0, KM_BC_OUTFLOW = 1, KM_BC_REFLECT = 2 };
+/* AXISYMMETRIC: the inner x1 boundary of cylindrical geometry, a mirror across r = 0
+ * that flips the r and phi vector components (boundary_conditions.cpp:23-60). */
+enum { KM_BC_PERIODIC = 0, KM_BC_OUTFLOW = 1, KM_BC_REFLECT = 2, KM_BC_AXISYMMETRIC = 3 };
 
 /* Parsed option set. Built from the reference's input-deck keys by km_cfg_set /
  * km_cfg_parse_deck (hydro.cpp:55-90, physics.cpp:18-24, kamayan_driver.cpp:25-27,
@@ -151,6 +153,12 @@ int km_calculate_emf(const km_cfg_t *cfg, const km_pack_t *u, void *stream);
  * for CT, dudt->face[d]. */
 int km_fluxes_to_dudt(const km_cfg_t *cfg, const km_pack_t *u, const km_pack_t *dudt,
                       void *stream);
+/* hydro::AddSourceTerms (hydro.cpp:199-240, AddTasksOneStep, between FluxesToDuDt and
+ * ApplyDuDt): in cylindrical geometry the r momentum of dudt gains
+ * (rho v_phi^2 + P) / r at each cell centre (primconsflux.hpp:144-159), from the
+ * primitives of u; nothing in Cartesian geometry. */
+int km_add_source_terms(const km_cfg_t *cfg, const km_pack_t *u, const km_pack_t *dudt,
+                        void *stream);
 /* grid::ApplyDuDt (grid.cpp:213-283):
  *   u0 <- beta*base + (1-beta)*u0 ;  u1 <- u0 + (beta*dt)*dudt
  * over the interior of every conserved cell var and (CT) every face element. */

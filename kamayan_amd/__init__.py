@@ -24,7 +24,7 @@ __all__ = [
     "lib", "Config", "Simulation", "DevicePack", "problem_fill", "Layout", "mesh_desc",
     "KmCfg", "KmPack", "KmBlock", "KmMeshDesc", "ncons", "nvar", "slot", "PATH_TASKS",
     "PATH_FUSED", "LIB_PATH", "device_count", "block_table", "AmrAdapt", "amr_adapt",
-    "amr_criterion", "amr_update_tree", "amr_tag_loehner", "amr_remesh",
+    "amr_criterion", "amr_update_tree", "amr_tag_loehner", "amr_remesh", "initial_refinement",
 ]
 
 _lib = None
@@ -53,6 +53,7 @@ def lib() -> C.CDLL:
         "km_calculate_fluxes": [P(KmCfg), P(KmPack), vp],
         "km_calculate_emf": [P(KmCfg), P(KmPack), vp],
         "km_fluxes_to_dudt": [P(KmCfg), P(KmPack), P(KmPack), vp],
+        "km_add_source_terms": [P(KmCfg), P(KmPack), P(KmPack), vp],
         "km_apply_dudt": [P(KmCfg), P(KmPack), P(KmPack), P(KmPack), P(KmPack), dbl, dbl, vp],
         "km_prepare_primitive": [P(KmCfg), P(KmPack), vp],
         "km_prepare_conserved": [P(KmCfg), P(KmPack), vp],
@@ -218,6 +219,49 @@ def amr_tag_loehner(pack: "DevicePack", crit: KmAmrCriterion, tags_dev: int, res
 def amr_remesh(old: "DevicePack", new: "DevicePack", origin_dev: int, v0: int, v1: int):
     check(lib().km_amr_remesh(C.byref(old.pack), C.byref(new.pack), C.c_void_p(origin_dev), v0,
                               v1, None), "km_amr_remesh")
+
+
+NO_MERGE = 1 << 30  # a derefine count no run reaches: the initial loop only refines
+
+
+def initial_refinement(cfg: Config, desc: KmMeshDesc, layout: Layout, problem: str, adapt,
+                       seed: int = 0, params=None, leaves=None):
+    """Parthenon's initial refinement (Mesh::Initialize with an adaptive mesh): fill the
+    problem on the leaves, fill ghost zones, tag with the criteria and refine, re-running
+    the problem generator on the new leaves until the tree stops changing (at most
+    numlevel - 1 rounds). Starts from the root grid unless `leaves` is given. Returns
+    (leaves, cell, faces) of the final tree."""
+    a = adapt.c if isinstance(adapt, AmrAdapt) else adapt
+    nd = layout.ndim
+    if leaves is None:
+        leaves = np.array([(0, i, j, k) for k in range(layout.nb[2] if nd > 2 else 1)
+                           for j in range(layout.nb[1] if nd > 1 else 1)
+                           for i in range(layout.nb[0])], dtype=np.int32)
+    for _ in range(max(a.numlevel, 1)):
+        lay = Layout(layout.ndim, layout.ng, layout.nx, layout.nb, nleaves=len(leaves))
+        cell, faces = problem_fill(problem, cfg, desc, lay, seed=seed, params=params,
+                                   leaves=leaves)
+        sim = Simulation(cfg, desc, lay, path=PATH_TASKS, leaves=leaves)
+        sim.set_primitive(cell, faces)
+        state, _ = sim.state()
+        sim.close()
+        dev = DevicePack(lay, nvar(cfg.c.mhd), blocks=amr_blocks(desc, leaves))
+        dev.upload(cell=state)
+        tags_d = dev._alloc(4 * len(leaves))
+        for c in range(a.ncrit):
+            amr_tag_loehner(dev, a.crit[c], tags_d, c == 0)
+        tags = np.full(len(leaves), -1, np.int32)
+        if a.ncrit:
+            check(lib().km_memcpy_d2h(_i32(tags), C.c_void_p(tags_d), 4 * len(leaves)), "d2h")
+        dev.close()
+        nl, _, _, changed = amr_update_tree(desc, a.numlevel, NO_MERGE, leaves, tags,
+                                            np.zeros(len(leaves), np.int32))
+        if not changed:
+            return leaves, cell, faces
+        leaves = nl
+    lay = Layout(layout.ndim, layout.ng, layout.nx, layout.nb, nleaves=len(leaves))
+    cell, faces = problem_fill(problem, cfg, desc, lay, seed=seed, params=params, leaves=leaves)
+    return leaves, cell, faces
 
 
 def problem_fill(name: str, cfg: Config, desc: KmMeshDesc, layout: Layout, seed: int = 0,

@@ -12,8 +12,9 @@ RECON = {"fog": 0, "plm": 1, "ppm": 2, "wenoz": 3}
 LIMITER = {"minmod": 0, "van_leer": 1, "mc": 2}
 RIEMANN = {"hll": 0, "hllc": 1, "hlld": 2}
 MHD = {"off": 0, "ct": 1}
+GEOMETRY = {"cartesian": 0, "cylindrical": 1}
 PRECISION = {"fp64": 0, "fp32_mixed": 1, "fp64_fma": 2}
-BC = {"periodic": 0, "outflow": 1, "reflect": 2}
+BC = {"periodic": 0, "outflow": 1, "reflect": 2, "axisymmetric": 3}
 PATH_TASKS, PATH_FUSED = 0, 1
 
 KM_DENS, KM_MOM0, KM_ENER, KM_MAGC0 = 0, 1, 4, 5

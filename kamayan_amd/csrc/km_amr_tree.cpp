@@ -81,7 +81,11 @@ std::string build_amr_tree(const km_mesh_desc_t &m, const int32_t *leaves, int n
     t.nb0[d] = active(m, d) ? m.nb[d] : 1;
     t.nbf[d] = active(m, d) ? (m.nb[d] << lmax) : 1;
   }
-  for (int d = 0; d < 6; ++d) t.bc[d] = m.bc[d];
+  for (int d = 0; d < 6; ++d) {
+    if (m.bc[d] == KM_BC_AXISYMMETRIC)
+      return "block-tree meshes take periodic, outflow or reflect boundaries";
+    t.bc[d] = m.bc[d];
+  }
   for (int d = 0; d < m.ndim; ++d) {
     if ((m.bc[2 * d] == KM_BC_PERIODIC) != (m.bc[2 * d + 1] == KM_BC_PERIODIC))
       return "periodic boundaries must be paired";

@@ -51,6 +51,51 @@ struct Geo {
 
 KM_HD double *cell_ptr(const km_pack_t &p, int b, int v) { return p.cell[(long)b * p.nvar + v]; }
 
+// ---- coordinates of a block ------------------------------------------------------
+// Parthenon's uniform coordinates: face i of dimension d sits at x0 + i dx with
+// x0 = xmin - lo dx (lo = first interior index), cell i's centre at x0 + (i + 1/2) dx.
+KM_HD double x_face(const km_block_t &b, const Geo &g, int d, int i) {
+  return (b.xmin[d] - g.lo[d] * b.dx[d]) + i * b.dx[d];
+}
+KM_HD double x_centre(const km_block_t &b, const Geo &g, int d, int i) {
+  return (b.xmin[d] - g.lo[d] * b.dx[d]) + (i + 0.5) * b.dx[d];
+}
+// Cylindrical (r, z, phi) geometry, kamayan's grid::Coordinates<cylindrical>
+// (geometry.hpp:35-167): Dx along phi is 2 pi; the r faces have area r dz 2 pi, the z
+// faces (r+^2 - r-^2)/2 2 pi, the cells (1/2) 2 pi (r+^2 - r-^2) dz; the radial centroid
+// is (2/3)(r+^2 |r+| - r-^2 |r-|)/(r+^2 - r-^2).
+constexpr double kTwoPi = 2.0 * 3.141592653589793;
+KM_HD double cyl_area_r(const km_block_t &b, const Geo &g, int i) {
+  return x_face(b, g, 0, i) * b.dx[1] * kTwoPi;
+}
+KM_HD double cyl_area_z(const km_block_t &b, const Geo &g, int i) {
+  const double rp = x_face(b, g, 0, i + 1), rm = x_face(b, g, 0, i);
+  return 0.5 * (rp * rp - rm * rm) * kTwoPi;
+}
+KM_HD double cyl_volume(const km_block_t &b, const Geo &g, int i) {
+  const double rp = x_face(b, g, 0, i + 1), rm = x_face(b, g, 0, i);
+  return 0.5 * kTwoPi * (rp * rp - rm * rm) * b.dx[1];
+}
+KM_HD double cyl_centroid_r(const km_block_t &b, const Geo &g, int i) {
+  const double r0 = x_face(b, g, 0, i), r1 = x_face(b, g, 0, i + 1);
+  const double r0sq = r0 * r0, r1sq = r1 * r1;
+  return (2.0 / 3.0) * (r1sq * __builtin_fabs(r1) - r0sq * __builtin_fabs(r0)) / (r1sq - r0sq);
+}
+// face area of the dir-face at cell (k, j, i) (the face on the cell's low side) and cell
+// volume, for either geometry
+KM_HD double face_area(const km_block_t &b, const Geo &g, bool cyl, int dir, int i) {
+  if (!cyl || dir == 2) return b.area[dir];
+  return dir == 0 ? cyl_area_r(b, g, i) : cyl_area_z(b, g, i);
+}
+KM_HD double cell_volume(const km_block_t &b, const Geo &g, bool cyl, int i) {
+  return cyl ? cyl_volume(b, g, i) : b.vol;
+}
+// utils::Ratio (robust.hpp:29-32): a / (b + sgn(b) 10 DBL_MIN)
+KM_HD double robust_ratio(double a, double b) {
+  const int sgn = (0.0 <= b) - (b < 0.0);
+  return a / (b + sgn * (10.0 * 2.2250738585072014e-308));
+}
+
 // ghost zones of one block, enumerated without the interior: the two x3 slabs (full
 // x1-x2 planes), then the two x2 slabs (interior x3 rows), then the two x1 slabs
 // (interior x2-x3 rows, ng cells at each end); x1 runs fastest within each piece.

@@ -70,6 +70,9 @@ void launch_fluxes_to_dudt(const km_cfg_t &cfg, const km_pack_t &u, const km_pac
 void launch_update(const km_cfg_t &cfg, const km_pack_t &base, const km_pack_t &u0,
                    const km_pack_t &u1, const km_pack_t &dudt, double beta, hipStream_t st,
                    const double *dt_dev, const int *inst);
+// hydro::AddSourceTerms: cylindrical geometric source into dudt (no-op for Cartesian)
+void launch_add_source_terms(const km_cfg_t &cfg, const km_pack_t &u, const km_pack_t &dudt,
+                             hipStream_t st);
 void launch_apply_dudt(const km_cfg_t &cfg, const km_pack_t &base, const km_pack_t &u0,
                        const km_pack_t &u1, const km_pack_t &dudt, double beta, double dt,
                        hipStream_t st, const double *dt_dev = nullptr, const int *inst = nullptr);

@@ -69,6 +69,12 @@ double xf(const km_block_t &b, int d, int idx, int lo) {
   const double x0 = b.xmin[d] - lo * b.dx[d];
   return x0 + idx * b.dx[d];
 }
+// radial centroid of cylindrical cell idx (geometry.hpp:55-63)
+double xc_cyl(const km_block_t &b, int idx, int lo) {
+  const double r0 = xf(b, 0, idx, lo), r1 = xf(b, 0, idx + 1, lo);
+  const double r0sq = r0 * r0, r1sq = r1 * r1;
+  return (2.0 / 3.0) * (r1sq * std::fabs(r1) - r0sq * std::fabs(r0)) / (r1sq - r0sq);
+}
 
 struct Mode {
   int k[3];
@@ -184,16 +190,19 @@ int km_problem_fill_blocks(const char *name, const km_cfg_t *cfg, const km_mesh_
   auto at = [&](int b, int v) { return cell + ((long)b * L.nvar + v) * L.ncell(); };
 
   if (pname == "isentropic_vortex") {
-    // VortexData::State (isentropic_vortex.hpp:31-51), cell centroids
+    // VortexData::State (isentropic_vortex.hpp:31-51) at cell centroids; in cylindrical
+    // geometry State<cylindrical> (:53-72): r^2 = x1^2 at the radial centroid and the
+    // vortex velocity is azimuthal, v_phi = r dv
+    const bool cyl = cfg->geometry == KM_GEOM_CYLINDRICAL;
     const double density = P(0, 1.0), pressure = P(1, 1.0), velx = P(2, 1.0),
                  vely = P(3, 1.0), strength = P(4, 5.0), gamma = cfg->gamma;
     for (int b = 0; b < L.nblocks; ++b)
       for (int k = L.lo(2); k <= L.hi(2); ++k)
         for (int j = L.lo(1); j <= L.hi(1); ++j)
           for (int i = L.lo(0); i <= L.hi(0); ++i) {
-            const double x = xc(blocks[b], 0, i, L.lo(0));
+            const double x = cyl ? xc_cyl(blocks[b], i, L.lo(0)) : xc(blocks[b], 0, i, L.lo(0));
             const double y = xc(blocks[b], 1, j, L.lo(1));
-            const double r2 = x * x + y * y;
+            const double r2 = cyl ? x * x : x * x + y * y;
             const double dv = strength * std::exp(0.5 * (1.0 - r2)) / (2.0 * kPi);
             const double T = pressure / density - (gamma - 1.0) * strength * strength *
                                                       std::exp(1.0 - r2) /
@@ -202,9 +211,15 @@ int km_problem_fill_blocks(const char *name, const km_cfg_t *cfg, const km_mesh_
             const long c = L.c(k, j, i);
             at(b, KM_DENS)[c] = rho;
             at(b, sP)[c] = T * rho;
-            at(b, sV + 0)[c] = velx - y * dv;
-            at(b, sV + 1)[c] = vely + x * dv;
-            at(b, sV + 2)[c] = 0.0;
+            if (cyl) {
+              at(b, sV + 0)[c] = 0.0;
+              at(b, sV + 1)[c] = 0.0;
+              at(b, sV + 2)[c] = x * dv;
+            } else {
+              at(b, sV + 0)[c] = velx - y * dv;
+              at(b, sV + 1)[c] = vely + x * dv;
+              at(b, sV + 2)[c] = 0.0;
+            }
           }
     return KM_OK;
   }

@@ -73,8 +73,12 @@ static int check_cfg(const km_cfg_t *cfg, const km_pack_t *u) {
     set_error("dispatch parm [Fluid] not handled");
     return KM_EINVAL;
   }
-  if (cfg->geometry != KM_GEOM_CARTESIAN) {
-    set_error("dispatch parm [Geometry] not handled: only cartesian is built");
+  if (cfg->geometry != KM_GEOM_CARTESIAN && cfg->geometry != KM_GEOM_CYLINDRICAL) {
+    set_error("dispatch parm [Geometry] not handled");
+    return KM_EINVAL;
+  }
+  if (cfg->geometry == KM_GEOM_CYLINDRICAL && (cfg->mhd != KM_MHD_OFF || u->ndim > 2)) {
+    set_error("cylindrical geometry is provided for hydro in 1-D and 2-D (r, z)");
     return KM_EINVAL;
   }
   if (cfg->mhd != KM_MHD_OFF && cfg->mhd != KM_MHD_CT) {
@@ -622,6 +626,14 @@ int km_fluxes_to_dudt(const km_cfg_t *cfg, const km_pack_t *u, const km_pack_t *
   return check_launch();
 }
 
+int km_add_source_terms(const km_cfg_t *cfg, const km_pack_t *u, const km_pack_t *dudt,
+                        void *stream) {
+  if (int rc = check_cfg(cfg, u)) return rc;
+  if (!dudt) return KM_EINVAL;
+  launch_add_source_terms(*cfg, *u, *dudt, as_stream(stream));
+  return check_launch();
+}
+
 int km_apply_dudt(const km_cfg_t *cfg, const km_pack_t *base, const km_pack_t *u0,
                   const km_pack_t *u1, const km_pack_t *dudt, double beta, double dt,
                   void *stream) {
@@ -675,6 +687,12 @@ int km_ghost_plan_uniform(const km_pack_t *shape, const int32_t nb[3], const int
       return KM_EINVAL;
     }
   }
+  for (int d = 0; d < 6; ++d)
+    if (bc[d] < KM_BC_PERIODIC || bc[d] > KM_BC_AXISYMMETRIC ||
+        (bc[d] == KM_BC_AXISYMMETRIC && d != 0)) {
+      set_error("boundary codes: periodic, outflow, reflect; axisymmetric at ix1 only");
+      return KM_EINVAL;
+    }
   auto *p = new km_ghost_plan_t();
   for (int d = 0; d < 3; ++d) p->grid.nb[d] = nb[d];
   for (int d = 0; d < 6; ++d) p->grid.bc[d] = bc[d];
@@ -780,6 +798,20 @@ static int sim_create(const km_cfg_t *cfg, const km_mesh_desc_t *meshes, int32_t
   s->meshes.assign(meshes, meshes + ninst);
   if (path == KM_PATH_FUSED && cfg->mhd != KM_MHD_OFF) {
     set_error("the fused stage path is built for hydro (MHD off)");
+    return KM_EINVAL;
+  }
+  if (cfg->geometry == KM_GEOM_CYLINDRICAL && (path == KM_PATH_FUSED || tree)) {
+    set_error("cylindrical geometry runs on the task path over a uniform block grid");
+    return KM_EINVAL;
+  }
+  for (int m = 0; m < ninst; ++m)
+    for (int d = 1; d < 6; ++d)
+      if (meshes[m].bc[d] == KM_BC_AXISYMMETRIC) {
+        set_error("the axisymmetric boundary is the inner x1 boundary (ix1_bc) only");
+        return KM_EINVAL;
+      }
+  if (meshes[0].bc[0] == KM_BC_AXISYMMETRIC && cfg->geometry != KM_GEOM_CYLINDRICAL) {
+    set_error("the axisymmetric boundary needs cylindrical geometry");
     return KM_EINVAL;
   }
   if (cfg->nstages < 1 || cfg->nstages > 3) {

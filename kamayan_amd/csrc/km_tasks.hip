@@ -71,7 +71,7 @@ __global__ void __launch_bounds__(256) k_flux(km_pack_t u, int lim, Box bx) {
 // stencil values the march reads, so the faces are bitwise the march's.
 // ---------------------------------------------------------------------------------
 template <class T, int RECON, int RIEM, int MHD>
-__global__ void __launch_bounds__(256) k_flux_x1(km_pack_t u, int lim, Box bx) {
+__global__ void __launch_bounds__(256) k_flux_x1(km_pack_t u, int lim, Box bx, int cyl) {
   constexpr int NR = HV<MHD>::NR, NC = HV<MHD>::NC;
   constexpr int W = ReconWidth<RECON>::value;
   const Geo g(u);
@@ -112,8 +112,16 @@ __global__ void __launch_bounds__(256) k_flux_x1(km_pack_t u, int lim, Box bx) {
   }
   T F[NC];
   riemann<RIEM, MHD, 0>(vL, vR, F);
+  double out[NC];
 #pragma unroll
-  for (int v = 0; v < NC; ++v) stg(flux_ptr(u, 0, b, v), row + c, double(F[v]));
+  for (int v = 0; v < NC; ++v) out[v] = double(F[v]);
+  if (cyl) {  // :108-117: the phi momentum flux carries r (angular momentum)
+    const double rf = x_face(u.blocks[b], g, 0, c);
+    out[C_MOM + 2] *= rf;
+    if constexpr (MHD) out[C_MAGC + 2] *= robust_ratio(1.0, rf);
+  }
+#pragma unroll
+  for (int v = 0; v < NC; ++v) stg(flux_ptr(u, 0, b, v), row + c, out[v]);
 }
 
 // ---------------------------------------------------------------------------------
@@ -173,23 +181,53 @@ __device__ __forceinline__ bool interior_cell(const Geo &g, int nblocks, int &b,
 }
 
 // grid::FluxDivergence (grid_update.hpp:15-57)
-__global__ void __launch_bounds__(256) k_flux_div(km_pack_t u, km_pack_t dudt, int nc) {
+__global__ void __launch_bounds__(256) k_flux_div(km_pack_t u, km_pack_t dudt, int nc, int cyl) {
   const Geo g(u);
   int b, k, j, i;
   if (!interior_cell(g, u.nblocks, b, k, j, i)) return;
   const km_block_t &blk = u.blocks[b];
   const long c = g.c(k, j, i);
   const long sc[3] = {g.sc[0], g.sc[1], g.sc[2]};
+  double am[3], ap[3];
+#pragma unroll
+  for (int dir = 0; dir < 3; ++dir) {
+    am[dir] = face_area(blk, g, cyl, dir, i);
+    ap[dir] = face_area(blk, g, cyl, dir, i + (dir == 0));
+  }
+  const double vol = cell_volume(blk, g, cyl, i);
   for (int var = 0; var < nc; ++var) {
     double du = 0.;
 #pragma unroll
     for (int dir = 0; dir < 3; ++dir) {
       if (dir < g.ndim) {
         const double *F = flux_ptr(u, dir, b, var);
-        du -= (blk.area[dir] * F[c + sc[dir]] - blk.area[dir] * F[c]);
+        du -= (ap[dir] * F[c + sc[dir]] - am[dir] * F[c]);
       }
     }
-    cell_ptr(dudt, b, var)[c] = du / blk.vol;
+    cell_ptr(dudt, b, var)[c] = du / vol;
+  }
+}
+
+// hydro::AddSourceTerms (hydro.cpp:199-234) with AddGeometricSource
+// (primconsflux.hpp:144-159): in cylindrical geometry the r momentum gains
+// (rho v_phi^2 + P) / r at the cell centre (+ (pmag - Bphi^2) / r with MHD)
+template <int MHD>
+__global__ void __launch_bounds__(256) k_geom_source(km_pack_t u, km_pack_t dudt) {
+  constexpr int nc = HV<MHD>::NC;
+  const Geo g(u);
+  int b, k, j, i;
+  if (!interior_cell(g, u.nblocks, b, k, j, i)) return;
+  const long c = g.c(k, j, i);
+  const double xi = x_centre(u.blocks[b], g, 0, i);
+  const double rho = cell_ptr(u, b, KM_DENS)[c], vphi = cell_ptr(u, b, nc + 2)[c];
+  double s = (rho * vphi * vphi + cell_ptr(u, b, nc + 3)[c]) / xi;
+  double *d = cell_ptr(dudt, b, KM_MOM0);
+  d[c] = d[c] + s;
+  if constexpr (MHD) {
+    const double b0 = cell_ptr(u, b, KM_MAGC0)[c], b1 = cell_ptr(u, b, KM_MAGC0 + 1)[c],
+                 b2 = cell_ptr(u, b, KM_MAGC0 + 2)[c];
+    const double pmag = 0.5 * (b0 * b0 + b1 * b1 + b2 * b2);
+    d[c] = d[c] + (pmag - b2 * b2) / xi;
   }
 }
 
@@ -300,10 +338,13 @@ __device__ __forceinline__ void face_average_b(const km_pack_t &u, const Geo &g,
   }
 }
 
-// primitives of one zone from its conserved values U (cell-centred B in U[5..7] for MHD)
+// primitives of one zone from its conserved values U (cell-centred B in U[5..7] for MHD);
+// rinv != 0: cylindrical geometry, where the stored phi momentum is r rho v_phi and
+// PreparePrimitive_impl (primconsflux.cpp:129-137) scales v_phi by Ratio(1, r_centroid)
+// after Cons2Prim (whose kinetic energy uses the stored phi momentum as is)
 template <int MHD>
 __device__ __forceinline__ void primitive_zone(const km_pack_t &u, int b, long c, const Eos &eos,
-                                               const double *U) {
+                                               const double *U, double rinv = 0.0) {
   constexpr int nc = HV<MHD>::NC;
   const double dens = U[0];
   const double idens = 1.0 / dens;
@@ -311,7 +352,7 @@ __device__ __forceinline__ void primitive_zone(const km_pack_t &u, int b, long c
 #pragma unroll
   for (int dir = 0; dir < 3; dir++) {
     const double v = idens * U[1 + dir];
-    cell_ptr(u, b, nc + dir)[c] = v;
+    cell_ptr(u, b, nc + dir)[c] = (dir == 2 && rinv != 0.0) ? v * rinv : v;
     ekin += v * v;
     if constexpr (MHD) {
       const double B = U[5 + dir];
@@ -330,7 +371,7 @@ __device__ __forceinline__ void primitive_zone(const km_pack_t &u, int b, long c
 
 // hydro PreparePrimitive_impl (primconsflux.cpp:82-142) + eos EosWrapped(ener)
 template <int MHD>
-__global__ void __launch_bounds__(256) k_prepare_primitive(km_pack_t u, Eos eos) {
+__global__ void __launch_bounds__(256) k_prepare_primitive(km_pack_t u, Eos eos, int cyl) {
   constexpr int nc = HV<MHD>::NC;
   const Geo g(u);
   int b, k, j, i;
@@ -340,7 +381,8 @@ __global__ void __launch_bounds__(256) k_prepare_primitive(km_pack_t u, Eos eos)
   double U[nc];
 #pragma unroll
   for (int v = 0; v < nc; ++v) U[v] = cell_ptr(u, b, v)[c];
-  primitive_zone<MHD>(u, b, c, eos, U);
+  primitive_zone<MHD>(u, b, c, eos, U,
+                      cyl ? robust_ratio(1.0, cyl_centroid_r(u.blocks[b], g, i)) : 0.0);
 }
 
 // FluxesToDuDt (grid_update.hpp:20-57), the cell part of ApplyDuDt (grid.cpp:213-241)
@@ -351,7 +393,7 @@ __global__ void __launch_bounds__(256) k_prepare_primitive(km_pack_t u, Eos eos)
 template <int MHD>
 __global__ void __launch_bounds__(256) k_update_cell(km_pack_t base, km_pack_t u0, km_pack_t u1,
                                                      double beta, double dt, const double *dtp,
-                                                     const int *inst, Eos eos) {
+                                                     const int *inst, Eos eos, int cyl) {
   constexpr int nc = HV<MHD>::NC;
   const Geo g(u0);
   int b, k, j, i;
@@ -360,6 +402,13 @@ __global__ void __launch_bounds__(256) k_update_cell(km_pack_t base, km_pack_t u
   const km_block_t &blk = u0.blocks[b];
   const long c = g.c(k, j, i);
   const long sc[3] = {g.sc[0], g.sc[1], g.sc[2]};
+  double am[3], ap[3];
+#pragma unroll
+  for (int dir = 0; dir < 3; ++dir) {
+    am[dir] = face_area(blk, g, cyl, dir, i);
+    ap[dir] = face_area(blk, g, cyl, dir, i + (dir == 0));
+  }
+  const double vol = cell_volume(blk, g, cyl, i);
   double du[nc], U[nc];
 #pragma unroll
   for (int var = 0; var < nc; ++var) {
@@ -368,10 +417,15 @@ __global__ void __launch_bounds__(256) k_update_cell(km_pack_t base, km_pack_t u
     for (int dir = 0; dir < 3; ++dir) {
       if (dir < g.ndim) {
         const double *F = flux_ptr(u0, dir, b, var);
-        d -= (blk.area[dir] * F[c + sc[dir]] - blk.area[dir] * F[c]);
+        d -= (ap[dir] * F[c + sc[dir]] - am[dir] * F[c]);
       }
     }
-    du[var] = d / blk.vol;
+    du[var] = d / vol;
+  }
+  if (cyl && !MHD) {  // hydro::AddSourceTerms between FluxesToDuDt and ApplyDuDt
+    const double xi = x_centre(blk, g, 0, i);
+    const double rho = cell_ptr(u0, b, KM_DENS)[c], vphi = cell_ptr(u0, b, nc + 2)[c];
+    du[1] = du[1] + (rho * vphi * vphi + cell_ptr(u0, b, nc + 3)[c]) / xi;
   }
 #pragma unroll
   for (int var = 0; var < nc; ++var)
@@ -382,13 +436,14 @@ __global__ void __launch_bounds__(256) k_update_cell(km_pack_t base, km_pack_t u
     U[var] = U[var] + beta * dt * du[var];
     cell_ptr(u1, b, var)[c] = U[var];
   }
-  if constexpr (!MHD) primitive_zone<0>(u1, b, c, eos, U);
+  if constexpr (!MHD)
+    primitive_zone<0>(u1, b, c, eos, U, cyl ? robust_ratio(1.0, cyl_centroid_r(blk, g, i)) : 0.0);
 }
 
 // eos PrepareConserved(dens_pres) then hydro ConvertToConserved_impl
 // (primconsflux.cpp:19-79)
 template <int MHD>
-__global__ void __launch_bounds__(256) k_prepare_conserved(km_pack_t u, Eos eos) {
+__global__ void __launch_bounds__(256) k_prepare_conserved(km_pack_t u, Eos eos, int cyl) {
   constexpr int nc = HV<MHD>::NC, NR = HV<MHD>::NR;
   const Geo g(u);
   int b, k, j, i;
@@ -404,6 +459,8 @@ __global__ void __launch_bounds__(256) k_prepare_conserved(km_pack_t u, Eos eos)
 #pragma unroll
   for (int r = 0; r < NR; ++r) V[r] = cell_ptr(u, b, recon_slot<MHD>(r))[c];
   prim2cons<MHD>(V, U);
+  // cylindrical: conserve angular momentum (primconsflux.cpp:70-74)
+  if (cyl) U[KM_MOM0 + 2] *= cyl_centroid_r(u.blocks[b], g, i);
 #pragma unroll
   for (int v = 0; v < nc; ++v) cell_ptr(u, b, v)[c] = U[v];
 }
@@ -428,7 +485,8 @@ __global__ void k_fill_value(double *p, double v, int n) {
 // hydro EstimateTimeStep (hydro_time_step.cpp:19-65): min over the interior of
 // dx/(|v| + cf); *out must hold DBL_MAX on entry
 template <int MHD>
-__global__ void __launch_bounds__(256) k_estimate_dt(km_pack_t u, double *out, const int *inst) {
+__global__ void __launch_bounds__(256) k_estimate_dt(km_pack_t u, double *out, const int *inst,
+                                                     int cyl) {
   constexpr int NR = HV<MHD>::NR;
   const Geo g(u);
   int b = 0, k, j, i;
@@ -443,6 +501,8 @@ __global__ void __launch_bounds__(256) k_estimate_dt(km_pack_t u, double *out, c
     dtl = kmin(dtl, blk.dx[0] / (kabs(V[R_VEL + 0]) + fast_speed<MHD, 0>(V)));
     if (g.ndim > 1) dtl = kmin(dtl, blk.dx[1] / (kabs(V[R_VEL + 1]) + fast_speed<MHD, 1>(V)));
     if (g.ndim > 2) dtl = kmin(dtl, blk.dx[2] / (kabs(V[R_VEL + 2]) + fast_speed<MHD, 2>(V)));
+    // cylindrical: the azimuthal crossing time r / |v_phi| (hydro_time_step.cpp:56-60)
+    if (cyl) dtl = kmin(dtl, x_centre(blk, g, 0, i) / kabs(V[R_VEL + 2]));
   }
   if (inst) {
     // instances may change inside a workgroup: reduce per wavefront when it holds one
@@ -494,8 +554,10 @@ __global__ void __launch_bounds__(256) k_fill_divb(km_pack_t u, int slot) {
 
 // ---------------------------------------------------------------------------------
 // Same-level ghost fill of a uniform block grid (Parthenon boundary exchange plus
-// periodic / outflow / reflect boundaries): every ghost zone, including edges and
-// corners, maps per dimension to a source zone of the same or a neighbouring block.
+// periodic / outflow / reflect / axisymmetric boundaries): every ghost zone, including
+// edges and corners, maps per dimension to a source zone of the same or a neighbouring
+// block. Reflect and axisymmetric mirror the zone and flip the normal vector component;
+// axisymmetric (Axisymmetric, boundary_conditions.cpp:23-60) flips the phi component too.
 // ---------------------------------------------------------------------------------
 __device__ __forceinline__ int map_cell(int g, int N, int lo, int hi, bool &flip) {
   if (g < 0) {
@@ -542,6 +604,7 @@ __global__ void __launch_bounds__(256) k_ghost_cells(km_pack_t u, UniformGrid gr
   const int bl[3] = {lb % gr.nb[0], (lb / gr.nb[0]) % gr.nb[1], lb / (gr.nb[0] * gr.nb[1])};
   int src[3], sb[3];
   bool flip[3] = {false, false, false};
+  bool axis = false;  // mirrored across an axisymmetric r = 0 boundary: phi flips too
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     if (d >= g.ndim) {
@@ -550,14 +613,17 @@ __global__ void __launch_bounds__(256) k_ghost_cells(km_pack_t u, UniformGrid gr
       continue;
     }
     const int N = gr.nb[d] * g.nx[d];
-    const int m = map_cell(bl[d] * g.nx[d] + (idx[d] - g.lo[d]), N, gr.bc[2 * d], gr.bc[2 * d + 1],
-                           flip[d]);
+    const int gi = bl[d] * g.nx[d] + (idx[d] - g.lo[d]);
+    if ((gi < 0 && gr.bc[2 * d] == KM_BC_AXISYMMETRIC) ||
+        (gi >= N && gr.bc[2 * d + 1] == KM_BC_AXISYMMETRIC))
+      axis = !axis;
+    const int m = map_cell(gi, N, gr.bc[2 * d], gr.bc[2 * d + 1], flip[d]);
     sb[d] = m / g.nx[d];
     src[d] = m % g.nx[d] + g.lo[d];
   }
   const int srcb = gr.first + (sb[2] * gr.nb[1] + sb[1]) * gr.nb[0] + sb[0];
   const long dc = g.c(idx[2], idx[1], idx[0]), scx = g.c(src[2], src[1], src[0]);
-  const bool fl0 = flip[0], fl1 = flip[1], fl2 = flip[2];
+  const bool fl0 = flip[0], fl1 = flip[1], fl2 = flip[2] != axis;
   // variables go in chunks whose loads are all issued before their stores
   constexpr int kChunk = 4;
   for (int v = v0; v < v1; v += kChunk) {
@@ -623,14 +689,15 @@ __global__ void __launch_bounds__(256) k_ghost_faces(km_pack_t u, UniformGrid gr
 static inline unsigned nblk(long n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
 
 template <class T, int RECON, int RIEM, int MHD>
-static void launch_flux_dirs(const km_pack_t &u, int lim, const Box &bx, hipStream_t st) {
+static void launch_flux_dirs(const km_pack_t &u, int lim, const Box &bx, int cyl,
+                             hipStream_t st) {
   const Geo g(u);
   auto cnt = [&](int da, int db) {
     return (long)(bx.hi[da] - bx.lo[da] + 1) * (bx.hi[db] - bx.lo[db] + 1) * u.nblocks;
   };
   const long nface_x1 = (long)(bx.hi[0] - bx.lo[0] + 2) * cnt(1, 2);
   hipLaunchKernelGGL((k_flux_x1<T, RECON, RIEM, MHD>), dim3(nblk(nface_x1)), dim3(256), 0, st, u,
-                     lim, bx);
+                     lim, bx, cyl);
   if (g.ndim > 1)
     hipLaunchKernelGGL((k_flux<T, RECON, RIEM, MHD, 1>), dim3(nblk(cnt(0, 2))), dim3(256), 0, st, u,
                        lim, bx);
@@ -641,27 +708,28 @@ static void launch_flux_dirs(const km_pack_t &u, int lim, const Box &bx, hipStre
 
 template <int RECON, int RIEM>
 static void launch_flux_mhd(const km_pack_t &u, int mhd, bool fp32, int lim, const Box &bx,
-                            hipStream_t st) {
+                            int cyl, hipStream_t st) {
   if (fp32) {
     if (mhd)
-      launch_flux_dirs<float, RECON, RIEM, 1>(u, lim, bx, st);
+      launch_flux_dirs<float, RECON, RIEM, 1>(u, lim, bx, cyl, st);
     else
-      launch_flux_dirs<float, RECON, RIEM, 0>(u, lim, bx, st);
+      launch_flux_dirs<float, RECON, RIEM, 0>(u, lim, bx, cyl, st);
   } else {
     if (mhd)
-      launch_flux_dirs<double, RECON, RIEM, 1>(u, lim, bx, st);
+      launch_flux_dirs<double, RECON, RIEM, 1>(u, lim, bx, cyl, st);
     else
-      launch_flux_dirs<double, RECON, RIEM, 0>(u, lim, bx, st);
+      launch_flux_dirs<double, RECON, RIEM, 0>(u, lim, bx, cyl, st);
   }
 }
 
 template <int RECON>
 static void launch_flux_riem(const km_pack_t &u, int riem, int mhd, bool fp32, int lim,
-                             const Box &bx, hipStream_t st) {
+                             const Box &bx, int cyl,
+                             hipStream_t st) {
   if (riem == RIEMANN_HLLC)
-    launch_flux_mhd<RECON, RIEMANN_HLLC>(u, mhd, fp32, lim, bx, st);
+    launch_flux_mhd<RECON, RIEMANN_HLLC>(u, mhd, fp32, lim, bx, cyl, st);
   else
-    launch_flux_mhd<RECON, RIEMANN_HLL>(u, mhd, fp32, lim, bx, st);
+    launch_flux_mhd<RECON, RIEMANN_HLL>(u, mhd, fp32, lim, bx, cyl, st);
 }
 
 void launch_calculate_fluxes(const km_cfg_t &cfg, const km_pack_t &u, const Box &bx,
@@ -670,11 +738,12 @@ void launch_calculate_fluxes(const km_cfg_t &cfg, const km_pack_t &u, const Box 
   // contraction only exists in the fused sweeps) and FP32-mixed
   const bool fp32 = cfg.precision == KM_PREC_FP32_MIXED;
   const int r = cfg.riemann, m = cfg.mhd, l = cfg.limiter;
+  const int cyl = cfg.geometry == KM_GEOM_CYLINDRICAL;
   switch (cfg.recon) {
-    case KM_RECON_FOG: launch_flux_riem<RECON_FOG>(u, r, m, fp32, l, bx, st); break;
-    case KM_RECON_PLM: launch_flux_riem<RECON_PLM>(u, r, m, fp32, l, bx, st); break;
-    case KM_RECON_PPM: launch_flux_riem<RECON_PPM>(u, r, m, fp32, l, bx, st); break;
-    default: launch_flux_riem<RECON_WENOZ>(u, r, m, fp32, l, bx, st); break;
+    case KM_RECON_FOG: launch_flux_riem<RECON_FOG>(u, r, m, fp32, l, bx, cyl, st); break;
+    case KM_RECON_PLM: launch_flux_riem<RECON_PLM>(u, r, m, fp32, l, bx, cyl, st); break;
+    case KM_RECON_PPM: launch_flux_riem<RECON_PPM>(u, r, m, fp32, l, bx, cyl, st); break;
+    default: launch_flux_riem<RECON_WENOZ>(u, r, m, fp32, l, bx, cyl, st); break;
   }
 }
 
@@ -697,7 +766,7 @@ void launch_fluxes_to_dudt(const km_cfg_t &cfg, const km_pack_t &u, const km_pac
   const Geo g(u);
   const int nc = cfg.mhd ? 8 : 5;
   hipLaunchKernelGGL(k_flux_div, dim3(nblk(interior_count(g, u.nblocks))), dim3(256), 0, st, u,
-                     dudt, nc);
+                     dudt, nc, cfg.geometry == KM_GEOM_CYLINDRICAL ? 1 : 0);
   if (cfg.mhd && g.ndim >= 2) {
     hipLaunchKernelGGL(k_stokes<0>, dim3(nblk(face_count(g, 0, u.nblocks))), dim3(256), 0, st,
                        u, dudt);
@@ -707,6 +776,17 @@ void launch_fluxes_to_dudt(const km_cfg_t &cfg, const km_pack_t &u, const km_pac
       hipLaunchKernelGGL(k_stokes<2>, dim3(nblk(face_count(g, 2, u.nblocks))), dim3(256), 0, st,
                          u, dudt);
   }
+}
+
+void launch_add_source_terms(const km_cfg_t &cfg, const km_pack_t &u, const km_pack_t &dudt,
+                             hipStream_t st) {
+  if (cfg.geometry != KM_GEOM_CYLINDRICAL) return;  // Cartesian: no source (hydro.cpp:201-205)
+  const Geo g(u);
+  const unsigned grid = nblk(interior_count(g, u.nblocks));
+  if (cfg.mhd)
+    hipLaunchKernelGGL(k_geom_source<1>, dim3(grid), dim3(256), 0, st, u, dudt);
+  else
+    hipLaunchKernelGGL(k_geom_source<0>, dim3(grid), dim3(256), 0, st, u, dudt);
 }
 
 void launch_apply_dudt(const km_cfg_t &cfg, const km_pack_t &base, const km_pack_t &u0,
@@ -733,13 +813,14 @@ void launch_update(const km_cfg_t &cfg, const km_pack_t &base, const km_pack_t &
   const Geo g(u0);
   const Eos eos = make_eos(cfg);
   const unsigned grid = nblk(interior_count(g, u0.nblocks));
+  const int cyl = cfg.geometry == KM_GEOM_CYLINDRICAL;
   if (!cfg.mhd) {
     hipLaunchKernelGGL(k_update_cell<0>, dim3(grid), dim3(256), 0, st, base, u0, u1, beta, 0.0,
-                       dt_dev, inst, eos);
+                       dt_dev, inst, eos, cyl);
     return;
   }
   hipLaunchKernelGGL(k_update_cell<1>, dim3(grid), dim3(256), 0, st, base, u0, u1, beta, 0.0,
-                     dt_dev, inst, eos);
+                     dt_dev, inst, eos, cyl);
   if (g.ndim >= 2) {
     hipLaunchKernelGGL(k_stokes<0>, dim3(nblk(face_count(g, 0, u0.nblocks))), dim3(256), 0, st,
                        u0, dudt);
@@ -756,27 +837,29 @@ void launch_update(const km_cfg_t &cfg, const km_pack_t &base, const km_pack_t &
       hipLaunchKernelGGL(k_apply_face<2>, dim3(nblk(face_count(g, 2, u0.nblocks))), dim3(256),
                          0, st, base, u0, u1, dudt, beta, 0.0, dt_dev, inst);
   }
-  hipLaunchKernelGGL(k_prepare_primitive<1>, dim3(grid), dim3(256), 0, st, u1, eos);
+  hipLaunchKernelGGL(k_prepare_primitive<1>, dim3(grid), dim3(256), 0, st, u1, eos, cyl);
 }
 
 void launch_prepare_primitive(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st) {
   const Geo g(u);
   const Eos eos = make_eos(cfg);
   const unsigned grid = nblk(interior_count(g, u.nblocks));
+  const int cyl = cfg.geometry == KM_GEOM_CYLINDRICAL;
   if (cfg.mhd)
-    hipLaunchKernelGGL(k_prepare_primitive<1>, dim3(grid), dim3(256), 0, st, u, eos);
+    hipLaunchKernelGGL(k_prepare_primitive<1>, dim3(grid), dim3(256), 0, st, u, eos, cyl);
   else
-    hipLaunchKernelGGL(k_prepare_primitive<0>, dim3(grid), dim3(256), 0, st, u, eos);
+    hipLaunchKernelGGL(k_prepare_primitive<0>, dim3(grid), dim3(256), 0, st, u, eos, cyl);
 }
 
 void launch_prepare_conserved(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st) {
   const Geo g(u);
   const Eos eos = make_eos(cfg);
   const unsigned grid = nblk(interior_count(g, u.nblocks));
+  const int cyl = cfg.geometry == KM_GEOM_CYLINDRICAL;
   if (cfg.mhd)
-    hipLaunchKernelGGL(k_prepare_conserved<1>, dim3(grid), dim3(256), 0, st, u, eos);
+    hipLaunchKernelGGL(k_prepare_conserved<1>, dim3(grid), dim3(256), 0, st, u, eos, cyl);
   else
-    hipLaunchKernelGGL(k_prepare_conserved<0>, dim3(grid), dim3(256), 0, st, u, eos);
+    hipLaunchKernelGGL(k_prepare_conserved<0>, dim3(grid), dim3(256), 0, st, u, eos, cyl);
 }
 
 void launch_estimate_dt(const km_cfg_t &cfg, const km_pack_t &u, double *dt_dev,
@@ -784,10 +867,11 @@ void launch_estimate_dt(const km_cfg_t &cfg, const km_pack_t &u, double *dt_dev,
   const Geo g(u);
   hipLaunchKernelGGL(k_fill_value, dim3(nblk(ninst, 64)), dim3(64), 0, st, dt_dev, DBL_MAX, ninst);
   const unsigned grid = nblk(interior_count(g, u.nblocks));
+  const int cyl = cfg.geometry == KM_GEOM_CYLINDRICAL;
   if (cfg.mhd)
-    hipLaunchKernelGGL(k_estimate_dt<1>, dim3(grid), dim3(256), 0, st, u, dt_dev, inst);
+    hipLaunchKernelGGL(k_estimate_dt<1>, dim3(grid), dim3(256), 0, st, u, dt_dev, inst, cyl);
   else
-    hipLaunchKernelGGL(k_estimate_dt<0>, dim3(grid), dim3(256), 0, st, u, dt_dev, inst);
+    hipLaunchKernelGGL(k_estimate_dt<0>, dim3(grid), dim3(256), 0, st, u, dt_dev, inst, cyl);
   hipLaunchKernelGGL(k_scale_value, dim3(nblk(ninst, 64)), dim3(64), 0, st, dt_dev,
                      cfg.cfl / static_cast<double>(g.ndim), ninst);
 }

@@ -388,6 +388,40 @@ struct Range {
 // CalculateFluxesNested::dispatch, hydro_add_flux_tasks.cpp:36-264 (the scratch-var
 // strategy, :266-385, evaluates the same faces from the same states)
 // ---------------------------------------------------------------------------------
+// ---- cylindrical (r, z, phi) geometry: grid::Coordinates<cylindrical>
+// (geometry.hpp:35-167) on Parthenon's uniform coordinates, whose x1 faces sit at
+// (xmin - is dx) + i dx; Dx along phi is 2 pi (:37-40)
+constexpr double kPhiSpan = 2.0 * 3.141592653589793;
+inline bool Cyl(const km_cfg_t &cfg) { return cfg.geometry == KM_GEOM_CYLINDRICAL; }
+inline double RFace(const km_block_t &blk, const Shape &sh, int i) {
+  return (blk.xmin[0] - sh.is * blk.dx[0]) + i * blk.dx[0];
+}
+inline double RCentre(const km_block_t &blk, const Shape &sh, int i) {  // Xi, :67-70
+  return (blk.xmin[0] - sh.is * blk.dx[0]) + (i + 0.5) * blk.dx[0];
+}
+inline double RCentroid(const km_block_t &blk, const Shape &sh, int i) {  // Xc, :55-63
+  const double r0 = RFace(blk, sh, i), r1 = RFace(blk, sh, i + 1);
+  const double r0sq = r0 * r0, r1sq = r1 * r1;
+  return (2.0 / 3.0) * (r1sq * std::abs(r1) - r0sq * std::abs(r0)) / (r1sq - r0sq);
+}
+// FaceArea<axis> at the low face of cell i (:124-137) and CellVolume (:154-160)
+inline double Area(const km_cfg_t &cfg, const km_block_t &blk, const Shape &sh, int dir, int i) {
+  if (!Cyl(cfg) || dir == 2) return blk.area[dir];
+  if (dir == 0) return RFace(blk, sh, i) * blk.dx[1] * kPhiSpan;
+  const double rp = RFace(blk, sh, i + 1), rm = RFace(blk, sh, i);
+  return 0.5 * (rp * rp - rm * rm) * kPhiSpan;
+}
+inline double Volume(const km_cfg_t &cfg, const km_block_t &blk, const Shape &sh, int i) {
+  if (!Cyl(cfg)) return blk.vol;
+  const double rp = RFace(blk, sh, i + 1), rm = RFace(blk, sh, i);
+  return 0.5 * kPhiSpan * (rp * rp - rm * rm) * blk.dx[1];
+}
+// utils::Ratio, robust.hpp:29-32
+inline double Ratio(double a, double b) {
+  const int sgn = (0.0 <= b) - (b < 0.0);
+  return a / (b + sgn * (10.0 * std::numeric_limits<double>::min()));
+}
+
 void CalculateFluxes(const km_cfg_t &cfg, const km_pack_t &u) {
   const Shape sh(u);
   const int mhd = cfg.mhd;
@@ -436,6 +470,11 @@ void CalculateFluxes(const km_cfg_t &cfg, const km_pack_t &u) {
               vR[R_MAGC + 0] = bf;
             }
             RiemannFlux(cfg.riemann, mhd, 0, vL, vR, F);
+            if (Cyl(cfg)) {  // :108-117, angular momentum flux
+              const double rf = RFace(u.blocks[b], sh, i);
+              F[KM_MOM0 + 2] *= rf;
+              if (mhd == KM_MHD_CT) F[KM_MAGC0 + 2] *= Ratio(1.0, rf);
+            }
             for (int v = 0; v < ncons(mhd); ++v) fluxp(u, 0, b, v)[sh.c(k, j, i)] = F[v];
           }
         }
@@ -577,9 +616,10 @@ void FluxesToDuDt(const km_cfg_t &cfg, const km_pack_t &u, const km_pack_t &dudt
             double du = 0.;
             for (int dir = 0; dir < ndim; ++dir) {
               const double *F = fluxp(u, dir, b, var);
-              du -= (blk.area[dir] * F[c + off[dir]] - blk.area[dir] * F[c]);
+              du -= (Area(cfg, blk, sh, dir, i + (dir == 0)) * F[c + off[dir]] -
+                     Area(cfg, blk, sh, dir, i) * F[c]);
             }
-            cellp(dudt, b, var)[c] = du / blk.vol;
+            cellp(dudt, b, var)[c] = du / Volume(cfg, blk, sh, i);
           }
         }
     }
@@ -624,6 +664,32 @@ void FluxesToDuDt(const km_cfg_t &cfg, const km_pack_t &u, const km_pack_t &dudt
 // ---------------------------------------------------------------------------------
 // ApplyDuDt_impl, grid.cpp:213-283
 // ---------------------------------------------------------------------------------
+// hydro::AddSourceTerms, hydro.cpp:199-234 (no-op in Cartesian geometry), with
+// AddGeometricSource, primconsflux.hpp:144-159: primitives of u, into dudt
+void AddSourceTerms(const km_cfg_t &cfg, const km_pack_t &u, const km_pack_t &dudt) {
+  if (!Cyl(cfg)) return;
+  const Shape sh(u);
+  const int mhd = cfg.mhd;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int b = 0; b < u.nblocks; ++b)
+    for (int k = sh.ks; k <= sh.ke; ++k)
+      for (int j = sh.js; j <= sh.je; ++j)
+        for (int i = sh.is; i <= sh.ie; ++i) {
+          const long c = sh.c(k, j, i);
+          const double r = RCentre(u.blocks[b], sh, i);
+          const double rho = cellp(u, b, KM_DENS)[c];
+          const double vphi = cellp(u, b, KM_VEL0(mhd) + 2)[c];
+          double &d = cellp(dudt, b, KM_MOM0)[c];
+          d += (rho * vphi * vphi + cellp(u, b, KM_PRES(mhd))[c]) / r;
+          if (mhd != KM_MHD_OFF) {
+            const double b0 = cellp(u, b, KM_MAGC0)[c], b1 = cellp(u, b, KM_MAGC0 + 1)[c],
+                         b2 = cellp(u, b, KM_MAGC0 + 2)[c];
+            const double pmag = 0.5 * (b0 * b0 + b1 * b1 + b2 * b2);
+            d += (pmag - b2 * b2) / r;
+          }
+        }
+}
+
 void ApplyDuDt(const km_cfg_t &cfg, const km_pack_t &base, const km_pack_t &u0,
                const km_pack_t &u1, const km_pack_t &dudt, double beta, double dt) {
   const Shape sh(u0);
@@ -709,6 +775,11 @@ void PreparePrimitive(const km_cfg_t &cfg, const km_pack_t &u) {
           const double eint = cellp(u, b, KM_ENER)[c] - ekin - emag;
           const double sie = idens * eint;
           cellp(u, b, KM_EINT(mhd))[c] = sie;
+          if (Cyl(cfg)) {  // primconsflux.cpp:129-137: v_phi from the angular momentum
+            double &vphi = cellp(u, b, KM_VEL0(mhd) + 2)[c];
+            vphi *= Ratio(1.0, RCentroid(u.blocks[b], sh, i));
+            if (mhd != KM_MHD_OFF) cellp(u, b, KM_MAGC0 + 2)[c] *= RCentroid(u.blocks[b], sh, i);
+          }
           // EosMode::ener -> temperature | pressure | bulk_modulus
           cellp(u, b, KM_PRES(mhd))[c] = eos.P(dens, sie);
           cellp(u, b, KM_TEMP(mhd))[c] = eos.T(dens, sie);
@@ -738,11 +809,13 @@ void PrepareConserved(const km_cfg_t &cfg, const km_pack_t &u) {
           double V[10], U[8];
           for (int r = 0; r < nrecon(mhd); ++r) V[r] = cellp(u, b, recon_slot(r, mhd))[c];
           Prim2Cons(mhd, V, U);
+          // cylindrical: conserve angular momentum, primconsflux.cpp:70-74
+          if (Cyl(cfg)) U[KM_MOM0 + 2] *= RCentroid(u.blocks[b], sh, i);
           for (int v = 0; v < ncons(mhd); ++v) cellp(u, b, v)[c] = U[v];
         }
 }
 
-// EstimateTimeStep::dispatch, hydro_time_step.cpp:19-65 (Cartesian)
+// EstimateTimeStep::dispatch, hydro_time_step.cpp:19-65
 double EstimateDt(const km_cfg_t &cfg, const km_pack_t &u) {
   const Shape sh(u);
   const int mhd = cfg.mhd;
@@ -762,6 +835,8 @@ double EstimateDt(const km_cfg_t &cfg, const km_pack_t &u) {
             const double cfast = FastSpeed(mhd, dir, V);
             dt_min = kmin(dt_min, blk.dx[dir] / (std::fabs(V[R_VEL + dir]) + cfast));
           }
+          if (Cyl(cfg))  // :56-60
+            dt_min = kmin(dt_min, RCentre(blk, sh, i) / std::fabs(V[R_VEL + 2]));
         }
     }
   return dt_min * cfl;
@@ -872,6 +947,9 @@ void GhostFill(const km_cfg_t &cfg, const Grid &grid, const km_pack_t &u, int v0
               continue;
             }
             const int g = bl[d] * sh.nx[d] + (idx[d] - st[d]);
+            // Axisymmetric (boundary_conditions.cpp:23-60): a mirror across r = 0 that
+            // also negates vector component 3 (phi)
+            if (d == 0 && g < 0 && grid.bc[0] == KM_BC_AXISYMMETRIC) flip[2] = !flip[2];
             const int m = map_cell(g, N[d], grid.bc[2 * d], grid.bc[2 * d + 1], &flip[d]);
             sb[d] = m / sh.nx[d];
             src[d] = m % sh.nx[d] + st[d];
@@ -2242,6 +2320,7 @@ void Cycle(km_cpu_sim_t &s) {
       if (s.cfg.mhd == KM_MHD_CT) AmrEmfCorrect(s.tree, p0);
     }
     FluxesToDuDt(s.cfg, p0, s.dudt.pack);
+    AddSourceTerms(s.cfg, p0, s.dudt.pack);  // AddTasksOneStep
     ApplyDuDt(s.cfg, s.base.pack, p0, md1.pack, s.dudt.pack, betas[stage - 1], dt);
     PreparePrimitive(s.cfg, md1.pack);
     if (stage == s.nstages) {
@@ -2340,6 +2419,11 @@ int km_cpu_calculate_emf(const km_cfg_t *cfg, const km_pack_t *u, void *) {
 int km_cpu_fluxes_to_dudt(const km_cfg_t *cfg, const km_pack_t *u, const km_pack_t *dudt,
                           void *) {
   FluxesToDuDt(*cfg, *u, *dudt);
+  return KM_OK;
+}
+int km_cpu_add_source_terms(const km_cfg_t *cfg, const km_pack_t *u, const km_pack_t *dudt,
+                            void *) {
+  AddSourceTerms(*cfg, *u, *dudt);
   return KM_OK;
 }
 int km_cpu_apply_dudt(const km_cfg_t *cfg, const km_pack_t *base, const km_pack_t *u0,

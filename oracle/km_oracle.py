@@ -45,6 +45,7 @@ def lib() -> C.CDLL:
         "km_cpu_calculate_fluxes": [P(KmCfg), P(KmPack), vp],
         "km_cpu_calculate_emf": [P(KmCfg), P(KmPack), vp],
         "km_cpu_fluxes_to_dudt": [P(KmCfg), P(KmPack), P(KmPack), vp],
+        "km_cpu_add_source_terms": [P(KmCfg), P(KmPack), P(KmPack), vp],
         "km_cpu_apply_dudt": [P(KmCfg), P(KmPack), P(KmPack), P(KmPack), P(KmPack), dbl, dbl,
                               vp],
         "km_cpu_prepare_primitive": [P(KmCfg), P(KmPack), vp],
@@ -175,6 +176,37 @@ def amr_update_tree(desc: KmMeshDesc, numlevel: int, derefine_count: int, leaves
 def amr_remesh(old: KmPack, new: KmPack, origin: np.ndarray, v0: int, v1: int):
     origin = np.ascontiguousarray(origin, dtype=np.int32)
     call("km_cpu_amr_remesh", C.byref(old), C.byref(new), _i32(origin), v0, v1)
+
+
+def initial_refinement(cfg, desc: KmMeshDesc, layout: Layout, problem_fill, adapt: KmAmrAdapt,
+                       leaves=None):
+    """The oracle's Parthenon initial-refinement loop: problem_fill(leaves) -> cell (the
+    library's generator, host code), ghost fill through the oracle's simulation, Loehner
+    tags and tree update, until the tree stops changing. Returns (leaves, cell)."""
+    nd = layout.ndim
+    if leaves is None:
+        leaves = np.array([(0, i, j, k) for k in range(layout.nb[2] if nd > 2 else 1)
+                           for j in range(layout.nb[1] if nd > 1 else 1)
+                           for i in range(layout.nb[0])], dtype=np.int32)
+    for _ in range(max(adapt.numlevel, 1)):
+        lay = Layout(layout.ndim, layout.ng, layout.nx, layout.nb, nleaves=len(leaves))
+        cell = problem_fill(leaves, lay)
+        sim = CpuSimulation(cfg, desc, lay, leaves=leaves)
+        sim.set_primitive(cell)
+        state, _ = sim.state()
+        sim.close()
+        hp = HostPack(lay, state.shape[1], blocks=amr_blocks(desc, leaves))
+        hp.cell[:] = state
+        tags = np.full(len(leaves), -1, np.int32)
+        for c in range(adapt.ncrit):
+            amr_tag_loehner(hp.pack, adapt.crit[c], tags, c == 0)
+        nl, _, _, changed = amr_update_tree(desc, adapt.numlevel, 1 << 30, leaves, tags,
+                                            np.zeros(len(leaves), np.int32))
+        if not changed:
+            return leaves, cell
+        leaves = nl
+    lay = Layout(layout.ndim, layout.ng, layout.nx, layout.nb, nleaves=len(leaves))
+    return leaves, problem_fill(leaves, lay)
 
 
 class HostPack:

@@ -21,6 +21,44 @@ def vortex(n=64, nblock=32, ng=3, recon="wenoz", limiter="minmod", riemann="hll"
     return cfg, desc, lay, cell, faces
 
 
+def vortex_cyl(n=64, nblock=8, ng=4, recon="wenoz", limiter="minmod", riemann="hll",
+               integrator="rk2", cfl=0.8):
+    """The reconstruction suite's cylindrical vortex (reconstruction.py:20-130,136,172-181):
+    1-D in r over [-0, 5] in 8-cell blocks, nghost 4, axisymmetric inner and outflow outer
+    boundary, x2 over [-5, 5] with one cell, the isentropic_vortex.in parameters."""
+    cfg = Config(**{"hydro/reconstruction": recon, "hydro/slope_limiter": limiter,
+                    "hydro/riemann": riemann, "parthenon/time/integrator": integrator,
+                    "hydro/cfl": cfl, "material/single/gamma": 1.4,
+                    "geometry/geometry": "cylindrical"})
+    cfg.c.ndim = 1
+    lay = Layout(1, ng, (nblock, 1, 1), (n // nblock, 1, 1))
+    desc = mesh_desc(lay, xmin=(-0.0, -5.0, -5.0), xmax=(5.0, 5.0, 5.0),
+                     bc=("axisymmetric", "outflow", "periodic", "periodic", "periodic",
+                         "periodic"))
+    cell, faces = km.problem_fill("isentropic_vortex", cfg, desc, lay)
+    return cfg, desc, lay, cell, faces
+
+
+def vortex_cyl_l1(cell, lay: Layout, blocks, var, strength=5.0):
+    """ErrorHistory (isentropic_vortex.hpp:150-200) in cylindrical geometry against the t=0
+    field: sum |exact(r_c) - q| V / (size_x1 size_x2), r_c the radial centroid, V the
+    cylindrical cell volume; var is the cell slot (the exact v_phi = r dv)."""
+    g = lay.ng
+    err = 0.0
+    for b in range(lay.nblocks):
+        bk = blocks[b]
+        x0 = bk.xmin[0] - g * bk.dx[0]
+        i = np.arange(g, g + lay.nx[0])
+        r0 = x0 + i * bk.dx[0]
+        r1 = x0 + (i + 1) * bk.dx[0]
+        rc = (2.0 / 3.0) * (r1 * r1 * np.abs(r1) - r0 * r0 * np.abs(r0)) / (r1 * r1 - r0 * r0)
+        vol = 0.5 * (2.0 * np.pi) * (r1 * r1 - r0 * r0) * bk.dx[1]
+        dv = strength * np.exp(0.5 * (1.0 - rc * rc)) / (2.0 * np.pi)
+        exact = {km.slot("VEL2"): rc * dv}[var]
+        err += float(np.sum(np.abs(exact - cell[b, var, 0, 0, g:g + lay.nx[0]]) * vol))
+    return err / (5.0 * 10.0)
+
+
 def smooth3d(n=32, nblock=16, ng=4, recon="wenoz", riemann="hllc", integrator="rk3",
              seed=20261204, precision="fp64"):
     """3-D ideal hydro with the seeded smooth Fourier-mode state, gamma=5/3, periodic."""

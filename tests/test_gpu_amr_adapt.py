@@ -150,3 +150,42 @@ def test_adaptive_run_bitwise(gpu, oracle, name, kw, numlevel, ncyc, dcount):
             assert sim.info() == ref.info()
     assert len(seen) > 1  # the tree changed during the run
     sim.close()
+
+
+def test_initial_refinement_then_adaptive_run_bitwise(gpu, oracle):
+    # Parthenon's initial loop on the sedov.py:38 shape (32^2 root in 8^2 blocks,
+    # nghost 4, numlevel 3), then adaptive cycles, against the oracle's loop and run
+    km = gpu
+    cfg = km.Config(**{"hydro/reconstruction": "wenoz", "hydro/slope_limiter": "minmod",
+                       "hydro/riemann": "hllc", "parthenon/time/integrator": "rk2",
+                       "material/single/gamma": 1.4})
+    cfg.c.ndim = 2
+    lay = Layout(2, 4, (8, 8, 1), (4, 4, 1))
+    desc = mesh_desc(lay, xmin=(-0.5,) * 3, xmax=(0.5,) * 3, bc=("outflow",) * 6)
+    params = [1.0, 0.05, 1.0, 1e-5]
+    adapt = amr_adapt(3, [amr_criterion(slot("PRES"), 3)])
+    leaves, cell, _ = km.initial_refinement(cfg, desc, lay, "sedov", adapt, params=params)
+
+    def fill(lv, lay_l):
+        c, _ = km.problem_fill("sedov", cfg, desc, lay_l, params=params, leaves=lv)
+        return c
+    rleaves, rcell = oracle.initial_refinement(cfg.c, desc, lay, fill, adapt)
+    print(f"initial refinement: {len(leaves)} leaves", flush=True)
+    _same(leaves, rleaves, "initial leaves")
+    _same(cell, rcell, "initial state")
+    assert max(leaves[:, 0]) == 2
+    lay_l = Layout(2, 4, (8, 8, 1), (4, 4, 1), nleaves=len(leaves))
+    sim = km.Simulation(cfg, desc, lay_l, path=km.PATH_TASKS, leaves=leaves)
+    sim.set_primitive(cell)
+    sim.set_adaptive(adapt)
+    ref = oracle.CpuSimulation(cfg.c, desc, lay_l, leaves=leaves)
+    ref.set_primitive(cell)
+    ref.set_adaptive(adapt)
+    sim.advance(8)
+    ref.advance(8)
+    _same(sim.leaves(), ref.leaves(), "leaves after 8 cycles")
+    g, _ = sim.state()
+    r, _ = ref.state()
+    _same(g, r, "state after 8 cycles")
+    assert sim.info() == ref.info()
+    sim.close()
