@@ -159,6 +159,10 @@ int km_fluxes_to_dudt(const km_cfg_t *cfg, const km_pack_t *u, const km_pack_t *
  * primitives of u; nothing in Cartesian geometry. */
 int km_add_source_terms(const km_cfg_t *cfg, const km_pack_t *u, const km_pack_t *dudt,
                         void *stream);
+/* hydro::PrepareConserved, the per-stage task on md0 between AddTasksOneStep and ApplyDuDt
+ * (hydro.cpp:244-286, kamayan_driver.cpp:224-231): with CT in cylindrical geometry the cell
+ * B_phi becomes B_phi / r_centroid; nothing otherwise. */
+int km_prepare_conserved_stage(const km_cfg_t *cfg, const km_pack_t *u, void *stream);
 /* grid::ApplyDuDt (grid.cpp:213-283):
  *   u0 <- beta*base + (1-beta)*u0 ;  u1 <- u0 + (beta*dt)*dudt
  * over the interior of every conserved cell var and (CT) every face element. */

@@ -54,6 +54,7 @@ def lib() -> C.CDLL:
         "km_calculate_emf": [P(KmCfg), P(KmPack), vp],
         "km_fluxes_to_dudt": [P(KmCfg), P(KmPack), P(KmPack), vp],
         "km_add_source_terms": [P(KmCfg), P(KmPack), P(KmPack), vp],
+        "km_prepare_conserved_stage": [P(KmCfg), P(KmPack), vp],
         "km_apply_dudt": [P(KmCfg), P(KmPack), P(KmPack), P(KmPack), P(KmPack), dbl, dbl, vp],
         "km_prepare_primitive": [P(KmCfg), P(KmPack), vp],
         "km_prepare_conserved": [P(KmCfg), P(KmPack), vp],

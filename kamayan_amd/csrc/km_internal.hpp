@@ -62,7 +62,7 @@ constexpr int KM_NBR_REFLECT = -2;
 // task launchers (km_tasks.hip)
 void launch_calculate_fluxes(const km_cfg_t &cfg, const km_pack_t &u, const Box &bx,
                              hipStream_t st);
-void launch_calculate_emf(const km_pack_t &u, hipStream_t st);
+void launch_calculate_emf(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st);
 void launch_fluxes_to_dudt(const km_cfg_t &cfg, const km_pack_t &u, const km_pack_t &dudt,
                            hipStream_t st);
 // FluxesToDuDt, ApplyDuDt and PreparePrimitive of one stage, with the stage's dt on the
@@ -81,6 +81,8 @@ void launch_prepare_conserved(const km_cfg_t &cfg, const km_pack_t &u, hipStream
 void launch_estimate_dt(const km_cfg_t &cfg, const km_pack_t &u, double *dt_dev, hipStream_t st,
                         const int *inst = nullptr, int ninst = 1);
 void launch_fill_divb(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st);
+// hydro::PrepareConserved per stage (cylindrical CT: md0's B_phi to B_phi / r)
+void launch_prepare_conserved_stage(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st);
 void launch_ghost_fill(const UniformGrid &gr, const km_pack_t &u, int v0, int v1, int nc,
                        bool faces, hipStream_t st, Instances in = Instances());
 

@@ -196,6 +196,60 @@ int km_problem_fill_blocks(const char *name, const km_cfg_t *cfg, const km_mesh_
     const bool cyl = cfg->geometry == KM_GEOM_CYLINDRICAL;
     const double density = P(0, 1.0), pressure = P(1, 1.0), velx = P(2, 1.0),
                  vely = P(3, 1.0), strength = P(4, 5.0), gamma = cfg->gamma;
+    const double mstr = P(5, 1.0);  // mhd_strength (isentropic_vortex.cpp:49)
+    if (mhd != KM_MHD_OFF) {
+      // StateMHD (isentropic_vortex.hpp:74-124): unit density, magnetic tension and
+      // centrifugal force balanced by pressure; B = (-y, x) dB (Cartesian) or B_phi = r dB
+      const double k8 = 8.0 * kPi * kPi;
+      for (int b = 0; b < L.nblocks; ++b)
+        for (int k = L.lo(2); k <= L.hi(2); ++k)
+          for (int j = L.lo(1); j <= L.hi(1); ++j)
+            for (int i = L.lo(0); i <= L.hi(0); ++i) {
+              const double x = cyl ? xc_cyl(blocks[b], i, L.lo(0)) : xc(blocks[b], 0, i, L.lo(0));
+              const double y = xc(blocks[b], 1, j, L.lo(1));
+              const double r2 = cyl ? x * x : x * x + y * y;
+              const double dv = strength * std::exp(0.5 * (1.0 - r2)) / (2.0 * kPi);
+              const double dB = mstr * std::exp(0.5 * (1.0 - r2)) / (2.0 * kPi);
+              const double ex = std::exp(1.0 - r2);
+              const long c = L.c(k, j, i);
+              at(b, KM_DENS)[c] = 1.0;
+              at(b, sP)[c] = pressure + (mstr * mstr / k8) * (1.0 - r2) * ex -
+                             (strength * strength / k8) * ex;
+              if (cyl) {
+                at(b, sV + 2)[c] = x * dv;
+                at(b, KM_MAGC0 + 2)[c] = x * dB;
+              } else {
+                at(b, sV + 0)[c] = velx - y * dv;
+                at(b, sV + 1)[c] = vely + x * dv;
+                at(b, KM_MAGC0 + 0)[c] = -y * dB;
+                at(b, KM_MAGC0 + 1)[c] = +x * dB;
+              }
+            }
+      if (mhd == KM_MHD_CT && L.ndim > 1 && !cyl && face && face[0] && face[1]) {
+        // div-free faces from Az = mhd_strength / (2 pi) exp((1 - r^2) / 2)
+        // (isentropic_vortex.cpp:154-174)
+        auto Az = [&](double xx, double yy) {
+          return mstr / (2. * kPi) * std::exp(0.5 * (1.0 - (xx * xx + yy * yy)));
+        };
+        const int k3 = L.ndim > 2 ? 1 : 0;
+        for (int b = 0; b < L.nblocks; ++b) {
+          const km_block_t &bk = blocks[b];
+          double *B1 = face[0] + (long)b * L.nstag(), *B2 = face[1] + (long)b * L.nstag();
+          for (int k = L.lo(2); k <= L.hi(2) + k3; ++k)
+            for (int j = L.lo(1); j <= L.hi(1) + 1; ++j)
+              for (int i = L.lo(0); i <= L.hi(0) + 1; ++i) {
+                const double xfx = xf(bk, 0, i, L.lo(0)), xfy = xc(bk, 1, j, L.lo(1));
+                const double dy = bk.dx[1];
+                B1[L.s(k, j, i)] = 1. / dy * (Az(xfx, xfy + 0.5 * dy) - Az(xfx, xfy - 0.5 * dy));
+                const double yfx = xc(bk, 0, i, L.lo(0)), yfy = xf(bk, 1, j, L.lo(1));
+                const double dx = bk.dx[0];
+                B2[L.s(k, j, i)] =
+                    -1. / dx * (Az(yfx + 0.5 * dx, yfy) - Az(yfx - 0.5 * dx, yfy));
+              }
+        }
+      }
+      return KM_OK;
+    }
     for (int b = 0; b < L.nblocks; ++b)
       for (int k = L.lo(2); k <= L.hi(2); ++k)
         for (int j = L.lo(1); j <= L.hi(1); ++j)

@@ -77,8 +77,8 @@ static int check_cfg(const km_cfg_t *cfg, const km_pack_t *u) {
     set_error("dispatch parm [Geometry] not handled");
     return KM_EINVAL;
   }
-  if (cfg->geometry == KM_GEOM_CYLINDRICAL && (cfg->mhd != KM_MHD_OFF || u->ndim > 2)) {
-    set_error("cylindrical geometry is provided for hydro in 1-D and 2-D (r, z)");
+  if (cfg->geometry == KM_GEOM_CYLINDRICAL && u->ndim > 2) {
+    set_error("cylindrical geometry is provided in 1-D and 2-D (r, z)");
     return KM_EINVAL;
   }
   if (cfg->mhd != KM_MHD_OFF && cfg->mhd != KM_MHD_CT) {
@@ -87,10 +87,6 @@ static int check_cfg(const km_cfg_t *cfg, const km_pack_t *u) {
   }
   if (u->ndim < 1 || u->ndim > 3 || u->nblocks < 0) {
     set_error("bad pack shape");
-    return KM_EINVAL;
-  }
-  if (cfg->mhd == KM_MHD_CT && u->ndim < 2) {
-    set_error("constrained transport needs ndim >= 2");
     return KM_EINVAL;
   }
   if (u->ng < km_cfg_min_ghost(cfg)) {
@@ -369,7 +365,7 @@ int task_stage(km_sim_t &s, int stage, double beta, double dt) {
   }
   const bool ct = cfg.mhd == KM_MHD_CT;
   launch_calculate_fluxes(cfg, p0, flux_box(cfg, p0), s.stream);
-  if (ct) launch_calculate_emf(p0, s.stream);
+  if (ct) launch_calculate_emf(cfg, p0, s.stream);
   // flux correction at level jumps (AddFluxCorrectionTasks, kamayan_driver.cpp:198-207):
   // cell fluxes and, for CT, the edge EMFs
   if (s.amr) {
@@ -614,7 +610,7 @@ int km_calculate_emf(const km_cfg_t *cfg, const km_pack_t *u, void *stream) {
     set_error("edge tables missing");
     return KM_EINVAL;
   }
-  launch_calculate_emf(*u, as_stream(stream));
+  launch_calculate_emf(*cfg, *u, as_stream(stream));
   return check_launch();
 }
 
@@ -631,6 +627,12 @@ int km_add_source_terms(const km_cfg_t *cfg, const km_pack_t *u, const km_pack_t
   if (int rc = check_cfg(cfg, u)) return rc;
   if (!dudt) return KM_EINVAL;
   launch_add_source_terms(*cfg, *u, *dudt, as_stream(stream));
+  return check_launch();
+}
+
+int km_prepare_conserved_stage(const km_cfg_t *cfg, const km_pack_t *u, void *stream) {
+  if (int rc = check_cfg(cfg, u)) return rc;
+  launch_prepare_conserved_stage(*cfg, *u, as_stream(stream));
   return check_launch();
 }
 

@@ -16,7 +16,7 @@ namespace km {
 // reconstructed and faces bx.lo..bx.hi+1 solved, as in the reference's loops.
 // ---------------------------------------------------------------------------------
 template <class T, int RECON, int RIEM, int MHD, int DIR>
-__global__ void __launch_bounds__(256) k_flux(km_pack_t u, int lim, Box bx) {
+__global__ void __launch_bounds__(256) k_flux(km_pack_t u, int lim, Box bx, int cyl) {
   constexpr int NR = HV<MHD>::NR, NC = HV<MHD>::NC;
   constexpr int DA = DIR == 0 ? 1 : 0;  // transverse dim mapped to consecutive lanes
   constexpr int DB = DIR == 2 ? 1 : 2;
@@ -58,8 +58,15 @@ __global__ void __launch_bounds__(256) k_flux(km_pack_t u, int lim, Box bx) {
         }
         T F[NC];
         riemann<RIEM, MHD, DIR>(vL, vR, F);
+        double out[NC];
 #pragma unroll
-        for (int v = 0; v < NC; ++v) stg(f[v], cb + c * cs, double(F[v]));
+        for (int v = 0; v < NC; ++v) out[v] = double(F[v]);
+        // cylindrical CT: the B_phi flux divided by the face's x2 position, as written
+        // (hydro_add_flux_tasks.cpp:171-178)
+        if (MHD && DIR == 1 && cyl)
+          out[C_MAGC + 2] *= robust_ratio(1.0, x_face(u.blocks[b], g, 1, c));
+#pragma unroll
+        for (int v = 0; v < NC; ++v) stg(f[v], cb + c * cs, out[v]);
       });
 }
 
@@ -129,7 +136,7 @@ __global__ void __launch_bounds__(256) k_flux_x1(km_pack_t u, int lim, Box bx, i
 // E3(i-1/2,j-1/2) = 1/4 [Fy(Bx)(i-1,j) + Fy(Bx)(i,j) - Fx(By)(i,j-1) - Fx(By)(i,j)] and
 // cyclic; each edge family over its own interior range.
 // ---------------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_emf(km_pack_t u) {
+__global__ void __launch_bounds__(256) k_emf(km_pack_t u, double esign) {
   const Geo g(u);
   const long n = g.nstag();
   long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -150,16 +157,16 @@ __global__ void __launch_bounds__(256) k_emf(km_pack_t u) {
   const long c = g.c(k, j, i);
   if (in_i1 && in_j1 && in_k) {
     const double *fyBx = flux_ptr(u, 1, b, C_MAGC + 0), *fxBy = flux_ptr(u, 0, b, C_MAGC + 1);
-    u.edge[2][b][s] = 1.0 * 0.25 * (fyBx[c - 1] + fyBx[c] - fxBy[c - sj] - fxBy[c]);
+    u.edge[2][b][s] = esign * 0.25 * (fyBx[c - 1] + fyBx[c] - fxBy[c - sj] - fxBy[c]);
   }
   if (g.ndim > 2) {
     if (in_i && in_j1 && in_k1) {
       const double *fzBy = flux_ptr(u, 2, b, C_MAGC + 1), *fyBz = flux_ptr(u, 1, b, C_MAGC + 2);
-      u.edge[0][b][s] = 1.0 * 0.25 * (fzBy[c - sj] + fzBy[c] - fyBz[c - sk] - fyBz[c]);
+      u.edge[0][b][s] = esign * 0.25 * (fzBy[c - sj] + fzBy[c] - fyBz[c - sk] - fyBz[c]);
     }
     if (in_i1 && in_j && in_k1) {
       const double *fxBz = flux_ptr(u, 0, b, C_MAGC + 2), *fzBx = flux_ptr(u, 2, b, C_MAGC + 0);
-      u.edge[1][b][s] = 1.0 * 0.25 * (fxBz[c - sk] + fxBz[c] - fzBx[c - 1] - fzBx[c]);
+      u.edge[1][b][s] = esign * 0.25 * (fxBz[c - sk] + fxBz[c] - fzBx[c - 1] - fzBx[c]);
     }
   }
 }
@@ -233,7 +240,7 @@ __global__ void __launch_bounds__(256) k_geom_source(km_pack_t u, km_pack_t dudt
 
 // grid::FluxStokes<geom, Face, edges...> (grid_update.hpp:80-137)
 template <int FACE>
-__global__ void __launch_bounds__(256) k_stokes(km_pack_t u, km_pack_t dudt) {
+__global__ void __launch_bounds__(256) k_stokes(km_pack_t u, km_pack_t dudt, int cyl) {
   const Geo g(u);
   const int di = FACE == 0, dj = FACE == 1, dk = FACE == 2;
   const int ni = g.nx[0] + di, nj = (g.ndim > 1 ? g.nx[1] : 1) + dj,
@@ -257,18 +264,22 @@ __global__ void __launch_bounds__(256) k_stokes(km_pack_t u, km_pack_t dudt) {
   constexpr int e1 = FACE == 0 ? 1 : FACE == 1 ? 0 : 1;
   constexpr int a1 = FACE == 0 ? 2 : FACE == 1 ? 2 : 0;
   constexpr double s1 = FACE == 0 ? -1.0 : 1.0;
+  // cylindrical r, z, phi is an odd permutation (:122-125): every sign flips; E3 edges
+  // have length |r| 2 pi (geometry.hpp:143-150), the others their Dx
+  const double gs = cyl ? -1.0 : 1.0;
+  auto len = [&](int e, int ii) {
+    return (cyl && e == 2) ? __builtin_fabs(x_face(blk, g, 0, ii)) * kTwoPi : blk.dx[e];
+  };
   double acc = 0.;
   {
     const double *E = u.edge[e0][b];
-    const double L = blk.dx[e0];
-    acc += s0 * (L * E[s] - L * E[s + g.ss[a0]]);
+    acc += (gs * s0) * (len(e0, i) * E[s] - len(e0, i + (a0 == 0)) * E[s + g.ss[a0]]);
   }
   if (g.ndim > 2) {
     const double *E = u.edge[e1][b];
-    const double L = blk.dx[e1];
-    acc += s1 * (L * E[s] - L * E[s + g.ss[a1]]);
+    acc += (gs * s1) * (len(e1, i) * E[s] - len(e1, i + (a1 == 0)) * E[s + g.ss[a1]]);
   }
-  acc *= 1. / (blk.area[FACE] + 1.e-36);
+  acc *= 1. / (face_area(blk, g, cyl, FACE, i) + 1.e-36);
   dudt.face[FACE][b][s] = acc;
 }
 
@@ -319,16 +330,21 @@ __global__ void __launch_bounds__(256) k_apply_face(km_pack_t base, km_pack_t u0
 // CT cell-centred B from face B (primconsflux.cpp:102-128)
 template <int MHD>
 __device__ __forceinline__ void face_average_b(const km_pack_t &u, const Geo &g, int b, int k,
-                                               int j, int i) {
+                                               int j, int i, int cyl) {
   if constexpr (MHD) {
     const km_block_t &blk = u.blocks[b];
     const long c = g.c(k, j, i), s = g.s(k, j, i);
     if (g.ndim > 1) {
       const double *B1 = u.face[0][b], *B2 = u.face[1][b];
-      cell_ptr(u, b, 5)[c] =
-          0.5 * blk.dx[0] * (blk.area[0] * B1[s + 1] + blk.area[0] * B1[s]) / blk.vol;
-      cell_ptr(u, b, 6)[c] =
-          0.5 * blk.dx[1] * (blk.area[1] * B2[s + g.ss[1]] + blk.area[1] * B2[s]) / blk.vol;
+      const double V = cell_volume(blk, g, cyl, i);
+      cell_ptr(u, b, 5)[c] = 0.5 * blk.dx[0] *
+                             (face_area(blk, g, cyl, 0, i + 1) * B1[s + 1] +
+                              face_area(blk, g, cyl, 0, i) * B1[s]) /
+                             V;
+      cell_ptr(u, b, 6)[c] = 0.5 * blk.dx[1] *
+                             (face_area(blk, g, cyl, 1, i) * B2[s + g.ss[1]] +
+                              face_area(blk, g, cyl, 1, i) * B2[s]) /
+                             V;
     }
     if (g.ndim > 2) {
       const double *B3 = u.face[2][b];
@@ -344,7 +360,8 @@ __device__ __forceinline__ void face_average_b(const km_pack_t &u, const Geo &g,
 // after Cons2Prim (whose kinetic energy uses the stored phi momentum as is)
 template <int MHD>
 __device__ __forceinline__ void primitive_zone(const km_pack_t &u, int b, long c, const Eos &eos,
-                                               const double *U, double rinv = 0.0) {
+                                               const double *U, double rinv = 0.0,
+                                               double rc = 0.0) {
   constexpr int nc = HV<MHD>::NC;
   const double dens = U[0];
   const double idens = 1.0 / dens;
@@ -359,6 +376,8 @@ __device__ __forceinline__ void primitive_zone(const km_pack_t &u, int b, long c
       emag += B * B;
     }
   }
+  if constexpr (MHD)
+    if (rinv != 0.0) cell_ptr(u, b, KM_MAGC0 + 2)[c] = U[7] * rc;  // B_phi back from B_phi / r
   ekin *= 0.5 * dens;
   emag *= 0.5;
   const double eint = U[4] - ekin - emag;
@@ -376,13 +395,24 @@ __global__ void __launch_bounds__(256) k_prepare_primitive(km_pack_t u, Eos eos,
   const Geo g(u);
   int b, k, j, i;
   if (!interior_cell(g, u.nblocks, b, k, j, i)) return;
-  face_average_b<MHD>(u, g, b, k, j, i);
+  face_average_b<MHD>(u, g, b, k, j, i, cyl);
   const long c = g.c(k, j, i);
   double U[nc];
 #pragma unroll
   for (int v = 0; v < nc; ++v) U[v] = cell_ptr(u, b, v)[c];
-  primitive_zone<MHD>(u, b, c, eos, U,
-                      cyl ? robust_ratio(1.0, cyl_centroid_r(u.blocks[b], g, i)) : 0.0);
+  const double rc = cyl ? cyl_centroid_r(u.blocks[b], g, i) : 0.0;
+  primitive_zone<MHD>(u, b, c, eos, U, cyl ? robust_ratio(1.0, rc) : 0.0, rc);
+}
+
+// hydro::PrepareConserved (hydro.cpp:244-286), the per-stage task before ApplyDuDt: with CT
+// in cylindrical geometry the cell B_phi of md0 becomes B_phi / r_centroid
+__global__ void __launch_bounds__(256) k_cyl_stage_conserved(km_pack_t u) {
+  const Geo g(u);
+  int b, k, j, i;
+  if (!interior_cell(g, u.nblocks, b, k, j, i)) return;
+  double *q = cell_ptr(u, b, KM_MAGC0 + 2);
+  const long c = g.c(k, j, i);
+  q[c] = q[c] * (1.0 / cyl_centroid_r(u.blocks[b], g, i));
 }
 
 // FluxesToDuDt (grid_update.hpp:20-57), the cell part of ApplyDuDt (grid.cpp:213-241)
@@ -454,7 +484,7 @@ __global__ void __launch_bounds__(256) k_prepare_conserved(km_pack_t u, Eos eos,
   cell_ptr(u, b, nc + 5)[c] = sie;
   cell_ptr(u, b, nc + 6)[c] = eos_temp(eos, dens, sie);
   cell_ptr(u, b, nc + 4)[c] = eos_bmod(eos, dens, sie);
-  face_average_b<MHD>(u, g, b, k, j, i);
+  face_average_b<MHD>(u, g, b, k, j, i, cyl);
   double V[NR], U[nc];
 #pragma unroll
   for (int r = 0; r < NR; ++r) V[r] = cell_ptr(u, b, recon_slot<MHD>(r))[c];
@@ -534,7 +564,7 @@ __global__ void k_scale_value(double *p, double s, int n) {
 }
 
 // hydro FillDerived_impl (hydro.cpp:150-194)
-__global__ void __launch_bounds__(256) k_fill_divb(km_pack_t u, int slot) {
+__global__ void __launch_bounds__(256) k_fill_divb(km_pack_t u, int slot, int cyl) {
   const Geo g(u);
   int b, k, j, i;
   if (!interior_cell(g, u.nblocks, b, k, j, i)) return;
@@ -546,10 +576,11 @@ __global__ void __launch_bounds__(256) k_fill_divb(km_pack_t u, int slot) {
   for (int dir = 0; dir < 3; dir++) {
     if (dir < g.ndim) {
       const double *B = u.face[dir][b];
-      divb += blk.area[dir] * B[s + ss[dir]] - blk.area[dir] * B[s];
+      divb += face_area(blk, g, cyl, dir, i + (dir == 0)) * B[s + ss[dir]] -
+              face_area(blk, g, cyl, dir, i) * B[s];
     }
   }
-  cell_ptr(u, b, slot)[g.c(k, j, i)] = divb / blk.vol;
+  cell_ptr(u, b, slot)[g.c(k, j, i)] = divb / cell_volume(blk, g, cyl, i);
 }
 
 // ---------------------------------------------------------------------------------
@@ -668,7 +699,15 @@ __global__ void __launch_bounds__(256) k_ghost_faces(km_pack_t u, UniformGrid gr
     const int N = gr.nb[d] * g.nx[d];
     const int gi = bl[d] * g.nx[d] + (idx[d] - g.lo[d]);
     if (d == face) {
-      const int m = map_face(gi, N, gr.bc[2 * d], gr.bc[2 * d + 1], flip[d]);
+      int m;
+      if (d == 0 && gi < 0 && gr.bc[0] == KM_BC_AXISYMMETRIC) {
+        // Axisymmetric on F1 faces (boundary_conditions.cpp:50-58): global face -g - 1,
+        // negated
+        m = -gi - 1;
+        flip[d] = !flip[d];
+      } else {
+        m = map_face(gi, N, gr.bc[2 * d], gr.bc[2 * d + 1], flip[d]);
+      }
       sb[d] = min(m / g.nx[d], gr.nb[d] - 1);
       src[d] = m - sb[d] * g.nx[d] + g.lo[d];
     } else {
@@ -700,10 +739,10 @@ static void launch_flux_dirs(const km_pack_t &u, int lim, const Box &bx, int cyl
                      lim, bx, cyl);
   if (g.ndim > 1)
     hipLaunchKernelGGL((k_flux<T, RECON, RIEM, MHD, 1>), dim3(nblk(cnt(0, 2))), dim3(256), 0, st, u,
-                       lim, bx);
+                       lim, bx, cyl);
   if (g.ndim > 2)
     hipLaunchKernelGGL((k_flux<T, RECON, RIEM, MHD, 2>), dim3(nblk(cnt(0, 1))), dim3(256), 0, st, u,
-                       lim, bx);
+                       lim, bx, cyl);
 }
 
 template <int RECON, int RIEM>
@@ -747,9 +786,12 @@ void launch_calculate_fluxes(const km_cfg_t &cfg, const km_pack_t &u, const Box 
   }
 }
 
-void launch_calculate_emf(const km_pack_t &u, hipStream_t st) {
+void launch_calculate_emf(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st) {
   const Geo g(u);
-  hipLaunchKernelGGL(k_emf, dim3(nblk(g.nstag() * u.nblocks)), dim3(256), 0, st, u);
+  if (g.ndim < 2) return;  // no edges to evolve in 1-D (hydro_add_flux_tasks.cpp:413)
+  // GetEdgeEMF's sign: -1 in cylindrical geometry (hydro_add_flux_tasks.cpp:398)
+  const double esign = cfg.geometry == KM_GEOM_CYLINDRICAL ? -1.0 : 1.0;
+  hipLaunchKernelGGL(k_emf, dim3(nblk(g.nstag() * u.nblocks)), dim3(256), 0, st, u, esign);
 }
 
 static long interior_count(const Geo &g, int nblocks) {
@@ -765,16 +807,17 @@ void launch_fluxes_to_dudt(const km_cfg_t &cfg, const km_pack_t &u, const km_pac
                            hipStream_t st) {
   const Geo g(u);
   const int nc = cfg.mhd ? 8 : 5;
+  const int cyl = cfg.geometry == KM_GEOM_CYLINDRICAL;
   hipLaunchKernelGGL(k_flux_div, dim3(nblk(interior_count(g, u.nblocks))), dim3(256), 0, st, u,
                      dudt, nc, cfg.geometry == KM_GEOM_CYLINDRICAL ? 1 : 0);
   if (cfg.mhd && g.ndim >= 2) {
     hipLaunchKernelGGL(k_stokes<0>, dim3(nblk(face_count(g, 0, u.nblocks))), dim3(256), 0, st,
-                       u, dudt);
+                       u, dudt, cyl);
     hipLaunchKernelGGL(k_stokes<1>, dim3(nblk(face_count(g, 1, u.nblocks))), dim3(256), 0, st,
-                       u, dudt);
+                       u, dudt, cyl);
     if (g.ndim > 2)
       hipLaunchKernelGGL(k_stokes<2>, dim3(nblk(face_count(g, 2, u.nblocks))), dim3(256), 0, st,
-                         u, dudt);
+                         u, dudt, cyl);
   }
 }
 
@@ -819,16 +862,29 @@ void launch_update(const km_cfg_t &cfg, const km_pack_t &base, const km_pack_t &
                        dt_dev, inst, eos, cyl);
     return;
   }
-  hipLaunchKernelGGL(k_update_cell<1>, dim3(grid), dim3(256), 0, st, base, u0, u1, beta, 0.0,
-                     dt_dev, inst, eos, cyl);
-  if (g.ndim >= 2) {
+  if (cyl) {
+    // the reference's task order, which the per-stage PrepareConserved on md0 splits:
+    // FluxesToDuDt, AddSourceTerms (reads md0's B_phi), PrepareConserved (md0's B_phi to
+    // B_phi / r), ApplyDuDt
+    launch_fluxes_to_dudt(cfg, u0, dudt, st);
+    launch_add_source_terms(cfg, u0, dudt, st);
+    hipLaunchKernelGGL(k_cyl_stage_conserved, dim3(grid), dim3(256), 0, st, u0);
+    hipLaunchKernelGGL(k_apply_cell, dim3(grid), dim3(256), 0, st, base, u0, u1, dudt, 8, beta,
+                       0.0, dt_dev, inst);
+  } else {
+    hipLaunchKernelGGL(k_update_cell<1>, dim3(grid), dim3(256), 0, st, base, u0, u1, beta, 0.0,
+                       dt_dev, inst, eos, cyl);
+  }
+  if (g.ndim >= 2 && !cyl) {
     hipLaunchKernelGGL(k_stokes<0>, dim3(nblk(face_count(g, 0, u0.nblocks))), dim3(256), 0, st,
-                       u0, dudt);
+                       u0, dudt, cyl);
     hipLaunchKernelGGL(k_stokes<1>, dim3(nblk(face_count(g, 1, u0.nblocks))), dim3(256), 0, st,
-                       u0, dudt);
+                       u0, dudt, cyl);
     if (g.ndim > 2)
       hipLaunchKernelGGL(k_stokes<2>, dim3(nblk(face_count(g, 2, u0.nblocks))), dim3(256), 0, st,
-                         u0, dudt);
+                         u0, dudt, cyl);
+  }
+  if (g.ndim >= 2) {
     hipLaunchKernelGGL(k_apply_face<0>, dim3(nblk(face_count(g, 0, u0.nblocks))), dim3(256), 0,
                        st, base, u0, u1, dudt, beta, 0.0, dt_dev, inst);
     hipLaunchKernelGGL(k_apply_face<1>, dim3(nblk(face_count(g, 1, u0.nblocks))), dim3(256), 0,
@@ -879,7 +935,14 @@ void launch_estimate_dt(const km_cfg_t &cfg, const km_pack_t &u, double *dt_dev,
 void launch_fill_divb(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st) {
   const Geo g(u);
   hipLaunchKernelGGL(k_fill_divb, dim3(nblk(interior_count(g, u.nblocks))), dim3(256), 0, st, u,
-                     KM_DIVB(cfg.mhd));
+                     KM_DIVB(cfg.mhd), cfg.geometry == KM_GEOM_CYLINDRICAL ? 1 : 0);
+}
+
+void launch_prepare_conserved_stage(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st) {
+  if (cfg.geometry != KM_GEOM_CYLINDRICAL || cfg.mhd == KM_MHD_OFF) return;
+  const Geo g(u);
+  hipLaunchKernelGGL(k_cyl_stage_conserved, dim3(nblk(interior_count(g, u.nblocks))), dim3(256),
+                     0, st, u);
 }
 
 void launch_ghost_fill(const UniformGrid &gr, const km_pack_t &u, int v0, int v1, int nc,
@@ -889,7 +952,7 @@ void launch_ghost_fill(const UniformGrid &gr, const km_pack_t &u, int v0, int v1
     hipLaunchKernelGGL(k_ghost_cells, dim3(nblk(ghost_count(g) * u.nblocks)), dim3(256), 0, st,
                        u, gr, v0, v1, nc, in);
   if (faces)
-    for (int f = 0; f < (g.ndim > 2 ? 3 : 2); ++f)
+    for (int f = 0; f < (g.ndim > 2 ? 3 : g.ndim); ++f)
       hipLaunchKernelGGL(k_ghost_faces, dim3(nblk(ghost_count(g, f) * u.nblocks)), dim3(256), 0,
                          st, u, gr, f, in);
 }

@@ -504,6 +504,11 @@ void CalculateFluxes(const km_cfg_t &cfg, const km_pack_t &u) {
                   vR[R_MAGC + 1] = bf;
                 }
                 RiemannFlux(cfg.riemann, mhd, 1, vL, vR, F);
+                if (Cyl(cfg) && mhd == KM_MHD_CT) {  // :171-178, divided by the face's x2
+                  const km_block_t &bk = u.blocks[b];
+                  const double zf = (bk.xmin[1] - sh.js * bk.dx[1]) + j * bk.dx[1];
+                  F[KM_MAGC0 + 2] *= Ratio(1.0, zf);
+                }
                 for (int v = 0; v < ncons(mhd); ++v) fluxp(u, 1, b, v)[sh.c(k, j, i)] = F[v];
               }
             }
@@ -558,6 +563,7 @@ void CalculateEMF(const km_cfg_t &cfg, const km_pack_t &u) {
   const int ndim = sh.ndim;
   const int bx = C_MAGC + 0, by = C_MAGC + 1, bz = C_MAGC + 2;
   const long sj = sh.E[0], sk = (long)sh.E[0] * sh.E[1];
+  const double esign = Cyl(cfg) ? -1.0 : 1.0;  // GetEdgeEMF, :398
 #pragma omp parallel for schedule(static)
   for (int b = 0; b < u.nblocks; ++b) {
     const double *fxBy = fluxp(u, 0, b, by), *fxBz = fluxp(u, 0, b, bz);
@@ -571,7 +577,7 @@ void CalculateEMF(const km_cfg_t &cfg, const km_pack_t &u) {
         for (int i = sh.is; i <= sh.ie + 1; ++i) {
           const long c = sh.c(k, j, i);
           e3[sh.s(k, j, i)] =
-              1.0 * 0.25 * (fyBx[c - 1] + fyBx[c] - fxBy[c - sj] - fxBy[c]);
+              esign * 0.25 * (fyBx[c - 1] + fyBx[c] - fxBy[c - sj] - fxBy[c]);
         }
     if (ndim > 2) {
       // E1 at (i, j-1/2, k-1/2): F3(By)(j-1) + F3(By) - F2(Bz)(k-1) - F2(Bz)
@@ -581,7 +587,7 @@ void CalculateEMF(const km_cfg_t &cfg, const km_pack_t &u) {
           for (int i = sh.is; i <= sh.ie; ++i) {
             const long c = sh.c(k, j, i);
             e1[sh.s(k, j, i)] =
-                1.0 * 0.25 * (fzBy[c - sj] + fzBy[c] - fyBz[c - sk] - fyBz[c]);
+                esign * 0.25 * (fzBy[c - sj] + fzBy[c] - fyBz[c - sk] - fyBz[c]);
           }
       // E2 at (i-1/2, j, k-1/2): F1(Bz)(k-1) + F1(Bz) - F3(Bx)(i-1) - F3(Bx)
       double *e2 = edgep(u, 1, b);
@@ -590,7 +596,7 @@ void CalculateEMF(const km_cfg_t &cfg, const km_pack_t &u) {
           for (int i = sh.is; i <= sh.ie + 1; ++i) {
             const long c = sh.c(k, j, i);
             e2[sh.s(k, j, i)] =
-                1.0 * 0.25 * (fxBz[c - sk] + fxBz[c] - fzBx[c - 1] - fzBx[c]);
+                esign * 0.25 * (fxBz[c - sk] + fxBz[c] - fzBx[c - 1] - fzBx[c]);
           }
     }
   }
@@ -643,17 +649,24 @@ void FluxesToDuDt(const km_cfg_t &cfg, const km_pack_t &u, const km_pack_t &dudt
     for (int b = 0; b < u.nblocks; ++b) {
       const km_block_t &blk = u.blocks[b];
       double *out = facep(dudt, face, b);
-      const double A = blk.area[face];
+      // cylindrical r, z, phi is an odd permutation (:122-125); E3 edges have length
+      // |r| 2 pi (geometry.hpp:143-150), the rest their Dx
+      const double gsign = Cyl(cfg) ? -1.0 : 1.0;
+      auto edge_len = [&](int e, int ii) {
+        return (Cyl(cfg) && e == 2) ? std::abs(RFace(blk, sh, ii)) * kPhiSpan : blk.dx[e];
+      };
       for (int k = sh.ks; k <= sh.ke + dk; ++k)
         for (int j = sh.js; j <= sh.je + dj; ++j)
           for (int i = sh.is; i <= sh.ie + di; ++i) {
             const long s = sh.s(k, j, i);
+            const double A = Area(cfg, blk, sh, face, i);
             out[s] = 0.;
             for (int t = 0; t < nterms; ++t) {
               const Term &tm = terms[face][t];
               const double *E = edgep(u, tm.edge, b);
-              const double L = blk.dx[tm.edge];  // EdgeLength<axis> = Dx<axis>
-              out[s] += tm.sign * (L * E[s] - L * E[s + soff[tm.axis]]);
+              const double Lm = edge_len(tm.edge, i);
+              const double Lp = edge_len(tm.edge, i + (tm.axis == 0));
+              out[s] += (gsign * tm.sign) * (Lm * E[s] - Lp * E[s + soff[tm.axis]]);
             }
             out[s] *= 1. / (A + 1.e-36);
           }
@@ -688,6 +701,19 @@ void AddSourceTerms(const km_cfg_t &cfg, const km_pack_t &u, const km_pack_t &du
             d += (pmag - b2 * b2) / r;
           }
         }
+}
+
+// hydro::PrepareConserved, the per-stage task before ApplyDuDt (hydro.cpp:244-286): with CT
+// in cylindrical geometry the cell B_phi of md0 becomes B_phi / r_centroid
+void StagePrepareConserved(const km_cfg_t &cfg, const km_pack_t &u) {
+  if (!Cyl(cfg) || cfg.mhd == KM_MHD_OFF) return;
+  const Shape sh(u);
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int b = 0; b < u.nblocks; ++b)
+    for (int k = sh.ks; k <= sh.ke; ++k)
+      for (int j = sh.js; j <= sh.je; ++j)
+        for (int i = sh.is; i <= sh.ie; ++i)
+          cellp(u, b, KM_MAGC0 + 2)[sh.c(k, j, i)] *= 1.0 / RCentroid(u.blocks[b], sh, i);
 }
 
 void ApplyDuDt(const km_cfg_t &cfg, const km_pack_t &base, const km_pack_t &u0,
@@ -725,16 +751,20 @@ void ApplyDuDt(const km_cfg_t &cfg, const km_pack_t &base, const km_pack_t &u0,
 }
 
 // CT cell-centred B from face B, primconsflux.cpp:102-128
-inline void FaceAverageB(const Shape &sh, const km_pack_t &u, int b, int k, int j, int i) {
+inline void FaceAverageB(const km_cfg_t &cfg, const Shape &sh, const km_pack_t &u, int b, int k,
+                         int j, int i) {
   const km_block_t &blk = u.blocks[b];
   const int ndim = sh.ndim;
   const long c = sh.c(k, j, i), s = sh.s(k, j, i);
   if (ndim > 1) {
     const double *B1 = facep(u, 0, b), *B2 = facep(u, 1, b);
+    const double V = Volume(cfg, blk, sh, i);
     cellp(u, b, KM_MAGC0 + 0)[c] =
-        0.5 * blk.dx[0] * (blk.area[0] * B1[s + 1] + blk.area[0] * B1[s]) / blk.vol;
+        0.5 * blk.dx[0] *
+        (Area(cfg, blk, sh, 0, i + 1) * B1[s + 1] + Area(cfg, blk, sh, 0, i) * B1[s]) / V;
     cellp(u, b, KM_MAGC0 + 1)[c] =
-        0.5 * blk.dx[1] * (blk.area[1] * B2[s + sh.S[0]] + blk.area[1] * B2[s]) / blk.vol;
+        0.5 * blk.dx[1] *
+        (Area(cfg, blk, sh, 1, i) * B2[s + sh.S[0]] + Area(cfg, blk, sh, 1, i) * B2[s]) / V;
   }
   if (ndim > 2) {
     const double *B3 = facep(u, 2, b);
@@ -756,7 +786,7 @@ void PreparePrimitive(const km_cfg_t &cfg, const km_pack_t &u) {
       for (int j = sh.js; j <= sh.je; ++j)
         for (int i = sh.is; i <= sh.ie; ++i) {
           const long c = sh.c(k, j, i);
-          if (mhd == KM_MHD_CT) FaceAverageB(sh, u, b, k, j, i);
+          if (mhd == KM_MHD_CT) FaceAverageB(cfg, sh, u, b, k, j, i);
           // Cons2Prim, primconsflux.hpp:76-98
           const double dens = cellp(u, b, KM_DENS)[c];
           const double idens = 1.0 / dens;
@@ -805,7 +835,7 @@ void PrepareConserved(const km_cfg_t &cfg, const km_pack_t &u) {
           cellp(u, b, KM_EINT(mhd))[c] = sie;
           cellp(u, b, KM_TEMP(mhd))[c] = eos.T(dens, sie);
           cellp(u, b, KM_BMOD(mhd))[c] = eos.B(dens, sie);
-          if (mhd == KM_MHD_CT) FaceAverageB(sh, u, b, k, j, i);
+          if (mhd == KM_MHD_CT) FaceAverageB(cfg, sh, u, b, k, j, i);
           double V[10], U[8];
           for (int r = 0; r < nrecon(mhd); ++r) V[r] = cellp(u, b, recon_slot(r, mhd))[c];
           Prim2Cons(mhd, V, U);
@@ -857,9 +887,10 @@ void FillDivB(const km_cfg_t &cfg, const km_pack_t &u) {
           double divb = 0.;
           for (int dir = 0; dir < sh.ndim; dir++) {
             const double *B = facep(u, dir, b);
-            divb += blk.area[dir] * B[s + soff[dir]] - blk.area[dir] * B[s];
+            divb += Area(cfg, blk, sh, dir, i + (dir == 0)) * B[s + soff[dir]] -
+                    Area(cfg, blk, sh, dir, i) * B[s];
           }
-          cellp(u, b, KM_DIVB(cfg.mhd))[sh.c(k, j, i)] = divb / blk.vol;
+          cellp(u, b, KM_DIVB(cfg.mhd))[sh.c(k, j, i)] = divb / Volume(cfg, blk, sh, i);
         }
     }
 }
@@ -988,7 +1019,14 @@ void GhostFill(const km_cfg_t &cfg, const Grid &grid, const km_pack_t &u, int v0
               const int g = bl[d] * sh.nx[d] + (idx[d] - st[d]);
               int m;
               if (d == face) {
-                m = map_face(g, N[d], grid.bc[2 * d], grid.bc[2 * d + 1], &flip[d]);
+                if (d == 0 && g < 0 && grid.bc[0] == KM_BC_AXISYMMETRIC) {
+                  // Axisymmetric on F1 (boundary_conditions.cpp:50-58): the ghost face at
+                  // index i takes -(face 2 is - 1 - i), i.e. global -g - 1
+                  m = -g - 1;
+                  flip[d] = !flip[d];
+                } else {
+                  m = map_face(g, N[d], grid.bc[2 * d], grid.bc[2 * d + 1], &flip[d]);
+                }
                 sb[d] = std::min(m / sh.nx[d], grid.nb[d] - 1);
                 src[d] = m - sb[d] * sh.nx[d] + st[d];
               } else {
@@ -2321,6 +2359,7 @@ void Cycle(km_cpu_sim_t &s) {
     }
     FluxesToDuDt(s.cfg, p0, s.dudt.pack);
     AddSourceTerms(s.cfg, p0, s.dudt.pack);  // AddTasksOneStep
+    StagePrepareConserved(s.cfg, p0);         // PrepareConserved on md0
     ApplyDuDt(s.cfg, s.base.pack, p0, md1.pack, s.dudt.pack, betas[stage - 1], dt);
     PreparePrimitive(s.cfg, md1.pack);
     if (stage == s.nstages) {
@@ -2424,6 +2463,10 @@ int km_cpu_fluxes_to_dudt(const km_cfg_t *cfg, const km_pack_t *u, const km_pack
 int km_cpu_add_source_terms(const km_cfg_t *cfg, const km_pack_t *u, const km_pack_t *dudt,
                             void *) {
   AddSourceTerms(*cfg, *u, *dudt);
+  return KM_OK;
+}
+int km_cpu_prepare_conserved_stage(const km_cfg_t *cfg, const km_pack_t *u, void *) {
+  StagePrepareConserved(*cfg, *u);
   return KM_OK;
 }
 int km_cpu_apply_dudt(const km_cfg_t *cfg, const km_pack_t *base, const km_pack_t *u0,

@@ -46,6 +46,7 @@ def lib() -> C.CDLL:
         "km_cpu_calculate_emf": [P(KmCfg), P(KmPack), vp],
         "km_cpu_fluxes_to_dudt": [P(KmCfg), P(KmPack), P(KmPack), vp],
         "km_cpu_add_source_terms": [P(KmCfg), P(KmPack), P(KmPack), vp],
+        "km_cpu_prepare_conserved_stage": [P(KmCfg), P(KmPack), vp],
         "km_cpu_apply_dudt": [P(KmCfg), P(KmPack), P(KmPack), P(KmPack), P(KmPack), dbl, dbl,
                               vp],
         "km_cpu_prepare_primitive": [P(KmCfg), P(KmPack), vp],

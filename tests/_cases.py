@@ -22,14 +22,14 @@ def vortex(n=64, nblock=32, ng=3, recon="wenoz", limiter="minmod", riemann="hll"
 
 
 def vortex_cyl(n=64, nblock=8, ng=4, recon="wenoz", limiter="minmod", riemann="hll",
-               integrator="rk2", cfl=0.8):
+               integrator="rk2", cfl=0.8, mhd="off"):
     """The reconstruction suite's cylindrical vortex (reconstruction.py:20-130,136,172-181):
     1-D in r over [-0, 5] in 8-cell blocks, nghost 4, axisymmetric inner and outflow outer
     boundary, x2 over [-5, 5] with one cell, the isentropic_vortex.in parameters."""
     cfg = Config(**{"hydro/reconstruction": recon, "hydro/slope_limiter": limiter,
                     "hydro/riemann": riemann, "parthenon/time/integrator": integrator,
                     "hydro/cfl": cfl, "material/single/gamma": 1.4,
-                    "geometry/geometry": "cylindrical"})
+                    "geometry/geometry": "cylindrical", "physics/MHD": mhd})
     cfg.c.ndim = 1
     lay = Layout(1, ng, (nblock, 1, 1), (n // nblock, 1, 1))
     desc = mesh_desc(lay, xmin=(-0.0, -5.0, -5.0), xmax=(5.0, 5.0, 5.0),
@@ -54,7 +54,7 @@ def vortex_cyl_l1(cell, lay: Layout, blocks, var, strength=5.0):
         rc = (2.0 / 3.0) * (r1 * r1 * np.abs(r1) - r0 * r0 * np.abs(r0)) / (r1 * r1 - r0 * r0)
         vol = 0.5 * (2.0 * np.pi) * (r1 * r1 - r0 * r0) * bk.dx[1]
         dv = strength * np.exp(0.5 * (1.0 - rc * rc)) / (2.0 * np.pi)
-        exact = {km.slot("VEL2"): rc * dv}[var]
+        exact = rc * dv  # v_phi; var is the VEL2 slot of the run's state
         err += float(np.sum(np.abs(exact - cell[b, var, 0, 0, g:g + lay.nx[0]]) * vol))
     return err / (5.0 * 10.0)
 

@@ -121,3 +121,109 @@ def test_cyl_refusals(gpu):
                              "periodic"))
     with pytest.raises(km.KmError, match="inner x1 boundary"):
         km.Simulation(c2, d3, lay2, path=km.PATH_TASKS)
+
+
+def _ct_rz(oracle):
+    from tests.test_oracle_cyl import _psi_faces
+    cfg = Config(**{"hydro/reconstruction": "plm", "hydro/slope_limiter": "mc",
+                    "hydro/riemann": "hll", "material/single/gamma": 5.0 / 3.0,
+                    "geometry/geometry": "cylindrical", "physics/MHD": "ct"})
+    cfg.c.ndim = 2
+    lay = Layout(2, 4, (8, 8, 1), (2, 2, 1))
+    desc = mesh_desc(lay, xmin=(0.0, 1.0, 0.0), xmax=(2.0, 3.0, 1.0),
+                     bc=("axisymmetric", "outflow", "periodic", "periodic", "periodic",
+                         "periodic"))
+    blocks = oracle.block_table(desc, lay.nblocks)
+    faces = _psi_faces(lay, blocks, lambda r, z: 0.25 * r * r * (1.0 + 0.3 * np.sin(np.pi * z))
+                       * np.exp(-r * r))
+    cell, _ = _swirl(lay, desc, oracle)
+    cell = np.concatenate([cell[:, :5], np.zeros_like(cell[:, :3]), cell[:, 5:],
+                           np.zeros_like(cell[:, :1])], axis=1)
+    cell[:, 7] = 0.05  # B_phi
+    return cfg, desc, lay, cell, faces
+
+
+def test_cyl_ct_tasks_bitwise(gpu, oracle):
+    km = gpu
+    cfg, desc, lay, cell, faces = _ct_rz(oracle)
+    print("cylindrical CT tasks", flush=True)
+    L, p = km.lib(), C.byref
+    blocks = oracle.block_table(desc, lay.nblocks)
+    dev = km.DevicePack(lay, 16, nflux=8, faces=True, edges=True, blocks=blocks)
+    host = oracle.HostPack(lay, 16, nflux=8, faces=True, edges=True, blocks=blocks)
+    dev.upload(cell=cell, face=faces)
+    host.cell[:] = cell
+    for d in range(3):
+        host.face[d][:] = faces[d]
+    km.check(L.km_prepare_conserved(p(cfg.c), p(dev.pack), None), "cons")
+    oracle.call("km_cpu_prepare_conserved", p(cfg.c), p(host.pack), None)
+    nb = (C.c_int32 * 3)(2, 2, 1)
+    bc = (C.c_int32 * 6)(*[desc.bc[d] for d in range(6)])
+    plan = C.c_void_p()
+    km.check(L.km_ghost_plan_uniform(p(dev.pack), nb, bc, p(plan)), "plan")
+    km.check(L.km_ghost_fill(plan, p(dev.pack), 0, 14, 1, None), "ghost")
+    L.km_ghost_plan_destroy(plan)
+    oracle.call("km_cpu_ghost_fill_uniform", p(cfg.c), p(host.pack), nb, bc, 0, 14, 1)
+    c0, _, f0, _ = dev.download()
+    _same(c0, host.cell, "cells after ghost fill")
+    for d in range(2):
+        _same(f0[d], host.face[d], f"faces[{d}] after ghost fill")
+    km.check(L.km_calculate_fluxes(p(cfg.c), p(dev.pack), None), "fluxes")
+    oracle.call("km_cpu_calculate_fluxes", p(cfg.c), p(host.pack), None)
+    km.check(L.km_calculate_emf(p(cfg.c), p(dev.pack), None), "emf")
+    oracle.call("km_cpu_calculate_emf", p(cfg.c), p(host.pack), None)
+    _, flux, _, edge = dev.download()
+    for d in range(2):
+        _same(flux[d], host.flux[d], f"flux[{d}]")
+    _same(edge[2], host.edge[2], "E3")
+    ddev = km.DevicePack(lay, 16, faces=True, blocks=blocks)
+    dhost = oracle.HostPack(lay, 16, faces=True, blocks=blocks)
+    km.check(L.km_fluxes_to_dudt(p(cfg.c), p(dev.pack), p(ddev.pack), None), "dudt")
+    oracle.call("km_cpu_fluxes_to_dudt", p(cfg.c), p(host.pack), p(dhost.pack), None)
+    km.check(L.km_add_source_terms(p(cfg.c), p(dev.pack), p(ddev.pack), None), "source")
+    oracle.call("km_cpu_add_source_terms", p(cfg.c), p(host.pack), p(dhost.pack), None)
+    km.check(L.km_prepare_conserved_stage(p(cfg.c), p(dev.pack), None), "stage cons")
+    oracle.call("km_cpu_prepare_conserved_stage", p(cfg.c), p(host.pack), None)
+    dc, _, df, _ = ddev.download()
+    _same(dc, dhost.cell, "dudt cells")
+    for d in range(2):
+        _same(df[d], dhost.face[d], f"dudt faces[{d}]")
+    u1d = km.DevicePack(lay, 16, faces=True, blocks=blocks)
+    u1h = oracle.HostPack(lay, 16, faces=True, blocks=blocks)
+    km.check(L.km_apply_dudt(p(cfg.c), p(dev.pack), p(dev.pack), p(u1d.pack), p(ddev.pack),
+                             0.5, 1e-3, None), "apply")
+    oracle.call("km_cpu_apply_dudt", p(cfg.c), p(host.pack), p(host.pack), p(u1h.pack),
+                p(dhost.pack), 0.5, 1e-3, None)
+    km.check(L.km_prepare_primitive(p(cfg.c), p(u1d.pack), None), "prim")
+    oracle.call("km_cpu_prepare_primitive", p(cfg.c), p(u1h.pack), None)
+    km.check(L.km_fill_divb(p(cfg.c), p(u1d.pack), None), "divb")
+    oracle.call("km_cpu_fill_divb", p(cfg.c), p(u1h.pack), None)
+    c1, _, f1, _ = u1d.download()
+    _same(c1, u1h.cell, "u1 cells")
+    for d in range(2):
+        _same(f1[d], u1h.face[d], f"u1 faces[{d}]")
+    for x in (dev, ddev, u1d):
+        x.close()
+
+
+@pytest.mark.parametrize("which,ncyc", [("ct_vortex1d", 30), ("ct_rz", 6)])
+def test_cyl_ct_sim_bitwise(gpu, oracle, which, ncyc):
+    km = gpu
+    if which == "ct_vortex1d":
+        cfg, desc, lay, cell, faces = _cases.vortex_cyl(mhd="ct")
+    else:
+        cfg, desc, lay, cell, faces = _ct_rz(oracle)
+    print(f"cylindrical CT run {which}", flush=True)
+    sim = km.Simulation(cfg, desc, lay, path=km.PATH_TASKS)
+    sim.set_primitive(cell, faces)
+    ref = oracle.CpuSimulation(cfg.c, desc, lay)
+    ref.set_primitive(cell, faces)
+    sim.advance(ncyc)
+    ref.advance(ncyc)
+    g, gf = sim.state()
+    r, rf = ref.state()
+    _same(g, r, which)
+    for d in range(3):
+        _same(gf[d], rf[d], f"{which} faces[{d}]")
+    assert sim.info() == ref.info()
+    sim.close()

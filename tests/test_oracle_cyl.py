@@ -167,3 +167,70 @@ def test_uniform_specific_angular_momentum_stays_uniform(oracle):
         worst = max(worst, float(np.max(np.abs(lz - ell))))
     print(f"max |r v_phi - l| = {worst:.3e}")
     assert worst < 5e-3 * ell  # measured 5.4e-4; without the factor r, 3.2e-2
+
+
+def test_cylindrical_ct_vortex_under_the_reference_ceiling(oracle):
+    # reconstruction.py:137 -- wenoz, cylindrical, CT: v_phi L1 error <= 0.35 at t = 10
+    cfg, desc, lay, cell, faces = _cases.vortex_cyl(mhd="ct")
+    sim = oracle.CpuSimulation(cfg.c, desc, lay)
+    sim.set_primitive(cell, faces)
+    sim.advance(100000, 10.0)
+    st, _ = sim.state()
+    blocks = oracle.block_table(desc, lay.nblocks)
+    # the MHD vortex keeps v_phi = r dv with B_phi = r dB (StateMHD<cylindrical>)
+    err = _cases.vortex_cyl_l1(st, lay, blocks, km.slot("VEL2", 1))
+    print(f"cylindrical CT vortex: v_phi L1 error {err:.3e}")
+    assert np.isfinite(err) and err < 0.35
+
+
+def _psi_faces(lay, blocks, psi):
+    """r-z faces of a divergence-free B from the flux function psi: A_r B_r =
+    -2 pi (psi(r, z+) - psi(r, z-)), A_z B_z = 2 pi (psi(r+, z) - psi(r-, z)), so the
+    cylindrical discrete divergence of every cell vanishes."""
+    faces = [np.zeros(lay.stag_shape()) for _ in range(3)]
+    g = lay.ng
+    tp = 2.0 * np.pi
+    for b in range(lay.nblocks):
+        bk = blocks[b]
+        r0, z0 = bk.xmin[0] - g * bk.dx[0], bk.xmin[1] - g * bk.dx[1]
+        for j in range(lay.S[1]):
+            for i in range(lay.S[0]):
+                rf, zf = r0 + i * bk.dx[0], z0 + j * bk.dx[1]
+                rc, zc = r0 + (i + 0.5) * bk.dx[0], z0 + (j + 0.5) * bk.dx[1]
+                ar = rf * bk.dx[1] * tp
+                faces[0][b, 0, j, i] = (-tp * (psi(rf, zf + bk.dx[1]) - psi(rf, zf)) / ar
+                                        if rf > 1e-12 else 0.0)
+                rp, rm = rf + bk.dx[0], rf
+                az = 0.5 * (rp * rp - rm * rm) * tp
+                faces[1][b, 0, j, i] = tp * (psi(rp, zf) - psi(rm, zf)) / az
+                del rc, zc
+    return faces
+
+
+def test_cylindrical_ct_keeps_divb_at_roundoff(oracle):
+    cfg = Config(**{"hydro/reconstruction": "plm", "hydro/slope_limiter": "mc",
+                    "hydro/riemann": "hll", "material/single/gamma": 5.0 / 3.0,
+                    "geometry/geometry": "cylindrical", "physics/MHD": "ct"})
+    cfg.c.ndim = 2
+    # z stays away from 0: the reference divides the x2-face B_phi flux by the face's z
+    # (hydro_add_flux_tasks.cpp:171-178), which a face at z = 0 turns into an overflow
+    lay = Layout(2, 4, (8, 8, 1), (2, 2, 1))
+    desc = mesh_desc(lay, xmin=(0.0, 1.0, 0.0), xmax=(2.0, 3.0, 1.0),
+                     bc=("axisymmetric", "outflow", "periodic", "periodic", "periodic", "periodic"))
+    blocks = oracle.block_table(desc, lay.nblocks)
+
+    def psi(r, z):
+        return 0.25 * r * r * (1.0 + 0.3 * np.sin(np.pi * z)) * np.exp(-r * r)
+    faces = _psi_faces(lay, blocks, psi)
+    cell, _ = _swirl(lay, desc, oracle)
+    cell = np.concatenate([cell[:, :5], np.zeros_like(cell[:, :3]), cell[:, 5:],
+                           np.zeros_like(cell[:, :1])], axis=1)
+    cell[:, km.slot("MAGC2", 1)] = 0.05
+    sim = oracle.CpuSimulation(cfg.c, desc, lay)
+    sim.set_primitive(cell, faces)
+    sim.advance(6)
+    st, f = sim.state()
+    divb = st[(slice(None), km.slot("DIVB", 1)) + lay.interior()]
+    bmax = max(float(np.max(np.abs(f[0]))), float(np.max(np.abs(f[1]))))
+    print(f"max |divB| = {np.max(np.abs(divb)):.3e}, max |B| = {bmax:.3e}")
+    assert np.max(np.abs(divb)) < 1e-12 * bmax / min(lay.nx[0], lay.nx[1])
