@@ -77,6 +77,7 @@ typedef struct km_cfg_t {
   double gamma;          /* material/single/gamma        default 5/3    */
   double abar;           /* material/single/Abar         default 1      */
   double cfl;            /* hydro/cfl                    default 0.8    */
+  int32_t nspecies;      /* entries of material/species  default 1 ("single") */
 } km_cfg_t;
 
 /* ---- cell-variable slots of a hydro state pack ------------------------------------
@@ -97,6 +98,13 @@ typedef struct km_cfg_t {
 #define KM_TEMP(mhd) (KM_NCONS(mhd) + 6)
 #define KM_DIVB(mhd) (KM_NCONS(mhd) + 7)
 #define KM_NVAR(mhd) ((mhd) == KM_MHD_CT ? 16 : 12)
+/* With more than one species (material/species), the mass fractions MFRAC follow:
+ * slot KM_MFRAC(mhd, s) = KM_NVAR(mhd) + s, and flux slot KM_NCONS(mhd) + s. They are
+ * conserved fields (stage containers own them): X_s between stages, rho X_s inside a
+ * stage's update (material.cpp:132-197). A pack then holds KM_NVAR(mhd) + nspecies cell
+ * vars and KM_NCONS(mhd) + nspecies flux vars. */
+#define KM_MFRAC(mhd, s) (KM_NVAR(mhd) + (s))
+#define KM_NSPEC_SLOTS(nspecies) ((nspecies) > 1 ? (nspecies) : 0)
 
 /* Per-block uniform-Cartesian metric: the values parthenon::UniformCartesian gives
  * to Dx<axis>(), FaceArea<axis>() and CellVolume() (geometry.hpp:35-167). */
@@ -143,14 +151,18 @@ int km_cfg_min_ghost(const km_cfg_t *cfg);
 /* hydro::CalculateFluxes -- Dispatcher<CalculateFluxesNested|Scratch>
  * (hydro_add_flux_tasks.cpp:29-385, called from AddFluxTasks :444-476).
  * Reads the reconstructed set DENS, VELOCITY, PRES, BMOD, EINT (+MAGC, face B for CT)
- * from u->cell / u->face and writes u->flux[d] for every conserved variable. */
+ * from u->cell / u->face and writes u->flux[d] for every conserved variable. With more
+ * than one species, each mass fraction is reconstructed the same way and its flux is the
+ * mass flux times the upwind side's mass fraction (:119-135); along x2 and x3 the last
+ * species' flux is left unwritten, as the reference's loops end one species early
+ * (:181-197, :240-256). The pack then needs KM_NCONS + nspecies flux slots. */
 int km_calculate_fluxes(const km_cfg_t *cfg, const km_pack_t *u, void *stream);
 /* hydro::CalculateEMF (hydro_add_flux_tasks.cpp:387-442): edge EMFs from the MAGC
  * face fluxes into u->edge[d]. No-op unless MHD == ct and ndim >= 2. */
 int km_calculate_emf(const km_cfg_t *cfg, const km_pack_t *u, void *stream);
 /* grid::FluxesToDuDt (grid.cpp:179-211 -> FluxDivergence grid_update.hpp:15-57 and
  * FluxStokes grid_update.hpp:80-137): dudt->cell[v] for the conserved cell vars and,
- * for CT, dudt->face[d]. */
+ * for CT, dudt->face[d]; with more than one species also each mass fraction's slot. */
 int km_fluxes_to_dudt(const km_cfg_t *cfg, const km_pack_t *u, const km_pack_t *dudt,
                       void *stream);
 /* hydro::AddSourceTerms (hydro.cpp:199-240, AddTasksOneStep, between FluxesToDuDt and
@@ -161,19 +173,26 @@ int km_add_source_terms(const km_cfg_t *cfg, const km_pack_t *u, const km_pack_t
                         void *stream);
 /* hydro::PrepareConserved, the per-stage task on md0 between AddTasksOneStep and ApplyDuDt
  * (hydro.cpp:244-286, kamayan_driver.cpp:224-231): with CT in cylindrical geometry the cell
- * B_phi becomes B_phi / r_centroid; nothing otherwise. */
+ * B_phi becomes B_phi / r_centroid. material::PrepareConserved (material.cpp:132-155), the
+ * same per-stage task: with more than one species each mass fraction X_s of u becomes
+ * rho X_s. Nothing otherwise. */
 int km_prepare_conserved_stage(const km_cfg_t *cfg, const km_pack_t *u, void *stream);
 /* grid::ApplyDuDt (grid.cpp:213-283):
  *   u0 <- beta*base + (1-beta)*u0 ;  u1 <- u0 + (beta*dt)*dudt
- * over the interior of every conserved cell var and (CT) every face element. */
+ * over the interior of every conserved cell var (mass fractions included) and (CT) every
+ * face element. */
 int km_apply_dudt(const km_cfg_t *cfg, const km_pack_t *base, const km_pack_t *u0,
                   const km_pack_t *u1, const km_pack_t *dudt, double beta, double dt,
                   void *stream);
 /* PreparePrimitive callbacks in DAG order: hydro PreparePrimitive_impl
- * (primconsflux.cpp:82-142) then eos EosWrapped(ener) (eos.cpp:71-114,164). */
+ * (primconsflux.cpp:82-142) then eos EosWrapped(ener) (eos.cpp:71-114,164), then
+ * material PreparePrimitive (material.cpp:157-197): with more than one species the mass
+ * fractions are divided by their sum. */
 int km_prepare_primitive(const km_cfg_t *cfg, const km_pack_t *u, void *stream);
 /* PostMeshInitialization in DAG order: eos PrepareConserved with mode_init=dens_pres
- * (eos.cpp:165-168) then hydro ConvertToConserved_impl (primconsflux.cpp:19-79). */
+ * (eos.cpp:165-168) then hydro ConvertToConserved_impl (primconsflux.cpp:19-79), then
+ * material PostMeshInitialization (material.cpp:104-130): the mass fractions are divided by
+ * their sum. */
 int km_prepare_conserved(const km_cfg_t *cfg, const km_pack_t *u, void *stream);
 /* hydro EstimateTimeStep (hydro_time_step.cpp:19-65): writes (cfl/ndim)*min(dx/(|v|+cf))
  * over the interior to *dt_dev (device). Does not block the host. */
@@ -214,7 +233,9 @@ int km_ghost_plan_amr(const km_pack_t *shape, const km_mesh_desc_t *mesh,
  * refinement_operations.hpp:52-121) or prolongated from a coarser one
  * (ProlongateSharedMinMod, :123-268). CT face arrays across levels (even nghost) take
  * RestrictAverage / ProlongateSharedMinMod on faces and ProlongateInternalTothAndRoe
- * (:270-398) for fine faces inside coarse cells; a block's own faces are never written. */
+ * (:270-398) for fine faces inside coarse cells; a block's own faces are never written.
+ * A pack with face tables (u->face[0] set) is an MHD pack: reflecting and axisymmetric
+ * boundaries negate its cell B vector (slots 5..7) as well as the momentum and velocity. */
 int km_ghost_fill(const km_ghost_plan_t *plan, const km_pack_t *u, int32_t v0, int32_t v1,
                   int32_t with_faces, void *stream);
 /* Flux correction (Parthenon AddFluxCorrectionTasks, kamayan_driver.cpp:198-207): every
@@ -249,10 +270,11 @@ int km_sim_create_amr(const km_cfg_t *cfg, const km_mesh_desc_t *mesh, const int
 /* time and dt of the first n instances (blocks the host) */
 int km_sim_ensemble_info(const km_sim_t *sim, double *times, double *dts, int32_t n);
 int km_sim_destroy(km_sim_t *sim);
-/* Host buffers in pack layout: cell = [nblocks][KM_NVAR][E3][E2][E1], face[d] =
- * [nblocks][S3][S2][S1] (NULL unless CT). km_sim_set_primitive takes DENS, VELOCITY,
- * PRES (+MAGC, face B) on the interior, as a problem generator leaves them, and runs
- * PostMeshInitialization, the initial ghost fill and the initial dt estimate. */
+/* Host buffers in pack layout: cell = [nblocks][nvar][E3][E2][E1] with nvar =
+ * KM_NVAR(mhd) + KM_NSPEC_SLOTS(nspecies), face[d] = [nblocks][S3][S2][S1] (NULL unless
+ * CT). km_sim_set_primitive takes DENS, VELOCITY, PRES (+MAGC, face B, mass fractions) on
+ * the interior, as a problem generator leaves them, and runs PostMeshInitialization, the
+ * initial ghost fill and the initial dt estimate. */
 int km_sim_set_primitive(km_sim_t *sim, const double *cell, const double *const face[3]);
 int km_sim_get_state(km_sim_t *sim, double *cell, double *const face[3]);
 /* Advance ncycles cycles (or until t >= tlim if tlim > 0). Blocks the host. */
@@ -272,7 +294,9 @@ int km_sim_set_tile(km_sim_t *sim, int32_t threads);
 /* names: "isentropic_vortex" (isentropic_vortex.cpp:105-175), "smooth3d", "mhd_random",
  * "kelvin_helmholtz", "sedov" (sedov.cpp:75-143, 3-D form), "mhd_blast"
  * (mhd_blast.cpp:47-160, Cartesian). Buffers as for
- * km_sim_set_primitive; `seed` feeds the random modes where a problem has them. */
+ * km_sim_set_primitive; `seed` feeds the random modes where a problem has them. With
+ * cfg->nspecies > 1 the mass fractions are set too: sedov puts species 0 inside the blast
+ * radius and species 1 outside (sedov.cpp:138-141), the others species 0 everywhere. */
 int km_problem_fill(const char *name, const km_cfg_t *cfg, const km_mesh_desc_t *mesh,
                     uint64_t seed, const double *params, int32_t nparams, double *cell,
                     double *const face[3]);

@@ -14,7 +14,8 @@ import numpy as np
 
 from ._abi import (BC, KM_OK, LIMITER, MHD, PATH_FUSED, PATH_TASKS, PRECISION, RECON,
                    RIEMANN, KmAmrAdapt, KmAmrCriterion, KmBlock, KmCfg, KmMeshDesc, KmPack,
-                   Layout, amr_adapt, amr_criterion, f64ptr, mesh_desc, ncons, nvar, slot)
+                   Layout, amr_adapt, amr_criterion, f64ptr, mesh_desc, mfrac, ncons, nvar,
+                   slot)
 
 HERE = pathlib.Path(__file__).resolve().parent
 # KAMAYAN_AMD_LIB selects another build of the library (e.g. a tuning variant)
@@ -22,7 +23,7 @@ LIB_PATH = pathlib.Path(os.environ.get("KAMAYAN_AMD_LIB", HERE / "libkm_hydro.so
 
 __all__ = [
     "lib", "Config", "Simulation", "DevicePack", "problem_fill", "Layout", "mesh_desc",
-    "KmCfg", "KmPack", "KmBlock", "KmMeshDesc", "ncons", "nvar", "slot", "PATH_TASKS",
+    "KmCfg", "KmPack", "KmBlock", "KmMeshDesc", "ncons", "nvar", "mfrac", "slot", "PATH_TASKS",
     "PATH_FUSED", "LIB_PATH", "device_count", "block_table", "AmrAdapt", "amr_adapt",
     "amr_criterion", "amr_update_tree", "amr_tag_loehner", "amr_remesh", "initial_refinement",
 ]
@@ -246,7 +247,7 @@ def initial_refinement(cfg: Config, desc: KmMeshDesc, layout: Layout, problem: s
         sim.set_primitive(cell, faces)
         state, _ = sim.state()
         sim.close()
-        dev = DevicePack(lay, nvar(cfg.c.mhd), blocks=amr_blocks(desc, leaves))
+        dev = DevicePack(lay, nvar(cfg.c.mhd, cfg.c.nspecies), blocks=amr_blocks(desc, leaves))
         dev.upload(cell=state)
         tags_d = dev._alloc(4 * len(leaves))
         for c in range(a.ncrit):
@@ -270,7 +271,7 @@ def problem_fill(name: str, cfg: Config, desc: KmMeshDesc, layout: Layout, seed:
     """Primitive initial state (cell array, face arrays or None) of a problem generator;
     with `leaves`, over the leaf blocks of a block tree."""
     mhd = cfg.c.mhd
-    cell = np.zeros(layout.cell_shape(nvar(mhd)))
+    cell = np.zeros(layout.cell_shape(nvar(mhd, cfg.c.nspecies)))
     faces = None
     face_ptrs = (C.c_void_p * 3)()
     if mhd == 1:
@@ -356,7 +357,7 @@ class Simulation:
         mhd = self.cfg.c.mhd
         if getattr(self, "_leaves", None) is not None:
             self.leaves()
-        cell = np.zeros(self.layout.cell_shape(nvar(mhd)))
+        cell = np.zeros(self.layout.cell_shape(nvar(mhd, self.cfg.c.nspecies)))
         fp = (C.c_void_p * 3)()
         faces = None
         if mhd == 1:

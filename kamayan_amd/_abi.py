@@ -24,8 +24,14 @@ def ncons(mhd: int) -> int:
     return 8 if mhd == 1 else 5
 
 
-def nvar(mhd: int) -> int:
-    return 16 if mhd == 1 else 12
+def nvar(mhd: int, nspecies: int = 1) -> int:
+    """cell vars of a state pack: the hydro/MHD slots plus the mass fractions when there is
+    more than one species"""
+    return (16 if mhd == 1 else 12) + (nspecies if nspecies > 1 else 0)
+
+
+def mfrac(mhd: int, s: int) -> int:
+    return (16 if mhd == 1 else 12) + s
 
 
 def slot(name: str, mhd: int = 0) -> int:
@@ -48,6 +54,7 @@ class KmCfg(C.Structure):
         ("fluid", C.c_int32), ("mhd", C.c_int32), ("geometry", C.c_int32),
         ("nstages", C.c_int32), ("precision", C.c_int32), ("ndim", C.c_int32),
         ("gamma", C.c_double), ("abar", C.c_double), ("cfl", C.c_double),
+        ("nspecies", C.c_int32),
     ]
 
 

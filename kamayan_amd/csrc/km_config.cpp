@@ -11,6 +11,7 @@
 #include <cstring>
 #include <sstream>
 #include <string>
+#include <vector>
 
 #include "../../include/km_hydro.h"
 #include "km_internal.hpp"
@@ -74,6 +75,7 @@ int km_cfg_default(km_cfg_t *cfg) {
   cfg->gamma = 5.0 / 3.0;              // eos.cpp:37-39
   cfg->abar = 1.0;
   cfg->cfl = 0.8;                      // hydro.cpp:90
+  cfg->nspecies = 1;                   // material.cpp:34 default "single"
   return KM_OK;
 }
 
@@ -149,6 +151,28 @@ int km_cfg_set(km_cfg_t *cfg, const char *block_key, const char *value) {
       km::set_error("invalid value '" + val + "' for " + key);
       return KM_EINVAL;
     }
+    return KM_OK;
+  }
+  if (key == "material/species") {  // comma-separated, unique (material.cpp:34-43)
+    int n = 0;
+    std::string item;
+    std::istringstream in(val);
+    std::vector<std::string> seen;
+    while (std::getline(in, item, ',')) {
+      item = trim(item);
+      for (const auto &x : seen)
+        if (x == item) {
+          km::set_error("material/species must be unique");
+          return KM_EINVAL;
+        }
+      seen.push_back(item);
+      ++n;
+    }
+    if (n < 1 || n > 64) {
+      km::set_error("invalid value '" + val + "' for " + key);
+      return KM_EINVAL;
+    }
+    cfg->nspecies = n;
     return KM_OK;
   }
   if (key == "kamayan_amd/precision") {

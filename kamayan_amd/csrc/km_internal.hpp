@@ -63,8 +63,9 @@ constexpr int KM_NBR_REFLECT = -2;
 void launch_calculate_fluxes(const km_cfg_t &cfg, const km_pack_t &u, const Box &bx,
                              hipStream_t st);
 void launch_calculate_emf(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st);
+// species: also the mass fractions' dU/dt (when there is more than one species)
 void launch_fluxes_to_dudt(const km_cfg_t &cfg, const km_pack_t &u, const km_pack_t &dudt,
-                           hipStream_t st);
+                           hipStream_t st, bool species = true);
 // FluxesToDuDt, ApplyDuDt and PreparePrimitive of one stage, with the stage's dt on the
 // device (dt_dev, indexed by inst when given)
 void launch_update(const km_cfg_t &cfg, const km_pack_t &base, const km_pack_t &u0,

@@ -178,7 +178,8 @@ int km_problem_fill_blocks(const char *name, const km_cfg_t *cfg, const km_mesh_
   if (!name || !cfg || !mesh || !blocks || !cell || nblocks < 0) return KM_EINVAL;
   const std::string pname(name);
   const int mhd = cfg->mhd;
-  Layout L(*mesh, KM_NVAR(mhd));
+  const int nsp = cfg->nspecies;
+  Layout L(*mesh, KM_NVAR(mhd) + KM_NSPEC_SLOTS(nsp));
   L.nblocks = nblocks;
   const int nc = KM_NCONS(mhd);
   const int sV = nc, sP = nc + 3;
@@ -188,6 +189,13 @@ int km_problem_fill_blocks(const char *name, const km_cfg_t *cfg, const km_mesh_
     for (int d = 0; d < 3; ++d)
       if (face[d]) std::memset(face[d], 0, sizeof(double) * L.nstag() * L.nblocks);
   auto at = [&](int b, int v) { return cell + ((long)b * L.nvar + v) * L.ncell(); };
+  // with more than one species the material is species 0 throughout, unless the problem
+  // sets the mass fractions itself
+  if (nsp > 1)
+    for (int b = 0; b < L.nblocks; ++b)
+      for (int k = L.lo(2); k <= L.hi(2); ++k)
+        for (int j = L.lo(1); j <= L.hi(1); ++j)
+          for (int i = L.lo(0); i <= L.hi(0); ++i) at(b, KM_MFRAC(mhd, 0))[L.c(k, j, i)] = 1.0;
 
   if (pname == "isentropic_vortex") {
     // VortexData::State (isentropic_vortex.hpp:31-51) at cell centroids; in cylindrical
@@ -338,6 +346,10 @@ int km_problem_fill_blocks(const char *name, const km_cfg_t *cfg, const km_mesh_
             const long c = L.c(k, j, i);
             at(b, KM_DENS)[c] = rho0;
             at(b, sP)[c] = r <= radius ? pexp : p0;
+            if (nsp > 1) {  // sedov.cpp:138-141: species 0 inside the blast, 1 outside
+              at(b, KM_MFRAC(mhd, 0))[c] = r <= radius ? 1.0 : 0.0;
+              at(b, KM_MFRAC(mhd, 1))[c] = r > radius ? 1.0 : 0.0;
+            }
           }
     return KM_OK;
   }

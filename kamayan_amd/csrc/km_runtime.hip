@@ -94,7 +94,11 @@ static int check_cfg(const km_cfg_t *cfg, const km_pack_t *u) {
               std::to_string(km_cfg_min_ghost(cfg)) + " the options need");
     return KM_EINVAL;
   }
-  if (u->nvar < KM_NVAR(cfg->mhd)) {
+  if (cfg->nspecies < 1 || cfg->nspecies > 64) {
+    set_error("material/species must name 1 to 64 species");
+    return KM_EINVAL;
+  }
+  if (u->nvar < KM_NVAR(cfg->mhd) + KM_NSPEC_SLOTS(cfg->nspecies)) {
     set_error("pack has fewer cell variables than the hydro state needs");
     return KM_EINVAL;
   }
@@ -187,6 +191,7 @@ struct km_sim_t {
   km_mesh_desc_t mesh;
   int path = KM_PATH_TASKS;
   int nb = 0, nvar = 0, nc = 0;
+  int nflux = 0;  // conserved fluxes plus one per species (more than one species)
   long ncell = 0, nstag = 0;
   std::vector<std::unique_ptr<DevBuf>> bufs;
   km_block_t *blocks_d = nullptr;
@@ -278,18 +283,22 @@ int make_cont(km_sim_t &s, Cont &c, bool full_cells, const Cont *share, bool fac
   c.pack.ng = s.mesh.ng;
   for (int d = 0; d < 3; ++d) c.pack.nx[d] = d < s.mesh.ndim ? s.mesh.nx[d] : 1;
   c.pack.nvar = s.nvar;
-  c.pack.nflux = s.nc;
+  c.pack.nflux = s.nflux;
   c.pack.blocks = s.blocks_d;
-  const int own = full_cells ? s.nvar : s.nc;
+  // a stage container owns the conserved fields (the hydro conserved slots and the mass
+  // fractions) and shares the derived ones with `share`
+  const int nhydro = KM_NVAR(s.cfg.mhd);
+  auto owned = [&](int v) { return full_cells || v < s.nc || v >= nhydro; };
+  const int own = full_cells ? s.nvar : s.nc + (s.nvar - nhydro);
   double *arena = nullptr;
   int rc = s.alloc(&arena, (size_t)s.nb * own * s.ncell);
   if (rc) return rc;
   c.cell_h.assign((size_t)s.nb * s.nvar, nullptr);
   for (int b = 0; b < s.nb; ++b)
-    for (int v = 0; v < s.nvar; ++v) {
+    for (int v = 0, o = 0; v < s.nvar; ++v) {
       const size_t t = (size_t)b * s.nvar + v;
-      if (v < own)
-        c.cell_h[t] = arena + ((size_t)b * own + v) * s.ncell;
+      if (owned(v))
+        c.cell_h[t] = arena + ((size_t)b * own + o++) * s.ncell;
       else if (share)
         c.cell_h[t] = share->cell_h[t];
     }
@@ -310,8 +319,8 @@ int make_cont(km_sim_t &s, Cont &c, bool full_cells, const Cont *share, bool fac
     for (int d = 0; d < 3; ++d) {
       if (d >= s.mesh.ndim) continue;
       double *fa = nullptr;
-      if ((rc = s.alloc(&fa, (size_t)s.nb * s.nc * s.ncell))) return rc;
-      c.flux_h[d].resize((size_t)s.nb * s.nc);
+      if ((rc = s.alloc(&fa, (size_t)s.nb * s.nflux * s.ncell))) return rc;
+      c.flux_h[d].resize((size_t)s.nb * s.nflux);
       for (size_t t = 0; t < c.flux_h[d].size(); ++t) c.flux_h[d][t] = fa + t * s.ncell;
       if ((rc = s.upload_table(&c.flux_d[d], c.flux_h[d]))) return rc;
       c.pack.flux[d] = c.flux_d[d];
@@ -369,7 +378,7 @@ int task_stage(km_sim_t &s, int stage, double beta, double dt) {
   // flux correction at level jumps (AddFluxCorrectionTasks, kamayan_driver.cpp:198-207):
   // cell fluxes and, for CT, the edge EMFs
   if (s.amr) {
-    launch_amr_flux_correct(s.agrid, p0, s.nc, s.stream);
+    launch_amr_flux_correct(s.agrid, p0, s.nflux, s.stream);
     if (ct) launch_amr_emf_correct(s.agrid, p0, s.stream);
   }
   // FluxesToDuDt, ApplyDuDt, PreparePrimitive
@@ -379,8 +388,9 @@ int task_stage(km_sim_t &s, int stage, double beta, double dt) {
     launch_estimate_dt(cfg, s.base.pack, s.est_d, s.stream, s.inst_d, s.ninst);
     if (ct) launch_fill_divb(cfg, s.base.pack, s.stream);
   }
-  // FillGhost fields: conserved, VELOCITY, PRES, BMOD, EINT, face B (TEMP is not)
+  // FillGhost fields: conserved, VELOCITY, PRES, BMOD, EINT, face B (TEMP is not), MFRAC
   ghost_fill(s, md1.pack, 0, KM_TEMP(cfg.mhd), ct);
+  if (s.nvar > KM_NVAR(cfg.mhd)) ghost_fill(s, md1.pack, KM_NVAR(cfg.mhd), s.nvar, false);
   return check_launch();
 }
 
@@ -510,6 +520,7 @@ int adapt_step(km_sim_t &s) {
   s.leaves = std::move(nl);
   s.deref = std::move(nd);
   ghost_fill(s, s.base.pack, 0, KM_TEMP(s.cfg.mhd), false);
+  if (s.nvar > KM_NVAR(s.cfg.mhd)) ghost_fill(s, s.base.pack, KM_NVAR(s.cfg.mhd), s.nvar, false);
   launch_estimate_dt(s.cfg, s.base.pack, s.est_d, s.stream, nullptr, 1);
   return check_launch();
 }
@@ -586,13 +597,23 @@ int km_synchronize(void) {
 }
 
 // ---- tasks ----
-int km_calculate_fluxes(const km_cfg_t *cfg, const km_pack_t *u, void *stream) {
-  if (int rc = check_cfg(cfg, u)) return rc;
+// flux tables for every active dimension, with a slot per conserved field and species
+static int check_flux_tables(const km_cfg_t *cfg, const km_pack_t *u) {
   for (int d = 0; d < u->ndim; ++d)
     if (!u->flux[d]) {
       set_error("flux tables missing");
       return KM_EINVAL;
     }
+  if (u->nflux < KM_NCONS(cfg->mhd) + KM_NSPEC_SLOTS(cfg->nspecies)) {
+    set_error("pack has fewer flux variables than the conserved fields and species need");
+    return KM_EINVAL;
+  }
+  return KM_OK;
+}
+
+int km_calculate_fluxes(const km_cfg_t *cfg, const km_pack_t *u, void *stream) {
+  if (int rc = check_cfg(cfg, u)) return rc;
+  if (int rc = check_flux_tables(cfg, u)) return rc;
   if (cfg->mhd == KM_MHD_CT)
     for (int d = 0; d < u->ndim; ++d)
       if (!u->face[d]) {
@@ -617,7 +638,12 @@ int km_calculate_emf(const km_cfg_t *cfg, const km_pack_t *u, void *stream) {
 int km_fluxes_to_dudt(const km_cfg_t *cfg, const km_pack_t *u, const km_pack_t *dudt,
                       void *stream) {
   if (int rc = check_cfg(cfg, u)) return rc;
+  if (int rc = check_flux_tables(cfg, u)) return rc;
   if (!dudt) return KM_EINVAL;
+  if (dudt->nvar < KM_NVAR(cfg->mhd) + KM_NSPEC_SLOTS(cfg->nspecies)) {
+    set_error("dU/dt pack has fewer cell variables than the state");
+    return KM_EINVAL;
+  }
   launch_fluxes_to_dudt(*cfg, *u, *dudt, as_stream(stream));
   return check_launch();
 }
@@ -739,7 +765,8 @@ int km_ghost_plan_amr(const km_pack_t *shape, const km_mesh_desc_t *mesh,
 int km_ghost_fill(const km_ghost_plan_t *plan, const km_pack_t *u, int32_t v0, int32_t v1,
                   int32_t with_faces, void *stream) {
   if (!plan || !u || v0 < 0 || v1 > u->nvar || v0 > v1) return KM_EINVAL;
-  const int nc = u->nvar >= 16 ? 8 : 5;
+  // a pack with face tables is an MHD pack: slots 5..7 hold the cell B vector
+  const int nc = u->face[0] ? 8 : 5;
   if (with_faces && (!u->face[0] || !u->face[1])) {
     set_error("face tables missing");
     return KM_EINVAL;
@@ -765,8 +792,7 @@ int km_flux_correct(const km_ghost_plan_t *plan, const km_pack_t *u, void *strea
       set_error("flux tables missing");
       return KM_EINVAL;
     }
-  const int nc = u->nflux < (u->nvar >= 16 ? 8 : 5) ? u->nflux : (u->nvar >= 16 ? 8 : 5);
-  launch_amr_flux_correct(plan->agrid, *u, nc, as_stream(stream));
+  launch_amr_flux_correct(plan->agrid, *u, u->nflux, as_stream(stream));
   // CT: the edge EMFs too (E3 always, E1 and E2 in 3-D)
   if (u->edge[2] && (u->ndim < 3 || (u->edge[0] && u->edge[1])))
     launch_amr_emf_correct(plan->agrid, *u, as_stream(stream));
@@ -802,6 +828,10 @@ static int sim_create(const km_cfg_t *cfg, const km_mesh_desc_t *meshes, int32_t
     set_error("the fused stage path is built for hydro (MHD off)");
     return KM_EINVAL;
   }
+  if (path == KM_PATH_FUSED && cfg->nspecies > 1) {
+    set_error("the fused stage path is built for one species; more run on the task path");
+    return KM_EINVAL;
+  }
   if (cfg->geometry == KM_GEOM_CYLINDRICAL && (path == KM_PATH_FUSED || tree)) {
     set_error("cylindrical geometry runs on the task path over a uniform block grid");
     return KM_EINVAL;
@@ -827,8 +857,9 @@ static int sim_create(const km_cfg_t *cfg, const km_mesh_desc_t *meshes, int32_t
     s->nb += meshes[m].nb[0] * meshes[m].nb[1] * meshes[m].nb[2];
   }
   if (tree) s->nb = (int)tree->blocks.size();
-  s->nvar = KM_NVAR(cfg->mhd);
+  s->nvar = KM_NVAR(cfg->mhd) + KM_NSPEC_SLOTS(cfg->nspecies);
   s->nc = KM_NCONS(cfg->mhd);
+  s->nflux = s->nc + KM_NSPEC_SLOTS(cfg->nspecies);
   km_pack_t shape;
   std::memset(&shape, 0, sizeof(shape));
   shape.nblocks = s->nb;
@@ -836,6 +867,7 @@ static int sim_create(const km_cfg_t *cfg, const km_mesh_desc_t *meshes, int32_t
   shape.ng = mesh->ng;
   for (int d = 0; d < 3; ++d) shape.nx[d] = d < mesh->ndim ? mesh->nx[d] : 1;
   shape.nvar = s->nvar;
+  shape.nflux = s->nflux;
   if (int rc = check_cfg(&s->cfg, &shape)) return rc;
   const Geo g(shape);
   s->ncell = g.ncell();
@@ -1017,6 +1049,8 @@ int km_sim_set_primitive(km_sim_t *s, const double *cell, const double *const fa
   if (int rc = copy_state(s, cell, face, nullptr, nullptr)) return rc;
   launch_prepare_conserved(s->cfg, s->base.pack, s->stream);
   ghost_fill(*s, s->base.pack, 0, KM_TEMP(s->cfg.mhd), s->cfg.mhd == KM_MHD_CT);
+  if (s->nvar > KM_NVAR(s->cfg.mhd))
+    ghost_fill(*s, s->base.pack, KM_NVAR(s->cfg.mhd), s->nvar, false);
   launch_estimate_dt(s->cfg, s->base.pack, s->est_d, s->stream, s->inst_d, s->ninst);
   hipMemcpyAsync(s->dt_d, s->est_d, s->ninst * sizeof(double), hipMemcpyDeviceToDevice,
                  s->stream);

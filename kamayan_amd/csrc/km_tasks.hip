@@ -187,6 +187,150 @@ __device__ __forceinline__ bool interior_cell(const Geo &g, int nblocks, int &b,
   return true;
 }
 
+// ---------------------------------------------------------------------------------
+// Mass scalars (MFRAC, material unit). Fluxes (hydro_add_flux_tasks.cpp:119-135,
+// 181-197, 240-256): one lane per face along DIR; each species is reconstructed as the
+// primitives are, and its flux is the face's mass flux times the upwind side's mass
+// fraction, F_s = F_rho > 0 ? F_rho X_s(left) : F_rho X_s(right). Along x2 and x3 the
+// reference's loops run s < upper bound, so the last species gets no flux there (its flux
+// arrays stay zero). slot0: cell slot of species 0; flux0: its flux slot.
+// ---------------------------------------------------------------------------------
+template <int RECON, int DIR>
+__global__ void __launch_bounds__(256) k_species_flux(km_pack_t u, int lim, Box bx, int nsp,
+                                                      int slot0, int flux0) {
+  constexpr int W = ReconWidth<RECON>::value;
+  constexpr int DA = DIR == 0 ? 1 : 0, DB = DIR == 2 ? 1 : 2;
+  const Geo g(u);
+  const int nf = bx.hi[DIR] - bx.lo[DIR] + 2;
+  const int na = bx.hi[DA] - bx.lo[DA] + 1, nbb = bx.hi[DB] - bx.lo[DB] + 1;
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= (long)nf * na * nbb * u.nblocks) return;
+  int idx[3];
+  idx[DIR] = bx.lo[DIR] + (int)(t % nf);
+  t /= nf;
+  idx[DA] = bx.lo[DA] + (int)(t % na);
+  t /= na;
+  idx[DB] = bx.lo[DB] + (int)(t % nbb);
+  const int b = (int)(t / nbb);
+  const long c = g.c(idx[2], idx[1], idx[0]);
+  const long cs = g.sc[DIR];
+  const double frho = flux_ptr(u, DIR, b, 0)[c];
+  const int ns = DIR == 0 ? nsp : nsp - 1;
+  for (int sp = 0; sp < ns; ++sp) {
+    const double *q = cell_ptr(u, b, slot0 + sp);
+    double st[2 * W + 2];  // cells c-1-W .. c+W along DIR
+#pragma unroll
+    for (int o = 0; o < 2 * W + 2; ++o) st[o] = ldg(q, c + (o - 1 - W) * cs);
+    double wl[5], wr[5];
+#pragma unroll
+    for (int o = -W; o <= W; ++o) {
+      wl[2 + o] = st[W + o];
+      wr[2 + o] = st[W + 1 + o];
+    }
+    double m, pl, pr;
+    reconstruct_rt<RECON>(lim, wl, m, pl);
+    reconstruct_rt<RECON>(lim, wr, pr, m);
+    stg(flux_ptr(u, DIR, b, flux0 + sp), c, frho > 0.0 ? frho * pl : frho * pr);
+  }
+}
+
+// FluxesToDuDt for the species: dudt slot slot0 + s from flux slot flux0 + s
+__global__ void __launch_bounds__(256) k_species_dudt(km_pack_t u, km_pack_t dudt, int nsp,
+                                                      int slot0, int flux0, int cyl) {
+  const Geo g(u);
+  int b, k, j, i;
+  if (!interior_cell(g, u.nblocks, b, k, j, i)) return;
+  const km_block_t &blk = u.blocks[b];
+  const long c = g.c(k, j, i);
+  const double vol = cell_volume(blk, g, cyl, i);
+  for (int sp = 0; sp < nsp; ++sp) {
+    double du = 0.;
+#pragma unroll
+    for (int dir = 0; dir < 3; ++dir) {
+      if (dir < g.ndim) {
+        const double *F = flux_ptr(u, dir, b, flux0 + sp);
+        du -= (face_area(blk, g, cyl, dir, i + (dir == 0)) * F[c + g.sc[dir]] -
+               face_area(blk, g, cyl, dir, i) * F[c]);
+      }
+    }
+    cell_ptr(dudt, b, slot0 + sp)[c] = du / vol;
+  }
+}
+
+// material::PrepareConserved, the per-stage task on md0 (material.cpp:132-155):
+// X_s -> rho X_s
+__global__ void __launch_bounds__(256) k_species_to_conserved(km_pack_t u, int nsp, int slot0) {
+  const Geo g(u);
+  int b, k, j, i;
+  if (!interior_cell(g, u.nblocks, b, k, j, i)) return;
+  const long c = g.c(k, j, i);
+  const double rho = cell_ptr(u, b, KM_DENS)[c];
+  for (int sp = 0; sp < nsp; ++sp) {
+    double *x = cell_ptr(u, b, slot0 + sp);
+    x[c] = x[c] * rho;
+  }
+}
+
+// material::PreparePrimitive (material.cpp:157-197) and PostMeshInitialization (:104-130):
+// X_s -> X_s / sum_s X_s, the sum taken in species order
+__global__ void __launch_bounds__(256) k_species_normalize(km_pack_t u, int nsp, int slot0) {
+  const Geo g(u);
+  int b, k, j, i;
+  if (!interior_cell(g, u.nblocks, b, k, j, i)) return;
+  const long c = g.c(k, j, i);
+  double sum = 0.0;
+  for (int sp = 0; sp < nsp; ++sp) sum += cell_ptr(u, b, slot0 + sp)[c];
+  for (int sp = 0; sp < nsp; ++sp) {
+    double *x = cell_ptr(u, b, slot0 + sp);
+    x[c] = x[c] * (1.0 / sum);
+  }
+}
+
+// One stage of the species in one pass per zone, in the reference's task order: md0's
+// X_s -> rho X_s with md0's density (the per-stage material::PrepareConserved, which runs
+// before ApplyDuDt mixes md0), the flux divergence, ApplyDuDt, and md1's normalisation
+// (material::PreparePrimitive). Every value rounds as in the separate tasks. It runs before
+// the hydro update, which rewrites md0's density. When base and md0 are one container the
+// mix reads the converted value back, as the reference's ApplyDuDt does.
+__global__ void __launch_bounds__(256) k_species_update(km_pack_t base, km_pack_t u0,
+                                                        km_pack_t u1, int nsp, int slot0,
+                                                        int flux0, double beta, double dt,
+                                                        const double *dtp, const int *inst,
+                                                        int cyl) {
+  const Geo g(u0);
+  int b, k, j, i;
+  if (!interior_cell(g, u0.nblocks, b, k, j, i)) return;
+  if (dtp) dt = dtp[inst ? inst[b] : 0];
+  const km_block_t &blk = u0.blocks[b];
+  const long c = g.c(k, j, i);
+  const double rho = cell_ptr(u0, b, KM_DENS)[c];
+  const double vol = cell_volume(blk, g, cyl, i);
+  double sum = 0.0;
+  for (int sp = 0; sp < nsp; ++sp) {
+    double *p0 = cell_ptr(u0, b, slot0 + sp);
+    p0[c] = p0[c] * rho;
+    double du = 0.;
+#pragma unroll
+    for (int dir = 0; dir < 3; ++dir) {
+      if (dir < g.ndim) {
+        const double *F = flux_ptr(u0, dir, b, flux0 + sp);
+        du -= (face_area(blk, g, cyl, dir, i + (dir == 0)) * F[c + g.sc[dir]] -
+               face_area(blk, g, cyl, dir, i) * F[c]);
+      }
+    }
+    du = du / vol;
+    const double mix = beta * cell_ptr(base, b, slot0 + sp)[c] + (1.0 - beta) * p0[c];
+    p0[c] = mix;
+    const double x1 = mix + beta * dt * du;
+    cell_ptr(u1, b, slot0 + sp)[c] = x1;
+    sum += x1;
+  }
+  for (int sp = 0; sp < nsp; ++sp) {
+    double *x = cell_ptr(u1, b, slot0 + sp);
+    x[c] = x[c] * (1.0 / sum);
+  }
+}
+
 // grid::FluxDivergence (grid_update.hpp:15-57)
 __global__ void __launch_bounds__(256) k_flux_div(km_pack_t u, km_pack_t dudt, int nc, int cyl) {
   const Geo g(u);
@@ -285,19 +429,19 @@ __global__ void __launch_bounds__(256) k_stokes(km_pack_t u, km_pack_t dudt, int
 
 // grid::ApplyDuDt_impl (grid.cpp:213-241), cell part
 __global__ void __launch_bounds__(256) k_apply_cell(km_pack_t base, km_pack_t u0, km_pack_t u1,
-                                                    km_pack_t dudt, int nc, double beta,
-                                                    double dt, const double *dtp,
+                                                    km_pack_t dudt, int v0, int v1,
+                                                    double beta, double dt, const double *dtp,
                                                     const int *inst) {
   const Geo g(u0);
   int b, k, j, i;
   if (!interior_cell(g, u0.nblocks, b, k, j, i)) return;
   if (dtp) dt = dtp[inst ? inst[b] : 0];
   const long c = g.c(k, j, i);
-  for (int var = 0; var < nc; ++var) {
+  for (int var = v0; var < v1; ++var) {
     double *p0 = cell_ptr(u0, b, var);
-    const double v0 = beta * cell_ptr(base, b, var)[c] + (1.0 - beta) * p0[c];
-    p0[c] = v0;
-    cell_ptr(u1, b, var)[c] = v0 + beta * dt * cell_ptr(dudt, b, var)[c];
+    const double mix = beta * cell_ptr(base, b, var)[c] + (1.0 - beta) * p0[c];
+    p0[c] = mix;
+    cell_ptr(u1, b, var)[c] = mix + beta * dt * cell_ptr(dudt, b, var)[c];
   }
 }
 
@@ -771,6 +915,43 @@ static void launch_flux_riem(const km_pack_t &u, int riem, int mhd, bool fp32, i
     launch_flux_mhd<RECON, RIEMANN_HLL>(u, mhd, fp32, lim, bx, cyl, st);
 }
 
+template <int RECON>
+static void launch_species_flux_recon(const km_pack_t &u, int lim, const Box &bx, int nsp,
+                                      int slot0, int flux0, hipStream_t st) {
+  const Geo g(u);
+  auto cnt = [&](int da, int db) {
+    return (long)(bx.hi[da] - bx.lo[da] + 1) * (bx.hi[db] - bx.lo[db] + 1) * u.nblocks;
+  };
+  hipLaunchKernelGGL((k_species_flux<RECON, 0>),
+                     dim3(nblk((long)(bx.hi[0] - bx.lo[0] + 2) * cnt(1, 2))), dim3(256), 0, st, u,
+                     lim, bx, nsp, slot0, flux0);
+  if (g.ndim > 1)
+    hipLaunchKernelGGL((k_species_flux<RECON, 1>),
+                       dim3(nblk((long)(bx.hi[1] - bx.lo[1] + 2) * cnt(0, 2))), dim3(256), 0, st,
+                       u, lim, bx, nsp, slot0, flux0);
+  if (g.ndim > 2)
+    hipLaunchKernelGGL((k_species_flux<RECON, 2>),
+                       dim3(nblk((long)(bx.hi[2] - bx.lo[2] + 2) * cnt(0, 1))), dim3(256), 0, st,
+                       u, lim, bx, nsp, slot0, flux0);
+}
+
+static void launch_species_fluxes(const km_cfg_t &cfg, const km_pack_t &u, const Box &bx,
+                                  hipStream_t st) {
+  const int nsp = cfg.nspecies, s0 = KM_MFRAC(cfg.mhd, 0), f0 = KM_NCONS(cfg.mhd);
+  switch (cfg.recon) {
+    case KM_RECON_FOG:
+      launch_species_flux_recon<RECON_FOG>(u, cfg.limiter, bx, nsp, s0, f0, st);
+      break;
+    case KM_RECON_PLM:
+      launch_species_flux_recon<RECON_PLM>(u, cfg.limiter, bx, nsp, s0, f0, st);
+      break;
+    case KM_RECON_PPM:
+      launch_species_flux_recon<RECON_PPM>(u, cfg.limiter, bx, nsp, s0, f0, st);
+      break;
+    default: launch_species_flux_recon<RECON_WENOZ>(u, cfg.limiter, bx, nsp, s0, f0, st); break;
+  }
+}
+
 void launch_calculate_fluxes(const km_cfg_t &cfg, const km_pack_t &u, const Box &bx,
                              hipStream_t st) {
   // the per-task path has two arithmetic forms: FP64 (also used for FP64_FMA, whose
@@ -784,6 +965,7 @@ void launch_calculate_fluxes(const km_cfg_t &cfg, const km_pack_t &u, const Box 
     case KM_RECON_PPM: launch_flux_riem<RECON_PPM>(u, r, m, fp32, l, bx, cyl, st); break;
     default: launch_flux_riem<RECON_WENOZ>(u, r, m, fp32, l, bx, cyl, st); break;
   }
+  if (cfg.nspecies > 1) launch_species_fluxes(cfg, u, bx, st);  // after the mass flux
 }
 
 void launch_calculate_emf(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st) {
@@ -804,12 +986,16 @@ static long face_count(const Geo &g, int face, int nblocks) {
 }
 
 void launch_fluxes_to_dudt(const km_cfg_t &cfg, const km_pack_t &u, const km_pack_t &dudt,
-                           hipStream_t st) {
+                           hipStream_t st, bool species) {
+  species = species && cfg.nspecies > 1;
   const Geo g(u);
   const int nc = cfg.mhd ? 8 : 5;
   const int cyl = cfg.geometry == KM_GEOM_CYLINDRICAL;
   hipLaunchKernelGGL(k_flux_div, dim3(nblk(interior_count(g, u.nblocks))), dim3(256), 0, st, u,
                      dudt, nc, cfg.geometry == KM_GEOM_CYLINDRICAL ? 1 : 0);
+  if (species)
+    hipLaunchKernelGGL(k_species_dudt, dim3(nblk(interior_count(g, u.nblocks))), dim3(256), 0,
+                       st, u, dudt, cfg.nspecies, KM_MFRAC(cfg.mhd, 0), nc, cyl);
   if (cfg.mhd && g.ndim >= 2) {
     hipLaunchKernelGGL(k_stokes<0>, dim3(nblk(face_count(g, 0, u.nblocks))), dim3(256), 0, st,
                        u, dudt, cyl);
@@ -838,7 +1024,11 @@ void launch_apply_dudt(const km_cfg_t &cfg, const km_pack_t &base, const km_pack
   const Geo g(u0);
   const int nc = cfg.mhd ? 8 : 5;
   hipLaunchKernelGGL(k_apply_cell, dim3(nblk(interior_count(g, u0.nblocks))), dim3(256), 0, st,
-                     base, u0, u1, dudt, nc, beta, dt, dt_dev, inst);
+                     base, u0, u1, dudt, 0, nc, beta, dt, dt_dev, inst);
+  if (cfg.nspecies > 1)
+    hipLaunchKernelGGL(k_apply_cell, dim3(nblk(interior_count(g, u0.nblocks))), dim3(256), 0,
+                       st, base, u0, u1, dudt, KM_MFRAC(cfg.mhd, 0),
+                       KM_MFRAC(cfg.mhd, cfg.nspecies), beta, dt, dt_dev, inst);
   if (cfg.mhd && g.ndim >= 2) {
     hipLaunchKernelGGL(k_apply_face<0>, dim3(nblk(face_count(g, 0, u0.nblocks))), dim3(256), 0,
                        st, base, u0, u1, dudt, beta, dt, dt_dev, inst);
@@ -857,6 +1047,10 @@ void launch_update(const km_cfg_t &cfg, const km_pack_t &base, const km_pack_t &
   const Eos eos = make_eos(cfg);
   const unsigned grid = nblk(interior_count(g, u0.nblocks));
   const int cyl = cfg.geometry == KM_GEOM_CYLINDRICAL;
+  if (cfg.nspecies > 1)
+    hipLaunchKernelGGL(k_species_update, dim3(grid), dim3(256), 0, st, base, u0, u1,
+                       cfg.nspecies, KM_MFRAC(cfg.mhd, 0), KM_NCONS(cfg.mhd), beta, 0.0,
+                       dt_dev, inst, cyl);
   if (!cfg.mhd) {
     hipLaunchKernelGGL(k_update_cell<0>, dim3(grid), dim3(256), 0, st, base, u0, u1, beta, 0.0,
                        dt_dev, inst, eos, cyl);
@@ -866,11 +1060,11 @@ void launch_update(const km_cfg_t &cfg, const km_pack_t &base, const km_pack_t &
     // the reference's task order, which the per-stage PrepareConserved on md0 splits:
     // FluxesToDuDt, AddSourceTerms (reads md0's B_phi), PrepareConserved (md0's B_phi to
     // B_phi / r), ApplyDuDt
-    launch_fluxes_to_dudt(cfg, u0, dudt, st);
+    launch_fluxes_to_dudt(cfg, u0, dudt, st, false);
     launch_add_source_terms(cfg, u0, dudt, st);
     hipLaunchKernelGGL(k_cyl_stage_conserved, dim3(grid), dim3(256), 0, st, u0);
-    hipLaunchKernelGGL(k_apply_cell, dim3(grid), dim3(256), 0, st, base, u0, u1, dudt, 8, beta,
-                       0.0, dt_dev, inst);
+    hipLaunchKernelGGL(k_apply_cell, dim3(grid), dim3(256), 0, st, base, u0, u1, dudt, 0, 8,
+                       beta, 0.0, dt_dev, inst);
   } else {
     hipLaunchKernelGGL(k_update_cell<1>, dim3(grid), dim3(256), 0, st, base, u0, u1, beta, 0.0,
                        dt_dev, inst, eos, cyl);
@@ -905,6 +1099,9 @@ void launch_prepare_primitive(const km_cfg_t &cfg, const km_pack_t &u, hipStream
     hipLaunchKernelGGL(k_prepare_primitive<1>, dim3(grid), dim3(256), 0, st, u, eos, cyl);
   else
     hipLaunchKernelGGL(k_prepare_primitive<0>, dim3(grid), dim3(256), 0, st, u, eos, cyl);
+  if (cfg.nspecies > 1)
+    hipLaunchKernelGGL(k_species_normalize, dim3(grid), dim3(256), 0, st, u, cfg.nspecies,
+                       KM_MFRAC(cfg.mhd, 0));
 }
 
 void launch_prepare_conserved(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st) {
@@ -916,6 +1113,10 @@ void launch_prepare_conserved(const km_cfg_t &cfg, const km_pack_t &u, hipStream
     hipLaunchKernelGGL(k_prepare_conserved<1>, dim3(grid), dim3(256), 0, st, u, eos, cyl);
   else
     hipLaunchKernelGGL(k_prepare_conserved<0>, dim3(grid), dim3(256), 0, st, u, eos, cyl);
+  // material::PostMeshInitialization: the mass fractions sum to one
+  if (cfg.nspecies > 1)
+    hipLaunchKernelGGL(k_species_normalize, dim3(grid), dim3(256), 0, st, u, cfg.nspecies,
+                       KM_MFRAC(cfg.mhd, 0));
 }
 
 void launch_estimate_dt(const km_cfg_t &cfg, const km_pack_t &u, double *dt_dev,
@@ -939,10 +1140,13 @@ void launch_fill_divb(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st) {
 }
 
 void launch_prepare_conserved_stage(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st) {
-  if (cfg.geometry != KM_GEOM_CYLINDRICAL || cfg.mhd == KM_MHD_OFF) return;
   const Geo g(u);
-  hipLaunchKernelGGL(k_cyl_stage_conserved, dim3(nblk(interior_count(g, u.nblocks))), dim3(256),
-                     0, st, u);
+  const unsigned grid = nblk(interior_count(g, u.nblocks));
+  if (cfg.geometry == KM_GEOM_CYLINDRICAL && cfg.mhd != KM_MHD_OFF)
+    hipLaunchKernelGGL(k_cyl_stage_conserved, dim3(grid), dim3(256), 0, st, u);
+  if (cfg.nspecies > 1)
+    hipLaunchKernelGGL(k_species_to_conserved, dim3(grid), dim3(256), 0, st, u, cfg.nspecies,
+                       KM_MFRAC(cfg.mhd, 0));
 }
 
 void launch_ghost_fill(const UniformGrid &gr, const km_pack_t &u, int v0, int v1, int nc,

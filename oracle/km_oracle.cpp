@@ -439,12 +439,17 @@ void CalculateFluxes(const km_cfg_t &cfg, const km_pack_t &u) {
   }
   const int nxb = sh.E[0];
   const int nb = u.nblocks;
-  int slot[10];
+  // reconstruct_vars = Reconstruct + MassScalars (:40-46): with more than one species the
+  // mass fractions follow the primitives in the pencils, at offset nr; flux slot nc + s
+  const int nsp = cfg.nspecies > 1 ? cfg.nspecies : 0;
+  const int nrt = nr + nsp, nc = ncons(mhd);
+  std::vector<int> slot(nrt);
   for (int r = 0; r < nr; ++r) slot[r] = recon_slot(r, mhd);
+  for (int sp = 0; sp < nsp; ++sp) slot[nr + sp] = KM_MFRAC(mhd, sp);
 
 #pragma omp parallel
   {
-    std::vector<double> bufM(nr * nxb), bufP(nr * nxb), bufMP(nr * nxb);
+    std::vector<double> bufM(nrt * nxb), bufP(nrt * nxb), bufMP(nrt * nxb);
     double vL[10], vR[10], F[8];
     // X-direction, :79-136
 #pragma omp for collapse(3) schedule(static)
@@ -452,7 +457,7 @@ void CalculateFluxes(const km_cfg_t &cfg, const km_pack_t &u) {
       for (int k = kb.s; k <= kb.e; ++k)
         for (int j = jb.s; j <= jb.e; ++j) {
           double *vM = bufM.data(), *vP = bufP.data();
-          for (int var = 0; var < nr; ++var) {
+          for (int var = 0; var < nrt; ++var) {
             const double *q = cellp(u, b, slot[var]);
             for (int i = ib.s - 1; i <= ib.e + 1; ++i) {
               Stencil st{q + sh.c(k, j, i), 1};
@@ -477,6 +482,14 @@ void CalculateFluxes(const km_cfg_t &cfg, const km_pack_t &u) {
             }
             for (int v = 0; v < ncons(mhd); ++v) fluxp(u, 0, b, v)[sh.c(k, j, i)] = F[v];
           }
+          // mass scalars, :121-135: upwind mass fraction times the mass flux
+          for (int sp = 0; sp < nsp; ++sp)
+            for (int i = ib.s; i <= ib.e + 1; ++i) {
+              const long c = sh.c(k, j, i);
+              const double rf = fluxp(u, 0, b, KM_DENS)[c];
+              fluxp(u, 0, b, nc + sp)[c] = rf > 0.0 ? rf * vP[(nr + sp) * nxb + i - 1]
+                                                    : rf * vM[(nr + sp) * nxb + i];
+            }
         }
     // Y-direction, :138-203
     if (ndim > 1) {
@@ -485,7 +498,7 @@ void CalculateFluxes(const km_cfg_t &cfg, const km_pack_t &u) {
         for (int k = kb.s; k <= kb.e; ++k) {
           double *vMP = bufMP.data(), *vM = bufM.data(), *vP = bufP.data();
           for (int j = jb.s - 1; j <= jb.e + 1; ++j) {
-            for (int var = 0; var < nr; ++var) {
+            for (int var = 0; var < nrt; ++var) {
               const double *q = cellp(u, b, slot[var]);
               for (int i = ib.s; i <= ib.e; ++i) {
                 Stencil st{q + sh.c(k, j, i), sh.E[0]};
@@ -511,6 +524,15 @@ void CalculateFluxes(const km_cfg_t &cfg, const km_pack_t &u) {
                 }
                 for (int v = 0; v < ncons(mhd); ++v) fluxp(u, 1, b, v)[sh.c(k, j, i)] = F[v];
               }
+              // mass scalars, :180-197: the loop ends at s < upper bound, so the last
+              // species has no x2 flux
+              for (int sp = 0; sp + 1 < nsp; ++sp)
+                for (int i = ib.s; i <= ib.e; ++i) {
+                  const long c = sh.c(k, j, i);
+                  const double rf = fluxp(u, 1, b, KM_DENS)[c];
+                  fluxp(u, 1, b, nc + sp)[c] = rf > 0.0 ? rf * vMP[(nr + sp) * nxb + i]
+                                                        : rf * vM[(nr + sp) * nxb + i];
+                }
             }
             std::swap(vMP, vP);
           }
@@ -523,7 +545,7 @@ void CalculateFluxes(const km_cfg_t &cfg, const km_pack_t &u) {
         for (int j = jb.s; j <= jb.e; ++j) {
           double *vMP = bufMP.data(), *vM = bufM.data(), *vP = bufP.data();
           for (int k = kb.s - 1; k <= kb.e + 1; ++k) {
-            for (int var = 0; var < nr; ++var) {
+            for (int var = 0; var < nrt; ++var) {
               const double *q = cellp(u, b, slot[var]);
               for (int i = ib.s; i <= ib.e; ++i) {
                 Stencil st{q + sh.c(k, j, i), (long)sh.E[0] * sh.E[1]};
@@ -544,6 +566,16 @@ void CalculateFluxes(const km_cfg_t &cfg, const km_pack_t &u) {
                 RiemannFlux(cfg.riemann, mhd, 2, vL, vR, F);
                 for (int v = 0; v < ncons(mhd); ++v) fluxp(u, 2, b, v)[sh.c(k, j, i)] = F[v];
               }
+              // mass scalars, :239-256, again without the last species. The reference's
+              // i loop runs to ib.e + 1, one x1 ghost column whose x3 flux no interior
+              // update reads; it is left out here.
+              for (int sp = 0; sp + 1 < nsp; ++sp)
+                for (int i = ib.s; i <= ib.e; ++i) {
+                  const long c = sh.c(k, j, i);
+                  const double rf = fluxp(u, 2, b, KM_DENS)[c];
+                  fluxp(u, 2, b, nc + sp)[c] = rf > 0.0 ? rf * vMP[(nr + sp) * nxb + i]
+                                                        : rf * vM[(nr + sp) * nxb + i];
+                }
             }
             std::swap(vMP, vP);
           }
@@ -610,6 +642,7 @@ void FluxesToDuDt(const km_cfg_t &cfg, const km_pack_t &u, const km_pack_t &dudt
   const Shape sh(u);
   const int ndim = sh.ndim;
   const int nc = ncons(cfg.mhd);
+  const int nsp = cfg.nspecies > 1 ? cfg.nspecies : 0;
   const long off[3] = {1, sh.E[0], (long)sh.E[0] * sh.E[1]};
 #pragma omp parallel for collapse(2) schedule(static)
   for (int b = 0; b < u.nblocks; ++b)
@@ -618,10 +651,13 @@ void FluxesToDuDt(const km_cfg_t &cfg, const km_pack_t &u, const km_pack_t &dudt
       for (int j = sh.js; j <= sh.je; ++j)
         for (int i = sh.is; i <= sh.ie; ++i) {
           const long c = sh.c(k, j, i);
-          for (int var = 0; var < nc; ++var) {
+          // every WithFluxes field: the conserved ones, then the mass fractions (flux
+          // slot nc + s, cell slot KM_MFRAC(s))
+          for (int fv = 0; fv < nc + nsp; ++fv) {
+            const int var = fv < nc ? fv : KM_MFRAC(cfg.mhd, fv - nc);
             double du = 0.;
             for (int dir = 0; dir < ndim; ++dir) {
-              const double *F = fluxp(u, dir, b, var);
+              const double *F = fluxp(u, dir, b, fv);
               du -= (Area(cfg, blk, sh, dir, i + (dir == 0)) * F[c + off[dir]] -
                      Area(cfg, blk, sh, dir, i) * F[c]);
             }
@@ -705,7 +741,10 @@ void AddSourceTerms(const km_cfg_t &cfg, const km_pack_t &u, const km_pack_t &du
 
 // hydro::PrepareConserved, the per-stage task before ApplyDuDt (hydro.cpp:244-286): with CT
 // in cylindrical geometry the cell B_phi of md0 becomes B_phi / r_centroid
+void SpeciesToConserved(const km_cfg_t &cfg, const km_pack_t &u);
+
 void StagePrepareConserved(const km_cfg_t &cfg, const km_pack_t &u) {
+  SpeciesToConserved(cfg, u);
   if (!Cyl(cfg) || cfg.mhd == KM_MHD_OFF) return;
   const Shape sh(u);
 #pragma omp parallel for collapse(2) schedule(static)
@@ -716,17 +755,55 @@ void StagePrepareConserved(const km_cfg_t &cfg, const km_pack_t &u) {
           cellp(u, b, KM_MAGC0 + 2)[sh.c(k, j, i)] *= 1.0 / RCentroid(u.blocks[b], sh, i);
 }
 
+// material::PrepareConserved, material.cpp:132-155: X_s -> rho X_s (more than one species)
+void SpeciesToConserved(const km_cfg_t &cfg, const km_pack_t &u) {
+  if (cfg.nspecies < 2) return;
+  const Shape sh(u);
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int b = 0; b < u.nblocks; ++b)
+    for (int k = sh.ks; k <= sh.ke; ++k)
+      for (int j = sh.js; j <= sh.je; ++j)
+        for (int i = sh.is; i <= sh.ie; ++i) {
+          const long c = sh.c(k, j, i);
+          for (int sp = 0; sp < cfg.nspecies; ++sp)
+            cellp(u, b, KM_MFRAC(cfg.mhd, sp))[c] *= cellp(u, b, KM_DENS)[c];
+        }
+}
+
+// material::PreparePrimitive (material.cpp:157-197) and PostMeshInitialization (:104-130):
+// each mass fraction divided by the sum over species, summed in species order. The
+// reference's PreparePrimitive indexes its per-row scratch sum by the interior i (which
+// starts at nghost) in a scratch of nx entries; the division by the sum is restated here
+// as PostMeshInitialization writes it.
+void SpeciesNormalize(const km_cfg_t &cfg, const km_pack_t &u) {
+  if (cfg.nspecies < 2) return;
+  const Shape sh(u);
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int b = 0; b < u.nblocks; ++b)
+    for (int k = sh.ks; k <= sh.ke; ++k)
+      for (int j = sh.js; j <= sh.je; ++j)
+        for (int i = sh.is; i <= sh.ie; ++i) {
+          const long c = sh.c(k, j, i);
+          double sum = 0.0;
+          for (int sp = 0; sp < cfg.nspecies; ++sp) sum += cellp(u, b, KM_MFRAC(cfg.mhd, sp))[c];
+          for (int sp = 0; sp < cfg.nspecies; ++sp)
+            cellp(u, b, KM_MFRAC(cfg.mhd, sp))[c] *= 1. / sum;
+        }
+}
+
 void ApplyDuDt(const km_cfg_t &cfg, const km_pack_t &base, const km_pack_t &u0,
                const km_pack_t &u1, const km_pack_t &dudt, double beta, double dt) {
   const Shape sh(u0);
   const int nc = ncons(cfg.mhd);
+  const int nsp = cfg.nspecies > 1 ? cfg.nspecies : 0;
 #pragma omp parallel for collapse(2) schedule(static)
   for (int b = 0; b < u0.nblocks; ++b)
     for (int k = sh.ks; k <= sh.ke; ++k)
       for (int j = sh.js; j <= sh.je; ++j)
         for (int i = sh.is; i <= sh.ie; ++i) {
           const long c = sh.c(k, j, i);
-          for (int var = 0; var < nc; ++var) {
+          for (int fv = 0; fv < nc + nsp; ++fv) {
+            const int var = fv < nc ? fv : KM_MFRAC(cfg.mhd, fv - nc);
             double &p0 = cellp(u0, b, var)[c];
             p0 = beta * cellp(base, b, var)[c] + (1.0 - beta) * p0;
             cellp(u1, b, var)[c] = p0 + beta * dt * cellp(dudt, b, var)[c];
@@ -815,6 +892,7 @@ void PreparePrimitive(const km_cfg_t &cfg, const km_pack_t &u) {
           cellp(u, b, KM_TEMP(mhd))[c] = eos.T(dens, sie);
           cellp(u, b, KM_BMOD(mhd))[c] = eos.B(dens, sie);
         }
+  SpeciesNormalize(cfg, u);  // material PreparePrimitive, after hydro and eos
 }
 
 // PostMeshInitialization: eos PrepareConserved(mode_init = dens_pres) (eos.cpp:165-168)
@@ -843,6 +921,7 @@ void PrepareConserved(const km_cfg_t &cfg, const km_pack_t &u) {
           if (Cyl(cfg)) U[KM_MOM0 + 2] *= RCentroid(u.blocks[b], sh, i);
           for (int v = 0; v < ncons(mhd); ++v) cellp(u, b, v)[c] = U[v];
         }
+  SpeciesNormalize(cfg, u);  // material PostMeshInitialization
 }
 
 // EstimateTimeStep::dispatch, hydro_time_step.cpp:19-65
@@ -1691,7 +1770,7 @@ void AmrEmfCorrect(const Tree &t, const km_pack_t &u) {
 
 void AmrFluxCorrect(const km_cfg_t &cfg, const Tree &t, const km_pack_t &u) {
   const Shape sh(u);
-  const int nc = ncons(cfg.mhd);
+  const int nc = ncons(cfg.mhd) + KM_NSPEC_SLOTS(cfg.nspecies);  // every WithFluxes field
   const int st[3] = {sh.is, sh.js, sh.ks};
   for (size_t b = 0; b < t.level.size(); ++b) {
     const int L = t.level[b];
@@ -2256,6 +2335,7 @@ struct km_cpu_sim_t {
   Tree tree;
   int nblocks = 0;
   int nvar, nc, nstages;
+  int nflux;  // nc plus one per species (more than one species)
   Container base, stage[2], dudt;
   double time = 0., dt = 0., dt_est = 0.;
   int ncycle = 0;
@@ -2279,15 +2359,17 @@ void MakeContainer(km_cpu_sim_t &s, Container &c, bool full, Container *share_pr
   p.ng = m.ng;
   for (int d = 0; d < 3; ++d) p.nx[d] = d < m.ndim ? m.nx[d] : 1;
   p.nvar = s.nvar;
-  p.nflux = s.nc;
+  p.nflux = s.nflux;
   p.blocks = s.blocks.data();
   Shape sh(p);
+  // stage containers own the conserved fields (hydro conserved slots and mass fractions)
+  const int nhydro = KM_NVAR(s.cfg.mhd);
   c.cell.assign((size_t)nbt * s.nvar, {});
   c.cell_tab.assign((size_t)nbt * s.nvar, nullptr);
   for (int b = 0; b < nbt; ++b)
     for (int v = 0; v < s.nvar; ++v) {
       const size_t t = (size_t)b * s.nvar + v;
-      if (v < s.nc || full) {
+      if (v < s.nc || v >= nhydro || full) {
         c.cell[t].assign(sh.ncell(), 0.0);
         c.cell_tab[t] = c.cell[t].data();
       } else if (share_prims) {
@@ -2297,8 +2379,8 @@ void MakeContainer(km_cpu_sim_t &s, Container &c, bool full, Container *share_pr
   p.cell = c.cell_tab.data();
   if (with_flux) {
     for (int d = 0; d < 3; ++d) {
-      c.flux[d].assign((size_t)nbt * s.nc, std::vector<double>(sh.ncell(), 0.0));
-      c.flux_tab[d].resize((size_t)nbt * s.nc);
+      c.flux[d].assign((size_t)nbt * s.nflux, std::vector<double>(sh.ncell(), 0.0));
+      c.flux_tab[d].resize((size_t)nbt * s.nflux);
       for (size_t t = 0; t < c.flux_tab[d].size(); ++t) c.flux_tab[d][t] = c.flux[d][t].data();
       p.flux[d] = c.flux_tab[d].data();
     }
@@ -2323,11 +2405,15 @@ void MakeContainer(km_cpu_sim_t &s, Container &c, bool full, Container *share_pr
 // hydro.cpp:102-105): conserved, VELOCITY, PRES, BMOD, EINT, and face B; TEMP is not
 // FillGhost (eos.cpp:67).
 void FillGhosts(km_cpu_sim_t &s, const km_pack_t &p) {
+  const int nh = KM_NVAR(s.cfg.mhd);  // mass fractions (FillGhost) follow at nh..nvar-1
   if (s.amr) {
     AmrGhostFill(s.cfg, s.tree, p, 0, KM_TEMP(s.cfg.mhd));
+    if (s.nvar > nh) AmrGhostFill(s.cfg, s.tree, p, nh, s.nvar);
     if (s.cfg.mhd == KM_MHD_CT) AmrGhostFaces(s.cfg, s.tree, p);
-  } else
+  } else {
     GhostFill(s.cfg, s.grid, p, 0, KM_TEMP(s.cfg.mhd), true);
+    if (s.nvar > nh) GhostFill(s.cfg, s.grid, p, nh, s.nvar, false);
+  }
 }
 
 const double kBetaRK1[1] = {1.0};
@@ -2513,8 +2599,9 @@ int km_cpu_sim_create(const km_cfg_t *cfg, const km_mesh_desc_t *mesh, km_cpu_si
   s->cfg = *cfg;
   s->mesh = *mesh;
   s->cfg.ndim = mesh->ndim;
-  s->nvar = KM_NVAR(cfg->mhd);
+  s->nvar = KM_NVAR(cfg->mhd) + KM_NSPEC_SLOTS(cfg->nspecies);
   s->nc = KM_NCONS(cfg->mhd);
+  s->nflux = s->nc + KM_NSPEC_SLOTS(cfg->nspecies);
   s->nstages = cfg->nstages;
   for (int d = 0; d < 3; ++d) s->grid.nb[d] = mesh->nb[d];
   for (int d = 0; d < 6; ++d) s->grid.bc[d] = mesh->bc[d];
@@ -2586,8 +2673,9 @@ int km_cpu_sim_create_amr(const km_cfg_t *cfg, const km_mesh_desc_t *mesh,
   s->cfg = *cfg;
   s->mesh = *mesh;
   s->cfg.ndim = mesh->ndim;
-  s->nvar = KM_NVAR(cfg->mhd);
+  s->nvar = KM_NVAR(cfg->mhd) + KM_NSPEC_SLOTS(cfg->nspecies);
   s->nc = KM_NCONS(cfg->mhd);
+  s->nflux = s->nc + KM_NSPEC_SLOTS(cfg->nspecies);
   s->nstages = cfg->nstages;
   const std::string err = BuildTree(*mesh, leaves, n, s->tree);
   if (!err.empty() || (cfg->mhd == KM_MHD_CT && mesh->ng % 2)) {

@@ -13,7 +13,7 @@ import subprocess
 import numpy as np
 
 from kamayan_amd._abi import (KmAmrAdapt, KmAmrCriterion, KmBlock, KmCfg, KmMeshDesc, KmPack,
-                              Layout, f64ptr)
+                              Layout, f64ptr, nvar)
 
 HERE = pathlib.Path(__file__).resolve().parent
 LIB_PATH = HERE / "libkm_oracle.so"
@@ -289,7 +289,7 @@ class CpuSimulation:
 
     def state(self):
         mhd = self.cfg.mhd
-        nv = 16 if mhd == 1 else 12
+        nv = nvar(mhd, self.cfg.nspecies)
         if getattr(self, "_leaves", None) is not None:
             self.leaves()
         cell = np.zeros(self.layout.cell_shape(nv))

@@ -138,13 +138,15 @@ def kh_ensemble(resolutions, nblock=32, ng=3, recon="wenoz", riemann="hllc", int
 def amr_case(problem="smooth3d", ndim=3, nroot=2, nblock=8, ng=2, lmax=1, center=(0.5,) * 3,
              radius=0.3, bc="periodic", recon="plm", limiter="mc", riemann="hllc",
              integrator="rk2", seed=20261206, params=None, xmin=(0.0,) * 3, xmax=(1.0,) * 3,
-             gamma=5.0 / 3.0):
+             gamma=5.0 / 3.0, species=None):
     """A block-tree mesh: a root grid of nroot^ndim blocks of nblock cells refined to lmax
     where blocks meet a ball, 2:1 balanced; the problem filled on every leaf at its own
     resolution. Returns (cfg, desc, layout, leaves, cell)."""
     cfg = Config(**{"hydro/reconstruction": recon, "hydro/slope_limiter": limiter,
                     "hydro/riemann": riemann, "parthenon/time/integrator": integrator,
                     "material/single/gamma": gamma})
+    if species:
+        cfg.set("material/species", species)
     cfg.c.ndim = ndim
     nxb = tuple(nblock if d < ndim else 1 for d in range(3))
     nbs = tuple(nroot if d < ndim else 1 for d in range(3))
@@ -198,3 +200,51 @@ def divb_cells(faces, lay, blocks):
             d = d + bk.area[2] * (f2[1:e3 + 1, :e2, :e1] - f2[:e3, :e2, :e1])
         out.append(d / bk.vol)
     return out
+
+
+def species_blobs(cell, lay: Layout, blocks, mhd: int, nsp: int, seed=11, normalised=False):
+    """Mass fractions on every block interior: a floor plus one Gaussian bump per species
+    at a seeded centre in the unit box (cell centres from the block metric). Unnormalised
+    unless asked; PostMeshInitialization divides by the sum."""
+    rng = np.random.default_rng(seed)
+    centres = rng.uniform(0.2, 0.8, size=(nsp, 3))
+    widths = rng.uniform(8.0, 30.0, size=nsp)
+    g = lay.ng
+    k_, j_, i_ = lay.interior()
+    for b in range(lay.nblocks):
+        bk = blocks[b]
+        ax = []
+        for d in range(3):
+            n = lay.E[d]
+            lo = g if d < lay.ndim else 0
+            ax.append(bk.xmin[d] + (np.arange(n) - lo + 0.5) * bk.dx[d])
+        Z, Y, X = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+        fr = []
+        for s_ in range(nsp):
+            r2 = (X - centres[s_, 0]) ** 2
+            if lay.ndim > 1:
+                r2 = r2 + (Y - centres[s_, 1]) ** 2
+            if lay.ndim > 2:
+                r2 = r2 + (Z - centres[s_, 2]) ** 2
+            fr.append(0.05 + np.exp(-widths[s_] * r2))
+        tot = sum(fr) if normalised else 1.0
+        for s_ in range(nsp):
+            cell[b, km.mfrac(mhd, s_)][k_, j_, i_] = (fr[s_] / tot)[k_, j_, i_]
+    return cell
+
+
+def sedov_species(ndim=2, n=32, nblock=16, ng=4, recon="wenoz", limiter="minmod",
+                  riemann="hllc", integrator="rk2", species="single,air,tracer"):
+    """sedov.in (wenoz, hllc, minmod, rk2, nghost 4, gamma 1.4, outflow on [-0.5, 0.5]^d)
+    on an n^d grid with material/species set: species 0 inside the blast radius, species 1
+    outside (sedov.cpp:138-141), the rest zero."""
+    cfg = Config(**{"hydro/reconstruction": recon, "hydro/slope_limiter": limiter,
+                    "hydro/riemann": riemann, "parthenon/time/integrator": integrator,
+                    "material/single/gamma": 1.4, "material/species": species})
+    cfg.c.ndim = ndim
+    nxb = tuple(nblock if d < ndim else 1 for d in range(3))
+    nbs = tuple(n // nblock if d < ndim else 1 for d in range(3))
+    lay = Layout(ndim, ng, nxb, nbs)
+    desc = mesh_desc(lay, xmin=(-0.5,) * 3, xmax=(0.5,) * 3, bc=("outflow",) * 6)
+    cell, faces = km.problem_fill("sedov", cfg, desc, lay)
+    return cfg, desc, lay, cell, faces

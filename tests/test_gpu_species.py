@@ -1,0 +1,218 @@
+"""GPU parity of the mass scalars (material/species) against the oracle, bit for bit: the
+task entry points one by one (species fluxes after the mass flux, their dU/dt, the
+per-stage conversion to partial densities, the stage update, the normalisation, the
+ghost fill of the fractions) and whole task-path runs with species on uniform grids,
+with CT MHD, in cylindrical geometry, on a static block tree and on an adaptive one."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from kamayan_amd import Config, Layout, amr_adapt, amr_criterion, mesh_desc, slot
+from tests import _cases
+
+pytestmark = pytest.mark.gpu
+
+
+def _same(a, b, what):
+    if not np.array_equal(a, b):
+        bad = np.argwhere(a != b)
+        i = tuple(bad[0])
+        raise AssertionError(f"{what}: {len(bad)} differ, first {i}: {a[i]!r} vs {b[i]!r}")
+
+
+@pytest.mark.parametrize("ndim,recon", [(2, "plm"), (3, "wenoz"), (2, "ppm")])
+def test_species_tasks_bitwise(gpu, oracle, ndim, recon):
+    km = gpu
+    nsp = 3
+    cfg = Config(**{"hydro/reconstruction": recon, "hydro/slope_limiter": "mc",
+                    "hydro/riemann": "hllc", "material/single/gamma": 1.4,
+                    "material/species": "a,b,c"})
+    cfg.c.ndim = ndim
+    nxb = tuple(8 if d < ndim else 1 for d in range(3))
+    nbs = tuple(2 if d < ndim else 1 for d in range(3))
+    lay = Layout(ndim, 3, nxb, nbs)
+    desc = mesh_desc(lay)
+    print(f"species tasks {ndim}d {recon}", flush=True)
+    cell, _ = km.problem_fill("smooth3d", cfg, desc, lay, seed=8)
+    blocks = oracle.block_table(desc, lay.nblocks)
+    _cases.species_blobs(cell, lay, blocks, 0, nsp)
+    nv, nc = km.nvar(0, nsp), km.ncons(0)
+    L, p = km.lib(), C.byref
+    dev = km.DevicePack(lay, nv, nflux=nc + nsp, blocks=blocks)
+    host = oracle.HostPack(lay, nv, nflux=nc + nsp, blocks=blocks)
+    dev.upload(cell=cell)
+    host.cell[:] = cell
+    km.check(L.km_prepare_conserved(p(cfg.c), p(dev.pack), None), "cons")
+    oracle.call("km_cpu_prepare_conserved", p(cfg.c), p(host.pack), None)
+    nb = (C.c_int32 * 3)(*[lay.nb[d] if d < lay.ndim else 1 for d in range(3)])
+    bc = (C.c_int32 * 6)(*[desc.bc[d] for d in range(6)])
+    plan = C.c_void_p()
+    km.check(L.km_ghost_plan_uniform(p(dev.pack), nb, bc, p(plan)), "plan")
+    for v0, v1 in ((0, 11), (12, nv)):
+        km.check(L.km_ghost_fill(plan, p(dev.pack), v0, v1, 0, None), "ghost")
+        oracle.call("km_cpu_ghost_fill_uniform", p(cfg.c), p(host.pack), nb, bc, v0, v1, 0)
+    L.km_ghost_plan_destroy(plan)
+    _same(dev.download()[0], host.cell, "conserved + normalised + ghost fill")
+    km.check(L.km_calculate_fluxes(p(cfg.c), p(dev.pack), None), "fluxes")
+    oracle.call("km_cpu_calculate_fluxes", p(cfg.c), p(host.pack), None)
+    _, flux, _, _ = dev.download()
+    for d in range(ndim):
+        _same(flux[d], host.flux[d], f"flux[{d}]")
+    ddev = km.DevicePack(lay, nv, blocks=blocks)
+    dhost = oracle.HostPack(lay, nv, blocks=blocks)
+    km.check(L.km_fluxes_to_dudt(p(cfg.c), p(dev.pack), p(ddev.pack), None), "dudt")
+    oracle.call("km_cpu_fluxes_to_dudt", p(cfg.c), p(host.pack), p(dhost.pack), None)
+    _same(ddev.download()[0], dhost.cell, "dudt")
+    # a second stage: base = md0 copy, md0 converted to partial densities, then the update
+    base_d = km.DevicePack(lay, nv, blocks=blocks)
+    base_h = oracle.HostPack(lay, nv, blocks=blocks)
+    base_d.upload(cell=host.cell)
+    base_h.cell[:] = host.cell
+    km.check(L.km_prepare_conserved_stage(p(cfg.c), p(base_d.pack), None), "stage cons base")
+    oracle.call("km_cpu_prepare_conserved_stage", p(cfg.c), p(base_h.pack), None)
+    km.check(L.km_prepare_conserved_stage(p(cfg.c), p(dev.pack), None), "stage cons")
+    oracle.call("km_cpu_prepare_conserved_stage", p(cfg.c), p(host.pack), None)
+    _same(dev.download()[0], host.cell, "md0 partial densities")
+    u1d = km.DevicePack(lay, nv, blocks=blocks)
+    u1h = oracle.HostPack(lay, nv, blocks=blocks)
+    km.check(L.km_apply_dudt(p(cfg.c), p(base_d.pack), p(dev.pack), p(u1d.pack), p(ddev.pack),
+                             0.5, 2e-3, None), "apply")
+    oracle.call("km_cpu_apply_dudt", p(cfg.c), p(base_h.pack), p(host.pack), p(u1h.pack),
+                p(dhost.pack), 0.5, 2e-3, None)
+    km.check(L.km_prepare_primitive(p(cfg.c), p(u1d.pack), None), "prim")
+    oracle.call("km_cpu_prepare_primitive", p(cfg.c), p(u1h.pack), None)
+    _same(u1d.download()[0], u1h.cell, "u1")
+    _same(dev.download()[0], host.cell, "md0 after the mix")
+    for x in (dev, ddev, base_d, u1d):
+        x.close()
+
+
+@pytest.mark.parametrize("ndim,n,recon,integrator,ncyc", [
+    (2, 32, "wenoz", "rk2", 12),   # sedov.in's options on a 32^2 grid
+    (3, 16, "plm", "rk3", 5),
+    (1, 64, "ppm", "rk2", 20),
+])
+def test_sedov_species_run_bitwise(gpu, oracle, ndim, n, recon, integrator, ncyc):
+    km = gpu
+    cfg, desc, lay, cell, _ = _cases.sedov_species(ndim=ndim, n=n, nblock=n // 2, recon=recon,
+                                                   integrator=integrator)
+    print(f"sedov species {ndim}d {recon} {integrator}", flush=True)
+    sim = km.Simulation(cfg, desc, lay, path=km.PATH_TASKS)
+    sim.set_primitive(cell)
+    ref = oracle.CpuSimulation(cfg.c, desc, lay)
+    ref.set_primitive(cell)
+    sim.advance(ncyc)
+    ref.advance(ncyc)
+    g, _ = sim.state()
+    r, _ = ref.state()
+    _same(g, r, "sedov species")
+    assert sim.info() == ref.info()
+    sim.close()
+
+
+def test_mhd_species_run_bitwise(gpu, oracle):
+    km = gpu
+    cfg = Config(**{"hydro/reconstruction": "ppm", "hydro/slope_limiter": "mc",
+                    "hydro/riemann": "hll", "parthenon/time/integrator": "rk2",
+                    "material/single/gamma": 5.0 / 3.0, "physics/MHD": "ct",
+                    "material/species": "a,b,c,d"})
+    cfg.c.ndim = 2
+    lay = Layout(2, 4, (8, 8, 1), (2, 2, 1))
+    desc = mesh_desc(lay)
+    print("mhd species 2d", flush=True)
+    cell, faces = km.problem_fill("mhd_random", cfg, desc, lay, seed=31)
+    _cases.species_blobs(cell, lay, oracle.block_table(desc, lay.nblocks), 1, 4)
+    sim = km.Simulation(cfg, desc, lay, path=km.PATH_TASKS)
+    sim.set_primitive(cell, faces)
+    ref = oracle.CpuSimulation(cfg.c, desc, lay)
+    ref.set_primitive(cell, faces)
+    sim.advance(8)
+    ref.advance(8)
+    g, gf = sim.state()
+    r, rf = ref.state()
+    _same(g, r, "mhd species cells")
+    for d in range(3):
+        _same(gf[d], rf[d], f"face {d}")
+    sim.close()
+
+
+def test_cylindrical_species_run_bitwise(gpu, oracle):
+    km = gpu
+    from tests.test_oracle_cyl import _rz, _swirl
+    cfg, desc, lay = _rz(bc_r="outflow")
+    cfg.set("material/species", "a,b")
+    base, _ = _swirl(lay, desc, oracle)
+    cell = np.zeros(lay.cell_shape(km.nvar(0, 2)))
+    cell[:, :12] = base
+    blocks = oracle.block_table(desc, lay.nblocks)
+    # bumps centred in [0.2, 0.8]^2 of the (r, z) plane, r in [0, 2], z in [-1, 1]
+    _cases.species_blobs(cell, lay, blocks, 0, 2)
+    print("cylindrical species", flush=True)
+    sim = km.Simulation(cfg, desc, lay, path=km.PATH_TASKS)
+    sim.set_primitive(cell)
+    ref = oracle.CpuSimulation(cfg.c, desc, lay)
+    ref.set_primitive(cell)
+    sim.advance(8)
+    ref.advance(8)
+    g, _ = sim.state()
+    r, _ = ref.state()
+    _same(g, r, "cylindrical species")
+    sim.close()
+
+
+def test_species_on_block_tree_bitwise(gpu, oracle):
+    km = gpu
+    cfg, desc, lay, leaves, cell = _cases.amr_case(
+        problem="smooth3d", ndim=2, nroot=2, nblock=8, ng=2, lmax=2, center=(0.4,) * 3,
+        radius=0.15, recon="plm", integrator="rk2", species="a,b,c")
+    _cases.species_blobs(cell, lay, oracle.amr_blocks(desc, leaves), 0, 3)
+    print(f"species on a block tree, {len(leaves)} leaves", flush=True)
+    sim = km.Simulation(cfg, desc, lay, path=km.PATH_TASKS, leaves=leaves)
+    sim.set_primitive(cell)
+    ref = oracle.CpuSimulation(cfg.c, desc, lay, leaves=leaves)
+    ref.set_primitive(cell)
+    sim.advance(6)
+    ref.advance(6)
+    g, _ = sim.state()
+    r, _ = ref.state()
+    _same(g, r, "species on a tree")
+    sim.close()
+
+
+def test_adaptive_sedov_species_bitwise(gpu, oracle):
+    # sedov.py:38 shape with two species; the tree follows the blast and the fractions
+    # move with every remesh
+    km = gpu
+    cfg, desc, lay, leaves, cell = _cases.amr_case(
+        "sedov", ndim=2, nroot=4, nblock=8, ng=4, recon="wenoz", limiter="minmod",
+        params=[1.0, 0.05, 1.0, 1e-5], center=(0.0,) * 3, radius=0.1, bc="outflow",
+        riemann="hllc", integrator="rk2", xmin=(-0.5,) * 3, xmax=(0.5,) * 3, gamma=1.4,
+        species="single,air")
+    adapt = amr_adapt(3, [amr_criterion(slot("PRES"), 3)], derefine_count=10)
+    print("adaptive sedov species", flush=True)
+    sim = km.Simulation(cfg, desc, lay, path=km.PATH_TASKS, leaves=leaves)
+    sim.set_primitive(cell)
+    sim.set_adaptive(adapt)
+    ref = oracle.CpuSimulation(cfg.c, desc, lay, leaves=leaves)
+    ref.set_primitive(cell)
+    ref.set_adaptive(adapt)
+    seen = {len(leaves)}
+    for c in range(16):
+        sim.advance(1)
+        ref.advance(1)
+        gl = sim.leaves()
+        _same(gl, ref.leaves(), f"leaves after cycle {c + 1}")
+        seen.add(len(gl))
+    g, _ = sim.state()
+    r, _ = ref.state()
+    _same(g, r, "adaptive sedov species")
+    assert len(seen) > 1
+    sim.close()
+
+
+def test_species_refusals(gpu):
+    km = gpu
+    cfg, desc, lay, _, _ = _cases.sedov_species(ndim=2, n=32, nblock=16)
+    with pytest.raises(km.KmError, match="fused stage path is built for one species"):
+        km.Simulation(cfg, desc, lay, path=km.PATH_FUSED)
