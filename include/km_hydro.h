@@ -385,6 +385,24 @@ int km_sim_set_adaptive(km_sim_t *sim, const km_amr_adapt_t *adapt);
  * entries of 4), count out; host buffers of get/set_state follow this count. */
 int km_sim_amr_leaves(const km_sim_t *sim, int32_t *leaves, int32_t *nleaves);
 
+/* ---- history outputs (Parthenon's "hst" output type) ------------------------------- */
+/* The isentropic vortex's history variables (isentropic_vortex.cpp:74-101): errors[v] =
+ * ErrorHistory (isentropic_vortex.hpp:150-200) of DENS, VEL0, VEL1, VEL2, PRES and, with
+ * MHD, MAGC0..2: sum over interior cells of |exact - q| * V / ((x1max - x1min) *
+ * (x2max - x2min)), exact = the vortex state at the cell centroid (the radial centroid in
+ * cylindrical geometry) with the parameters of km_problem_fill("isentropic_vortex"). u holds
+ * primitives on the device; mesh gives the domain. 5 or 8 values; blocks the host. */
+int km_history_vortex_errors(const km_cfg_t *cfg, const km_pack_t *u, const km_mesh_desc_t *mesh,
+                             const double *params, int32_t nparams, double *errors,
+                             void *stream);
+/* A history file for a simulation (parthenon/outputN/file_type = hst, dt): '#' header
+ * lines, then one row per output of time, dt (of the next step), cycle, block count and
+ * the problem's history variables ("isentropic_vortex": the errors above; NULL or "": none).
+ * A row is written now, after every cycle that reaches the next output time (which then
+ * advances by dt), and when km_sim_advance reaches tlim. Single runs only (no ensembles). */
+int km_sim_set_history(km_sim_t *sim, const char *path, double dt, const char *problem,
+                       const double *params, int32_t nparams);
+
 /* ---- device memory helpers (so callers need no other runtime) --------------------- */
 int km_device_count(int32_t *count);
 int km_malloc(void **ptr, size_t bytes);

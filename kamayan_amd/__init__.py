@@ -26,6 +26,7 @@ __all__ = [
     "KmCfg", "KmPack", "KmBlock", "KmMeshDesc", "ncons", "nvar", "mfrac", "slot", "PATH_TASKS",
     "PATH_FUSED", "LIB_PATH", "device_count", "block_table", "AmrAdapt", "amr_adapt",
     "amr_criterion", "amr_update_tree", "amr_tag_loehner", "amr_remesh", "initial_refinement",
+    "history_vortex_errors",
 ]
 
 _lib = None
@@ -100,6 +101,9 @@ def lib() -> C.CDLL:
         "km_amr_remesh": [P(KmPack), P(KmPack), vp, i32, i32, vp],
         "km_sim_set_adaptive": [vp, P(KmAmrAdapt)],
         "km_sim_amr_leaves": [vp, P(i32), P(i32)],
+        "km_history_vortex_errors": [P(KmCfg), P(KmPack), P(KmMeshDesc), P(dbl), i32, P(dbl),
+                                     vp],
+        "km_sim_set_history": [vp, C.c_char_p, dbl, C.c_char_p, P(dbl), i32],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -223,6 +227,24 @@ def amr_remesh(old: "DevicePack", new: "DevicePack", origin_dev: int, v0: int, v
                               v1, None), "km_amr_remesh")
 
 
+def _params(params):
+    if params is None:
+        return None, 0
+    arr = np.ascontiguousarray(params, dtype=np.float64)
+    return arr, len(arr)
+
+
+def history_vortex_errors(cfg: Config, pack: "DevicePack", desc: KmMeshDesc, params=None):
+    """The isentropic vortex's history variables (ErrorHistory of DENS, VEL0..2, PRES and,
+    with MHD, MAGC0..2) of the primitives in a device pack."""
+    arr, n = _params(params)
+    out = (C.c_double * 8)()
+    pp = arr.ctypes.data_as(C.POINTER(C.c_double)) if arr is not None else None
+    check(lib().km_history_vortex_errors(C.byref(cfg.c), C.byref(pack.pack), C.byref(desc), pp,
+                                         n, out, None), "km_history_vortex_errors")
+    return np.array(out[: 8 if cfg.c.mhd == 1 else 5])
+
+
 NO_MERGE = 1 << 30  # a derefine count no run reaches: the initial loop only refines
 
 
@@ -342,6 +364,15 @@ class Simulation:
         """Adaptive refinement (AmrAdapt or KmAmrAdapt) of a block-tree simulation."""
         c = adapt.c if isinstance(adapt, AmrAdapt) else adapt
         check(lib().km_sim_set_adaptive(self.h, C.byref(c)), "set_adaptive")
+
+    def set_history(self, path, dt: float, problem: str = "isentropic_vortex", params=None):
+        """Write a history file (Parthenon's hst output) with a row now and at every output
+        time dt; `problem` names the history variables ("isentropic_vortex" or "")."""
+        arr, n = _params(params)
+        pp = arr.ctypes.data_as(C.POINTER(C.c_double)) if arr is not None else None
+        check(lib().km_sim_set_history(self.h, str(path).encode(), dt,
+                                       problem.encode() if problem else None, pp, n),
+              "set_history")
 
     def leaves(self) -> np.ndarray:
         """Current leaves; also resizes `layout` to the current leaf count."""

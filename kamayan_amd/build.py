@@ -46,6 +46,7 @@ UNITS = [
     ("km_problems.cpp", "off"),
     ("km_amr.hip", "off"),
     ("km_amr_tree.cpp", "off"),
+    ("km_history.hip", "off"),
 ]
 
 

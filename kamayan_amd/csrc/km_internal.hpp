@@ -84,6 +84,13 @@ void launch_estimate_dt(const km_cfg_t &cfg, const km_pack_t &u, double *dt_dev,
 void launch_fill_divb(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st);
 // hydro::PrepareConserved per stage (cylindrical CT: md0's B_phi to B_phi / r)
 void launch_prepare_conserved_stage(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st);
+// isentropic_vortex parameters of the history reduction (km_history.hip)
+struct HistVortex {
+  double density, pressure, velx, vely, strength, mhd_strength, gamma;
+};
+// per-block sums of |exact - q| V of DENS, VEL0..2, PRES (+ MAGC0..2): part[b * nv + v]
+void launch_history_vortex(const km_cfg_t &cfg, const km_pack_t &u, const HistVortex &p,
+                           double *part, hipStream_t st);
 void launch_ghost_fill(const UniformGrid &gr, const km_pack_t &u, int v0, int v1, int nc,
                        bool faces, hipStream_t st, Instances in = Instances());
 

@@ -245,6 +245,12 @@ struct km_sim_t {
   std::vector<int32_t> leaves, deref;
   int *tags_d = nullptr;
   double *err2_d = nullptr;
+  // history output file
+  std::FILE *hist = nullptr;
+  double hist_dt = 0., hist_next = 0.;
+  bool hist_vortex = false;
+  HistVortex hist_p{};
+  int hist_last_cycle = -1;
 
   template <class T>
   int alloc(T **out, size_t count) {
@@ -270,6 +276,7 @@ struct km_sim_t {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (stream) (void)hipStreamDestroy(stream);
+    if (hist) std::fclose(hist);
   }
 };
 
@@ -523,6 +530,65 @@ int adapt_step(km_sim_t &s) {
   if (s.nvar > KM_NVAR(s.cfg.mhd)) ghost_fill(s, s.base.pack, KM_NVAR(s.cfg.mhd), s.nvar, false);
   launch_estimate_dt(s.cfg, s.base.pack, s.est_d, s.stream, nullptr, 1);
   return check_launch();
+}
+
+// isentropic_vortex parameters as km_problem_fill reads them (isentropic_vortex.cpp:55-65)
+HistVortex vortex_params(const km_cfg_t &cfg, const double *params, int n) {
+  auto P = [&](int i, double d) { return (params && i < n) ? params[i] : d; };
+  HistVortex p;
+  p.density = P(0, 1.0);
+  p.pressure = P(1, 1.0);
+  p.velx = P(2, 1.0);
+  p.vely = P(3, 1.0);
+  p.strength = P(4, 5.0);
+  p.mhd_strength = P(5, 1.0);
+  p.gamma = cfg.gamma;
+  return p;
+}
+
+// ErrorHistory of every history variable: block sums on the device, added on the host in
+// block order, over the x1-x2 extent of the domain (isentropic_vortex.hpp:158-199)
+int vortex_errors(const km_cfg_t &cfg, const km_pack_t &u, const km_mesh_desc_t &mesh,
+                  const HistVortex &p, hipStream_t st, double *out) {
+  const int nv = cfg.mhd != KM_MHD_OFF ? 8 : 5;
+  const size_t n = (size_t)(u.nblocks > 0 ? u.nblocks : 1) * nv;
+  DevBuf part;
+  KM_HIP(hipMalloc(&part.p, n * sizeof(double)));
+  launch_history_vortex(cfg, u, p, static_cast<double *>(part.p), st);
+  if (int rc = check_launch()) return rc;
+  std::vector<double> h(n, 0.0);
+  KM_HIP(hipMemcpyAsync(h.data(), part.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
+  KM_HIP(hipStreamSynchronize(st));
+  const double area = (mesh.xmax[0] - mesh.xmin[0]) * (mesh.xmax[1] - mesh.xmin[1]);
+  for (int v = 0; v < nv; ++v) {
+    double sum = 0.0;
+    for (int b = 0; b < u.nblocks; ++b) sum += h[(size_t)b * nv + v];
+    out[v] = sum / area;
+  }
+  return KM_OK;
+}
+
+// one history row of the current base state: time, dt, cycle, block count, variables
+int history_row(km_sim_t &s) {
+  double t = 0., dt = 0.;
+  KM_HIP(hipMemcpyAsync(&t, s.time_d, sizeof(double), hipMemcpyDeviceToHost, s.stream));
+  KM_HIP(hipMemcpyAsync(&dt, s.dt_d, sizeof(double), hipMemcpyDeviceToHost, s.stream));
+  KM_HIP(hipStreamSynchronize(s.stream));
+  std::fprintf(s.hist, "%.14e %.14e %d %d", t, dt, s.ncycle, s.nb);
+  if (s.hist_vortex) {
+    if (s.path == KM_PATH_FUSED && !s.prims_from_ic) {  // it keeps conserved interiors only
+      launch_ghost_fill(s.grid, s.base.pack, 0, s.nc, s.nc, false, s.stream,
+                        Instances{s.inst_d, s.grids_d});
+      launch_prepare_primitive(s.cfg, s.base.pack, s.stream);
+    }
+    double e[8];
+    if (int rc = vortex_errors(s.cfg, s.base.pack, s.mesh, s.hist_p, s.stream, e)) return rc;
+    for (int v = 0; v < (s.cfg.mhd != KM_MHD_OFF ? 8 : 5); ++v) std::fprintf(s.hist, " %.14e", e[v]);
+  }
+  std::fputc('\n', s.hist);
+  std::fflush(s.hist);
+  s.hist_last_cycle = s.ncycle;
+  return KM_OK;
 }
 
 // one cycle; dt_host is the step dt when the task path needs it by value
@@ -1096,7 +1162,7 @@ int km_sim_advance(km_sim_t *s, int32_t ncycles, double tlim) {
   if (!s) return KM_EINVAL;
   for (int n = 0; n < ncycles; ++n) {
     const bool need_host =
-        s->path == KM_PATH_TASKS || s->prims_from_ic || s->dt_keep || tlim > 0.;
+        s->path == KM_PATH_TASKS || s->prims_from_ic || s->dt_keep || tlim > 0. || s->hist;
     if (need_host) {
       double dt = 0., time = 0.;
       KM_HIP(hipMemcpyAsync(&dt, s->dt_d, sizeof(double), hipMemcpyDeviceToHost, s->stream));
@@ -1114,6 +1180,17 @@ int km_sim_advance(km_sim_t *s, int32_t ncycles, double tlim) {
         }
       }
       if (int rc = run_cycle(*s, dt)) return rc;
+      if (s->hist) {  // history rows: at the next output time, and at tlim
+        double t = 0.;
+        KM_HIP(hipMemcpyAsync(&t, s->time_d, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+        KM_HIP(hipStreamSynchronize(s->stream));
+        if (t >= s->hist_next) {
+          if (int rc = history_row(*s)) return rc;
+          s->hist_next += s->hist_dt;
+        } else if (tlim > 0. && t >= tlim) {
+          if (int rc = history_row(*s)) return rc;
+        }
+      }
       continue;
     }
     // fused path, no host decisions: capture one cycle in a graph and replay it
@@ -1241,6 +1318,52 @@ int km_sim_amr_leaves(const km_sim_t *s, int32_t *leaves, int32_t *nleaves) {
     return KM_EINVAL;
   }
   std::memcpy(leaves, s->leaves.data(), s->leaves.size() * sizeof(int32_t));
+  return KM_OK;
+}
+
+int km_history_vortex_errors(const km_cfg_t *cfg, const km_pack_t *u, const km_mesh_desc_t *mesh,
+                             const double *params, int32_t nparams, double *errors,
+                             void *stream) {
+  if (int rc = check_cfg(cfg, u)) return rc;
+  if (!mesh || !errors) return KM_EINVAL;
+  return vortex_errors(*cfg, *u, *mesh, vortex_params(*cfg, params, nparams),
+                       as_stream(stream), errors);
+}
+
+int km_sim_set_history(km_sim_t *s, const char *path, double dt, const char *problem,
+                       const double *params, int32_t nparams) {
+  if (!s || !path || !(dt > 0.)) return KM_EINVAL;
+  if (s->ninst > 1) {
+    set_error("history output is for single runs, not ensembles");
+    return KM_EINVAL;
+  }
+  const std::string prob = problem ? problem : "";
+  if (!prob.empty() && prob != "isentropic_vortex") {
+    set_error("no history variables for problem '" + prob + "'");
+    return KM_EINVAL;
+  }
+  if (s->hist) std::fclose(s->hist);
+  s->hist = std::fopen(path, "w");
+  if (!s->hist) {
+    set_error(std::string("cannot open ") + path);
+    return KM_EINVAL;
+  }
+  s->hist_vortex = prob == "isentropic_vortex";
+  s->hist_p = vortex_params(s->cfg, params, nparams);
+  s->hist_dt = dt;
+  static const char *const names[8] = {"density error", "velx error", "vely error",
+                                        "velz error", "pressure error", "magx error",
+                                        "magy error", "magz error"};
+  std::fprintf(s->hist, "# History data\n# [1]=time [2]=dt [3]=cycle [4]=nbtotal");
+  if (s->hist_vortex)
+    for (int v = 0; v < (s->cfg.mhd != KM_MHD_OFF ? 8 : 5); ++v)
+      std::fprintf(s->hist, " [%d]=%s", v + 5, names[v]);
+  std::fputc('\n', s->hist);
+  KM_HIP(hipStreamSynchronize(s->stream));
+  double t = 0.;
+  KM_HIP(hipMemcpy(&t, s->time_d, sizeof(double), hipMemcpyDeviceToHost));
+  if (int rc = history_row(*s)) return rc;
+  s->hist_next = t + dt;
   return KM_OK;
 }
 

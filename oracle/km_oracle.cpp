@@ -2577,6 +2577,68 @@ int km_cpu_fill_divb(const km_cfg_t *cfg, const km_pack_t *u, void *) {
   FillDivB(*cfg, *u);
   return KM_OK;
 }
+// isentropic_vortex history variables: ErrorHistory (isentropic_vortex.hpp:150-200) of
+// DENS, VELOCITY(0..2), PRES (+ MAGC(0..2)) against VortexData::State / StateMHD
+// (:31-135) at the cell centroids, divided by the x1-x2 extent; params as the problem's
+// (density, pressure, velx, vely, strength, mhd_strength)
+int km_cpu_history_vortex_errors(const km_cfg_t *cfg, const km_pack_t *u,
+                                 const km_mesh_desc_t *mesh, const double *params,
+                                 int32_t nparams, double *errors) {
+  auto P = [&](int i, double d) { return (params && i < nparams) ? params[i] : d; };
+  const double density = P(0, 1.0), pressure = P(1, 1.0), velx = P(2, 1.0), vely = P(3, 1.0),
+               strength = P(4, 5.0), mstr = P(5, 1.0), gamma = cfg->gamma;
+  const double pi = 3.141592653589793;
+  const int mhd = cfg->mhd;
+  const int nv = mhd == KM_MHD_CT ? 8 : 5;
+  const int slot[8] = {KM_DENS,       KM_VEL0(mhd) + 0, KM_VEL0(mhd) + 1, KM_VEL0(mhd) + 2,
+                       KM_PRES(mhd),  KM_MAGC0,         KM_MAGC0 + 1,     KM_MAGC0 + 2};
+  const Shape sh(*u);
+  double err[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int b = 0; b < u->nblocks; ++b) {
+    const km_block_t &blk = u->blocks[b];
+    for (int k = sh.ks; k <= sh.ke; ++k)
+      for (int j = sh.js; j <= sh.je; ++j)
+        for (int i = sh.is; i <= sh.ie; ++i) {
+          const double x = Cyl(*cfg) ? RCentroid(blk, sh, i)
+                                     : (blk.xmin[0] - sh.is * blk.dx[0]) + (i + 0.5) * blk.dx[0];
+          const double y = (blk.xmin[1] - sh.js * blk.dx[1]) + (j + 0.5) * blk.dx[1];
+          const double r2 = Cyl(*cfg) ? x * x : x * x + y * y;
+          const double dv = strength * std::exp(0.5 * (1.0 - r2)) / (2.0 * pi);
+          double st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+          if (mhd == KM_MHD_OFF) {
+            const double T = pressure / density - (gamma - 1.0) * strength * strength *
+                                                      std::exp(1.0 - r2) /
+                                                      (8.0 * gamma * pi * pi);
+            st[0] = std::pow(T, 1.0 / (gamma - 1.0));
+            st[4] = T * st[0];
+          } else {
+            const double dB = mstr * std::exp(0.5 * (1.0 - r2)) / (2.0 * pi);
+            const double ex = std::exp(1.0 - r2);
+            st[0] = 1.0;
+            st[4] = pressure + (mstr * mstr / (8.0 * pi * pi)) * (1.0 - r2) * ex -
+                    (strength * strength / (8.0 * pi * pi)) * ex;
+            if (Cyl(*cfg)) {
+              st[7] = x * dB;
+            } else {
+              st[5] = -y * dB;
+              st[6] = x * dB;
+            }
+          }
+          if (Cyl(*cfg)) {
+            st[3] = x * dv;
+          } else {
+            st[1] = velx - y * dv;
+            st[2] = vely + x * dv;
+          }
+          const long c = sh.c(k, j, i);
+          const double vol = Volume(*cfg, blk, sh, i);
+          for (int v = 0; v < nv; ++v) err[v] += std::abs(st[v] - cellp(*u, b, slot[v])[c]) * vol;
+        }
+  }
+  const double area = (mesh->xmax[0] - mesh->xmin[0]) * (mesh->xmax[1] - mesh->xmin[1]);
+  for (int v = 0; v < nv; ++v) errors[v] = err[v] / area;
+  return KM_OK;
+}
 int km_cpu_ghost_fill_uniform(const km_cfg_t *cfg, const km_pack_t *u, const int32_t nb[3],
                               const int32_t bc[6], int32_t v0, int32_t v1,
                               int32_t with_faces) {

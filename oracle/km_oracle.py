@@ -53,6 +53,8 @@ def lib() -> C.CDLL:
         "km_cpu_prepare_conserved": [P(KmCfg), P(KmPack), vp],
         "km_cpu_estimate_dt": [P(KmCfg), P(KmPack), P(dbl), vp],
         "km_cpu_fill_divb": [P(KmCfg), P(KmPack), vp],
+        "km_cpu_history_vortex_errors": [P(KmCfg), P(KmPack), P(KmMeshDesc), P(dbl), C.c_int32,
+                                         P(dbl)],
         "km_cpu_ghost_fill_uniform": [P(KmCfg), P(KmPack), P(i32), P(i32), i32, i32, i32],
         "km_cpu_mesh_blocks": [P(KmMeshDesc), vp],
         "km_cpu_sim_create": [P(KmCfg), P(KmMeshDesc), P(vp)],
@@ -327,3 +329,13 @@ class CpuSimulation:
 
 def set_threads(n: int) -> None:
     os.environ["OMP_NUM_THREADS"] = str(n)
+
+
+def history_vortex_errors(cfg: KmCfg, pack: KmPack, desc: KmMeshDesc, params=None) -> np.ndarray:
+    """ErrorHistory of the isentropic vortex's history variables for a host pack."""
+    out = (C.c_double * 8)()
+    arr = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
+    pp = arr.ctypes.data_as(C.POINTER(C.c_double)) if arr is not None else None
+    call("km_cpu_history_vortex_errors", C.byref(cfg), C.byref(pack), C.byref(desc), pp,
+         0 if arr is None else len(arr), out)
+    return np.array(out[: 8 if cfg.mhd == 1 else 5])
