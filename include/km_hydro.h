@@ -370,16 +370,23 @@ int km_amr_update_tree(const km_mesh_desc_t *mesh, int32_t numlevel, int32_t der
  * for cell slots [v0, v1): a kept leaf copies every zone, a split leaf's children take
  * ProlongateSharedMinMod (refinement_operations.hpp:123-268) of the parent's zones (its
  * ghost zones give the slopes at its edges) over their interior, a merged parent the
- * RestrictAverage (:52-121) of its children over its interior. origin_dev: the origin
- * table of km_amr_update_tree in device memory. Both packs share ndim, ng and nx. */
+ * RestrictAverage (:52-121) of its children over its interior. When both packs carry all
+ * three face tables (CT), the face field moves too: kept faces are copied, a child's faces
+ * on parent faces take ProlongateSharedMinMod with slopes across the face, those inside
+ * parent cells ProlongateInternalTothAndRoe (:270-398), and a merged parent's faces the
+ * area-weighted RestrictAverage of the child faces on them (the upper child's copy of a
+ * face two children share). Ghost zones and ghost faces of new leaves are left for the
+ * ghost fill. origin_dev: the origin table of km_amr_update_tree in device memory. Both
+ * packs share ndim, ng and nx. */
 int km_amr_remesh(const km_pack_t *old_u, const km_pack_t *new_u, const int32_t *origin_dev,
                   int32_t v0, int32_t v1, void *stream);
-/* Adaptive refinement of a block-tree simulation (hydro): after every cycle each leaf is
- * tagged by the criteria (km_amr_tag_loehner on the filled base state), the tree is
- * updated (km_amr_update_tree) and, when it changed, every cell field moves to the new
- * leaves (km_amr_remesh), ghost zones are refilled and dt is re-estimated on the new
- * mesh before the time-step rule (EvolutionDriver::Execute and
- * InitializeBlockTimeStepsAfterAMR in Parthenon). Resets every derefine count. */
+/* Adaptive refinement of a block-tree simulation (hydro or CT): after every cycle each leaf
+ * is tagged by the criteria (km_amr_tag_loehner on the filled base state), the tree is
+ * updated (km_amr_update_tree) and, when it changed, every cell field and the CT face
+ * field move to the new leaves (km_amr_remesh), ghost zones (and faces) are refilled, divB
+ * is recomputed (CT) and dt is re-estimated on the new mesh before the time-step rule
+ * (EvolutionDriver::Execute and InitializeBlockTimeStepsAfterAMR in Parthenon). Resets
+ * every derefine count. */
 int km_sim_set_adaptive(km_sim_t *sim, const km_amr_adapt_t *adapt);
 /* Current leaves of a block-tree simulation in block order: *nleaves capacity in (leaf
  * entries of 4), count out; host buffers of get/set_state follow this count. */

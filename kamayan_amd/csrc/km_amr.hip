@@ -115,6 +115,21 @@ __device__ __forceinline__ double grad_minmod(double fc, double fm, double fp, d
   return 0.5 * (sgn(gxm) + sgn(gxp)) * kmin(kabs(gxm), kabs(gxp));
 }
 
+// the child formulas of ProlongateSharedGeneral (refinement_operations.hpp:241-259): the
+// fine element at offsets (oi, oj, ok) in its coarse element, from the coarse value and
+// the slope terms a = gradient x half the fine spacing along each included axis
+__device__ __forceinline__ double child_value(double fc, double a, double bb, double c3, int oi,
+                                              int oj, int ok) {
+  if (!ok && !oj && !oi) return fc - (a + bb + c3);
+  if (!ok && !oj) return fc + (a - bb - c3);
+  if (!ok && !oi) return fc - (a - bb + c3);
+  if (!ok) return fc + (a + bb - c3);
+  if (!oj && !oi) return fc - (a + bb - c3);
+  if (!oj) return fc + (a - bb + c3);
+  if (!oi) return fc - (a - bb - c3);
+  return fc + (a + bb + c3);
+}
+
 }  // namespace
 
 // Zones a level-L or a level-L+1 leaf covers: copy or RestrictAverage. Zones a coarser
@@ -547,35 +562,19 @@ __global__ void __launch_bounds__(256) k_amr_faces_shared(km_pack_t u, AmrGrid t
   const int oi = d != 0 ? (G[0] & 1) : 0;
   const int oj = (d != 1 && g.ndim > 1) ? (G[1] & 1) : 0;
   const int ok = (d != 2 && g.ndim > 2) ? (G[2] & 1) : 0;
-  double x;
-  if (!ok && !oj && !oi) x = fc - (a[0] + a[1] + a[2]);
-  else if (!ok && !oj) x = fc + (a[0] - a[1] - a[2]);
-  else if (!ok && !oi) x = fc - (a[0] - a[1] + a[2]);
-  else if (!ok) x = fc + (a[0] + a[1] - a[2]);
-  else if (!oj && !oi) x = fc - (a[0] + a[1] - a[2]);
-  else if (!oj) x = fc + (a[0] - a[1] + a[2]);
-  else if (!oi) x = fc - (a[0] - a[1] - a[2]);
-  else x = fc + (a[0] + a[1] + a[2]);
+  double x = child_value(fc, a[0], a[1], a[2], oi, oj, ok);
   if (d < g.ndim && fl[d]) x = -x;
   u.face[d][b][dst] = x;
 }
 
-// phase 2: ProlongateInternalTothAndRoe (refinement_operations.hpp:278-398), Cartesian, for
-// the fine faces of component d inside a coarse cell under a coarser leaf. The fine
-// element (kk, jj, ii) of the reference's loop is reached through its permuted offsets;
-// the coarse cell spacings enter unpermuted, as the reference writes them.
-__global__ void __launch_bounds__(256) k_amr_faces_internal(km_pack_t u, AmrGrid t, int d) {
-  const Geo g(u);
-  int b, idx[3];
-  if (!decode_stag(g, u.nblocks, b, idx)) return;
-  if (!ghost_face(g, d, idx)) return;
-  if (d < g.ndim && !((idx[d] - g.lo[d]) & 1)) return;  // shared position
-  const km_block_t &blk = u.blocks[b];
-  int G[3];
-  bool fl[3];
-  face_global(t, g, blk, d, idx, G, fl);
-  const Face f = locate_face(u, t, g, blk.level, d, G, face_up(g, d, idx));
-  if (f.kind != 2) return;
+namespace {
+// ProlongateInternalTothAndRoe (refinement_operations.hpp:278-398), Cartesian: the value of
+// the fine face of component d at idx, inside a coarse cell of block b whose shared fine
+// faces (all three components) are already in place. The fine element (kk, jj, ii) of the
+// reference's loop is reached through its permuted offsets; the coarse cell spacings
+// enter unpermuted, as the reference writes them.
+__device__ double toth_roe(const km_pack_t &u, const Geo &g, int b, int d, const int idx[3],
+                           const km_block_t &blk) {
   const int g2 = g.ndim > 1, g3 = g.ndim > 2;
   // fine lower corner of the coarse cell
   int fc[3];
@@ -628,7 +627,25 @@ __global__ void __launch_bounds__(256) k_amr_faces_internal(km_pack_t u, AmrGrid
   double x = 0.5 * (fine(0, ok, oj, 0) * 1.0 + fine(0, ok, oj, 2) * 1.0) + Uxx + sg(ok) * Vxyz +
              sg(oj) * Wxyz;
   x *= 1.0;
-  u.face[d][b][g.s(idx[2], idx[1], idx[0])] = x;
+  return x;
+}
+}  // namespace
+
+// phase 2: ProlongateInternalTothAndRoe for the ghost fine faces of component d inside a
+// coarse cell under a coarser leaf
+__global__ void __launch_bounds__(256) k_amr_faces_internal(km_pack_t u, AmrGrid t, int d) {
+  const Geo g(u);
+  int b, idx[3];
+  if (!decode_stag(g, u.nblocks, b, idx)) return;
+  if (!ghost_face(g, d, idx)) return;
+  if (d < g.ndim && !((idx[d] - g.lo[d]) & 1)) return;  // shared position
+  const km_block_t &blk = u.blocks[b];
+  int G[3];
+  bool fl[3];
+  face_global(t, g, blk, d, idx, G, fl);
+  const Face f = locate_face(u, t, g, blk.level, d, G, face_up(g, d, idx));
+  if (f.kind != 2) return;
+  u.face[d][b][g.s(idx[2], idx[1], idx[0])] = toth_roe(u, g, b, d, idx, blk);
 }
 
 // EMF correction at level jumps: every edge on a block's boundary that a finer leaf also
@@ -824,19 +841,8 @@ __global__ void __launch_bounds__(256) k_amr_remesh(km_pack_t o, km_pack_t nw,
         const double dxc = 2.0 * blk.dx[d];
         gr[d] = grad_minmod(fc, q[cc - g.sc[d]], q[cc + g.sc[d]], dxc, dxc);
       }
-      // the child formulas of ProlongateSharedGeneral (refinement_operations.hpp:241-259)
-      const double a = gr[0] * df[0], bb = gr[1] * df[1], c3 = gr[2] * df[2];
-      const int oi = bit[0], oj = bit[1], ok = bit[2];
-      double x;
-      if (!ok && !oj && !oi) x = fc - (a + bb + c3);
-      else if (!ok && !oj) x = fc + (a - bb - c3);
-      else if (!ok && !oi) x = fc - (a - bb + c3);
-      else if (!ok) x = fc + (a + bb - c3);
-      else if (!oj && !oi) x = fc - (a + bb - c3);
-      else if (!oj) x = fc + (a - bb + c3);
-      else if (!oi) x = fc - (a - bb - c3);
-      else x = fc + (a + bb + c3);
-      cell_ptr(nw, b, v)[z] = x;
+      cell_ptr(nw, b, v)[z] =
+          child_value(fc, gr[0] * df[0], gr[1] * df[1], gr[2] * df[2], bit[0], bit[1], bit[2]);
     }
     return;
   }
@@ -853,6 +859,106 @@ __global__ void __launch_bounds__(256) k_amr_remesh(km_pack_t o, km_pack_t nw,
   const double vol = o.blocks[cb].vol;
   const long fz = g.c(f[2], f[1], f[0]);
   for (int v = v0; v < v1; ++v) cell_ptr(nw, b, v)[z] = restrict_cell(cell_ptr(o, cb, v), vol, g, fz);
+}
+
+// The same data movement for the CT face field, component d, one lane per face element
+// of a new block. A kept leaf copies the element (ghost faces included). A split leaf's
+// child takes, at every own face that lies on a parent face, ProlongateSharedMinMod of
+// that parent face with minmod slopes across it from the parent's neighbouring faces
+// (interior or ghost); its faces inside parent cells are left to
+// k_amr_remesh_faces_internal. A merged parent takes the area-weighted RestrictAverage of
+// the 2^(d-1) child faces on each of its own faces. Ghost faces of new leaves are left to
+// the ghost fill.
+__global__ void __launch_bounds__(256) k_amr_remesh_faces_shared(km_pack_t o, km_pack_t nw,
+                                                                 const int32_t *origin, int d) {
+  const Geo g(nw);
+  int b, idx[3];
+  if (!decode_stag(g, nw.nblocks, b, idx)) return;
+  const int32_t *org = origin + 9 * (long)b;
+  const long dst = g.s(idx[2], idx[1], idx[0]);
+  if (org[0] == 0) {
+    nw.face[d][b][dst] = o.face[d][org[1]][dst];
+    return;
+  }
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    const int hi = e < g.ndim ? g.hi[e] + (e == d ? 1 : 0) : 0;
+    if (idx[e] < g.lo[e] || idx[e] > hi) return;  // ghost face or padding
+  }
+  const km_block_t &blk = nw.blocks[b];
+  if (org[0] == 1) {
+    const int pb = org[1];
+    const km_block_t &par = o.blocks[pb];
+    int G[3] = {0, 0, 0}, pc[3] = {0, 0, 0};
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+      if (e >= g.ndim) continue;
+      G[e] = blk.lx[e] * g.nx[e] + (idx[e] - g.lo[e]);
+      pc[e] = (G[e] >> 1) - par.lx[e] * g.nx[e] + g.lo[e];
+    }
+    if (d < g.ndim && (G[d] & 1)) return;  // inside a parent cell: phase 2
+    const double *q = o.face[d][pb];
+    const long pcs = g.s(pc[2], pc[1], pc[0]);
+    const double fc = q[pcs];
+    double a[3] = {0., 0., 0.};
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+      if (e == d || e >= g.ndim) continue;
+      const double dxc = 2.0 * blk.dx[e];
+      a[e] = grad_minmod(fc, q[pcs - g.ss[e]], q[pcs + g.ss[e]], dxc, dxc) * (0.5 * blk.dx[e]);
+    }
+    const int oi = d != 0 ? (G[0] & 1) : 0;
+    const int oj = (d != 1 && g.ndim > 1) ? (G[1] & 1) : 0;
+    const int ok = (d != 2 && g.ndim > 2) ? (G[2] & 1) : 0;
+    nw.face[d][b][dst] = child_value(fc, a[0], a[1], a[2], oi, oj, ok);
+    return;
+  }
+  int ch = 0, f[3] = {0, 0, 0};
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    if (e >= g.ndim) continue;
+    const int lc = idx[e] - g.lo[e];
+    int half = (2 * lc) / g.nx[e];
+    half = half > 1 ? 1 : half;  // the upper face of the block lies in the upper child
+    ch |= half << e;
+    f[e] = 2 * lc - half * g.nx[e] + g.lo[e];
+  }
+  Face src;
+  src.kind = 1;
+  src.leaf = org[1 + ch];
+  src.idx = g.s(f[2], f[1], f[0]);
+  nw.face[d][b][dst] = face_value(o, g, d, src);
+}
+
+// phase 2 of the face remesh: ProlongateInternalTothAndRoe for the faces of a split leaf's
+// child that lie inside parent cells, from the child's shared faces of phase 1
+__global__ void __launch_bounds__(256) k_amr_remesh_faces_internal(km_pack_t nw,
+                                                                   const int32_t *origin, int d) {
+  const Geo g(nw);
+  int b, idx[3];
+  if (!decode_stag(g, nw.nblocks, b, idx)) return;
+  if (origin[9 * (long)b] != 1 || d >= g.ndim) return;
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    const int hi = e < g.ndim ? g.hi[e] + (e == d ? 1 : 0) : 0;
+    if (idx[e] < g.lo[e] || idx[e] > hi) return;
+  }
+  if (!((idx[d] - g.lo[d]) & 1)) return;  // shared position (block offsets are even)
+  nw.face[d][b][g.s(idx[2], idx[1], idx[0])] = toth_roe(nw, g, b, d, idx, nw.blocks[b]);
+}
+
+void launch_amr_remesh_faces(const km_pack_t &old_u, const km_pack_t &new_u,
+                             const int32_t *origin, hipStream_t st) {
+  const Geo g(new_u);
+  const long n = g.nstag() * new_u.nblocks;
+  if (n == 0) return;
+  const unsigned grid = (unsigned)((n + 255) / 256);
+  for (int d = 0; d < 3; ++d)
+    hipLaunchKernelGGL(k_amr_remesh_faces_shared, dim3(grid), dim3(256), 0, st, old_u, new_u,
+                       origin, d);
+  for (int d = 0; d < g.ndim; ++d)
+    hipLaunchKernelGGL(k_amr_remesh_faces_internal, dim3(grid), dim3(256), 0, st, new_u, origin,
+                       d);
 }
 
 void launch_amr_remesh(const km_pack_t &old_u, const km_pack_t &new_u, const int32_t *origin,

@@ -221,6 +221,9 @@ void launch_amr_tag_loehner(const km_pack_t &u, const km_amr_criterion_t &crit, 
                             bool reset, double *err2_scratch, hipStream_t st);
 void launch_amr_remesh(const km_pack_t &old_u, const km_pack_t &new_u, const int32_t *origin,
                        int v0, int v1, hipStream_t st);
+// the CT face field's remesh (all three components; new ghost faces are left unset)
+void launch_amr_remesh_faces(const km_pack_t &old_u, const km_pack_t &new_u,
+                             const int32_t *origin, hipStream_t st);
 
 void launch_amr_ghost_fill(const AmrGrid &t, const km_pack_t &u, int v0, int v1, int nc,
                            hipStream_t st);

@@ -491,6 +491,7 @@ int amr_remesh(km_sim_t &s, const std::vector<int32_t> &nl, const std::vector<in
   KM_HIP(hipMemcpy(origin_d, origin.data(), origin.size() * sizeof(int32_t),
                    hipMemcpyHostToDevice));
   launch_amr_remesh(old_pack, s.base.pack, origin_d, 0, s.nvar, s.stream);
+  if (s.cfg.mhd == KM_MHD_CT) launch_amr_remesh_faces(old_pack, s.base.pack, origin_d, s.stream);
   if ((rc = check_launch())) return rc;
   KM_HIP(hipStreamSynchronize(s.stream));
   s.mesh_bufs.swap(fresh);  // the old mesh's buffers go with `fresh`
@@ -526,8 +527,10 @@ int adapt_step(km_sim_t &s) {
   if (int rc = amr_remesh(s, nl, org)) return rc;
   s.leaves = std::move(nl);
   s.deref = std::move(nd);
-  ghost_fill(s, s.base.pack, 0, KM_TEMP(s.cfg.mhd), false);
+  const bool ct = s.cfg.mhd == KM_MHD_CT;
+  ghost_fill(s, s.base.pack, 0, KM_TEMP(s.cfg.mhd), ct);
   if (s.nvar > KM_NVAR(s.cfg.mhd)) ghost_fill(s, s.base.pack, KM_NVAR(s.cfg.mhd), s.nvar, false);
+  if (ct) launch_fill_divb(s.cfg, s.base.pack, s.stream);  // divB of the new mesh's faces
   launch_estimate_dt(s.cfg, s.base.pack, s.est_d, s.stream, nullptr, 1);
   return check_launch();
 }
@@ -1270,10 +1273,6 @@ int km_sim_set_adaptive(km_sim_t *s, const km_amr_adapt_t *a) {
     set_error("adaptive refinement needs a block-tree simulation (km_sim_create_amr)");
     return KM_EINVAL;
   }
-  if (s->cfg.mhd == KM_MHD_CT) {
-    set_error("adaptive refinement is provided for hydro; CT meshes keep their leaves");
-    return KM_EINVAL;
-  }
   if (s->mesh.ng < 2) {
     set_error("the Loehner estimator reads two ghost layers: nghost must be at least 2");
     return KM_EINVAL;
@@ -1395,6 +1394,9 @@ int km_amr_remesh(const km_pack_t *old_u, const km_pack_t *new_u, const int32_t 
       return KM_EINVAL;
     }
   launch_amr_remesh(*old_u, *new_u, origin_dev, v0, v1, as_stream(stream));
+  bool faces = true;
+  for (int d = 0; d < 3; ++d) faces = faces && old_u->face[d] && new_u->face[d];
+  if (faces) launch_amr_remesh_faces(*old_u, *new_u, origin_dev, as_stream(stream));
   return check_launch();
 }
 

@@ -1514,6 +1514,64 @@ bool GhostFaceElem(const Tree &t, const Shape &sh, int d, const int idx[3]) {
   return ghost;
 }
 
+// ProlongateInternalTothAndRoe (refinement_operations.hpp:278-398), Cartesian: the fine
+// face (component d) at idx = (i, j, k) inside the coarse cell whose fine lower corner is
+// c, from the coarse cell's shared fine faces of all three components in block b. The
+// reference's (kk, jj, ii) loop order is reached by permuting the offsets per component;
+// the coarse spacings enter unpermuted, as the reference writes them.
+double TothRoe(const Shape &sh, const km_pack_t &u, int b, const km_block_t &blk, int d,
+               const int c[3], const int idx[3]) {
+  const int g2 = sh.ndim > 1, g3 = sh.ndim > 2;
+  const int fi = c[0], fj = c[1], fk = c[2];
+  auto elem = [&](int ok, int oj, int oi, int out[3]) {  // {k, j, i}
+    if (d == 0) {
+      out[0] = fk + ok * g3, out[1] = fj + oj * g2, out[2] = fi + oi;
+    } else if (d == 1) {
+      out[0] = fk + oj * g3, out[1] = fj + oi * g2, out[2] = fi + ok;
+    } else {
+      out[0] = fk + oi * g3, out[1] = fj + ok * g2, out[2] = fi + oj;
+    }
+  };
+  auto fine = [&](int eidx, int ok, int oj, int oi) {
+    int q[3];
+    elem(ok, oj, oi, q);
+    return facep(u, (d + eidx) % 3, b)[sh.s(q[0], q[1], q[2])];
+  };
+  int ok = 0, oj = 0;
+  const int o[3] = {idx[0] - fi, idx[1] - fj, idx[2] - fk};
+  if (d == 0) {
+    ok = o[2];
+    oj = o[1];
+  } else if (d == 1) {
+    ok = o[0];
+    oj = o[2];
+  } else {
+    ok = o[1];
+    oj = o[0];
+  }
+  auto sg = [](int off) -> double { return off == 0 ? -1.0 : 1.0; };
+  double Uxx = 0.0, Vxyz = 0.0, Wxyz = 0.0;
+  for (const int v : {0, 1})
+    for (const int uu : {0, 2})
+      for (const int tt : {0, 1}) {
+        const double fine2 = fine(1, v, uu, tt) * 1.0;
+        const double fine3 = fine(2, uu, v, tt) * 1.0;
+        Uxx += sg(tt) * sg(uu) * (fine2 + fine3);
+        Vxyz += sg(tt) * sg(uu) * sg(v) * fine2;
+        Wxyz += sg(tt) * sg(uu) * sg(v) * fine3;
+      }
+  Uxx *= 0.125;
+  const double cdx = 2.0 * blk.dx[0], cdy = 2.0 * blk.dx[1], cdz = 2.0 * blk.dx[2];
+  const double dx2 = cdx * cdx, dy2 = cdy * cdy, dz2 = cdz * cdz;
+  Vxyz *= 0.125 * dz2 / (dx2 + dz2);
+  Wxyz *= 0.125 * dy2 / (dx2 + dy2);
+  double x =
+      0.5 * (fine(0, ok, oj, 0) * 1.0 + fine(0, ok, oj, 2) * 1.0) + Uxx + sg(ok) * Vxyz +
+      sg(oj) * Wxyz;
+  x *= 1.0;
+  return x;
+}
+
 void AmrGhostFaces(const km_cfg_t &cfg, const Tree &t, const km_pack_t &u) {
   (void)cfg;
   const Shape sh(u);
@@ -1588,7 +1646,6 @@ void AmrGhostFaces(const km_cfg_t &cfg, const Tree &t, const km_pack_t &u) {
     }
   }
   // phase 2: Toth & Roe for fine faces inside coarse cells
-  const int g2 = t.ndim > 1, g3 = t.ndim > 2;
   for (int d = 0; d < 3; ++d) {
 #pragma omp parallel for schedule(dynamic, 4) reduction(| : bad)
     for (int b = 0; b < nleaf; ++b) {
@@ -1620,53 +1677,7 @@ void AmrGhostFaces(const km_cfg_t &cfg, const Tree &t, const km_pack_t &u) {
               const int off = idx[e] - lo[e];
               c[e] = lo[e] + off - (((off % 2) + 2) % 2);
             }
-            const int fi = c[0], fj = c[1], fk = c[2];
-            auto elem = [&](int ok, int oj, int oi, int out[3]) {  // {k, j, i}
-              if (d == 0) {
-                out[0] = fk + ok * g3, out[1] = fj + oj * g2, out[2] = fi + oi;
-              } else if (d == 1) {
-                out[0] = fk + oj * g3, out[1] = fj + oi * g2, out[2] = fi + ok;
-              } else {
-                out[0] = fk + oi * g3, out[1] = fj + ok * g2, out[2] = fi + oj;
-              }
-            };
-            auto fine = [&](int eidx, int ok, int oj, int oi) {
-              int q[3];
-              elem(ok, oj, oi, q);
-              return facep(u, (d + eidx) % 3, b)[sh.s(q[0], q[1], q[2])];
-            };
-            int ok = 0, oj = 0;
-            const int o[3] = {i - fi, j - fj, k - fk};
-            if (d == 0) {
-              ok = o[2];
-              oj = o[1];
-            } else if (d == 1) {
-              ok = o[0];
-              oj = o[2];
-            } else {
-              ok = o[1];
-              oj = o[0];
-            }
-            auto sg = [](int off) -> double { return off == 0 ? -1.0 : 1.0; };
-            double Uxx = 0.0, Vxyz = 0.0, Wxyz = 0.0;
-            for (const int v : {0, 1})
-              for (const int uu : {0, 2})
-                for (const int tt : {0, 1}) {
-                  const double fine2 = fine(1, v, uu, tt) * 1.0;
-                  const double fine3 = fine(2, uu, v, tt) * 1.0;
-                  Uxx += sg(tt) * sg(uu) * (fine2 + fine3);
-                  Vxyz += sg(tt) * sg(uu) * sg(v) * fine2;
-                  Wxyz += sg(tt) * sg(uu) * sg(v) * fine3;
-                }
-            Uxx *= 0.125;
-            const double cdx = 2.0 * blk.dx[0], cdy = 2.0 * blk.dx[1], cdz = 2.0 * blk.dx[2];
-            const double dx2 = cdx * cdx, dy2 = cdy * cdy, dz2 = cdz * cdz;
-            Vxyz *= 0.125 * dz2 / (dx2 + dz2);
-            Wxyz *= 0.125 * dy2 / (dx2 + dy2);
-            double x = 0.5 * (fine(0, ok, oj, 0) * 1.0 + fine(0, ok, oj, 2) * 1.0) + Uxx +
-                       sg(ok) * Vxyz + sg(oj) * Wxyz;
-            x *= 1.0;
-            facep(u, d, b)[sh.s(k, j, i)] = x;
+            facep(u, d, b)[sh.s(k, j, i)] = TothRoe(sh, u, b, blk, d, c, idx);
           }
     }
   }
@@ -2313,6 +2324,133 @@ void Remesh(const km_pack_t &o, const km_pack_t &nw, const int32_t *origin, int 
   }
 }
 
+// The CT face field's share of RedistributeAndRefineMeshBlocks, written as Parthenon's
+// refinement operators loop: a kept leaf copies its face arrays; a split leaf's child
+// runs ProlongateSharedMinMod (refinement_operations.hpp:123-268) over the parent's faces
+// in its quadrant -- each coarse face writes the fine faces on it, with minmod slopes
+// across the face from the parent's neighbouring faces -- then
+// ProlongateInternalTothAndRoe over each coarse cell of the quadrant; a merged parent
+// takes RestrictAverage (:52-121) of each child's faces over the child's quadrant, the
+// children in x1-fastest order, so the upper child writes a face two children share.
+void RemeshFaces(const km_pack_t &o, const km_pack_t &nw, const int32_t *origin) {
+  const Shape sh(nw);
+  const int nd = sh.ndim;
+  const int lo[3] = {sh.is, sh.js, sh.ks};
+  const int n[3] = {sh.nx[0], nd > 1 ? sh.nx[1] : 1, nd > 2 ? sh.nx[2] : 1};
+  const int hn[3] = {n[0] / 2, nd > 1 ? n[1] / 2 : 1, nd > 2 ? n[2] / 2 : 1};
+  const long ss[3] = {1, sh.S[0], (long)sh.S[0] * sh.S[1]};
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int b = 0; b < nw.nblocks; ++b) {
+    const int32_t *org = origin + 9 * b;
+    const km_block_t &blk = nw.blocks[b];
+    if (org[0] == 0) {
+      for (int d = 0; d < 3; ++d)
+        std::memcpy(facep(nw, d, b), facep(o, d, org[1]), sh.nstag() * sizeof(double));
+      continue;
+    }
+    if (org[0] == 1) {
+      const int pb = org[1];
+      int q0[3] = {0, 0, 0};  // the child's quadrant: first parent cell, parent-local
+      for (int e = 0; e < nd; ++e) q0[e] = lo[e] + (blk.lx[e] & 1) * hn[e];
+      for (int d = 0; d < 3; ++d) {
+        const double *q = facep(o, d, pb);
+        double *f = facep(nw, d, b);
+        const int cn[3] = {hn[0] + (d == 0), hn[1] + (d == 1 && nd > 1),
+                           hn[2] + (d == 2 && nd > 2)};
+        for (int ck = 0; ck < cn[2]; ++ck)
+          for (int cj = 0; cj < cn[1]; ++cj)
+            for (int ci = 0; ci < cn[0]; ++ci) {
+              const int cc[3] = {ci, cj, ck};
+              const long ps = sh.s(q0[2] + ck, q0[1] + cj, q0[0] + ci);
+              const double fc = q[ps];
+              double a[3] = {0., 0., 0.};
+              for (int e = 0; e < nd; ++e) {
+                if (e == d) continue;
+                const double dxc = 2.0 * blk.dx[e];
+                a[e] = GradMinMod(fc, q[ps - ss[e]], q[ps + ss[e]], dxc, dxc) * (0.5 * blk.dx[e]);
+              }
+              const int ni = (d != 0) ? 2 : 1, nj = (d != 1 && nd > 1) ? 2 : 1,
+                        nk = (d != 2 && nd > 2) ? 2 : 1;
+              for (int ok = 0; ok < nk; ++ok)
+                for (int oj = 0; oj < nj; ++oj)
+                  for (int oi = 0; oi < ni; ++oi) {
+                    double x;
+                    if (!oi && !oj && !ok) x = fc - (a[0] + a[1] + a[2]);
+                    else if (oi && !oj && !ok) x = fc + (a[0] - a[1] - a[2]);
+                    else if (!oi && oj && !ok) x = fc - (a[0] - a[1] + a[2]);
+                    else if (oi && oj && !ok) x = fc + (a[0] + a[1] - a[2]);
+                    else if (!oi && !oj && ok) x = fc - (a[0] + a[1] - a[2]);
+                    else if (oi && !oj && ok) x = fc + (a[0] - a[1] + a[2]);
+                    else if (!oi && oj && ok) x = fc - (a[0] - a[1] - a[2]);
+                    else x = fc + (a[0] + a[1] + a[2]);
+                    int fi[3] = {0, 0, 0};
+                    const int off[3] = {oi, oj, ok};
+                    for (int e = 0; e < nd; ++e) fi[e] = lo[e] + 2 * cc[e] + off[e];
+                    f[sh.s(fi[2], fi[1], fi[0])] = x;
+                  }
+            }
+      }
+      // Toth & Roe over every coarse cell of the quadrant (all shared faces are in place)
+      for (int d = 0; d < nd; ++d)
+        for (int ck = 0; ck < hn[2]; ++ck)
+          for (int cj = 0; cj < hn[1]; ++cj)
+            for (int ci = 0; ci < hn[0]; ++ci) {
+              const int c[3] = {lo[0] + 2 * ci, nd > 1 ? lo[1] + 2 * cj : 0,
+                                nd > 2 ? lo[2] + 2 * ck : 0};
+              const int na = (d != 0 ? 2 : 1), nb2 = (d != 1 && nd > 1 ? 2 : 1),
+                        nc2 = (d != 2 && nd > 2 ? 2 : 1);
+              for (int ok = 0; ok < nc2; ++ok)
+                for (int oj = 0; oj < nb2; ++oj)
+                  for (int oi = 0; oi < na; ++oi) {
+                    int idx[3] = {c[0] + oi, c[1] + oj, c[2] + ok};
+                    idx[d] = c[d] + 1;  // the face inside the coarse cell
+                    facep(nw, d, b)[sh.s(idx[2], idx[1], idx[0])] =
+                        TothRoe(sh, nw, b, blk, d, c, idx);
+                  }
+            }
+      continue;
+    }
+    // merged parent: each child restricts its faces into the parent's quadrant
+    for (int ch = 0; ch < (1 << nd); ++ch) {
+      const int cb = org[1 + ch];
+      const double A[3] = {o.blocks[cb].area[0], o.blocks[cb].area[1], o.blocks[cb].area[2]};
+      int q0[3] = {0, 0, 0};
+      for (int e = 0; e < nd; ++e) q0[e] = lo[e] + ((ch >> e) & 1) * hn[e];
+      for (int d = 0; d < 3; ++d) {
+        const double *q = facep(o, d, cb);
+        const bool inc[3] = {d != 0, d != 1 && nd > 1, d != 2 && nd > 2};
+        const int cn[3] = {hn[0] + (d == 0), hn[1] + (d == 1 && nd > 1),
+                           hn[2] + (d == 2 && nd > 2)};
+        for (int ck = 0; ck < cn[2]; ++ck)
+          for (int cj = 0; cj < cn[1]; ++cj)
+            for (int ci = 0; ci < cn[0]; ++ci) {
+              const int cc[3] = {ci, cj, ck};
+              int fl[3] = {0, 0, 0};
+              for (int e = 0; e < nd; ++e) fl[e] = lo[e] + 2 * cc[e];
+              double V[2][2][2], T[2][2][2];
+              for (int ok = 0; ok < 2; ++ok)
+                for (int oj = 0; oj < 2; ++oj)
+                  for (int oi = 0; oi < 2; ++oi) {
+                    V[ok][oj][oi] = T[ok][oj][oi] = 0.0;
+                    if ((ok == 0 || inc[2]) && (oj == 0 || inc[1]) && (oi == 0 || inc[0])) {
+                      V[ok][oj][oi] = A[d];
+                      T[ok][oj][oi] = A[d] * q[sh.s(fl[2] + ok, fl[1] + oj, fl[0] + oi)];
+                    }
+                  }
+              const double tvol = ((V[0][0][0] + V[0][1][0]) + (V[0][0][1] + V[0][1][1])) +
+                                  ((V[1][0][0] + V[1][1][0]) + (V[1][0][1] + V[1][1][1]));
+              const double x =
+                  tvol > 0.0 ? (((T[0][0][0] + T[0][1][0]) + (T[0][0][1] + T[0][1][1])) +
+                                ((T[1][0][0] + T[1][1][0]) + (T[1][0][1] + T[1][1][1]))) /
+                                   tvol
+                             : 0.0;
+              facep(nw, d, b)[sh.s(q0[2] + ck, q0[1] + cj, q0[0] + ci)] = x;
+            }
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------------
 // Whole-mesh driver: KamayanDriver::MakeTaskCollection / BuildTaskListRKStage
 // (kamayan_driver.cpp:121-248) over a single MeshData partition.
@@ -2480,6 +2618,7 @@ void AdaptStep(km_cpu_sim_t &s) {
   s.nblocks = n;
   MakeContainer(s, s.base, true, nullptr, false);
   Remesh(old_base.pack, s.base.pack, org.data(), 0, s.nvar);
+  if (s.cfg.mhd == KM_MHD_CT) RemeshFaces(old_base.pack, s.base.pack, org.data());
   for (int i = 0; i < 2; ++i) MakeContainer(s, s.stage[i], false, &s.base, false);
   MakeContainer(s, s.dudt, false, nullptr, true);
   s.leaves = std::move(nl);
@@ -2874,11 +3013,14 @@ int km_cpu_amr_update_tree(const km_mesh_desc_t *mesh, int32_t numlevel,
 int km_cpu_amr_remesh(const km_pack_t *old_u, const km_pack_t *new_u, const int32_t *origin,
                       int32_t v0, int32_t v1) {
   Remesh(*old_u, *new_u, origin, v0, v1);
+  bool faces = true;
+  for (int d = 0; d < 3; ++d) faces = faces && old_u->face[d] && new_u->face[d];
+  if (faces) RemeshFaces(*old_u, *new_u, origin);
   return KM_OK;
 }
 
 int km_cpu_sim_set_adaptive(km_cpu_sim_t *s, const km_amr_adapt_t *a) {
-  if (!s->amr || s->cfg.mhd == KM_MHD_CT || a->ncrit < 0 || a->ncrit > KM_AMR_MAX_CRIT ||
+  if (!s->amr || a->ncrit < 0 || a->ncrit > KM_AMR_MAX_CRIT ||
       a->numlevel < 1)
     return KM_EINVAL;
   s->adaptive = true;
