@@ -227,6 +227,17 @@ int km_ghost_plan_destroy(km_ghost_plan_t *plan);
  * and prolongation stencils inside blocks adjacent to the block being filled. */
 int km_ghost_plan_amr(const km_pack_t *shape, const km_mesh_desc_t *mesh,
                       const int32_t *leaves, int32_t nleaves, km_ghost_plan_t **plan);
+/* The same plan for a geometry (KM_GEOM_*; km_ghost_plan_amr is Cartesian). Cylindrical
+ * (r, z) meshes take kamayan's geometry-templated operators (refinement_operations.hpp:
+ * 52-398): restriction weighted by the r-dependent cell volumes and face areas (and phi
+ * edge lengths |r| 2 pi), r and z faces prolongated as r B (values times their radius,
+ * the child divided by its radius + DBL_EPSILON) and Toth-Roe on r B with the phi
+ * spacing 2 pi; an axisymmetric inner x1 boundary mirrors zones and x1 faces
+ * (boundary_conditions.cpp:23-71). Flux correction on such a plan weights the fine
+ * fluxes by their cylindrical face areas. */
+int km_ghost_plan_amr_geom(const km_pack_t *shape, const km_mesh_desc_t *mesh,
+                           const int32_t *leaves, int32_t nleaves, int32_t geometry,
+                           km_ghost_plan_t **plan);
 /* Fill every ghost zone (faces, edges, corners) of cell vars [v0, v1) and, when
  * with_faces != 0, of the CT face arrays. With a block-tree plan a ghost zone is copied
  * from a same-level neighbour, restricted from a finer one (RestrictAverage,
@@ -293,7 +304,7 @@ int km_sim_set_tile(km_sim_t *sim, int32_t threads);
 /* ---- problem generators (host-side; fill primitives on the interior) -------------- */
 /* names: "isentropic_vortex" (isentropic_vortex.cpp:105-175), "smooth3d", "mhd_random",
  * "kelvin_helmholtz", "sedov" (sedov.cpp:75-143, 3-D form), "mhd_blast"
- * (mhd_blast.cpp:47-160, Cartesian). Buffers as for
+ * (mhd_blast.cpp:47-160; Cartesian, and r-z cylindrical). Buffers as for
  * km_sim_set_primitive; `seed` feeds the random modes where a problem has them. With
  * cfg->nspecies > 1 the mass fractions are set too: sedov puts species 0 inside the blast
  * radius and species 1 outside (sedov.cpp:138-141), the others species 0 everywhere. */
@@ -380,6 +391,12 @@ int km_amr_update_tree(const km_mesh_desc_t *mesh, int32_t numlevel, int32_t der
  * packs share ndim, ng and nx. */
 int km_amr_remesh(const km_pack_t *old_u, const km_pack_t *new_u, const int32_t *origin_dev,
                   int32_t v0, int32_t v1, void *stream);
+/* The same for a geometry (km_amr_remesh is Cartesian); cylindrical geometry takes the
+ * operators of km_ghost_plan_amr_geom and needs the mesh description (its x1min places
+ * the radii). */
+int km_amr_remesh_geom(const km_pack_t *old_u, const km_pack_t *new_u, const int32_t *origin_dev,
+                       int32_t v0, int32_t v1, const km_mesh_desc_t *mesh, int32_t geometry,
+                       void *stream);
 /* Adaptive refinement of a block-tree simulation (hydro or CT): after every cycle each leaf
  * is tagged by the criteria (km_amr_tag_loehner on the filled base state), the tree is
  * updated (km_amr_update_tree) and, when it changed, every cell field and the CT face

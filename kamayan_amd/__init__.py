@@ -84,6 +84,7 @@ def lib() -> C.CDLL:
         "km_amr_blocks": [P(KmMeshDesc), P(i32), i32, vp],
         "km_amr_refine_ball": [P(KmMeshDesc), i32, P(dbl), dbl, P(i32), P(i32)],
         "km_ghost_plan_amr": [P(KmPack), P(KmMeshDesc), P(i32), i32, P(vp)],
+        "km_ghost_plan_amr_geom": [P(KmPack), P(KmMeshDesc), P(i32), i32, i32, P(vp)],
         "km_flux_correct": [vp, P(KmPack), vp],
         "km_sim_create_amr": [P(KmCfg), P(KmMeshDesc), P(i32), i32, i32, P(vp)],
         "km_device_count": [P(i32)],
@@ -99,6 +100,7 @@ def lib() -> C.CDLL:
         "km_amr_update_tree": [P(KmMeshDesc), i32, i32, P(i32), i32, P(i32), P(i32), P(i32),
                                P(i32), P(i32), P(i32), P(i32)],
         "km_amr_remesh": [P(KmPack), P(KmPack), vp, i32, i32, vp],
+        "km_amr_remesh_geom": [P(KmPack), P(KmPack), vp, i32, i32, P(KmMeshDesc), i32, vp],
         "km_sim_set_adaptive": [vp, P(KmAmrAdapt)],
         "km_sim_amr_leaves": [vp, P(i32), P(i32)],
         "km_history_vortex_errors": [P(KmCfg), P(KmPack), P(KmMeshDesc), P(dbl), i32, P(dbl),
@@ -222,9 +224,13 @@ def amr_tag_loehner(pack: "DevicePack", crit: KmAmrCriterion, tags_dev: int, res
                                    1 if reset else 0, None), "km_amr_tag_loehner")
 
 
-def amr_remesh(old: "DevicePack", new: "DevicePack", origin_dev: int, v0: int, v1: int):
-    check(lib().km_amr_remesh(C.byref(old.pack), C.byref(new.pack), C.c_void_p(origin_dev), v0,
-                              v1, None), "km_amr_remesh")
+def amr_remesh(old: "DevicePack", new: "DevicePack", origin_dev: int, v0: int, v1: int,
+               desc: KmMeshDesc = None, geometry: int = 0):
+    """km_amr_remesh_geom: cell slots [v0, v1) and, when both packs carry them, the faces;
+    geometry 1 (cylindrical) needs the mesh description"""
+    check(lib().km_amr_remesh_geom(C.byref(old.pack), C.byref(new.pack), C.c_void_p(origin_dev),
+                                   v0, v1, C.byref(desc) if desc is not None else None,
+                                   geometry, None), "km_amr_remesh")
 
 
 def _params(params):

@@ -63,12 +63,19 @@ __device__ __forceinline__ void map_zone(const AmrGrid &t, const Geo &g, int L, 
     flip[d] = false;
     G[d] = d < g.ndim ? amr_map(t, d, Gin[d], amr_zones(t, g, d, L), flip[d]) : 0;
   }
+  // the axisymmetric mirror also negates the phi (x3) component of a vector
+  // (boundary_conditions.cpp:66-69); x3 is inactive in the geometries that use it
+  if (Gin[0] < 0 && t.bc[0] == KM_BC_AXISYMMETRIC) flip[2] = !flip[2];
 }
 
-// RestrictAverage (refinement_operations.hpp:52-121) of the children of a zone: cell
-// volumes of a Cartesian block are all equal; the pairwise sum order is the reference's
-__device__ __forceinline__ double restrict_cell(const double *q, double vol, const Geo &g,
-                                                long idx) {
+// RestrictAverage (refinement_operations.hpp:52-121) of the children of a zone (lower
+// corner idx in fine block fb): weights are the children's cell volumes (equal in a
+// Cartesian block, r-dependent in cylindrical geometry); the pairwise sum order is the
+// reference's
+__device__ __forceinline__ double restrict_cell(const double *q, const km_block_t &fb,
+                                                const Geo &g, long idx, bool cyl) {
+  const int i0 = (int)(idx % g.E[0]);
+  const double w[2] = {cell_volume(fb, g, cyl, i0), cell_volume(fb, g, cyl, i0 + 1)};
   double V[2][2][2], Tm[2][2][2];
 #pragma unroll
   for (int ok = 0; ok < 2; ++ok)
@@ -79,8 +86,8 @@ __device__ __forceinline__ double restrict_cell(const double *q, double vol, con
         V[ok][oj][oi] = 0.;
         Tm[ok][oj][oi] = 0.;
         if ((ok == 0 || g.ndim > 2) && (oj == 0 || g.ndim > 1)) {
-          V[ok][oj][oi] = vol;
-          Tm[ok][oj][oi] = vol * q[idx + ok * g.sc[2] + oj * g.sc[1] + oi];
+          V[ok][oj][oi] = w[oi];
+          Tm[ok][oj][oi] = w[oi] * q[idx + ok * g.sc[2] + oj * g.sc[1] + oi];
         }
       }
   const double tvol = ((V[0][0][0] + V[0][1][0]) + (V[0][0][1] + V[0][1][1])) +
@@ -92,9 +99,10 @@ __device__ __forceinline__ double restrict_cell(const double *q, double vol, con
 }
 
 // value of a located zone (coarse leaf zone or restriction of finer children)
-__device__ __forceinline__ double value_at(const km_pack_t &u, const Geo &g, const Src &s, int v) {
+__device__ __forceinline__ double value_at(const km_pack_t &u, const Geo &g, const Src &s, int v,
+                                           bool cyl) {
   const double *q = cell_ptr(u, s.leaf, v);
-  return s.kind == 1 ? restrict_cell(q, u.blocks[s.leaf].vol, g, s.idx) : q[s.idx];
+  return s.kind == 1 ? restrict_cell(q, u.blocks[s.leaf], g, s.idx, cyl) : q[s.idx];
 }
 
 // axis a vector component points along (momentum, velocity, cell-centred B), else -1
@@ -156,7 +164,7 @@ __global__ void __launch_bounds__(256) k_amr_ghost(km_pack_t u, AmrGrid t, int v
   // variables go in chunks: every load of a chunk is issued before its stores, so the
   // loads of one chunk overlap instead of waiting on the previous variable's store
   constexpr int kChunk = 4;
-  const double vol = s.kind == 1 ? u.blocks[s.leaf].vol : 0.0;
+  const km_block_t &sb = u.blocks[s.leaf];
   for (int v = v0; v < v1; v += kChunk) {
     double x[kChunk];
 #pragma unroll
@@ -164,7 +172,7 @@ __global__ void __launch_bounds__(256) k_amr_ghost(km_pack_t u, AmrGrid t, int v
       x[q] = 0.0;
       if (v + q < v1) {
         const double *src = cell_ptr(u, s.leaf, v + q);
-        x[q] = s.kind == 1 ? restrict_cell(src, vol, g, s.idx) : src[s.idx];
+        x[q] = s.kind == 1 ? restrict_cell(src, sb, g, s.idx, t.cyl != 0) : src[s.idx];
       }
     }
 #pragma unroll
@@ -267,7 +275,8 @@ __global__ void __launch_bounds__(256) k_amr_prolong(km_pack_t u, AmrGrid t, int
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       if (d < g.ndim) {
-        double fm = value_at(u, g, nbm[d], v), fp = value_at(u, g, nbp[d], v);
+        double fm = value_at(u, g, nbm[d], v, t.cyl != 0);
+        double fp = value_at(u, g, nbp[d], v, t.cyl != 0);
         if (ax >= 0 && flm[d][ax]) fm = -fm;
         if (ax >= 0 && flp[d][ax]) fp = -fp;
         const double dxc = 2.0 * blk.dx[d];
@@ -351,7 +360,9 @@ __global__ void __launch_bounds__(256) k_amr_flux_correct(km_pack_t u, AmrGrid t
   ci[dir] = side ? g.hi[dir] + 1 : g.lo[dir];
   const long cidx = g.c(ci[2], ci[1], ci[0]);
   const bool inc[3] = {dir != 0, dir != 1 && g.ndim > 1, dir != 2 && g.ndim > 2};
-  const double A = fb.area[dir];
+  // fine face areas (r-dependent for the r and z faces in cylindrical geometry)
+  const double A[2] = {face_area(fb, g, t.cyl != 0, dir, fi[0]),
+                       face_area(fb, g, t.cyl != 0, dir, fi[0] + (dir != 0))};
   for (int v = 0; v < nc; ++v) {
     const double *F = flux_ptr(u, dir, fl, v);
     double V[2][2][2], Tm[2][2][2];
@@ -364,8 +375,8 @@ __global__ void __launch_bounds__(256) k_amr_flux_correct(km_pack_t u, AmrGrid t
           V[ok][oj][oi] = 0.;
           Tm[ok][oj][oi] = 0.;
           if ((ok == 0 || inc[2]) && (oj == 0 || inc[1]) && (oi == 0 || inc[0])) {
-            V[ok][oj][oi] = A;
-            Tm[ok][oj][oi] = A * F[fidx + ok * g.sc[2] + oj * g.sc[1] + oi];
+            V[ok][oj][oi] = A[oi];
+            Tm[ok][oj][oi] = A[oi] * F[fidx + ok * g.sc[2] + oj * g.sc[1] + oi];
           }
         }
     const double tvol = ((V[0][0][0] + V[0][1][0]) + (V[0][0][1] + V[0][1][1])) +
@@ -455,11 +466,14 @@ __device__ __forceinline__ Face locate_face(const km_pack_t &u, const AmrGrid &t
 // RestrictAverage of the fine faces of component d behind a located face (kind 1): the
 // transverse active axes are averaged with the fine face areas as weights
 __device__ __forceinline__ double face_value(const km_pack_t &u, const Geo &g, int d,
-                                             const Face &f) {
+                                             const Face &f, bool cyl) {
   const double *q = u.face[d][f.leaf];
   if (f.kind != 1) return q[f.idx];
   const bool inc[3] = {d != 0, d != 1 && g.ndim > 1, d != 2 && g.ndim > 2};
-  const double A = u.blocks[f.leaf].area[d];
+  // fine face areas: r-dependent for the r and z faces in cylindrical geometry
+  const km_block_t &fb = u.blocks[f.leaf];
+  const int i0 = (int)(f.idx % g.S[0]);
+  const double A[2] = {face_area(fb, g, cyl, d, i0), face_area(fb, g, cyl, d, i0 + (d != 0))};
   double V[2][2][2], Tm[2][2][2];
 #pragma unroll
   for (int ok = 0; ok < 2; ++ok)
@@ -470,8 +484,8 @@ __device__ __forceinline__ double face_value(const km_pack_t &u, const Geo &g, i
         V[ok][oj][oi] = 0.;
         Tm[ok][oj][oi] = 0.;
         if ((ok == 0 || inc[2]) && (oj == 0 || inc[1]) && (oi == 0 || inc[0])) {
-          V[ok][oj][oi] = A;
-          Tm[ok][oj][oi] = A * q[f.idx + ok * g.ss[2] + oj * g.ss[1] + oi];
+          V[ok][oj][oi] = A[oi];
+          Tm[ok][oj][oi] = A[oi] * q[f.idx + ok * g.ss[2] + oj * g.ss[1] + oi];
         }
       }
   const double tvol = ((V[0][0][0] + V[0][1][0]) + (V[0][0][1] + V[0][1][1])) +
@@ -530,14 +544,20 @@ __global__ void __launch_bounds__(256) k_amr_faces_shared(km_pack_t u, AmrGrid t
   const Face f = locate_face(u, t, g, L, d, G, up);
   const long dst = g.s(idx[2], idx[1], idx[0]);
   if (f.kind != 2) {
-    double x = face_value(u, g, d, f);
+    double x = face_value(u, g, d, f, t.cyl != 0);
     if (d < g.ndim && fl[d]) x = -x;
     u.face[d][b][dst] = x;
     return;
   }
   if (d < g.ndim && (G[d] & 1)) return;  // inside a coarse cell: phase 2
-  // shared fine face on coarse face (G[d]/2, C): transverse minmod slopes at level L-1
-  const double fc = u.face[d][f.leaf][f.idx];
+  // shared fine face on coarse face (G[d]/2, C): transverse minmod slopes at level L-1.
+  // Cylindrical r and z faces prolongate r B (SCALE_BY_R, refinement_operations.hpp:
+  // 165-173,228-233): each coarse value is scaled by its element's radius (logical
+  // index, negative past the axis) and the child divided by its own radius + DBL_EPSILON.
+  const bool sr = t.cyl && d < 2;
+  const double dxc0 = 2.0 * blk.dx[0];
+  auto rc = [&](int C0) { return sr ? amr_radius(t.x1min, C0, d == 1, dxc0) : 1.0; };
+  const double fc = u.face[d][f.leaf][f.idx] * rc(G[0] >> 1);
   double a[3] = {0., 0., 0.};
 #pragma unroll
   for (int e = 0; e < 3; ++e) {
@@ -554,7 +574,7 @@ __global__ void __launch_bounds__(256) k_amr_faces_shared(km_pack_t u, AmrGrid t
       for (int q = 0; q < 3; ++q)
         M[q] = (q == e) ? amr_map(t, q, C[q], amr_zones(t, g, q, L - 1), nf[q]) : C[q];
       const Face s = locate_face(u, t, g, L - 1, d, M, up);
-      fv[side] = face_value(u, g, d, s);
+      fv[side] = face_value(u, g, d, s, t.cyl != 0) * rc(C[0]);
     }
     const double dxc = 2.0 * blk.dx[e];
     a[e] = grad_minmod(fc, fv[0], fv[1], dxc, dxc) * (0.5 * blk.dx[e]);
@@ -563,6 +583,7 @@ __global__ void __launch_bounds__(256) k_amr_faces_shared(km_pack_t u, AmrGrid t
   const int oj = (d != 1 && g.ndim > 1) ? (G[1] & 1) : 0;
   const int ok = (d != 2 && g.ndim > 2) ? (G[2] & 1) : 0;
   double x = child_value(fc, a[0], a[1], a[2], oi, oj, ok);
+  if (sr) x *= 1.0 / (amr_radius(t.x1min, G[0], d == 1, blk.dx[0]) + 2.220446049250313e-16);
   if (d < g.ndim && fl[d]) x = -x;
   u.face[d][b][dst] = x;
 }
@@ -574,7 +595,7 @@ namespace {
 // reference's loop is reached through its permuted offsets; the coarse cell spacings
 // enter unpermuted, as the reference writes them.
 __device__ double toth_roe(const km_pack_t &u, const Geo &g, int b, int d, const int idx[3],
-                           const km_block_t &blk) {
+                           const km_block_t &blk, bool cyl, double x1min) {
   const int g2 = g.ndim > 1, g3 = g.ndim > 2;
   // fine lower corner of the coarse cell
   int fc[3];
@@ -591,10 +612,18 @@ __device__ double toth_roe(const km_pack_t &u, const Geo &g, int b, int d, const
       out[0] = fk + oi * g3, out[1] = fj + ok * g2, out[2] = fi + oj;
     }
   };
+  // cylindrical: the divergence constraint holds for r B, so every fine face enters
+  // multiplied by its radius (an r face's own, a z face's cell-centre radius, 1 for the
+  // phi faces of a 1-D or 2-D mesh) and the result is divided by the target's radius
+  auto radius = [&](int comp, int ii) {
+    if (!cyl || comp == 2) return 1.0;
+    return amr_radius(x1min, blk.lx[0] * g.nx[0] + (ii - g.lo[0]), comp == 1, blk.dx[0]);
+  };
   auto fine = [&](int eidx, int ok, int oj, int oi) {
     int p[3];
     kji(ok, oj, oi, p);
-    return u.face[(d + eidx) % 3][b][g.s(p[0], p[1], p[2])];
+    const int comp = (d + eidx) % 3;
+    return u.face[comp][b][g.s(p[0], p[1], p[2])] * radius(comp, p[2]);
   };
   // this element's (ok, oj) in the reference's write loop
   int ok = 0, oj = 0;
@@ -620,13 +649,15 @@ __device__ double toth_roe(const km_pack_t &u, const Geo &g, int b, int d, const
         Wxyz += sg(tt) * sg(uu) * sg(v) * fine3;
       }
   Uxx *= 0.125;
-  const double cdx = 2.0 * blk.dx[0], cdy = 2.0 * blk.dx[1], cdz = 2.0 * blk.dx[2];
+  // coarse spacings; the cylindrical Dx along phi is 2 pi (geometry.hpp:35-43)
+  const double cdx = 2.0 * blk.dx[0], cdy = 2.0 * blk.dx[1];
+  const double cdz = cyl ? kTwoPi : 2.0 * blk.dx[2];
   const double dx2 = cdx * cdx, dy2 = cdy * cdy, dz2 = cdz * cdz;
   Vxyz *= 0.125 * dz2 / (dx2 + dz2);
   Wxyz *= 0.125 * dy2 / (dx2 + dy2);
   double x = 0.5 * (fine(0, ok, oj, 0) * 1.0 + fine(0, ok, oj, 2) * 1.0) + Uxx + sg(ok) * Vxyz +
              sg(oj) * Wxyz;
-  x *= 1.0;
+  x *= cyl ? 1.0 / (radius(d, idx[0]) + 2.220446049250313e-16) : 1.0;
   return x;
 }
 }  // namespace
@@ -645,7 +676,7 @@ __global__ void __launch_bounds__(256) k_amr_faces_internal(km_pack_t u, AmrGrid
   face_global(t, g, blk, d, idx, G, fl);
   const Face f = locate_face(u, t, g, blk.level, d, G, face_up(g, d, idx));
   if (f.kind != 2) return;
-  u.face[d][b][g.s(idx[2], idx[1], idx[0])] = toth_roe(u, g, b, d, idx, blk);
+  u.face[d][b][g.s(idx[2], idx[1], idx[0])] = toth_roe(u, g, b, d, idx, blk, t.cyl != 0, t.x1min);
 }
 
 // EMF correction at level jumps: every edge on a block's boundary that a finer leaf also
@@ -793,7 +824,8 @@ void launch_amr_tag_loehner(const km_pack_t &u, const km_amr_criterion_t &crit, 
 // zones of the parent); a merged parent takes the RestrictAverage of the 2^d zones of
 // the child holding 2C. Ghost zones of new leaves are left for the ghost fill.
 __global__ void __launch_bounds__(256) k_amr_remesh(km_pack_t o, km_pack_t nw,
-                                                    const int32_t *origin, int v0, int v1) {
+                                                    const int32_t *origin, int v0, int v1,
+                                                    int cyl) {
   const Geo g(nw);
   const long n = g.ncell();
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -856,9 +888,9 @@ __global__ void __launch_bounds__(256) k_amr_remesh(km_pack_t o, km_pack_t nw,
     f[d] = 2 * lc - half * g.nx[d] + g.lo[d];
   }
   const int cb = org[1 + ch];
-  const double vol = o.blocks[cb].vol;
   const long fz = g.c(f[2], f[1], f[0]);
-  for (int v = v0; v < v1; ++v) cell_ptr(nw, b, v)[z] = restrict_cell(cell_ptr(o, cb, v), vol, g, fz);
+  for (int v = v0; v < v1; ++v)
+    cell_ptr(nw, b, v)[z] = restrict_cell(cell_ptr(o, cb, v), o.blocks[cb], g, fz, cyl != 0);
 }
 
 // The same data movement for the CT face field, component d, one lane per face element
@@ -870,7 +902,8 @@ __global__ void __launch_bounds__(256) k_amr_remesh(km_pack_t o, km_pack_t nw,
 // the 2^(d-1) child faces on each of its own faces. Ghost faces of new leaves are left to
 // the ghost fill.
 __global__ void __launch_bounds__(256) k_amr_remesh_faces_shared(km_pack_t o, km_pack_t nw,
-                                                                 const int32_t *origin, int d) {
+                                                                 const int32_t *origin, int d,
+                                                                 int cyl, double x1min) {
   const Geo g(nw);
   int b, idx[3];
   if (!decode_stag(g, nw.nblocks, b, idx)) return;
@@ -899,18 +932,26 @@ __global__ void __launch_bounds__(256) k_amr_remesh_faces_shared(km_pack_t o, km
     if (d < g.ndim && (G[d] & 1)) return;  // inside a parent cell: phase 2
     const double *q = o.face[d][pb];
     const long pcs = g.s(pc[2], pc[1], pc[0]);
-    const double fc = q[pcs];
+    // cylindrical r and z faces prolongate r B, as in the ghost fill
+    const bool sr = cyl && d < 2;
+    const double dxc0 = 2.0 * blk.dx[0];
+    auto rc = [&](int C0) { return sr ? amr_radius(x1min, C0, d == 1, dxc0) : 1.0; };
+    const int C0 = G[0] >> 1;
+    const double fc = q[pcs] * rc(C0);
     double a[3] = {0., 0., 0.};
 #pragma unroll
     for (int e = 0; e < 3; ++e) {
       if (e == d || e >= g.ndim) continue;
       const double dxc = 2.0 * blk.dx[e];
-      a[e] = grad_minmod(fc, q[pcs - g.ss[e]], q[pcs + g.ss[e]], dxc, dxc) * (0.5 * blk.dx[e]);
+      const double fm = q[pcs - g.ss[e]] * rc(C0 - (e == 0)), fp = q[pcs + g.ss[e]] * rc(C0 + (e == 0));
+      a[e] = grad_minmod(fc, fm, fp, dxc, dxc) * (0.5 * blk.dx[e]);
     }
     const int oi = d != 0 ? (G[0] & 1) : 0;
     const int oj = (d != 1 && g.ndim > 1) ? (G[1] & 1) : 0;
     const int ok = (d != 2 && g.ndim > 2) ? (G[2] & 1) : 0;
-    nw.face[d][b][dst] = child_value(fc, a[0], a[1], a[2], oi, oj, ok);
+    double x = child_value(fc, a[0], a[1], a[2], oi, oj, ok);
+    if (sr) x *= 1.0 / (amr_radius(x1min, G[0], d == 1, blk.dx[0]) + 2.220446049250313e-16);
+    nw.face[d][b][dst] = x;
     return;
   }
   int ch = 0, f[3] = {0, 0, 0};
@@ -927,13 +968,14 @@ __global__ void __launch_bounds__(256) k_amr_remesh_faces_shared(km_pack_t o, km
   src.kind = 1;
   src.leaf = org[1 + ch];
   src.idx = g.s(f[2], f[1], f[0]);
-  nw.face[d][b][dst] = face_value(o, g, d, src);
+  nw.face[d][b][dst] = face_value(o, g, d, src, cyl != 0);
 }
 
 // phase 2 of the face remesh: ProlongateInternalTothAndRoe for the faces of a split leaf's
 // child that lie inside parent cells, from the child's shared faces of phase 1
 __global__ void __launch_bounds__(256) k_amr_remesh_faces_internal(km_pack_t nw,
-                                                                   const int32_t *origin, int d) {
+                                                                   const int32_t *origin, int d,
+                                                                   int cyl, double x1min) {
   const Geo g(nw);
   int b, idx[3];
   if (!decode_stag(g, nw.nblocks, b, idx)) return;
@@ -944,30 +986,31 @@ __global__ void __launch_bounds__(256) k_amr_remesh_faces_internal(km_pack_t nw,
     if (idx[e] < g.lo[e] || idx[e] > hi) return;
   }
   if (!((idx[d] - g.lo[d]) & 1)) return;  // shared position (block offsets are even)
-  nw.face[d][b][g.s(idx[2], idx[1], idx[0])] = toth_roe(nw, g, b, d, idx, nw.blocks[b]);
+  nw.face[d][b][g.s(idx[2], idx[1], idx[0])] =
+      toth_roe(nw, g, b, d, idx, nw.blocks[b], cyl != 0, x1min);
 }
 
 void launch_amr_remesh_faces(const km_pack_t &old_u, const km_pack_t &new_u,
-                             const int32_t *origin, hipStream_t st) {
+                             const int32_t *origin, bool cyl, double x1min, hipStream_t st) {
   const Geo g(new_u);
   const long n = g.nstag() * new_u.nblocks;
   if (n == 0) return;
   const unsigned grid = (unsigned)((n + 255) / 256);
   for (int d = 0; d < 3; ++d)
     hipLaunchKernelGGL(k_amr_remesh_faces_shared, dim3(grid), dim3(256), 0, st, old_u, new_u,
-                       origin, d);
+                       origin, d, cyl ? 1 : 0, x1min);
   for (int d = 0; d < g.ndim; ++d)
     hipLaunchKernelGGL(k_amr_remesh_faces_internal, dim3(grid), dim3(256), 0, st, new_u, origin,
-                       d);
+                       d, cyl ? 1 : 0, x1min);
 }
 
 void launch_amr_remesh(const km_pack_t &old_u, const km_pack_t &new_u, const int32_t *origin,
-                       int v0, int v1, hipStream_t st) {
+                       int v0, int v1, bool cyl, hipStream_t st) {
   const Geo g(new_u);
   const long n = g.ncell() * new_u.nblocks;
   if (n == 0 || v0 >= v1) return;
   hipLaunchKernelGGL(k_amr_remesh, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, old_u,
-                     new_u, origin, v0, v1);
+                     new_u, origin, v0, v1, cyl ? 1 : 0);
 }
 
 }  // namespace km

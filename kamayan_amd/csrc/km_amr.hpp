@@ -21,7 +21,16 @@ struct AmrGrid {
   int bc[6];          // ix1, ox1, ix2, ox2, ix3, ox3
   int nbf[3];         // lookup extent: nb0 << lmax along active dimensions
   const int *lookup;  // [nbf2][nbf1][nbf0] -> leaf block id
+  int cyl;            // cylindrical (r, z) geometry: r-weighted refinement operators
+  double x1min;       // domain's lower x1 edge (the radius of global face 0)
 };
+
+// Radius of a level-L element along x1 for the r-weighted face operators: global face
+// index G (centre = 0) or the centre of cell G (centre = 1), h the level's spacing. Ghost
+// positions past the axis have negative radii.
+KM_HD double amr_radius(double x1min, int G, int centre, double h) {
+  return centre ? x1min + (G + 0.5) * h : x1min + G * h;
+}
 
 // global zone count along dimension d at level L
 KM_HD int amr_zones(const AmrGrid &t, const Geo &g, int d, int L) {
@@ -35,7 +44,7 @@ KM_HD int amr_map(const AmrGrid &t, int d, int G, int N, bool &flip) {
     const int bc = t.bc[2 * d];
     if (bc == KM_BC_PERIODIC) return ((G % N) + N) % N;
     if (bc == KM_BC_OUTFLOW) return 0;
-    flip = !flip;
+    flip = !flip;  // reflect and axisymmetric mirror the zones
     return -G - 1;
   }
   if (G >= N) {
@@ -55,7 +64,9 @@ KM_HD int amr_map_face(const AmrGrid &t, int d, int G, int N, bool &flip) {
     if (bc == KM_BC_PERIODIC) return ((G % N) + N) % N;
     if (bc == KM_BC_OUTFLOW) return 0;
     flip = !flip;
-    return -G;
+    // the axisymmetric boundary maps x1 faces as it maps cells (boundary_conditions.cpp:
+    // 40-58): ghost face -g takes face g - 1
+    return bc == KM_BC_AXISYMMETRIC ? -G - 1 : -G;
   }
   if (G > N) {
     const int bc = t.bc[2 * d + 1];
@@ -165,7 +176,9 @@ KM_HD bool amr_emf_edge(const km_pack_t &u, const AmrGrid &t, int te, int b, con
       }
     }
     const bool inc[3] = {te == 0, te == 1 && g.ndim > 1, te == 2 && g.ndim > 2};
-    const double len = fb.dx[te];
+    // edge length: dx along the edge, |r| 2 pi for phi edges in cylindrical geometry
+    const double len =
+        (t.cyl && te == 2) ? __builtin_fabs(x_face(fb, g, 0, fe[0])) * kTwoPi : fb.dx[te];
     const double *E = u.edge[te][fl];
     const long fidx = g.s(fe[2], fe[1], fe[0]);
     double V[2][2][2], Tm[2][2][2];
@@ -196,6 +209,8 @@ KM_HD bool amr_emf_edge(const km_pack_t &u, const AmrGrid &t, int te, int b, con
 // host-side tree: leaf metrics in the caller's leaf order and the finest-level lookup
 struct AmrTreeHost {
   int nb0[3] = {1, 1, 1}, nbf[3] = {1, 1, 1}, lmax = 0, bc[6] = {0, 0, 0, 0, 0, 0};
+  int cyl = 0;        // set by the caller from the configuration's geometry
+  double x1min = 0.;
   std::vector<km_block_t> blocks;
   std::vector<int> lookup;
   AmrGrid grid(const int *lookup_dev) const;
@@ -220,10 +235,11 @@ std::string update_amr_tree(const km_mesh_desc_t &mesh, int numlevel, int thresh
 void launch_amr_tag_loehner(const km_pack_t &u, const km_amr_criterion_t &crit, int *tags,
                             bool reset, double *err2_scratch, hipStream_t st);
 void launch_amr_remesh(const km_pack_t &old_u, const km_pack_t &new_u, const int32_t *origin,
-                       int v0, int v1, hipStream_t st);
-// the CT face field's remesh (all three components; new ghost faces are left unset)
+                       int v0, int v1, bool cyl, hipStream_t st);
+// the CT face field's remesh (all three components; new ghost faces are left unset);
+// x1min: the domain's lower x1 edge (radii of the cylindrical face operators)
 void launch_amr_remesh_faces(const km_pack_t &old_u, const km_pack_t &new_u,
-                             const int32_t *origin, hipStream_t st);
+                             const int32_t *origin, bool cyl, double x1min, hipStream_t st);
 
 void launch_amr_ghost_fill(const AmrGrid &t, const km_pack_t &u, int v0, int v1, int nc,
                            hipStream_t st);

@@ -82,10 +82,11 @@ std::string build_amr_tree(const km_mesh_desc_t &m, const int32_t *leaves, int n
     t.nbf[d] = active(m, d) ? (m.nb[d] << lmax) : 1;
   }
   for (int d = 0; d < 6; ++d) {
-    if (m.bc[d] == KM_BC_AXISYMMETRIC)
-      return "block-tree meshes take periodic, outflow or reflect boundaries";
+    if (m.bc[d] == KM_BC_AXISYMMETRIC && (d != 0 || m.ndim > 2))
+      return "the axisymmetric boundary is the inner x1 boundary of a 1-D or 2-D mesh";
     t.bc[d] = m.bc[d];
   }
+  t.x1min = m.xmin[0];
   for (int d = 0; d < m.ndim; ++d) {
     if ((m.bc[2 * d] == KM_BC_PERIODIC) != (m.bc[2 * d + 1] == KM_BC_PERIODIC))
       return "periodic boundaries must be paired";
@@ -160,6 +161,8 @@ AmrGrid AmrTreeHost::grid(const int *lookup_dev) const {
   g.lmax = lmax;
   for (int d = 0; d < 6; ++d) g.bc[d] = bc[d];
   g.lookup = lookup_dev;
+  g.cyl = cyl;
+  g.x1min = x1min;
   return g;
 }
 

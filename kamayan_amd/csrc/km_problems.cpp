@@ -357,11 +357,39 @@ int km_problem_fill_blocks(const char *name, const km_cfg_t *cfg, const km_mesh_
   if (pname == "mhd_blast") {
     // mhd_blast.cpp:47-160, Cartesian: rho = 1, v = 0, P = p_explosion inside
     // r = sqrt(x^2 + y^2) <= radius else p_ambient, cell B = (bx, 0, 0), face B from
-    // Az = bx y: B1 = (Az(y_{j+1}) - Az(y_j)) / dy, B2 = 0 (and B3 = 0 in 3-D)
+    // Az = bx y: B1 = (Az(y_{j+1}) - Az(y_j)) / dy, B2 = 0 (and B3 = 0 in 3-D).
+    // Cylindrical (r, z) (:122-152): the distance uses the radial centroid and z, the field
+    // lies along the axis: cell B = (0, bx, 0), z faces bx, r faces 0.
     if (mhd != KM_MHD_CT || !face || !face[0] || !face[1] || !face[2] || L.ndim < 2)
       return KM_EINVAL;
     const double rho = P(0, 1.0), pamb = P(1, 1.0e-1), pexp = P(2, 1.0e1), bx = P(3, 1.0),
                  radius = P(4, 0.1);
+    if (cfg->geometry == KM_GEOM_CYLINDRICAL) {
+      if (L.ndim != 2) return KM_EINVAL;
+      for (int b = 0; b < L.nblocks; ++b) {
+        const km_block_t &bk = blocks[b];
+        double *B1 = face[0] + (long)b * L.nstag(), *B2 = face[1] + (long)b * L.nstag();
+        for (int j = L.lo(1); j <= L.hi(1) + 1; ++j)
+          for (int i = L.lo(0); i <= L.hi(0) + 1; ++i) {
+            const long s = L.s(0, j, i);
+            if (i <= L.hi(0) && j <= L.hi(1)) {
+              const double r = xc_cyl(bk, i, L.lo(0)), z = xc(bk, 1, j, L.lo(1));
+              const long c = L.c(0, j, i);
+              at(b, KM_DENS)[c] = rho;
+              at(b, sV + 0)[c] = 0.;
+              at(b, sV + 1)[c] = 0.;
+              at(b, sV + 2)[c] = 0.;
+              at(b, sP)[c] = std::sqrt(r * r + z * z) <= radius ? pexp : pamb;
+              at(b, KM_MAGC0 + 0)[c] = 0.;
+              at(b, KM_MAGC0 + 1)[c] = bx;
+              at(b, KM_MAGC0 + 2)[c] = 0.;
+            }
+            if (i <= L.hi(0)) B2[s] = bx;
+            if (j <= L.hi(1)) B1[s] = 0.0;
+          }
+      }
+      return KM_OK;
+    }
     for (int b = 0; b < L.nblocks; ++b) {
       const km_block_t &bk = blocks[b];
       double *B1 = face[0] + (long)b * L.nstag(), *B2 = face[1] + (long)b * L.nstag();

@@ -467,6 +467,7 @@ int amr_remesh(km_sim_t &s, const std::vector<int32_t> &nl, const std::vector<in
     set_error("remesh: " + err);
     return KM_EINVAL;
   }
+  t.cyl = s.cfg.geometry == KM_GEOM_CYLINDRICAL;
   std::vector<std::unique_ptr<DevBuf>> fresh;
   s.target = &fresh;
   const km_pack_t old_pack = s.base.pack;
@@ -490,8 +491,10 @@ int amr_remesh(km_sim_t &s, const std::vector<int32_t> &nl, const std::vector<in
   if ((rc = s.alloc(&origin_d, origin.size()))) return rc;
   KM_HIP(hipMemcpy(origin_d, origin.data(), origin.size() * sizeof(int32_t),
                    hipMemcpyHostToDevice));
-  launch_amr_remesh(old_pack, s.base.pack, origin_d, 0, s.nvar, s.stream);
-  if (s.cfg.mhd == KM_MHD_CT) launch_amr_remesh_faces(old_pack, s.base.pack, origin_d, s.stream);
+  const bool cyl = s.cfg.geometry == KM_GEOM_CYLINDRICAL;
+  launch_amr_remesh(old_pack, s.base.pack, origin_d, 0, s.nvar, cyl, s.stream);
+  if (s.cfg.mhd == KM_MHD_CT)
+    launch_amr_remesh_faces(old_pack, s.base.pack, origin_d, cyl, s.mesh.xmin[0], s.stream);
   if ((rc = check_launch())) return rc;
   KM_HIP(hipStreamSynchronize(s.stream));
   s.mesh_bufs.swap(fresh);  // the old mesh's buffers go with `fresh`
@@ -805,6 +808,12 @@ int km_ghost_plan_destroy(km_ghost_plan_t *plan) {
 
 int km_ghost_plan_amr(const km_pack_t *shape, const km_mesh_desc_t *mesh,
                       const int32_t *leaves, int32_t nleaves, km_ghost_plan_t **plan) {
+  return km_ghost_plan_amr_geom(shape, mesh, leaves, nleaves, KM_GEOM_CARTESIAN, plan);
+}
+
+int km_ghost_plan_amr_geom(const km_pack_t *shape, const km_mesh_desc_t *mesh,
+                           const int32_t *leaves, int32_t nleaves, int32_t geometry,
+                           km_ghost_plan_t **plan) {
   if (!shape || !mesh || !leaves || !plan) return KM_EINVAL;
   if (nleaves != shape->nblocks || shape->ndim != mesh->ndim || shape->ng != mesh->ng) {
     set_error("pack shape does not match the mesh and its leaves");
@@ -815,12 +824,17 @@ int km_ghost_plan_amr(const km_pack_t *shape, const km_mesh_desc_t *mesh,
       set_error("pack block size does not match the mesh");
       return KM_EINVAL;
     }
+  if (geometry != KM_GEOM_CARTESIAN && geometry != KM_GEOM_CYLINDRICAL) {
+    set_error("dispatch parm [Geometry] not handled");
+    return KM_EINVAL;
+  }
   AmrTreeHost t;
   const std::string err = build_amr_tree(*mesh, leaves, nleaves, t);
   if (!err.empty()) {
     set_error(err);
     return KM_EINVAL;
   }
+  t.cyl = geometry == KM_GEOM_CYLINDRICAL;
   auto p = std::make_unique<km_ghost_plan_t>();
   p->amr = true;
   KM_HIP(hipMalloc(&p->lookup_d, t.lookup.size() * sizeof(int)));
@@ -901,8 +915,8 @@ static int sim_create(const km_cfg_t *cfg, const km_mesh_desc_t *meshes, int32_t
     set_error("the fused stage path is built for one species; more run on the task path");
     return KM_EINVAL;
   }
-  if (cfg->geometry == KM_GEOM_CYLINDRICAL && (path == KM_PATH_FUSED || tree)) {
-    set_error("cylindrical geometry runs on the task path over a uniform block grid");
+  if (cfg->geometry == KM_GEOM_CYLINDRICAL && path == KM_PATH_FUSED) {
+    set_error("cylindrical geometry runs on the task path");
     return KM_EINVAL;
   }
   for (int m = 0; m < ninst; ++m)
@@ -1061,6 +1075,7 @@ int km_sim_create_amr(const km_cfg_t *cfg, const km_mesh_desc_t *mesh, const int
     set_error("constrained transport across refinement levels needs an even nghost");
     return KM_EINVAL;
   }
+  t.cyl = cfg->geometry == KM_GEOM_CYLINDRICAL;
   const int rc = sim_create(cfg, mesh, 1, path, &t, out);
   if (rc == KM_OK) {
     (*out)->leaves.assign(leaves, leaves + 4 * (size_t)nleaves);
@@ -1383,6 +1398,18 @@ int km_amr_tag_loehner(const km_pack_t *u, const km_amr_criterion_t *crit, int32
 
 int km_amr_remesh(const km_pack_t *old_u, const km_pack_t *new_u, const int32_t *origin_dev,
                   int32_t v0, int32_t v1, void *stream) {
+  return km_amr_remesh_geom(old_u, new_u, origin_dev, v0, v1, nullptr, KM_GEOM_CARTESIAN,
+                            stream);
+}
+
+int km_amr_remesh_geom(const km_pack_t *old_u, const km_pack_t *new_u, const int32_t *origin_dev,
+                       int32_t v0, int32_t v1, const km_mesh_desc_t *mesh, int32_t geometry,
+                       void *stream) {
+  const bool cyl = geometry == KM_GEOM_CYLINDRICAL;
+  if ((cyl && !mesh) || (geometry != KM_GEOM_CARTESIAN && !cyl)) {
+    set_error("km_amr_remesh: cylindrical geometry needs the mesh description");
+    return KM_EINVAL;
+  }
   if (!old_u || !new_u || !origin_dev || old_u->ndim != new_u->ndim || old_u->ng != new_u->ng ||
       v0 < 0 || v1 > old_u->nvar || v1 > new_u->nvar) {
     set_error("km_amr_remesh: packs must share ndim, ng, nx and hold the slots");
@@ -1393,10 +1420,12 @@ int km_amr_remesh(const km_pack_t *old_u, const km_pack_t *new_u, const int32_t 
       set_error("km_amr_remesh: packs must share nx");
       return KM_EINVAL;
     }
-  launch_amr_remesh(*old_u, *new_u, origin_dev, v0, v1, as_stream(stream));
+  launch_amr_remesh(*old_u, *new_u, origin_dev, v0, v1, cyl, as_stream(stream));
   bool faces = true;
   for (int d = 0; d < 3; ++d) faces = faces && old_u->face[d] && new_u->face[d];
-  if (faces) launch_amr_remesh_faces(*old_u, *new_u, origin_dev, as_stream(stream));
+  if (faces)
+    launch_amr_remesh_faces(*old_u, *new_u, origin_dev, cyl, cyl ? mesh->xmin[0] : 0.0,
+                            as_stream(stream));
   return check_launch();
 }
 

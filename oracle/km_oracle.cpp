@@ -1142,6 +1142,8 @@ void GhostFill(const km_cfg_t &cfg, const Grid &grid, const km_pack_t &u, int v0
 // ---------------------------------------------------------------------------------
 struct Tree {
   int ndim = 1, nx[3] = {1, 1, 1}, nb0[3] = {1, 1, 1}, lmax = 0, bc[6] = {0, 0, 0, 0, 0, 0};
+  bool cyl = false;   // cylindrical (r, z): geometry-templated refinement operators
+  double x1min = 0.;  // the domain's lower x1 edge
   std::vector<int> level, loc;  // per leaf: level, loc[3*b + d]
   // leaf id by (level, block location) key
   std::vector<std::vector<int>> at;  // at[L][flat block index at level L] (-1: none)
@@ -1160,6 +1162,7 @@ std::string BuildTree(const km_mesh_desc_t &m, const int32_t *leaves, int n, Tre
     t.nb0[d] = d < m.ndim ? m.nb[d] : 1;
   }
   for (int d = 0; d < 6; ++d) t.bc[d] = m.bc[d];
+  t.x1min = m.xmin[0];
   t.lmax = 0;
   for (int b = 0; b < n; ++b) t.lmax = std::max(t.lmax, (int)leaves[4 * b]);
   t.at.assign(t.lmax + 1, {});
@@ -1215,6 +1218,28 @@ void LeafMetric(const km_mesh_desc_t &m, int L, const int l[3], km_block_t &blk)
   blk.vol = blk.dx[0] * blk.dx[1] * blk.dx[2];
   blk.level = L;
 }
+
+// Coordinates<geom> element measures of leaf blk at local index i along x1
+// (geometry.hpp:124-191): cylindrical cell volumes 0.5 2pi (r+^2 - r-^2) dz, r-face areas
+// r dz 2pi, z-face areas 0.5 (r+^2 - r-^2) 2pi; Cartesian ones from the block metric
+double CellVol(bool cyl, const km_block_t &blk, const Shape &sh, int i) {
+  if (!cyl) return blk.vol;
+  const double rp = RFace(blk, sh, i + 1), rm = RFace(blk, sh, i);
+  return 0.5 * kPhiSpan * (rp * rp - rm * rm) * blk.dx[1];
+}
+double FaceArea(bool cyl, const km_block_t &blk, const Shape &sh, int d, int i) {
+  if (!cyl || d == 2) return blk.area[d];
+  if (d == 0) return RFace(blk, sh, i) * blk.dx[1] * kPhiSpan;
+  const double rp = RFace(blk, sh, i + 1), rm = RFace(blk, sh, i);
+  return 0.5 * (rp * rp - rm * rm) * kPhiSpan;
+}
+// radius of the element at global index G (a face, or a cell centre) with spacing h, for
+// the r-weighted face operators: measured from the domain's x1 edge, negative past the
+// axis (the coarse buffer's own coordinates)
+double RadiusAt(const Tree &t, int G, bool centre, double h) {
+  return centre ? t.x1min + (G + 0.5) * h : t.x1min + G * h;
+}
+constexpr double kDblEps = 2.220446049250313e-16;
 
 // composite per-level data of one ghost fill (variable independent part)
 struct Composite {
@@ -1278,7 +1303,9 @@ double RestrictFrom(const Tree &t, const km_pack_t &u, const Composite &c, int L
         const int ch[3] = {2 * z[0] + oi, x2 ? 2 * z[1] + oj : 0, x3 ? 2 * z[2] + ok : 0};
         const long ci = ZoneIndex(t, L + 1, ch);
         const int lf = c.leaf[L + 1][ci];
-        vol[ok][oj][oi] = u.blocks[lf].vol;
+        const Shape sh(u);
+        vol[ok][oj][oi] =
+            CellVol(t.cyl, u.blocks[lf], sh, ch[0] - t.loc[3 * lf] * t.nx[0] + sh.is);
         terms[ok][oj][oi] = vol[ok][oj][oi] * c.val[L + 1][ci];
       }
   const double tvol = ((vol[0][0][0] + vol[0][1][0]) + (vol[0][0][1] + vol[0][1][1])) +
@@ -1326,6 +1353,7 @@ double GradMinMod(double fc, double fm, double fp, double dxm, double dxp) {
   return 0.5 * (Sign(gxm) + Sign(gxp)) * kmin(std::fabs(gxm), std::fabs(gxp));
 }
 
+
 void AmrGhostFill(const km_cfg_t &cfg, const Tree &t, const km_pack_t &u, int v0, int v1) {
   const Shape sh(u);
   const int mhd = cfg.mhd;
@@ -1358,9 +1386,12 @@ void AmrGhostFill(const km_cfg_t &cfg, const Tree &t, const km_pack_t &u, int v0
             if (!ghost) continue;
             int z[3] = {0, 0, 0};
             bool flip[3] = {false, false, false};
-            for (int d = 0; d < t.ndim; ++d)
-              z[d] = map_cell(t.loc[3 * b + d] * t.nx[d] + (idx[d] - st[d]), t.zones(L, d),
-                              t.bc[2 * d], t.bc[2 * d + 1], &flip[d]);
+            for (int d = 0; d < t.ndim; ++d) {
+              const int g0 = t.loc[3 * b + d] * t.nx[d] + (idx[d] - st[d]);
+              z[d] = map_cell(g0, t.zones(L, d), t.bc[2 * d], t.bc[2 * d + 1], &flip[d]);
+              // the axisymmetric mirror also negates a vector's phi component
+              if (d == 0 && g0 < 0 && t.bc[0] == KM_BC_AXISYMMETRIC) flip[2] = !flip[2];
+            }
             double x;
             if (c.depth[L][ZoneIndex(t, L, z)] >= 0) {
               x = ReadR(t, u, c, L, z, &bad);
@@ -1386,6 +1417,7 @@ void AmrGhostFill(const km_cfg_t &cfg, const Tree &t, const km_pack_t &u, int v0
                   int n[3] = {cz[0], cz[1], cz[2]};
                   bool nf[3] = {false, false, false};
                   n[d] += side ? 1 : -1;
+                  if (d == 0 && n[0] < 0 && t.bc[0] == KM_BC_AXISYMMETRIC) nf[2] = !nf[2];
                   n[d] = map_cell(n[d], t.zones(Lc, d), t.bc[2 * d], t.bc[2 * d + 1], &nf[d]);
                   f[side] = ReadR(t, u, c, Lc, n, &bad);
                   if (ax >= 0 && nf[ax]) f[side] = -f[side];
@@ -1489,7 +1521,7 @@ double FaceValue(const Tree &t, const Shape &sh, const km_pack_t &u, int L, int 
     for (int oj = 0; oj < 1 + inc[1]; ++oj)
       for (int oi = 0; oi < 1 + inc[0]; ++oi) {
         const int li[3] = {f.li[0] + oi, f.li[1] + oj, f.li[2] + ok};
-        vol[ok][oj][oi] = u.blocks[f.leaf].area[d];
+        vol[ok][oj][oi] = FaceArea(t.cyl, u.blocks[f.leaf], sh, d, li[0] + sh.is);
         terms[ok][oj][oi] = vol[ok][oj][oi] * FaceAt(sh, u, d, f.leaf, li);
       }
   const double tvol = ((vol[0][0][0] + vol[0][1][0]) + (vol[0][0][1] + vol[0][1][1])) +
@@ -1520,7 +1552,7 @@ bool GhostFaceElem(const Tree &t, const Shape &sh, int d, const int idx[3]) {
 // reference's (kk, jj, ii) loop order is reached by permuting the offsets per component;
 // the coarse spacings enter unpermuted, as the reference writes them.
 double TothRoe(const Shape &sh, const km_pack_t &u, int b, const km_block_t &blk, int d,
-               const int c[3], const int idx[3]) {
+               const int c[3], const int idx[3], bool cyl, double x1min) {
   const int g2 = sh.ndim > 1, g3 = sh.ndim > 2;
   const int fi = c[0], fj = c[1], fk = c[2];
   auto elem = [&](int ok, int oj, int oi, int out[3]) {  // {k, j, i}
@@ -1532,10 +1564,18 @@ double TothRoe(const Shape &sh, const km_pack_t &u, int b, const km_block_t &blk
       out[0] = fk + oi * g3, out[1] = fj + ok * g2, out[2] = fi + oj;
     }
   };
+  // cylindrical (:336-356): the constraint holds for r B; r faces enter times their radius,
+  // z faces times their cell-centre radius, phi faces (1-D and 2-D meshes) times 1
+  auto radius = [&](int comp, int ii) -> double {
+    if (!cyl || comp == 2) return 1.0;
+    const int G = blk.lx[0] * sh.nx[0] + (ii - sh.is);
+    return comp == 1 ? x1min + (G + 0.5) * blk.dx[0] : x1min + G * blk.dx[0];
+  };
   auto fine = [&](int eidx, int ok, int oj, int oi) {
     int q[3];
     elem(ok, oj, oi, q);
-    return facep(u, (d + eidx) % 3, b)[sh.s(q[0], q[1], q[2])];
+    const int comp = (d + eidx) % 3;
+    return facep(u, comp, b)[sh.s(q[0], q[1], q[2])] * radius(comp, q[2]);
   };
   int ok = 0, oj = 0;
   const int o[3] = {idx[0] - fi, idx[1] - fj, idx[2] - fk};
@@ -1561,14 +1601,16 @@ double TothRoe(const Shape &sh, const km_pack_t &u, int b, const km_block_t &blk
         Wxyz += sg(tt) * sg(uu) * sg(v) * fine3;
       }
   Uxx *= 0.125;
-  const double cdx = 2.0 * blk.dx[0], cdy = 2.0 * blk.dx[1], cdz = 2.0 * blk.dx[2];
+  // coarse Dx per axis; Dx along phi is 2 pi in cylindrical geometry (geometry.hpp:35-43)
+  const double cdx = 2.0 * blk.dx[0], cdy = 2.0 * blk.dx[1];
+  const double cdz = cyl ? kPhiSpan : 2.0 * blk.dx[2];
   const double dx2 = cdx * cdx, dy2 = cdy * cdy, dz2 = cdz * cdz;
   Vxyz *= 0.125 * dz2 / (dx2 + dz2);
   Wxyz *= 0.125 * dy2 / (dx2 + dy2);
   double x =
       0.5 * (fine(0, ok, oj, 0) * 1.0 + fine(0, ok, oj, 2) * 1.0) + Uxx + sg(ok) * Vxyz +
       sg(oj) * Wxyz;
-  x *= 1.0;
+  x *= cyl ? 1.0 / (radius(d, idx[0]) + kDblEps) : 1.0;
   return x;
 }
 
@@ -1593,6 +1635,12 @@ void AmrGhostFaces(const km_cfg_t &cfg, const Tree &t, const km_pack_t &u) {
             bool fl[3] = {false, false, false};
             for (int e = 0; e < t.ndim; ++e) {
               const int G0 = t.loc[3 * b + e] * t.nx[e] + (idx[e] - lo[e]);
+              if (e == d && e == 0 && G0 < 0 && t.bc[0] == KM_BC_AXISYMMETRIC) {
+                // axisymmetric: x1 faces mirror like cells (boundary_conditions.cpp:60)
+                G[e] = -G0 - 1;
+                fl[e] = !fl[e];
+                continue;
+              }
               G[e] = e == d ? map_face(G0, t.zones(L, e), t.bc[2 * e], t.bc[2 * e + 1], &fl[e])
                             : map_cell(G0, t.zones(L, e), t.bc[2 * e], t.bc[2 * e + 1], &fl[e]);
             }
@@ -1602,12 +1650,15 @@ void AmrGhostFaces(const km_cfg_t &cfg, const Tree &t, const km_pack_t &u) {
               bad |= 1;
               continue;
             }
+            // cylindrical r and z faces prolongate r B: radii of the coarse elements
+            const bool sr = t.cyl && d < 2;
+            auto rc = [&](int C0) { return sr ? RadiusAt(t, C0, d == 1, 2.0 * blk.dx[0]) : 1.0; };
             double x;
             if (f.lev >= L) {
               x = FaceValue(t, sh, u, L, d, f);
             } else {
               if (d < t.ndim && (G[d] & 1)) continue;  // inside a coarse cell
-              const double fc = FaceAt(sh, u, d, f.leaf, f.li);
+              const double fc = FaceAt(sh, u, d, f.leaf, f.li) * rc(G[0] / 2);
               double a[3] = {0., 0., 0.};
               for (int e = 0; e < t.ndim; ++e) {
                 if (e == d) continue;
@@ -1617,6 +1668,7 @@ void AmrGhostFaces(const km_cfg_t &cfg, const Tree &t, const km_pack_t &u) {
                   bool nf[3] = {false, false, false};
                   for (int q = 0; q < t.ndim; ++q) C[q] = G[q] / 2;
                   C[e] += side ? 1 : -1;
+                  const double rn = rc(C[0]);  // the neighbour's logical radius
                   C[e] = map_cell(C[e], t.zones(L - 1, e), t.bc[2 * e], t.bc[2 * e + 1], &nf[e]);
                   FaceRef s2;
                   if (!FindFace(t, L - 1, d, C, up, &s2) || s2.lev < L - 1) {
@@ -1624,7 +1676,7 @@ void AmrGhostFaces(const km_cfg_t &cfg, const Tree &t, const km_pack_t &u) {
                     fv[side] = 0.;
                     continue;
                   }
-                  fv[side] = FaceValue(t, sh, u, L - 1, d, s2);
+                  fv[side] = FaceValue(t, sh, u, L - 1, d, s2) * rn;
                 }
                 const double dxc = 2.0 * blk.dx[e];
                 a[e] = GradMinMod(fc, fv[0], fv[1], dxc, dxc) * (0.5 * blk.dx[e]);
@@ -1639,6 +1691,7 @@ void AmrGhostFaces(const km_cfg_t &cfg, const Tree &t, const km_pack_t &u) {
               else if (oi && !oj && ok) x = fc + (a[0] - a[1] + a[2]);
               else if (!oi && oj && ok) x = fc - (a[0] - a[1] - a[2]);
               else x = fc + (a[0] + a[1] + a[2]);
+              if (sr) x *= 1.0 / (RadiusAt(t, G[0], d == 1, blk.dx[0]) + kDblEps);
             }
             if (d < t.ndim && fl[d]) x = -x;
             facep(u, d, b)[sh.s(k, j, i)] = x;
@@ -1661,6 +1714,11 @@ void AmrGhostFaces(const km_cfg_t &cfg, const Tree &t, const km_pack_t &u) {
             bool fl[3] = {false, false, false};
             for (int e = 0; e < t.ndim; ++e) {
               const int G0 = t.loc[3 * b + e] * t.nx[e] + (idx[e] - lo[e]);
+              if (e == d && e == 0 && G0 < 0 && t.bc[0] == KM_BC_AXISYMMETRIC) {
+                G[e] = -G0 - 1;
+                fl[e] = !fl[e];
+                continue;
+              }
               G[e] = e == d ? map_face(G0, t.zones(L, e), t.bc[2 * e], t.bc[2 * e + 1], &fl[e])
                             : map_cell(G0, t.zones(L, e), t.bc[2 * e], t.bc[2 * e + 1], &fl[e]);
             }
@@ -1677,7 +1735,7 @@ void AmrGhostFaces(const km_cfg_t &cfg, const Tree &t, const km_pack_t &u) {
               const int off = idx[e] - lo[e];
               c[e] = lo[e] + off - (((off % 2) + 2) % 2);
             }
-            facep(u, d, b)[sh.s(k, j, i)] = TothRoe(sh, u, b, blk, d, c, idx);
+            facep(u, d, b)[sh.s(k, j, i)] = TothRoe(sh, u, b, blk, d, c, idx, t.cyl, t.x1min);
           }
     }
   }
@@ -1744,7 +1802,11 @@ void AmrEmfCorrect(const Tree &t, const km_pack_t &u) {
                   fe[e] = fnode[e] - t.loc[3 * fb + e] * t.nx[e] + lo[e];
               }
               const bool inc[3] = {te == 0, te == 1 && t.ndim > 1, te == 2 && t.ndim > 2};
-              const double len = u.blocks[fb].dx[te];
+              // edge length: dx along the edge, |r| 2 pi for phi edges in cylindrical
+              // geometry (geometry.hpp:143-150)
+              const double len =
+                  (t.cyl && te == 2) ? std::abs(RFace(u.blocks[fb], sh, fe[0])) * kPhiSpan
+                                     : u.blocks[fb].dx[te];
               const double *E = edgep(u, te, fb);
               double vol[2][2][2], terms[2][2][2];
               for (int ok = 0; ok < 2; ++ok)
@@ -1825,7 +1887,7 @@ void AmrFluxCorrect(const km_cfg_t &cfg, const Tree &t, const km_pack_t &u) {
                   fi[tb] = st[tb] + 2 * jb;
                 }
                 const bool inc[3] = {dir != 0, dir != 1 && t.ndim > 1, dir != 2 && t.ndim > 2};
-                const double A = u.blocks[fb].area[dir];
+                const km_block_t &fbk = u.blocks[fb];
                 for (int v = 0; v < nc; ++v) {
                   const double *F = fluxp(u, dir, fb, v);
                   double vol[2][2][2], terms[2][2][2];
@@ -1835,7 +1897,7 @@ void AmrFluxCorrect(const km_cfg_t &cfg, const Tree &t, const km_pack_t &u) {
                   for (int ok = 0; ok < 1 + inc[2]; ++ok)
                     for (int oj = 0; oj < 1 + inc[1]; ++oj)
                       for (int oi = 0; oi < 1 + inc[0]; ++oi) {
-                        vol[ok][oj][oi] = A;
+                        vol[ok][oj][oi] = FaceArea(t.cyl, fbk, sh, dir, fi[0] + oi);
                         terms[ok][oj][oi] =
                             vol[ok][oj][oi] * F[sh.c(fi[2] + ok, fi[1] + oj, fi[0] + oi)];
                       }
@@ -2241,7 +2303,8 @@ bool UpdateTree(const km_mesh_desc_t &m, int numlevel, int threshold,
 }
 
 // RedistributeAndRefineMeshBlocks data movement for cell slots [v0, v1)
-void Remesh(const km_pack_t &o, const km_pack_t &nw, const int32_t *origin, int v0, int v1) {
+void Remesh(const km_pack_t &o, const km_pack_t &nw, const int32_t *origin, int v0, int v1,
+            bool cyl) {
   const Shape sh(nw);
   const int st[3] = {sh.is, sh.js, sh.ks};
   const int nd = sh.ndim;
@@ -2300,13 +2363,13 @@ void Remesh(const km_pack_t &o, const km_pack_t &nw, const int32_t *origin, int 
               }
               const int cb = org[1 + c];
               const double *q = cellp(o, cb, v);
-              const double vol = o.blocks[cb].vol;
               double V[2][2][2], T[2][2][2];
               for (int ok = 0; ok < 2; ++ok)
                 for (int oj = 0; oj < 2; ++oj)
                   for (int oi = 0; oi < 2; ++oi) {
                     V[ok][oj][oi] = T[ok][oj][oi] = 0.0;
                     if ((ok == 0 || nd > 2) && (oj == 0 || nd > 1)) {
+                      const double vol = CellVol(cyl, o.blocks[cb], sh, f[0] + oi);
                       V[ok][oj][oi] = vol;
                       T[ok][oj][oi] = vol * q[sh.c(f[2] + ok, f[1] + oj, f[0] + oi)];
                     }
@@ -2332,7 +2395,11 @@ void Remesh(const km_pack_t &o, const km_pack_t &nw, const int32_t *origin, int 
 // ProlongateInternalTothAndRoe over each coarse cell of the quadrant; a merged parent
 // takes RestrictAverage (:52-121) of each child's faces over the child's quadrant, the
 // children in x1-fastest order, so the upper child writes a face two children share.
-void RemeshFaces(const km_pack_t &o, const km_pack_t &nw, const int32_t *origin) {
+void RemeshFaces(const km_pack_t &o, const km_pack_t &nw, const int32_t *origin, bool cyl,
+                 double x1min) {
+  Tree tr;  // the radii of the r-weighted operators
+  tr.cyl = cyl;
+  tr.x1min = x1min;
   const Shape sh(nw);
   const int nd = sh.ndim;
   const int lo[3] = {sh.is, sh.js, sh.ks};
@@ -2362,12 +2429,18 @@ void RemeshFaces(const km_pack_t &o, const km_pack_t &nw, const int32_t *origin)
             for (int ci = 0; ci < cn[0]; ++ci) {
               const int cc[3] = {ci, cj, ck};
               const long ps = sh.s(q0[2] + ck, q0[1] + cj, q0[0] + ci);
-              const double fc = q[ps];
+              // cylindrical r and z faces prolongate r B (global coarse index along x1)
+              const bool sr = cyl && d < 2;
+              const int C0 = blk.lx[0] * hn[0] + ci;
+              auto rc = [&](int Cq) { return sr ? RadiusAt(tr, Cq, d == 1, 2.0 * blk.dx[0]) : 1.0; };
+              const double fc = q[ps] * rc(C0);
               double a[3] = {0., 0., 0.};
               for (int e = 0; e < nd; ++e) {
                 if (e == d) continue;
                 const double dxc = 2.0 * blk.dx[e];
-                a[e] = GradMinMod(fc, q[ps - ss[e]], q[ps + ss[e]], dxc, dxc) * (0.5 * blk.dx[e]);
+                const double fm = q[ps - ss[e]] * rc(C0 - (e == 0)),
+                             fp = q[ps + ss[e]] * rc(C0 + (e == 0));
+                a[e] = GradMinMod(fc, fm, fp, dxc, dxc) * (0.5 * blk.dx[e]);
               }
               const int ni = (d != 0) ? 2 : 1, nj = (d != 1 && nd > 1) ? 2 : 1,
                         nk = (d != 2 && nd > 2) ? 2 : 1;
@@ -2386,6 +2459,10 @@ void RemeshFaces(const km_pack_t &o, const km_pack_t &nw, const int32_t *origin)
                     int fi[3] = {0, 0, 0};
                     const int off[3] = {oi, oj, ok};
                     for (int e = 0; e < nd; ++e) fi[e] = lo[e] + 2 * cc[e] + off[e];
+                    if (sr) {
+                      const int Gf = blk.lx[0] * n[0] + (fi[0] - lo[0]);
+                      x *= 1.0 / (RadiusAt(tr, Gf, d == 1, blk.dx[0]) + kDblEps);
+                    }
                     f[sh.s(fi[2], fi[1], fi[0])] = x;
                   }
             }
@@ -2405,7 +2482,7 @@ void RemeshFaces(const km_pack_t &o, const km_pack_t &nw, const int32_t *origin)
                     int idx[3] = {c[0] + oi, c[1] + oj, c[2] + ok};
                     idx[d] = c[d] + 1;  // the face inside the coarse cell
                     facep(nw, d, b)[sh.s(idx[2], idx[1], idx[0])] =
-                        TothRoe(sh, nw, b, blk, d, c, idx);
+                        TothRoe(sh, nw, b, blk, d, c, idx, cyl, x1min);
                   }
             }
       continue;
@@ -2413,7 +2490,6 @@ void RemeshFaces(const km_pack_t &o, const km_pack_t &nw, const int32_t *origin)
     // merged parent: each child restricts its faces into the parent's quadrant
     for (int ch = 0; ch < (1 << nd); ++ch) {
       const int cb = org[1 + ch];
-      const double A[3] = {o.blocks[cb].area[0], o.blocks[cb].area[1], o.blocks[cb].area[2]};
       int q0[3] = {0, 0, 0};
       for (int e = 0; e < nd; ++e) q0[e] = lo[e] + ((ch >> e) & 1) * hn[e];
       for (int d = 0; d < 3; ++d) {
@@ -2433,8 +2509,9 @@ void RemeshFaces(const km_pack_t &o, const km_pack_t &nw, const int32_t *origin)
                   for (int oi = 0; oi < 2; ++oi) {
                     V[ok][oj][oi] = T[ok][oj][oi] = 0.0;
                     if ((ok == 0 || inc[2]) && (oj == 0 || inc[1]) && (oi == 0 || inc[0])) {
-                      V[ok][oj][oi] = A[d];
-                      T[ok][oj][oi] = A[d] * q[sh.s(fl[2] + ok, fl[1] + oj, fl[0] + oi)];
+                      const double A = FaceArea(cyl, o.blocks[cb], sh, d, fl[0] + oi);
+                      V[ok][oj][oi] = A;
+                      T[ok][oj][oi] = A * q[sh.s(fl[2] + ok, fl[1] + oj, fl[0] + oi)];
                     }
                   }
               const double tvol = ((V[0][0][0] + V[0][1][0]) + (V[0][0][1] + V[0][1][1])) +
@@ -2610,6 +2687,7 @@ void AdaptStep(km_cpu_sim_t &s) {
   Tree t;
   const std::string err = BuildTree(s.mesh, nl.data(), n, t);
   if (!err.empty()) throw std::runtime_error("remesh produced an invalid tree: " + err);
+  t.cyl = Cyl(s.cfg);
   std::vector<km_block_t> old_blocks = std::move(s.blocks);
   Container old_base = std::move(s.base);
   s.blocks.assign(n, km_block_t{});
@@ -2617,8 +2695,9 @@ void AdaptStep(km_cpu_sim_t &s) {
   s.tree = std::move(t);
   s.nblocks = n;
   MakeContainer(s, s.base, true, nullptr, false);
-  Remesh(old_base.pack, s.base.pack, org.data(), 0, s.nvar);
-  if (s.cfg.mhd == KM_MHD_CT) RemeshFaces(old_base.pack, s.base.pack, org.data());
+  Remesh(old_base.pack, s.base.pack, org.data(), 0, s.nvar, Cyl(s.cfg));
+  if (s.cfg.mhd == KM_MHD_CT)
+    RemeshFaces(old_base.pack, s.base.pack, org.data(), Cyl(s.cfg), s.mesh.xmin[0]);
   for (int i = 0; i < 2; ++i) MakeContainer(s, s.stage[i], false, &s.base, false);
   MakeContainer(s, s.dudt, false, nullptr, true);
   s.leaves = std::move(nl);
@@ -2831,6 +2910,7 @@ int km_cpu_amr_ghost_fill(const km_cfg_t *cfg, const km_pack_t *u, const km_mesh
                           const int32_t *leaves, int32_t n, int32_t v0, int32_t v1) {
   Tree t;
   const std::string err = BuildTree(*mesh, leaves, n, t);
+  t.cyl = Cyl(*cfg);
   if (!err.empty() || n != u->nblocks) {
     std::fprintf(stderr, "km_cpu_amr_ghost_fill: %s\n", err.empty() ? "leaf count" : err.c_str());
     return KM_EINVAL;
@@ -2848,6 +2928,7 @@ int km_cpu_flux_correct(const km_cfg_t *cfg, const km_pack_t *u, const km_mesh_d
                         const int32_t *leaves, int32_t n) {
   Tree t;
   const std::string err = BuildTree(*mesh, leaves, n, t);
+  t.cyl = Cyl(*cfg);
   if (!err.empty() || n != u->nblocks) return KM_EINVAL;
   AmrFluxCorrect(*cfg, t, *u);
   if (u->edge[2] && (u->ndim < 3 || (u->edge[0] && u->edge[1]))) AmrEmfCorrect(t, *u);
@@ -2858,6 +2939,7 @@ int km_cpu_amr_ghost_faces(const km_cfg_t *cfg, const km_pack_t *u, const km_mes
                            const int32_t *leaves, int32_t n) {
   Tree t;
   const std::string err = BuildTree(*mesh, leaves, n, t);
+  t.cyl = Cyl(*cfg);
   if (!err.empty() || n != u->nblocks || !u->face[0] || !u->face[1] || !u->face[2]) return KM_EINVAL;
   try {
     AmrGhostFaces(*cfg, t, *u);
@@ -2879,6 +2961,7 @@ int km_cpu_sim_create_amr(const km_cfg_t *cfg, const km_mesh_desc_t *mesh,
   s->nflux = s->nc + KM_NSPEC_SLOTS(cfg->nspecies);
   s->nstages = cfg->nstages;
   const std::string err = BuildTree(*mesh, leaves, n, s->tree);
+  s->tree.cyl = Cyl(*cfg);
   if (!err.empty() || (cfg->mhd == KM_MHD_CT && mesh->ng % 2)) {
     std::fprintf(stderr, "km_cpu_sim_create_amr: %s\n", err.empty() ? "CT needs even nghost" : err.c_str());
     delete s;
@@ -3010,12 +3093,21 @@ int km_cpu_amr_update_tree(const km_mesh_desc_t *mesh, int32_t numlevel,
   return KM_OK;
 }
 
+int km_cpu_amr_remesh_geom(const km_pack_t *old_u, const km_pack_t *new_u, const int32_t *origin,
+                           int32_t v0, int32_t v1, const km_mesh_desc_t *mesh, int32_t geometry);
 int km_cpu_amr_remesh(const km_pack_t *old_u, const km_pack_t *new_u, const int32_t *origin,
                       int32_t v0, int32_t v1) {
-  Remesh(*old_u, *new_u, origin, v0, v1);
+  return km_cpu_amr_remesh_geom(old_u, new_u, origin, v0, v1, nullptr, KM_GEOM_CARTESIAN);
+}
+
+int km_cpu_amr_remesh_geom(const km_pack_t *old_u, const km_pack_t *new_u, const int32_t *origin,
+                           int32_t v0, int32_t v1, const km_mesh_desc_t *mesh, int32_t geometry) {
+  const bool cyl = geometry == KM_GEOM_CYLINDRICAL;
+  if (cyl && !mesh) return KM_EINVAL;
+  Remesh(*old_u, *new_u, origin, v0, v1, cyl);
   bool faces = true;
   for (int d = 0; d < 3; ++d) faces = faces && old_u->face[d] && new_u->face[d];
-  if (faces) RemeshFaces(*old_u, *new_u, origin);
+  if (faces) RemeshFaces(*old_u, *new_u, origin, cyl, cyl ? mesh->xmin[0] : 0.0);
   return KM_OK;
 }
 

@@ -73,6 +73,7 @@ def lib() -> C.CDLL:
         "km_cpu_amr_update_tree": [P(KmMeshDesc), i32, i32, P(i32), i32, P(i32), P(i32),
                                    P(i32), P(i32), P(i32), P(i32), P(i32)],
         "km_cpu_amr_remesh": [P(KmPack), P(KmPack), P(i32), i32, i32],
+        "km_cpu_amr_remesh_geom": [P(KmPack), P(KmPack), P(i32), i32, i32, P(KmMeshDesc), i32],
         "km_cpu_sim_set_adaptive": [vp, P(KmAmrAdapt)],
         "km_cpu_sim_amr_leaves": [vp, P(i32), P(i32)],
     }
@@ -176,9 +177,13 @@ def amr_update_tree(desc: KmMeshDesc, numlevel: int, derefine_count: int, leaves
     return nl[:k].copy(), nd[:k].copy(), org[:k].copy(), bool(ch.value)
 
 
-def amr_remesh(old: KmPack, new: KmPack, origin: np.ndarray, v0: int, v1: int):
+def amr_remesh(old: KmPack, new: KmPack, origin: np.ndarray, v0: int, v1: int, desc=None,
+               geometry: int = 0):
+    """remesh of cell slots [v0, v1) (and faces when both packs have them); geometry 1
+    (cylindrical) takes the r-weighted operators and needs the mesh description"""
     origin = np.ascontiguousarray(origin, dtype=np.int32)
-    call("km_cpu_amr_remesh", C.byref(old), C.byref(new), _i32(origin), v0, v1)
+    call("km_cpu_amr_remesh_geom", C.byref(old), C.byref(new), _i32(origin), v0, v1,
+         C.byref(desc) if desc is not None else None, geometry)
 
 
 def initial_refinement(cfg, desc: KmMeshDesc, layout: Layout, problem_fill, adapt: KmAmrAdapt,
