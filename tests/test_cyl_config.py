@@ -13,9 +13,17 @@ def test_cylindrical_parses(km):
         km.Config(**{"geometry/geometry": "spherical"})
 
 
-def test_block_trees_refuse_the_axisymmetric_boundary(km):
+def test_block_trees_take_the_axisymmetric_boundary_at_the_inner_x1_edge_only(km):
     lay = Layout(2, 2, (8, 8, 1), (2, 2, 1))
     desc = mesh_desc(lay, bc=("axisymmetric", "outflow", "periodic", "periodic", "periodic",
                               "periodic"))
-    with pytest.raises(km.KmError, match="block-tree meshes take"):
-        km.amr_blocks(desc, km.refine_ball(mesh_desc(lay), 1, (0.5, 0.5, 0.5), 0.1))
+    leaves = km.refine_ball(mesh_desc(lay), 1, (0.5, 0.5, 0.5), 0.1)
+    assert len(km.amr_blocks(desc, leaves)) == len(leaves)
+    bad = mesh_desc(lay, bc=("outflow", "axisymmetric", "periodic", "periodic", "periodic",
+                             "periodic"))
+    with pytest.raises(km.KmError, match="inner x1 boundary"):
+        km.amr_blocks(bad, leaves)
+    lay3 = Layout(3, 2, (8, 8, 8), (2, 2, 2))
+    bad3 = mesh_desc(lay3, bc=("axisymmetric", "outflow") + ("periodic",) * 4)
+    with pytest.raises(km.KmError, match="inner x1 boundary"):
+        km.amr_blocks(bad3, km.refine_ball(mesh_desc(lay3), 1, (0.5, 0.5, 0.5), 0.1))
