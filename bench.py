@@ -223,6 +223,38 @@ def _secondary(km, args):
         ref.set_adaptive(adapt)
         sample_cycles = 1
         cpu_zones = len(leaves) * 16 ** 3
+    elif args.workload == "vortex2d":
+        # the reference's CPU-runnable case: isentropic_vortex.in at 128^2 in 32^2 blocks,
+        # PLM-MC + HLLC, RK2, CFL 0.4; a small mesh where launch latency counts, so both
+        # paths replay one captured graph per cycle
+        cfg = km.Config(**{"hydro/reconstruction": "plm", "hydro/slope_limiter": "mc",
+                           "hydro/riemann": "hllc", "parthenon/time/integrator": "rk2",
+                           "hydro/cfl": 0.4, "material/single/gamma": 1.4})
+        cfg.c.ndim = 2
+        lay = km.Layout(2, 4, (32, 32, 1), (4, 4, 1))
+        desc = km.mesh_desc(lay, xmin=(-5.0,) * 3, xmax=(5.0,) * 3)
+        cell, _ = km.problem_fill("isentropic_vortex", cfg, desc, lay)
+        zones = 128 * 128
+        stage_bytes = 8 * 3 * 5
+        nst = 2
+        # the task path (one lane per face) leads here: 512 pencils are too few lanes
+        # for the fused sweeps
+        sim = km.Simulation(cfg, desc, lay, path=km.PATH_TASKS)
+        sim.set_primitive(cell)
+        dt = _timed(km, sim, args.steps, args.warmup)
+        sim.close()
+        sim = km.Simulation(cfg, desc, lay, path=km.PATH_FUSED)
+        sim.set_primitive(cell)
+        dtf = _timed(km, sim, args.steps, args.warmup)
+        sim.close()
+        out["variants"] = {"fused_path": {"value": zones * args.steps / dtf,
+                                          "ms_per_step": dtf * 1e3 / args.steps}}
+        out["config"] = {"workload": "vortex2d_128c_32b_plm_mc_hllc_rk2_cfl04", "zones": zones,
+                         "block": [32, 32], "nghost": 4, "path": "tasks",
+                         "parallelism": "single GPU"}
+        ref = km_oracle.CpuSimulation(cfg.c, desc, lay)
+        ref.set_primitive(cell)
+        sample_cycles = 200
     elif args.workload == "kh_ensemble":
         sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
         rng = np.random.default_rng(20261207)
@@ -294,7 +326,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--workload", default="hydro3d",
-                    choices=["hydro3d", "mhd3d", "sedov_amr", "sedov_amr_adaptive",
+                    choices=["hydro3d", "mhd3d", "sedov_amr", "sedov_amr_adaptive", "vortex2d",
                              "kh_ensemble"])
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="after the run, copy BYTES between two device buffers with a plain "

@@ -1179,8 +1179,10 @@ int km_sim_set_dt(km_sim_t *s, double dt) {
 int km_sim_advance(km_sim_t *s, int32_t ncycles, double tlim) {
   if (!s) return KM_EINVAL;
   for (int n = 0; n < ncycles; ++n) {
-    const bool need_host =
-        s->path == KM_PATH_TASKS || s->prims_from_ic || s->dt_keep || tlim > 0. || s->hist;
+    // a cycle needs the host when it reads dt there (tlim, an override), writes history,
+    // starts from the initial primitives or may remesh; every other cycle of either path
+    // is one captured graph replayed (the task path keeps dt on the device as well)
+    const bool need_host = s->adaptive || s->prims_from_ic || s->dt_keep || tlim > 0. || s->hist;
     if (need_host) {
       double dt = 0., time = 0.;
       KM_HIP(hipMemcpyAsync(&dt, s->dt_d, sizeof(double), hipMemcpyDeviceToHost, s->stream));
@@ -1211,7 +1213,7 @@ int km_sim_advance(km_sim_t *s, int32_t ncycles, double tlim) {
       }
       continue;
     }
-    // fused path, no host decisions: capture one cycle in a graph and replay it
+    // no host decisions: capture one cycle in a graph and replay it
     if (s->use_graph && !s->profile) {
       if (!s->graph_exec) {
         KM_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeRelaxed));
