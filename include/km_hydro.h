@@ -299,8 +299,6 @@ int km_sim_profile(km_sim_t *sim, int32_t on);
 /* Sum of recorded kernel times (ms) and launch counts per kind: 0..2 = sweep along
  * x1..x3 (the last one also updates u1), 3 = ghost fill, 4 = whole stage. Blocks. */
 int km_sim_kernel_times(km_sim_t *sim, double *ms, int32_t *count, int32_t nkinds);
-/* Threads per workgroup of the fused sweeps (64..256, multiple of 64). */
-int km_sim_set_tile(km_sim_t *sim, int32_t threads);
 /* ---- problem generators (host-side; fill primitives on the interior) -------------- */
 /* names: "isentropic_vortex" (isentropic_vortex.cpp:105-175), "smooth3d", "mhd_random",
  * "kelvin_helmholtz", "sedov" (sedov.cpp:75-143, 3-D form), "mhd_blast"

@@ -76,7 +76,6 @@ def lib() -> C.CDLL:
         "km_sim_set_dt": [vp, dbl],
         "km_sim_profile": [vp, i32],
         "km_sim_kernel_times": [vp, P(dbl), P(i32), i32],
-        "km_sim_set_tile": [vp, i32],
         "km_problem_fill": [C.c_char_p, P(KmCfg), P(KmMeshDesc), u64, P(dbl), i32, vp, P(vp)],
         "km_problem_fill_blocks": [C.c_char_p, P(KmCfg), P(KmMeshDesc), vp, i32, u64, P(dbl),
                                    i32, vp, P(vp)],
@@ -433,9 +432,6 @@ class Simulation:
         cnt = (C.c_int32 * 5)()
         check(lib().km_sim_kernel_times(self.h, ms, cnt, 5), "kernel_times")
         return {k: (ms[i], cnt[i]) for i, k in enumerate(self.KERNEL_KINDS)}
-
-    def set_tile(self, threads: int):
-        check(lib().km_sim_set_tile(self.h, threads), "set_tile")
 
 
 class DevicePack:

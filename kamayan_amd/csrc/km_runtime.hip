@@ -211,7 +211,6 @@ struct km_sim_t {
   int ncycle = 0;
   bool prims_from_ic = false;  // first stage must read the initial primitives
   bool dt_keep = false;
-  int tile = 256;
   bool use_graph = true;
   hipGraphExec_t graph_exec = nullptr;
   hipGraph_t graph = nullptr;
@@ -436,11 +435,11 @@ int fused_stage(km_sim_t &s, int stage, double beta) {
     }
     int rc;
     if (cfg.precision == KM_PREC_FP32_MIXED)
-      rc = fp32::launch_fused_sweep(cfg, a, ndim, dir, last, s.tile, s.stream);
+      rc = fp32::launch_fused_sweep(cfg, a, ndim, dir, last, s.stream);
     else if (cfg.precision == KM_PREC_FP64_FMA)
-      rc = fma::launch_fused_sweep(cfg, a, ndim, dir, last, s.tile, s.stream);
+      rc = fma::launch_fused_sweep(cfg, a, ndim, dir, last, s.stream);
     else
-      rc = exact::launch_fused_sweep(cfg, a, ndim, dir, last, s.tile, s.stream);
+      rc = exact::launch_fused_sweep(cfg, a, ndim, dir, last, s.stream);
     if (rc) return rc;
     if (s.profile) {
       (void)hipEventRecord(e1, s.stream);
@@ -1265,21 +1264,6 @@ int km_sim_kernel_times(km_sim_t *s, double *ms, int32_t *count, int32_t nkinds)
     KM_HIP(hipEventElapsedTime(&t, r.a, r.b));
     ms[r.kind] += t;
     count[r.kind] += 1;
-  }
-  return KM_OK;
-}
-
-int km_sim_set_tile(km_sim_t *s, int32_t threads) {
-  if (!s || threads < 64 || threads > 256 || threads % 64) {
-    set_error("tile must be 64, 128, 192 or 256 threads");
-    return KM_EINVAL;
-  }
-  s->tile = threads;
-  if (s->graph_exec) {
-    (void)hipGraphExecDestroy(s->graph_exec);
-    (void)hipGraphDestroy(s->graph);
-    s->graph_exec = nullptr;
-    s->graph = nullptr;
   }
   return KM_OK;
 }
