@@ -2,6 +2,7 @@
 // benchmark configurations (host-side; they fill primitives on block interiors the way
 // a kamayan ProblemGenerator does before PostMeshInitialization).
 #include <array>
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -82,6 +83,17 @@ struct Mode {
 };
 
 std::vector<Mode> draw_modes(Rng &rng, int n, int kmax, int ndim) {
+  // at most as many modes as there are distinct +-k pairs with 0 < |k| <= kmax (four in
+  // one dimension for kmax 4)
+  int avail = 0;
+  for (int a = -kmax; a <= kmax; ++a)
+    for (int b = ndim > 1 ? -kmax : 0; b <= (ndim > 1 ? kmax : 0); ++b)
+      for (int c = ndim > 2 ? -kmax : 0; c <= (ndim > 2 ? kmax : 0); ++c) {
+        const int k2 = a * a + b * b + c * c;
+        const int first = a != 0 ? a : b != 0 ? b : c;
+        if (k2 > 0 && k2 <= kmax * kmax && first > 0) ++avail;
+      }
+  n = std::min(n, avail);
   std::vector<Mode> modes;
   while ((int)modes.size() < n) {
     Mode m;

@@ -16,9 +16,12 @@
 // captured once in a hipGraph and replayed.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -214,6 +217,15 @@ struct km_sim_t {
   bool use_graph = true;
   hipGraphExec_t graph_exec = nullptr;
   hipGraph_t graph = nullptr;
+  // kernel of the fused path's x1 sweep: -1 picked by the autotuner at the first cycle
+  // that starts from conserved data, 0 the march, k >= 1 row-tile shape k
+  int x1 = -1;
+  void drop_graph() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+  }
   // optional HIP-event timing of the fused-stage kernels (kinds: sweep x1, x2, x3,
   // ghost fill, whole stage)
   bool profile = false;
@@ -400,22 +412,102 @@ int task_stage(km_sim_t &s, int stage, double beta, double dt) {
   return check_launch();
 }
 
-int fused_stage(km_sim_t &s, int stage, double beta) {
-  const km_cfg_t &cfg = s.cfg;
-  Cont &md0 = container(s, stage - 1), &md1 = container(s, stage);
-  const bool last = stage == cfg.nstages;
+FusedArgs fused_args(km_sim_t &s, int stage, double beta) {
   FusedArgs a;
-  a.u0 = md0.pack;
+  a.u0 = container(s, stage - 1).pack;
   a.base = s.base.pack;
-  a.u1 = md1.pack;
+  a.u1 = container(s, stage).pack;
   a.du = s.du;
   a.dt = s.dt_d;
   a.dt_est = s.est_d;
   a.inst = s.inst_d;
   a.nbr = s.nbr_d;
   a.beta = beta;
-  a.lim = cfg.limiter;
-  a.eos = make_eos(cfg);
+  a.lim = s.cfg.limiter;
+  a.eos = make_eos(s.cfg);
+  return a;
+}
+
+int launch_sweep(const km_sim_t &s, const FusedArgs &a, int dir, bool last, int x1) {
+  const km_cfg_t &cfg = s.cfg;
+  const int ndim = s.mesh.ndim;
+  if (cfg.precision == KM_PREC_FP32_MIXED)
+    return fp32::launch_fused_sweep(cfg, a, ndim, dir, last, x1, s.stream);
+  if (cfg.precision == KM_PREC_FP64_FMA)
+    return fma::launch_fused_sweep(cfg, a, ndim, dir, last, x1, s.stream);
+  return exact::launch_fused_sweep(cfg, a, ndim, dir, last, x1, s.stream);
+}
+
+int x1_shapes(const km_cfg_t &cfg) {
+  if (cfg.precision == KM_PREC_FP32_MIXED) return fp32::x1_shapes();
+  if (cfg.precision == KM_PREC_FP64_FMA) return fma::x1_shapes();
+  return exact::x1_shapes();
+}
+
+// Autotuner of the fused x1 sweep. Every candidate kernel (the march and each row-tile
+// shape the arithmetic form offers) runs the first stage's x1 sweep on the simulation's
+// own data, which only writes the du accumulator that the stage's own x1 sweep then
+// rewrites; the fastest of three timed launches per candidate wins. Choices are kept
+// per (form, reconstruction, limiter, Riemann solver, block shape, block count) in a
+// process-wide table, so later simulations of the same shape reuse them.
+std::mutex g_tune_mu;
+std::map<std::vector<int>, int> g_tune;
+
+int autotune_x1(km_sim_t &s) {
+  if (s.x1 >= 0) return KM_OK;
+  const km_cfg_t &cfg = s.cfg;
+  const int nshape = x1_shapes(cfg);
+  if (s.path != KM_PATH_FUSED || s.mesh.ndim < 2 || nshape == 0) {
+    s.x1 = 0;
+    return KM_OK;
+  }
+  const std::vector<int> key = {cfg.precision, cfg.recon, cfg.limiter, cfg.riemann,
+                                s.mesh.ndim, s.mesh.nx[0], s.mesh.nx[1], s.mesh.nx[2], s.nb};
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  auto it = g_tune.find(key);
+  if (it != g_tune.end()) {
+    s.x1 = it->second;
+    return KM_OK;
+  }
+  const FusedArgs a = fused_args(s, 1, 1.0);
+  hipEvent_t e0 = s.ev(), e1 = s.ev();
+  int best = 0;
+  float best_ms = 0.f;
+  for (int x1 = 0; x1 <= nshape; ++x1) {
+    // warm-up; a shape whose LDS does not fit the blocks, or that the device refuses,
+    // is skipped
+    if (int rc = launch_sweep(s, a, 0, false, x1)) {
+      if (x1 == 0) return rc;
+      continue;
+    }
+    if (x1 > 0 && hipGetLastError() != hipSuccess) continue;
+    float ms = 0.f;
+    for (int r = 0; r < 3; ++r) {
+      KM_HIP(hipEventRecord(e0, s.stream));
+      if (int rc = launch_sweep(s, a, 0, false, x1)) return rc;
+      KM_HIP(hipEventRecord(e1, s.stream));
+      KM_HIP(hipEventSynchronize(e1));
+      float t = 0.f;
+      KM_HIP(hipEventElapsedTime(&t, e0, e1));
+      ms = r == 0 ? t : std::min(ms, t);
+    }
+    if (int rc = check_launch()) return rc;
+    if (x1 == 0 || ms < best_ms) {
+      best = x1;
+      best_ms = ms;
+    }
+  }
+  s.ev_free.push_back(e0);
+  s.ev_free.push_back(e1);
+  g_tune[key] = best;
+  s.x1 = best;
+  return KM_OK;
+}
+
+int fused_stage(km_sim_t &s, int stage, double beta) {
+  const km_cfg_t &cfg = s.cfg;
+  const bool last = stage == cfg.nstages;
+  const FusedArgs a = fused_args(s, stage, beta);
   const int ndim = s.mesh.ndim;
   hipEvent_t st0 = nullptr, st1 = nullptr;
   if (s.profile) {
@@ -433,14 +525,7 @@ int fused_stage(km_sim_t &s, int stage, double beta) {
       e1 = s.ev();
       (void)hipEventRecord(e0, s.stream);
     }
-    int rc;
-    if (cfg.precision == KM_PREC_FP32_MIXED)
-      rc = fp32::launch_fused_sweep(cfg, a, ndim, dir, last, s.stream);
-    else if (cfg.precision == KM_PREC_FP64_FMA)
-      rc = fma::launch_fused_sweep(cfg, a, ndim, dir, last, s.stream);
-    else
-      rc = exact::launch_fused_sweep(cfg, a, ndim, dir, last, s.stream);
-    if (rc) return rc;
+    if (int rc = launch_sweep(s, a, dir, last, s.x1 > 0 ? s.x1 : 0)) return rc;
     if (s.profile) {
       (void)hipEventRecord(e1, s.stream);
       s.recs.push_back({dir, e0, e1});
@@ -1178,6 +1263,8 @@ int km_sim_set_dt(km_sim_t *s, double dt) {
 int km_sim_advance(km_sim_t *s, int32_t ncycles, double tlim) {
   if (!s) return KM_EINVAL;
   for (int n = 0; n < ncycles; ++n) {
+    if (!s->prims_from_ic && s->x1 < 0)
+      if (int rc = autotune_x1(*s)) return rc;
     // a cycle needs the host when it reads dt there (tlim, an override), writes history,
     // starts from the initial primitives or may remesh; every other cycle of either path
     // is one captured graph replayed (the task path keeps dt on the device as well)
@@ -1236,6 +1323,25 @@ int km_sim_advance(km_sim_t *s, int32_t ncycles, double tlim) {
     }
   }
   KM_HIP(hipStreamSynchronize(s->stream));
+  return KM_OK;
+}
+
+int km_sim_set_fused_x1(km_sim_t *s, int32_t x1) {
+  if (!s) return KM_EINVAL;
+  if (x1 < -1 || x1 > x1_shapes(s->cfg)) {
+    set_error("fused x1 kernel: -1 (autotune), 0 (march) or a row-tile shape 1.." +
+              std::to_string(x1_shapes(s->cfg)) + " of this arithmetic form");
+    return KM_EINVAL;
+  }
+  KM_HIP(hipStreamSynchronize(s->stream));
+  s->x1 = x1;
+  s->drop_graph();
+  return KM_OK;
+}
+
+int km_sim_fused_x1(km_sim_t *s, int32_t *x1) {
+  if (!s || !x1) return KM_EINVAL;
+  *x1 = s->x1;
   return KM_OK;
 }
 
