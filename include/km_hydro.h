@@ -299,16 +299,16 @@ int km_sim_profile(km_sim_t *sim, int32_t on);
 /* Sum of recorded kernel times (ms) and launch counts per kind: 0..2 = sweep along
  * x1..x3 (the last one also updates u1), 3 = ghost fill, 4 = whole stage. Blocks. */
 int km_sim_kernel_times(km_sim_t *sim, double *ms, int32_t *count, int32_t nkinds);
-/* Kernel of the fused path's x1 sweep in more than one dimension: -1 = chosen by the
- * library's autotuner (the default; it times every candidate on the simulation's own
- * data at the first cycle that starts from conserved values and keeps the choice per
- * arithmetic form, method and block shape for the process), 0 = the row march, k >= 1 =
- * row-tile shape k (the FP32-mixed form offers 4). The reference has no counterpart:
+/* Kernel of the fused path's sweep along x(dir+1) (dir = -1: all three): -1 = chosen by
+ * the library's autotuner (the default; at the first cycle that starts from conserved
+ * values it times every candidate on the simulation's own data and keeps the choice per
+ * arithmetic form, method, block shape and direction for the process), 0 = the pencil
+ * march, k >= 1 = pencil-tile shape k of 4. The reference has no counterpart:
  * CalculateFluxesNested/Scratch (hydro_add_flux_tasks.cpp:29-385) fix their loop shape
  * in source. */
-int km_sim_set_fused_x1(km_sim_t *sim, int32_t x1);
-/* The x1 kernel in use (-1 until the autotuner has run). */
-int km_sim_fused_x1(km_sim_t *sim, int32_t *x1);
+int km_sim_set_fused_kernel(km_sim_t *sim, int32_t dir, int32_t kernel);
+/* The kernel in use for the sweep along x(dir+1) (-1 until the autotuner has run). */
+int km_sim_fused_kernel(km_sim_t *sim, int32_t dir, int32_t *kernel);
 /* ---- problem generators (host-side; fill primitives on the interior) -------------- */
 /* names: "isentropic_vortex" (isentropic_vortex.cpp:105-175), "smooth3d", "mhd_random",
  * "kelvin_helmholtz", "sedov" (sedov.cpp:75-143, 3-D form), "mhd_blast"

@@ -76,8 +76,8 @@ def lib() -> C.CDLL:
         "km_sim_set_dt": [vp, dbl],
         "km_sim_profile": [vp, i32],
         "km_sim_kernel_times": [vp, P(dbl), P(i32), i32],
-        "km_sim_set_fused_x1": [vp, i32],
-        "km_sim_fused_x1": [vp, P(i32)],
+        "km_sim_set_fused_kernel": [vp, i32, i32],
+        "km_sim_fused_kernel": [vp, i32, P(i32)],
         "km_problem_fill": [C.c_char_p, P(KmCfg), P(KmMeshDesc), u64, P(dbl), i32, vp, P(vp)],
         "km_problem_fill_blocks": [C.c_char_p, P(KmCfg), P(KmMeshDesc), vp, i32, u64, P(dbl),
                                    i32, vp, P(vp)],
@@ -426,13 +426,14 @@ class Simulation:
     def profile(self, on: bool = True):
         check(lib().km_sim_profile(self.h, 1 if on else 0), "profile")
 
-    def set_fused_x1(self, x1: int):
-        """x1 sweep kernel of the fused path: -1 autotune, 0 march, k row-tile shape k."""
-        check(lib().km_sim_set_fused_x1(self.h, x1), "set_fused_x1")
+    def set_fused_kernel(self, dir: int, kern: int):
+        """Kernel of the fused sweep along x(dir+1) (dir -1: all): -1 autotune, 0 march,
+        k pencil-tile shape k."""
+        check(lib().km_sim_set_fused_kernel(self.h, dir, kern), "set_fused_kernel")
 
-    def fused_x1(self) -> int:
+    def fused_kernel(self, dir: int) -> int:
         v = C.c_int32()
-        check(lib().km_sim_fused_x1(self.h, C.byref(v)), "fused_x1")
+        check(lib().km_sim_fused_kernel(self.h, dir, C.byref(v)), "fused_kernel")
         return v.value
 
     KERNEL_KINDS = ("sweep_x1", "sweep_x2", "sweep_x3_update", "ghost_fill", "stage")

@@ -100,18 +100,18 @@ void launch_ghost_fill(const UniformGrid &gr, const km_pack_t &u, int v0, int v1
 //   fp32  : FP32 reconstruction/Riemann, FP64 conversion, accumulation and stage combine
 namespace exact {
 int launch_fused_sweep(const km_cfg_t &cfg, const FusedArgs &a, int ndim, int dir, bool with_dt,
-                       int x1, hipStream_t st);
-int x1_shapes();
+                       int kern, hipStream_t st);
+int tile_shapes();
 }
 namespace fma {
 int launch_fused_sweep(const km_cfg_t &cfg, const FusedArgs &a, int ndim, int dir, bool with_dt,
-                       int x1, hipStream_t st);
-int x1_shapes();
+                       int kern, hipStream_t st);
+int tile_shapes();
 }
 namespace fp32 {
 int launch_fused_sweep(const km_cfg_t &cfg, const FusedArgs &a, int ndim, int dir, bool with_dt,
-                       int x1, hipStream_t st);
-int x1_shapes();
+                       int kern, hipStream_t st);
+int tile_shapes();
 }
 
 // thread-local error message for km_last_error()

@@ -217,9 +217,9 @@ struct km_sim_t {
   bool use_graph = true;
   hipGraphExec_t graph_exec = nullptr;
   hipGraph_t graph = nullptr;
-  // kernel of the fused path's x1 sweep: -1 picked by the autotuner at the first cycle
-  // that starts from conserved data, 0 the march, k >= 1 row-tile shape k
-  int x1 = -1;
+  // kernel of each fused sweep: -1 picked by the autotuner at the first cycle that starts
+  // from conserved data, 0 the march, k >= 1 pencil-tile shape k
+  int kern[3] = {-1, -1, -1};
   void drop_graph() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -428,79 +428,85 @@ FusedArgs fused_args(km_sim_t &s, int stage, double beta) {
   return a;
 }
 
-int launch_sweep(const km_sim_t &s, const FusedArgs &a, int dir, bool last, int x1) {
+int launch_sweep(const km_sim_t &s, const FusedArgs &a, int dir, bool last, int kern) {
   const km_cfg_t &cfg = s.cfg;
   const int ndim = s.mesh.ndim;
   if (cfg.precision == KM_PREC_FP32_MIXED)
-    return fp32::launch_fused_sweep(cfg, a, ndim, dir, last, x1, s.stream);
+    return fp32::launch_fused_sweep(cfg, a, ndim, dir, last, kern, s.stream);
   if (cfg.precision == KM_PREC_FP64_FMA)
-    return fma::launch_fused_sweep(cfg, a, ndim, dir, last, x1, s.stream);
-  return exact::launch_fused_sweep(cfg, a, ndim, dir, last, x1, s.stream);
+    return fma::launch_fused_sweep(cfg, a, ndim, dir, last, kern, s.stream);
+  return exact::launch_fused_sweep(cfg, a, ndim, dir, last, kern, s.stream);
 }
 
-int x1_shapes(const km_cfg_t &cfg) {
-  if (cfg.precision == KM_PREC_FP32_MIXED) return fp32::x1_shapes();
-  if (cfg.precision == KM_PREC_FP64_FMA) return fma::x1_shapes();
-  return exact::x1_shapes();
+int tile_shapes(const km_cfg_t &cfg) {
+  if (cfg.precision == KM_PREC_FP32_MIXED) return fp32::tile_shapes();
+  if (cfg.precision == KM_PREC_FP64_FMA) return fma::tile_shapes();
+  return exact::tile_shapes();
 }
 
-// Autotuner of the fused x1 sweep. Every candidate kernel (the march and each row-tile
-// shape the arithmetic form offers) runs the first stage's x1 sweep on the simulation's
-// own data, which only writes the du accumulator that the stage's own x1 sweep then
-// rewrites; the fastest of three timed launches per candidate wins. Choices are kept
-// per (form, reconstruction, limiter, Riemann solver, block shape, block count) in a
-// process-wide table, so later simulations of the same shape reuse them.
+// Autotuner of the fused sweeps. For each direction every candidate kernel (the march
+// and each pencil-tile shape) runs that direction's sweep of the first stage on the
+// simulation's own data; the fastest of three timed launches wins. The sweeps only
+// write scratch the next stage rewrites before reading it: the du accumulator, and u1
+// of stage 1, which is a stage container unless the integrator has one stage. With one
+// stage u1 is the base state, so the final sweep keeps the march there. Choices are kept
+// per (form, reconstruction, limiter, Riemann solver, block shape, block count,
+// direction) in a process-wide table, so later simulations of the same shape reuse them.
 std::mutex g_tune_mu;
 std::map<std::vector<int>, int> g_tune;
 
-int autotune_x1(km_sim_t &s) {
-  if (s.x1 >= 0) return KM_OK;
+int autotune(km_sim_t &s) {
   const km_cfg_t &cfg = s.cfg;
-  const int nshape = x1_shapes(cfg);
-  if (s.path != KM_PATH_FUSED || s.mesh.ndim < 2 || nshape == 0) {
-    s.x1 = 0;
-    return KM_OK;
-  }
-  const std::vector<int> key = {cfg.precision, cfg.recon, cfg.limiter, cfg.riemann,
-                                s.mesh.ndim, s.mesh.nx[0], s.mesh.nx[1], s.mesh.nx[2], s.nb};
+  const int ndim = s.mesh.ndim;
+  const int nshape = tile_shapes(cfg);
   std::lock_guard<std::mutex> lock(g_tune_mu);
-  auto it = g_tune.find(key);
-  if (it != g_tune.end()) {
-    s.x1 = it->second;
-    return KM_OK;
-  }
   const FusedArgs a = fused_args(s, 1, 1.0);
-  hipEvent_t e0 = s.ev(), e1 = s.ev();
-  int best = 0;
-  float best_ms = 0.f;
-  for (int x1 = 0; x1 <= nshape; ++x1) {
-    // warm-up; a shape whose LDS does not fit the blocks, or that the device refuses,
-    // is skipped
-    if (int rc = launch_sweep(s, a, 0, false, x1)) {
-      if (x1 == 0) return rc;
+  for (int dir = 0; dir < 3; ++dir) {
+    if (s.kern[dir] >= 0) continue;
+    const bool final_sweep = dir == ndim - 1;
+    if (s.path != KM_PATH_FUSED || dir >= ndim || (final_sweep && cfg.nstages < 2)) {
+      s.kern[dir] = 0;
       continue;
     }
-    if (x1 > 0 && hipGetLastError() != hipSuccess) continue;
-    float ms = 0.f;
-    for (int r = 0; r < 3; ++r) {
-      KM_HIP(hipEventRecord(e0, s.stream));
-      if (int rc = launch_sweep(s, a, 0, false, x1)) return rc;
-      KM_HIP(hipEventRecord(e1, s.stream));
-      KM_HIP(hipEventSynchronize(e1));
-      float t = 0.f;
-      KM_HIP(hipEventElapsedTime(&t, e0, e1));
-      ms = r == 0 ? t : std::min(ms, t);
+    const std::vector<int> key = {cfg.precision, cfg.recon, cfg.limiter, cfg.riemann, ndim,
+                                  s.mesh.nx[0], s.mesh.nx[1], s.mesh.nx[2], s.nb, dir};
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+      s.kern[dir] = it->second;
+      continue;
     }
-    if (int rc = check_launch()) return rc;
-    if (x1 == 0 || ms < best_ms) {
-      best = x1;
-      best_ms = ms;
+    hipEvent_t e0 = s.ev(), e1 = s.ev();
+    int best = 0;
+    float best_ms = 0.f;
+    for (int k = 0; k <= nshape; ++k) {
+      // warm-up; a shape whose LDS does not fit the blocks, or that the device refuses,
+      // is skipped
+      if (int rc = launch_sweep(s, a, dir, false, k)) {
+        if (k == 0) return rc;
+        continue;
+      }
+      if (k > 0 && hipGetLastError() != hipSuccess) continue;
+      float ms = 0.f;
+      for (int r = 0; r < 3; ++r) {
+        KM_HIP(hipEventRecord(e0, s.stream));
+        if (int rc = launch_sweep(s, a, dir, false, k)) return rc;
+        KM_HIP(hipEventRecord(e1, s.stream));
+        KM_HIP(hipEventSynchronize(e1));
+        float t = 0.f;
+        KM_HIP(hipEventElapsedTime(&t, e0, e1));
+        ms = r == 0 ? t : std::min(ms, t);
+      }
+      if (int rc = check_launch()) return rc;
+      if (k == 0 || ms < best_ms) {
+        best = k;
+        best_ms = ms;
+      }
     }
+    s.ev_free.push_back(e0);
+    s.ev_free.push_back(e1);
+    g_tune[key] = best;
+    s.kern[dir] = best;
   }
-  s.ev_free.push_back(e0);
-  s.ev_free.push_back(e1);
-  g_tune[key] = best;
-  s.x1 = best;
   return KM_OK;
 }
 
@@ -525,7 +531,7 @@ int fused_stage(km_sim_t &s, int stage, double beta) {
       e1 = s.ev();
       (void)hipEventRecord(e0, s.stream);
     }
-    if (int rc = launch_sweep(s, a, dir, last, s.x1 > 0 ? s.x1 : 0)) return rc;
+    if (int rc = launch_sweep(s, a, dir, last, s.kern[dir] > 0 ? s.kern[dir] : 0)) return rc;
     if (s.profile) {
       (void)hipEventRecord(e1, s.stream);
       s.recs.push_back({dir, e0, e1});
@@ -1263,8 +1269,8 @@ int km_sim_set_dt(km_sim_t *s, double dt) {
 int km_sim_advance(km_sim_t *s, int32_t ncycles, double tlim) {
   if (!s) return KM_EINVAL;
   for (int n = 0; n < ncycles; ++n) {
-    if (!s->prims_from_ic && s->x1 < 0)
-      if (int rc = autotune_x1(*s)) return rc;
+    if (!s->prims_from_ic && (s->kern[0] < 0 || s->kern[1] < 0 || s->kern[2] < 0))
+      if (int rc = autotune(*s)) return rc;
     // a cycle needs the host when it reads dt there (tlim, an override), writes history,
     // starts from the initial primitives or may remesh; every other cycle of either path
     // is one captured graph replayed (the task path keeps dt on the device as well)
@@ -1326,22 +1332,23 @@ int km_sim_advance(km_sim_t *s, int32_t ncycles, double tlim) {
   return KM_OK;
 }
 
-int km_sim_set_fused_x1(km_sim_t *s, int32_t x1) {
+int km_sim_set_fused_kernel(km_sim_t *s, int32_t dir, int32_t kern) {
   if (!s) return KM_EINVAL;
-  if (x1 < -1 || x1 > x1_shapes(s->cfg)) {
-    set_error("fused x1 kernel: -1 (autotune), 0 (march) or a row-tile shape 1.." +
-              std::to_string(x1_shapes(s->cfg)) + " of this arithmetic form");
+  if (dir < -1 || dir > 2 || kern < -1 || kern > tile_shapes(s->cfg)) {
+    set_error("fused sweep kernel: direction 0..2 (or -1 for all) and kernel -1 (autotune), "
+              "0 (march) or a pencil-tile shape 1.." + std::to_string(tile_shapes(s->cfg)));
     return KM_EINVAL;
   }
   KM_HIP(hipStreamSynchronize(s->stream));
-  s->x1 = x1;
+  for (int d = 0; d < 3; ++d)
+    if (dir < 0 || d == dir) s->kern[d] = kern;
   s->drop_graph();
   return KM_OK;
 }
 
-int km_sim_fused_x1(km_sim_t *s, int32_t *x1) {
-  if (!s || !x1) return KM_EINVAL;
-  *x1 = s->x1;
+int km_sim_fused_kernel(km_sim_t *s, int32_t dir, int32_t *kern) {
+  if (!s || !kern || dir < 0 || dir > 2) return KM_EINVAL;
+  *kern = s->kern[dir];
   return KM_OK;
 }
 

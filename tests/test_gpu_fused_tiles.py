@@ -1,5 +1,5 @@
-"""The fused path's x1 sweep kernels: the row march and every row-tile shape must give
-the same stage, and the autotuner's pick must be one of them. In the FP64 form every
+"""The fused path's sweep kernels: the pencil march and every pencil-tile shape, in each
+direction, must give the same stage, and the autotuner's picks must be among them. In the FP64 form every
 shape is compared with the oracle bit for bit. In the FP32-mixed form each shape is
 compared with the march (relative L1 per conserved variable below 1e-6: both evaluate
 the same FP32 expressions per value, so only the compiler's contraction choices may
@@ -12,16 +12,21 @@ from tests import _cases
 
 pytestmark = pytest.mark.gpu
 
-NSHAPES = 4  # row-tile shapes
-SHAPES = [(4, 256), (8, 256), (8, 512), (16, 1024)]  # (rows, lanes) per workgroup
+NSHAPES = 4  # pencil-tile shapes
+SHAPES = [(4, 256), (8, 256), (8, 512), (16, 1024)]  # (pencils, lanes) per workgroup
+INFO_BYTES = 176  # per-pencil table of a tile workgroup
 WIDTH = {"wenoz": 2, "ppm": 2, "plm": 1, "fog": 0}
 
 
-def _fits(shape, n0, recon, size):
+def _fits(shape, lay, recon, size):
+    """Whether tile shape `shape` fits every direction's pencils in LDS."""
     rows = SHAPES[shape - 1][0]
     w = WIDTH[recon]
-    floats = rows * (7 * (n0 + 2 + 2 * w) + 7 * (n0 + 2) + 5 * (n0 + 1))
-    return floats * size + 72 * rows <= 160 * 1024
+    for n in lay.nx[:lay.ndim]:
+        vals = rows * (7 * (n + 2 + 2 * w) + 7 * (n + 2) + 5 * (n + 1))
+        if vals * size + INFO_BYTES * rows > 160 * 1024:
+            return False
+    return True
 
 
 def _case(kind, precision):
@@ -54,17 +59,23 @@ KINDS = ["smooth3d-wenoz-hllc", "smooth3d-ppm-hll-64", "vortex2d-plm-mc-hllc",
          "wenoz-12x8x8-reflect-outflow"]
 
 
-def _run(km, case, x1, ncyc):
+def _run(km, case, kern, ncyc):
+    """kern: one kernel for every direction, a tuple per direction, or None (autotune)."""
     cfg, desc, lay, cell, faces = case
     sim = km.Simulation(cfg, desc, lay, path=km.PATH_FUSED)
-    if x1 is not None:
-        sim.set_fused_x1(x1)
-    sim.set_primitive(cell, faces)
-    sim.advance(ncyc)
-    g, _ = sim.state()
-    info = sim.info()
-    used = sim.fused_x1()
-    sim.close()
+    if isinstance(kern, tuple):
+        for d, k in enumerate(kern):
+            sim.set_fused_kernel(d, k)
+    elif kern is not None:
+        sim.set_fused_kernel(-1, kern)
+    try:
+        sim.set_primitive(cell, faces)
+        sim.advance(ncyc)
+        g, _ = sim.state()
+        info = sim.info()
+        used = tuple(sim.fused_kernel(d) for d in range(3))
+    finally:
+        sim.close()
     return g, info, used
 
 
@@ -73,7 +84,7 @@ def _recon(cfg):
 
 
 @pytest.mark.parametrize("kind", KINDS)
-def test_x1_shapes_fp64_bitwise(gpu, oracle, kind):
+def test_tile_shapes_fp64_bitwise(gpu, oracle, kind):
     import kamayan_amd as km
     case = _case(kind, "fp64")
     cfg, desc, lay, cell, faces = case
@@ -84,20 +95,24 @@ def test_x1_shapes_fp64_bitwise(gpu, oracle, kind):
     r, _ = ref.state()
     rinfo = ref.info()
     sl = (slice(None), slice(0, 5)) + lay.interior()
-    for shape in [0] + list(range(1, NSHAPES + 1)) + [-1]:
-        print(f"[x1 rows fp64] {kind} shape {shape}", flush=True)
-        if shape > 0 and not _fits(shape, lay.nx[0], _recon(cfg), 8):
+    for shape in [0] + list(range(1, NSHAPES + 1)) + [(1, 3, 2), (2, 0, 1), None]:
+        print(f"[tiles fp64] {kind} kernels {shape}", flush=True)
+        if isinstance(shape, int) and shape > 0 and not _fits(shape, lay, _recon(cfg), 8):
             with pytest.raises(km.KmError):
                 _run(gpu, case, shape, ncyc)
             continue
         g, info, used = _run(gpu, case, shape, ncyc)
-        assert used == shape or (shape == -1 and 0 <= used <= NSHAPES)
+        if shape is None:
+            assert all(0 <= u <= NSHAPES for u in used)
+        else:
+            want = shape if isinstance(shape, tuple) else (shape,) * 3
+            assert used[:lay.ndim] == want[:lay.ndim]
         assert info == rinfo
         assert np.array_equal(g[sl], r[sl]), (shape, float(np.max(np.abs(g[sl] - r[sl]))))
 
 
 @pytest.mark.parametrize("kind", KINDS)
-def test_x1_shapes_fp32_mixed(gpu, oracle, kind):
+def test_tile_shapes_fp32_mixed(gpu, oracle, kind):
     case = _case(kind, "fp32_mixed")
     cfg, desc, lay, cell, faces = case
     ncyc = 3
@@ -106,14 +121,14 @@ def test_x1_shapes_fp32_mixed(gpu, oracle, kind):
     ref.advance(ncyc)
     r, _ = ref.state()
     march, _, used = _run(gpu, case, 0, ncyc)
-    assert used == 0
+    assert used == (0, 0, 0)
     l1 = _cases.rel_l1(march, r, lay, 5)
     assert np.all(l1 < 1e-5), ("march vs oracle", l1)
-    for shape in range(1, NSHAPES + 1):
-        print(f"[x1 rows fp32] {kind} shape {shape}", flush=True)
-        assert _fits(shape, lay.nx[0], _recon(cfg), 4)
+    for shape in list(range(1, NSHAPES + 1)) + [(3, 1, 4)]:
+        print(f"[tiles fp32] {kind} kernels {shape}", flush=True)
         g, _, used = _run(gpu, case, shape, ncyc)
-        assert used == shape
+        want = shape if isinstance(shape, tuple) else (shape,) * 3
+        assert used[:lay.ndim] == want[:lay.ndim]
         sl = (slice(None), slice(0, 5)) + lay.interior()
         assert np.all(np.isfinite(g[sl]))
         d = _cases.rel_l1(g, march, lay, 5)
@@ -121,29 +136,37 @@ def test_x1_shapes_fp32_mixed(gpu, oracle, kind):
         l1 = _cases.rel_l1(g, r, lay, 5)
         assert np.all(l1 < 1e-5), (shape, l1)
     auto, _, used = _run(gpu, case, None, ncyc)
-    assert 0 <= used <= NSHAPES
+    assert all(0 <= u <= NSHAPES for u in used)
     assert np.all(_cases.rel_l1(auto, march, lay, 5) < 1e-6)
 
 
-def test_x1_choice_refusals(gpu):
+def test_kernel_choice_refusals_and_one_dimension(gpu, oracle):
     import kamayan_amd as km
     case = _cases.smooth3d(n=16, nblock=8, precision="fp32_mixed")
     cfg, desc, lay, cell, faces = case
     sim = gpu.Simulation(cfg, desc, lay, path=gpu.PATH_FUSED)
     with pytest.raises(km.KmError):
-        sim.set_fused_x1(NSHAPES + 1)
+        sim.set_fused_kernel(0, NSHAPES + 1)
     with pytest.raises(km.KmError):
-        sim.set_fused_x1(-2)
+        sim.set_fused_kernel(0, -2)
+    with pytest.raises(km.KmError):
+        sim.set_fused_kernel(3, 1)
     sim.close()
-    # one dimension: x1 is the final sweep and always the march
+    # one dimension, RK3: x1 is both the first and the final sweep; every shape is
+    # bitwise the oracle in FP64
     cfg = km.Config(**{"hydro/reconstruction": "wenoz", "hydro/riemann": "hllc",
                        "parthenon/time/integrator": "rk3", "material/single/gamma": 1.4})
     cfg.c.ndim = 1
     lay = km.Layout(1, 4, (32, 1, 1), (4, 1, 1))
     desc = km.mesh_desc(lay)
     cell, faces = km.problem_fill("smooth3d", cfg, desc, lay, seed=3)
-    sim = gpu.Simulation(cfg, desc, lay, path=gpu.PATH_FUSED)
-    sim.set_primitive(cell, faces)
-    sim.advance(2)
-    assert sim.fused_x1() == 0
-    sim.close()
+    ref = oracle.CpuSimulation(cfg.c, desc, lay)
+    ref.set_primitive(cell, faces)
+    ref.advance(3)
+    r, _ = ref.state()
+    sl = (slice(None), slice(0, 5)) + lay.interior()
+    for shape in range(0, NSHAPES + 1):
+        g, info, used = _run(gpu, (cfg, desc, lay, cell, faces), shape, 3)
+        assert used[0] == shape
+        assert info == ref.info()
+        assert np.array_equal(g[sl], r[sl]), shape
