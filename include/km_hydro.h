@@ -303,7 +303,7 @@ int km_sim_kernel_times(km_sim_t *sim, double *ms, int32_t *count, int32_t nkind
  * the library's autotuner (the default; at the first cycle that starts from conserved
  * values it times every candidate on the simulation's own data and keeps the choice per
  * arithmetic form, method, block shape and direction for the process), 0 = the pencil
- * march, k >= 1 = pencil-tile shape k of 4. The reference has no counterpart:
+ * march, k >= 1 = pencil-tile shape k of 5. The reference has no counterpart:
  * CalculateFluxesNested/Scratch (hydro_add_flux_tasks.cpp:29-385) fix their loop shape
  * in source. */
 int km_sim_set_fused_kernel(km_sim_t *sim, int32_t dir, int32_t kernel);

@@ -12,8 +12,8 @@ from tests import _cases
 
 pytestmark = pytest.mark.gpu
 
-NSHAPES = 4  # pencil-tile shapes
-SHAPES = [(4, 256), (8, 256), (8, 512), (16, 1024)]  # (pencils, lanes) per workgroup
+NSHAPES = 5  # pencil-tile shapes
+SHAPES = [(4, 256), (8, 256), (8, 512), (16, 1024), (32, 1024)]  # (pencils, lanes)
 INFO_BYTES = 176  # per-pencil table of a tile workgroup
 WIDTH = {"wenoz": 2, "ppm": 2, "plm": 1, "fog": 0}
 
@@ -23,7 +23,7 @@ def _fits(shape, lay, recon, size):
     rows = SHAPES[shape - 1][0]
     w = WIDTH[recon]
     for n in lay.nx[:lay.ndim]:
-        vals = rows * (7 * (n + 2 + 2 * w) + 7 * (n + 2) + 5 * (n + 1))
+        vals = rows * (7 * (n + 2 + 2 * w) + 7 * (n + 2))
         if vals * size + INFO_BYTES * rows > 160 * 1024:
             return False
     return True
