@@ -25,14 +25,43 @@ N, NB, NG = 256, 64, 4
 SEED = 20261204
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_ZONE_STAGE = 8 * 3 * 5  # stage input + RK base read, stage output written
-# HBM bytes per RK stage of this workload measured with rocprofv3 FETCH_SIZE / WRITE_SIZE,
-# calibrated against the library's 1 GiB copy kernel (bench.py --calibrate-copy); the
-# summary is profiles/fused_stage_traffic_hydro3d_256c_wenoz_hllc_rk3.md
-MEASURED_TRAFFIC_PER_STAGE = 9.10e9
-# VALU instructions per zone per stage (SQ_INSTS_VALU x 64 / zones, same profile): x1 + x2
-# + the RK3 average of the two x3 forms; peak issue rate of FP64 VALU work on MI355X:
-# 256 CUs x 64 lanes x 2.4 GHz (78.6 TFLOP/s FP64 vector = one FMA per lane per clock)
-VALU_INSTR_PER_ZONE_STAGE = 1497 + 1433 + (2 * 1745 + 1866) / 3
+# Per-launch figures of each FP64 sweep kernel on this workload, measured with rocprofv3
+# (profiles/fused_stage_traffic_hydro3d_256c_wenoz_hllc_rk3.md): HBM bytes read
+# (FETCH_SIZE x 2, the factor a 1 GiB copy by the library's copy kernel calibrates) and
+# written (WRITE_SIZE x 1), and VALU instructions per zone (SQ_INSTS_VALU x 64 / zones).
+# The final sweep has a plain form and, on the last RK stage, the CFL-minimum form.
+# Kernels are named as _kernel_names() reports the autotuner's choices.
+MEASURED = {
+    "x1": {"march": (2.3494e9 + 0.8032e9, 1535.8),
+           "tile 4 pencils x 256 lanes": (1.0908e9 + 0.6711e9, 1972.8),
+           "tile 8 pencils x 256 lanes": (1.0908e9 + 0.6711e9, 1734.4),
+           "tile 8 pencils x 512 lanes": (1.0908e9 + 0.6711e9, 1773.4),
+           "tile 16 pencils x 1024 lanes": (1.0908e9 + 0.6711e9, 1683.1)},
+    "x2": {"march": (1.6589e9 + 0.9476e9, 1433.3),
+           "tile 4 pencils x 256 lanes": (1.4997e9 + 0.6711e9, 1921.2),
+           "tile 8 pencils x 256 lanes": (1.4997e9 + 0.6711e9, 1672.1),
+           "tile 8 pencils x 512 lanes": (1.4997e9 + 0.6711e9, 1724.1),
+           "tile 16 pencils x 1024 lanes": (1.4997e9 + 0.6711e9, 1633.8)},
+    "x3": {"march": (2.4187e9 + 0.6711e9, 1745.3)},
+}
+MEASURED_X3_DT = {"march": (2.8539e9 + 0.6712e9, 1865.8)}
+
+
+def _stage_figures(kern):
+    """(HBM bytes, VALU instructions per zone) of one RK3 stage with the given sweep
+    kernels (two plain final sweeps and one CFL-minimum final sweep per cycle), or None
+    for a kernel without measurements."""
+    try:
+        x1, x2 = MEASURED["x1"][kern["x1"]], MEASURED["x2"][kern["x2"]]
+        x3p, x3d = MEASURED["x3"][kern["x3"]], MEASURED_X3_DT[kern["x3"]]
+    except KeyError:
+        return None
+    x3 = [(2 * x3p[i] + x3d[i]) / 3 for i in range(2)]
+    return x1[0] + x2[0] + x3[0], x1[1] + x2[1] + x3[1]
+
+
+# peak issue rate of FP64 VALU work on MI355X: 256 CUs x 64 lanes x 2.4 GHz (78.6 TFLOP/s
+# FP64 vector = one FMA per lane per clock)
 VALU_PEAK_LANE_INSTR_PER_S = 256 * 64 * 2.4e9
 
 
@@ -47,11 +76,25 @@ def _setup(km, precision="fp64", n=N, nb=NB):
     return cfg, desc, lay, cell
 
 
+# pencil-tile shapes of the fused sweeps (pencils, lanes per workgroup), numbered from 1;
+# 0 is the pencil march
+TILE_SHAPES = [(4, 256), (8, 256), (8, 512), (16, 1024), (32, 1024)]
+
+
+def _kernel_names(sim, ndim):
+    names = {}
+    for d in range(ndim):
+        k = sim.fused_kernel(d)
+        names[f"x{d + 1}"] = ("march" if k <= 0 else
+                              f"tile {TILE_SHAPES[k - 1][0]} pencils x {TILE_SHAPES[k - 1][1]} lanes")
+    return names
+
+
 def _run_gpu(km, cell, precision, steps, warmup, profile):
     cfg, desc, lay, _ = _setup(km, precision)
     sim = km.Simulation(cfg, desc, lay, path=km.PATH_FUSED)
     sim.set_primitive(cell)
-    sim.advance(warmup)
+    sim.advance(warmup)  # the autotuner picks each sweep's kernel in the second cycle
     if profile:
         sim.profile(True)
     km.check(km.lib().km_synchronize(), "sync")
@@ -61,8 +104,9 @@ def _run_gpu(km, cell, precision, steps, warmup, profile):
     t1 = time.perf_counter()
     times = sim.kernel_times() if profile else None
     state, _ = sim.state()
+    kern = _kernel_names(sim, 3)
     sim.close()
-    return t1 - t0, times, state, lay
+    return t1 - t0, times, state, lay, kern
 
 
 def _rel_l1(a, b, lay):
@@ -109,10 +153,10 @@ def _mhd_setup(km, precision):
     return cfg, desc, lay, cell, faces
 
 
-def _sedov_setup(km):
+def _sedov_setup(km, precision="fp64"):
     cfg = km.Config(**{"hydro/reconstruction": "plm", "hydro/slope_limiter": "mc",
                        "hydro/riemann": "hllc", "parthenon/time/integrator": "rk2",
-                       "material/single/gamma": 1.4})
+                       "material/single/gamma": 1.4, "kamayan_amd/precision": precision})
     cfg.c.ndim = 3
     root = km.Layout(3, 4, (16, 16, 16), (4, 4, 4))
     desc = km.mesh_desc(root, xmin=(-0.5,) * 3, xmax=(0.5,) * 3, bc=("outflow",) * 6)
@@ -180,7 +224,17 @@ def _secondary(km, args):
         sim = km.Simulation(cfg, desc, lay, path=km.PATH_TASKS, leaves=leaves)
         sim.set_primitive(cell)
         dt = _timed(km, sim, args.steps, args.warmup)
+        s64, _ = sim.state()
         sim.close()
+        cfg32, _, _, _, _ = _sedov_setup(km, "fp32_mixed")
+        sim = km.Simulation(cfg32, desc, lay, path=km.PATH_TASKS, leaves=leaves)
+        sim.set_primitive(cell)
+        dt32 = _timed(km, sim, args.steps, args.warmup)
+        s32, _ = sim.state()
+        sim.close()
+        out["variants"] = {"fp32_mixed": {"value": zones * args.steps / dt32,
+                                          "speedup_vs_fp64": dt / dt32,
+                                          "rel_l1_vs_fp64_per_var": _rel_l1(s32, s64, lay)}}
         lv = np.bincount(leaves[:, 0], minlength=3)
         out["config"] = {"workload": "sedov3d_amr_root64c_16b_3lev_plm_mc_hllc_rk2",
                          "zones": zones, "leaves": int(len(leaves)),
@@ -280,10 +334,24 @@ def _secondary(km, args):
         sim = km.Simulation(cfg, descs[0], lay, path=km.PATH_FUSED, descs=descs)
         sim.set_primitive(cell)
         dt = _timed(km, sim, args.steps, args.warmup)
+        s64, _ = sim.state()
+        kern64 = _kernel_names(sim, 2)
         sim.close()
+        cfg.set("kamayan_amd/precision", "fp32_mixed")
+        sim = km.Simulation(cfg, descs[0], lay, path=km.PATH_FUSED, descs=descs)
+        sim.set_primitive(cell)
+        dt32 = _timed(km, sim, args.steps, args.warmup)
+        s32, _ = sim.state()
+        kern32 = _kernel_names(sim, 2)
+        sim.close()
+        cfg.set("kamayan_amd/precision", "fp64")
+        out["variants"] = {"fp32_mixed": {"value": zones * args.steps / dt32,
+                                          "speedup_vs_fp64": dt / dt32,
+                                          "rel_l1_vs_fp64_per_var": _rel_l1(s32, s64, lay),
+                                          "sweep_kernels": kern32}}
         out["config"] = {"workload": "kh2d_ensemble_256runs_64to512_32b_wenoz_hllc_rk3",
                          "zones": zones, "runs": len(res), "block": [32, 32], "nghost": 4,
-                         "path": "fused", "parallelism": "single GPU"}
+                         "path": "fused", "sweep_kernels": kern64, "parallelism": "single GPU"}
         # CPU sample: the largest member alone
         big = int(np.argmax(res))
         lay1 = km.Layout(2, 4, (32, 32, 1), (res[big] // 32, res[big] // 32, 1))
@@ -333,8 +401,9 @@ def main() -> int:
                          "copy kernel (the reference for scaling HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the "
-              "sharding design is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: this build "
+              "runs on a single GPU (the path would shard by MeshBlock ranges, with face-"
+              "neighbour slabs exchanged before each stage)", file=sys.stderr)
         return 2
     import kamayan_amd as km
     km.lib()
@@ -346,13 +415,16 @@ def main() -> int:
 
     _, _, lay, cell = _setup(km)
     zones = N ** 3
-    dt64, times, s64, lay = _run_gpu(km, cell, "fp64", args.steps, args.warmup, profile=True)
+    dt64, times, s64, lay, kern64 = _run_gpu(km, cell, "fp64", args.steps, args.warmup,
+                                             profile=True)
     value = zones * args.steps / dt64
     st_ms, st_n = times["stage"]
     stage_ms = st_ms / max(st_n, 1)
     achieved = BYTES_PER_ZONE_STAGE * zones / (stage_ms * 1e-3) / 1e9
     kernels = {k: {"avg_ms": (ms / n if n else 0.0), "launches": n}
-               for k, (ms, n) in times.items()}
+               for k, (ms, n) in times.items() if n}
+    fig = _stage_figures(kern64)
+    traffic, valu_zone = fig if fig else (None, None)
     out = {
         "metric": "zone-cycles/s on 1 MI355X, FP64 vs FP32-mixed at equal L1 error; % HBM peak",
         "value": value,
@@ -369,27 +441,30 @@ def main() -> int:
         "config": {"workload": WORKLOAD, "zones": zones, "block": [NB] * 3, "nghost": NG,
                    "reconstruction": "wenoz", "riemann": "hllc", "integrator": "rk3",
                    "cfl": 0.8, "gamma": 5.0 / 3.0, "seed": SEED, "path": "fused",
-                   "parallelism": "single GPU"},
+                   "sweep_kernels": kern64, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC_PER_STAGE,
-                     "kernel": "one RK stage: sweep_x1 + sweep_x2 + sweep_x3_update + ghost_fill",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "one RK stage: sweep_x1 + sweep_x2 + sweep_x3_update (the fused "
+                               "path needs no ghost exchange on a single-level mesh)",
                      "algorithmic_bytes_per_launch": BYTES_PER_ZONE_STAGE * zones,
                      "avg_launch_ms": stage_ms},
         "kernels": kernels,
-        "valu": {"bound": "fp64 valu issue",
-                 "achieved": VALU_INSTR_PER_ZONE_STAGE * zones / (stage_ms * 1e-3),
-                 "peak": VALU_PEAK_LANE_INSTR_PER_S, "unit": "lane-instr/s",
-                 "frac": VALU_INSTR_PER_ZONE_STAGE * zones / (stage_ms * 1e-3)
-                 / VALU_PEAK_LANE_INSTR_PER_S,
-                 "instr_per_zone_stage": VALU_INSTR_PER_ZONE_STAGE},
     }
+    if valu_zone:
+        rate = valu_zone * zones / (stage_ms * 1e-3)
+        out["valu"] = {"bound": "fp64 valu issue", "achieved": rate,
+                       "peak": VALU_PEAK_LANE_INSTR_PER_S, "unit": "lane-instr/s",
+                       "frac": rate / VALU_PEAK_LANE_INSTR_PER_S,
+                       "instr_per_zone_stage": valu_zone}
     if not args.no_variants:
         var = {}
         for prec in ("fp32_mixed", "fp64_fma"):
-            dtv, _, sv, _ = _run_gpu(km, cell, prec, args.steps, args.warmup, profile=False)
+            dtv, _, sv, _, kv = _run_gpu(km, cell, prec, args.steps, args.warmup,
+                                         profile=False)
             var[prec] = {"value": zones * args.steps / dtv,
                          "speedup_vs_fp64": dt64 / dtv,
-                         "rel_l1_vs_fp64_per_var": _rel_l1(sv, s64, lay)}
+                         "rel_l1_vs_fp64_per_var": _rel_l1(sv, s64, lay),
+                         "sweep_kernels": kv}
         out["variants"] = var
     if not args.no_cpu:
         cpu = _cpu_baseline(km, cell, args.cpu_cycles)
