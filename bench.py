@@ -228,15 +228,23 @@ def _secondary(km, args):
         dt = _timed(km, sim, args.steps, args.warmup)
         s64, _ = sim.state()
         sim.close()
-        cfg32, _, _, _, _ = _sedov_setup(km, "fp32_mixed")
-        sim = km.Simulation(cfg32, desc, lay, path=km.PATH_TASKS, leaves=leaves)
-        sim.set_primitive(cell)
-        dt32 = _timed(km, sim, args.steps, args.warmup)
-        s32, _ = sim.state()
-        sim.close()
-        out["variants"] = {"fp32_mixed": {"value": zones * args.steps / dt32,
-                                          "speedup_vs_fp64": dt / dt32,
-                                          "rel_l1_vs_fp64_per_var": _rel_l1(s32, s64, lay)}}
+        var = {}
+        # the same tree on the task path in FP32-mixed, and on the fused path (pencil
+        # tiles over every leaf in one launch per direction, autotuned) in both forms
+        for name, path, prec in (("tasks_fp32_mixed", km.PATH_TASKS, "fp32_mixed"),
+                                 ("fused_fp64", km.PATH_FUSED, "fp64"),
+                                 ("fused_fp32_mixed", km.PATH_FUSED, "fp32_mixed")):
+            cfgv, _, _, _, _ = _sedov_setup(km, prec)
+            sim = km.Simulation(cfgv, desc, lay, path=path, leaves=leaves)
+            sim.set_primitive(cell)
+            dtv = _timed(km, sim, args.steps, args.warmup)
+            sv, _ = sim.state()
+            var[name] = {"value": zones * args.steps / dtv, "speedup_vs_fp64_tasks": dt / dtv,
+                         "rel_l1_vs_fp64_per_var": _rel_l1(sv, s64, lay)}
+            if path == km.PATH_FUSED:
+                var[name]["sweep_kernels"] = _kernel_names(sim, 3)
+            sim.close()
+        out["variants"] = var
         lv = np.bincount(leaves[:, 0], minlength=3)
         out["config"] = {"workload": "sedov3d_amr_root64c_16b_3lev_plm_mc_hllc_rk2",
                          "zones": zones, "leaves": int(len(leaves)),

@@ -272,10 +272,13 @@ int km_sim_create(const km_cfg_t *cfg, const km_mesh_desc_t *mesh, int32_t path,
  * one launch. Ensembles advance by cycle count (tlim must be 0). */
 int km_sim_create_ensemble(const km_cfg_t *cfg, const km_mesh_desc_t *meshes, int32_t ninst,
                            int32_t path, km_sim_t **sim);
-/* A block-tree (AMR) mesh with the leaves described at km_ghost_plan_amr, on the task
- * path (hydro or CT MHD; CT needs an even nghost): each stage adds flux (and EMF)
- * correction after the fluxes and fills ghost zones across levels. Host buffers hold the
- * blocks in leaf order. */
+/* A block-tree (AMR) mesh with the leaves described at km_ghost_plan_amr. On the task
+ * path (hydro or CT MHD; CT needs an even nghost) each stage adds flux (and EMF)
+ * correction after the fluxes and fills ghost zones across levels. On the fused path
+ * (hydro, one species, Cartesian) the sweeps run as pencil tiles over every leaf in one
+ * launch per direction, reading the exchanged primitives; faces bordering a finer leaf
+ * take the restricted fine flux, computed first; the results are the task path's bits.
+ * Host buffers hold the blocks in leaf order. */
 int km_sim_create_amr(const km_cfg_t *cfg, const km_mesh_desc_t *mesh, const int32_t *leaves,
                       int32_t nleaves, int32_t path, km_sim_t **sim);
 /* time and dt of the first n instances (blocks the host) */
@@ -303,7 +306,7 @@ int km_sim_kernel_times(km_sim_t *sim, double *ms, int32_t *count, int32_t nkind
  * the library's autotuner (the default; at the first cycle that starts from conserved
  * values it times every candidate on the simulation's own data and keeps the choice per
  * arithmetic form, method, block shape and direction for the process), 0 = the pencil
- * march, k >= 1 = pencil-tile shape k of 5. The reference has no counterpart:
+ * march (single-level meshes only), k >= 1 = pencil-tile shape k of 5. The reference has no counterpart:
  * CalculateFluxesNested/Scratch (hydro_add_flux_tasks.cpp:29-385) fix their loop shape
  * in source. */
 int km_sim_set_fused_kernel(km_sim_t *sim, int32_t dir, int32_t kernel);

@@ -7,6 +7,7 @@
 #include <string>
 
 #include "../../include/km_hydro.h"
+#include "km_amr.hpp"
 #include "km_physics.hpp"
 
 namespace km {
@@ -54,6 +55,11 @@ struct FusedArgs {
   double beta;
   int lim;
   Eos eos;
+  // block-tree mesh: cells past a block edge are the block's own (filled) ghost zones,
+  // primitives are read from the pack, and a face bordering a finer leaf takes the
+  // restricted fine flux from the pack's flux arrays
+  int amr;
+  AmrGrid agrid;
 };
 
 constexpr int KM_NBR_OUTFLOW = -1;
@@ -102,16 +108,19 @@ namespace exact {
 int launch_fused_sweep(const km_cfg_t &cfg, const FusedArgs &a, int ndim, int dir, bool with_dt,
                        int kern, hipStream_t st);
 int tile_shapes();
+int launch_fine_face_fluxes(const km_cfg_t &cfg, const FusedArgs &a, hipStream_t st);
 }
 namespace fma {
 int launch_fused_sweep(const km_cfg_t &cfg, const FusedArgs &a, int ndim, int dir, bool with_dt,
                        int kern, hipStream_t st);
 int tile_shapes();
+int launch_fine_face_fluxes(const km_cfg_t &cfg, const FusedArgs &a, hipStream_t st);
 }
 namespace fp32 {
 int launch_fused_sweep(const km_cfg_t &cfg, const FusedArgs &a, int ndim, int dir, bool with_dt,
                        int kern, hipStream_t st);
 int tile_shapes();
+int launch_fine_face_fluxes(const km_cfg_t &cfg, const FusedArgs &a, hipStream_t st);
 }
 
 // thread-local error message for km_last_error()

@@ -425,6 +425,10 @@ FusedArgs fused_args(km_sim_t &s, int stage, double beta) {
   a.beta = beta;
   a.lim = s.cfg.limiter;
   a.eos = make_eos(s.cfg);
+  a.amr = s.amr ? 1 : 0;
+  a.agrid = s.agrid;
+  if (s.amr)  // fluxes at level jumps live in the task path's flux arrays
+    for (int d = 0; d < 3; ++d) a.u0.flux[d] = s.scratch.pack.flux[d];
   return a;
 }
 
@@ -436,6 +440,12 @@ int launch_sweep(const km_sim_t &s, const FusedArgs &a, int dir, bool last, int 
   if (cfg.precision == KM_PREC_FP64_FMA)
     return fma::launch_fused_sweep(cfg, a, ndim, dir, last, kern, s.stream);
   return exact::launch_fused_sweep(cfg, a, ndim, dir, last, kern, s.stream);
+}
+
+int launch_fine_faces(const km_sim_t &s, const FusedArgs &a) {
+  if (s.cfg.precision == KM_PREC_FP32_MIXED) return fp32::launch_fine_face_fluxes(s.cfg, a, s.stream);
+  if (s.cfg.precision == KM_PREC_FP64_FMA) return fma::launch_fine_face_fluxes(s.cfg, a, s.stream);
+  return exact::launch_fine_face_fluxes(s.cfg, a, s.stream);
 }
 
 int tile_shapes(const km_cfg_t &cfg) {
@@ -465,24 +475,25 @@ int autotune(km_sim_t &s) {
     if (s.kern[dir] >= 0) continue;
     const bool final_sweep = dir == ndim - 1;
     if (s.path != KM_PATH_FUSED || dir >= ndim || (final_sweep && cfg.nstages < 2)) {
-      s.kern[dir] = 0;
+      s.kern[dir] = s.amr && s.path == KM_PATH_FUSED ? 1 : 0;
       continue;
     }
     const std::vector<int> key = {cfg.precision, cfg.recon, cfg.limiter, cfg.riemann, ndim,
-                                  s.mesh.nx[0], s.mesh.nx[1], s.mesh.nx[2], s.nb, dir};
+                                  s.mesh.nx[0], s.mesh.nx[1], s.mesh.nx[2], s.nb, dir,
+                                  s.amr ? 1 : 0};
     auto it = g_tune.find(key);
     if (it != g_tune.end()) {
       s.kern[dir] = it->second;
       continue;
     }
     hipEvent_t e0 = s.ev(), e1 = s.ev();
-    int best = 0;
+    int best = -1;
     float best_ms = 0.f;
-    for (int k = 0; k <= nshape; ++k) {
+    for (int k = s.amr ? 1 : 0; k <= nshape; ++k) {  // block trees run tiles only
       // warm-up; a shape whose LDS does not fit the blocks, or that the device refuses,
       // is skipped
-      if (int rc = launch_sweep(s, a, dir, false, k)) {
-        if (k == 0) return rc;
+      if (launch_sweep(s, a, dir, false, k)) {
+        (void)hipGetLastError();
         continue;
       }
       if (k > 0 && hipGetLastError() != hipSuccess) continue;
@@ -497,17 +508,74 @@ int autotune(km_sim_t &s) {
         ms = r == 0 ? t : std::min(ms, t);
       }
       if (int rc = check_launch()) return rc;
-      if (k == 0 || ms < best_ms) {
+      if (best < 0 || ms < best_ms) {
         best = k;
         best_ms = ms;
       }
     }
     s.ev_free.push_back(e0);
     s.ev_free.push_back(e1);
+    if (best < 0) {
+      set_error("no fused sweep kernel fits these blocks");
+      return KM_EINVAL;
+    }
     g_tune[key] = best;
     s.kern[dir] = best;
   }
   return KM_OK;
+}
+
+// One RK stage of the fused path on a block-tree mesh. The stage input's ghost zones and
+// primitives are current (the previous stage's PreparePrimitive and ghost exchange, as on
+// the task path), so the sweeps read the stored primitives of every cell they need. The
+// fluxes of faces where a block borders a coarser leaf are computed first and restricted
+// onto the coarse face (AddFluxCorrectionTasks), and the coarse block's sweep takes that
+// flux at its end face: every face flux, and so du and u1, is the task path's. The final
+// sweep stops at the stage combine; PreparePrimitive, the CFL estimate and the ghost
+// exchange are the task path's kernels (the primitive arrays are shared by the stage
+// containers and still read by other tiles during the sweep).
+int amr_fused_stage(km_sim_t &s, int stage, double beta) {
+  const km_cfg_t &cfg = s.cfg;
+  Cont &md1 = container(s, stage);
+  const FusedArgs a = fused_args(s, stage, beta);
+  const int ndim = s.mesh.ndim;
+  hipEvent_t st0 = nullptr, st1 = nullptr;
+  if (s.profile) {
+    st0 = s.ev();
+    st1 = s.ev();
+    (void)hipEventRecord(st0, s.stream);
+  }
+  if (int rc = launch_fine_faces(s, a)) return rc;
+  launch_amr_flux_correct(s.agrid, a.u0, s.nflux, s.stream);
+  for (int dir = 0; dir < ndim; ++dir) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (s.profile) {
+      e0 = s.ev();
+      e1 = s.ev();
+      (void)hipEventRecord(e0, s.stream);
+    }
+    if (int rc = launch_sweep(s, a, dir, false, s.kern[dir] > 0 ? s.kern[dir] : 1)) return rc;
+    if (s.profile) {
+      (void)hipEventRecord(e1, s.stream);
+      s.recs.push_back({dir, e0, e1});
+    }
+  }
+  launch_prepare_primitive(cfg, md1.pack, s.stream);
+  if (stage == cfg.nstages) launch_estimate_dt(cfg, s.base.pack, s.est_d, s.stream, s.inst_d, s.ninst);
+  hipEvent_t g0 = nullptr, g1 = nullptr;
+  if (s.profile) {
+    g0 = s.ev();
+    g1 = s.ev();
+    (void)hipEventRecord(g0, s.stream);
+  }
+  ghost_fill(s, md1.pack, 0, KM_TEMP(cfg.mhd), false);
+  if (s.profile) {
+    (void)hipEventRecord(g1, s.stream);
+    s.recs.push_back({3, g0, g1});
+    (void)hipEventRecord(st1, s.stream);
+    s.recs.push_back({4, st0, st1});
+  }
+  return check_launch();
 }
 
 int fused_stage(km_sim_t &s, int stage, double beta) {
@@ -577,6 +645,11 @@ int amr_remesh(km_sim_t &s, const std::vector<int32_t> &nl, const std::vector<in
   if ((rc = make_cont(s, s.scratch, false, nullptr, false, true))) return rc;
   if ((rc = s.alloc(&s.tags_d, n))) return rc;
   if ((rc = s.alloc(&s.err2_d, n))) return rc;
+  if (s.path == KM_PATH_FUSED) {
+    const long nint = (long)s.mesh.nx[0] * (s.mesh.ndim > 1 ? s.mesh.nx[1] : 1) *
+                      (s.mesh.ndim > 2 ? s.mesh.nx[2] : 1);
+    if ((rc = s.alloc(&s.du, (size_t)n * 5 * nint))) return rc;
+  }
   int32_t *origin_d = nullptr;
   if ((rc = s.alloc(&origin_d, origin.size()))) return rc;
   KM_HIP(hipMemcpy(origin_d, origin.data(), origin.size() * sizeof(int32_t),
@@ -672,7 +745,7 @@ int history_row(km_sim_t &s) {
   KM_HIP(hipStreamSynchronize(s.stream));
   std::fprintf(s.hist, "%.14e %.14e %d %d", t, dt, s.ncycle, s.nb);
   if (s.hist_vortex) {
-    if (s.path == KM_PATH_FUSED && !s.prims_from_ic) {  // it keeps conserved interiors only
+    if (s.path == KM_PATH_FUSED && !s.amr && !s.prims_from_ic) {  // conserved interiors only
       launch_ghost_fill(s.grid, s.base.pack, 0, s.nc, s.nc, false, s.stream,
                         Instances{s.inst_d, s.grids_d});
       launch_prepare_primitive(s.cfg, s.base.pack, s.stream);
@@ -695,6 +768,8 @@ int run_cycle(km_sim_t &s, double dt_host) {
     int rc;
     if (s.path == KM_PATH_TASKS || s.prims_from_ic)
       rc = task_stage(s, stage, betas[stage - 1], dt_host);
+    else if (s.amr)
+      rc = amr_fused_stage(s, stage, betas[stage - 1]);
     else
       rc = fused_stage(s, stage, betas[stage - 1]);
     s.prims_from_ic = false;
@@ -1097,6 +1172,8 @@ static int sim_create(const km_cfg_t *cfg, const km_mesh_desc_t *meshes, int32_t
   if (path == KM_PATH_FUSED) {
     const long nint = (long)shape.nx[0] * shape.nx[1] * shape.nx[2];
     if ((rc = s->alloc(&s->du, (size_t)s->nb * 5 * nint))) return rc;
+  }
+  if (path == KM_PATH_FUSED && !tree) {
     // face-neighbour table of every instance's uniform block grid (periodic wraps; other
     // boundaries are mapped inside the sweep)
     std::vector<int> nbr((size_t)s->nb * 6);
@@ -1151,8 +1228,8 @@ int km_sim_create(const km_cfg_t *cfg, const km_mesh_desc_t *mesh, int32_t path,
 int km_sim_create_amr(const km_cfg_t *cfg, const km_mesh_desc_t *mesh, const int32_t *leaves,
                       int32_t nleaves, int32_t path, km_sim_t **out) {
   if (!cfg || !mesh || !leaves || !out) return KM_EINVAL;
-  if (path != KM_PATH_TASKS) {
-    set_error("block-tree meshes run on the task path");
+  if (path != KM_PATH_TASKS && path != KM_PATH_FUSED) {
+    set_error("unknown path");
     return KM_EINVAL;
   }
   AmrTreeHost t;
@@ -1247,9 +1324,9 @@ int km_sim_set_primitive(km_sim_t *s, const double *cell, const double *const fa
 int km_sim_get_state(km_sim_t *s, double *cell, double *const face[3]) {
   if (!s || !cell) return KM_EINVAL;
   KM_HIP(hipStreamSynchronize(s->stream));
-  if (s->path == KM_PATH_FUSED) {
-    // the fused path keeps only the conserved interior current; fill the ghost zones
-    // and derive the primitives
+  if (s->path == KM_PATH_FUSED && !s->amr) {
+    // the single-level fused path keeps only the conserved interior current; fill the
+    // ghost zones and derive the primitives
     launch_ghost_fill(s->grid, s->base.pack, 0, s->nc, s->nc, false, s->stream,
                       Instances{s->inst_d, s->grids_d});
     launch_prepare_primitive(s->cfg, s->base.pack, s->stream);
@@ -1337,6 +1414,11 @@ int km_sim_set_fused_kernel(km_sim_t *s, int32_t dir, int32_t kern) {
   if (dir < -1 || dir > 2 || kern < -1 || kern > tile_shapes(s->cfg)) {
     set_error("fused sweep kernel: direction 0..2 (or -1 for all) and kernel -1 (autotune), "
               "0 (march) or a pencil-tile shape 1.." + std::to_string(tile_shapes(s->cfg)));
+    return KM_EINVAL;
+  }
+  if (kern == 0 && s->amr && s->path == KM_PATH_FUSED) {
+    set_error("the pencil march reads face neighbours of a single-level mesh; block trees "
+              "run the fused sweeps as pencil tiles");
     return KM_EINVAL;
   }
   KM_HIP(hipStreamSynchronize(s->stream));
