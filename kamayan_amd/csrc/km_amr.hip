@@ -142,8 +142,14 @@ __device__ __forceinline__ double child_value(double fc, double a, double bb, do
 
 // Zones a level-L or a level-L+1 leaf covers: copy or RestrictAverage. Zones a coarser
 // leaf covers are left to k_amr_prolong.
+// variable k of the range [v0, v1) with the slots [s0, s1) left out (v0 <= s0 <= s1 <= v1)
+__device__ __forceinline__ int fill_var(int k, int v0, int s0, int s1) {
+  const int v = v0 + k;
+  return v >= s0 ? v + (s1 - s0) : v;
+}
+
 __global__ void __launch_bounds__(256) k_amr_ghost(km_pack_t u, AmrGrid t, int v0, int v1,
-                                                   int nc) {
+                                                   int nc, int s0, int s1) {
   const Geo g(u);
   const long n = ghost_count(g);
   const long tt = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -165,21 +171,23 @@ __global__ void __launch_bounds__(256) k_amr_ghost(km_pack_t u, AmrGrid t, int v
   // loads of one chunk overlap instead of waiting on the previous variable's store
   constexpr int kChunk = 4;
   const km_block_t &sb = u.blocks[s.leaf];
-  for (int v = v0; v < v1; v += kChunk) {
+  const int nv = (v1 - v0) - (s1 - s0);
+  for (int k = 0; k < nv; k += kChunk) {
     double x[kChunk];
 #pragma unroll
     for (int q = 0; q < kChunk; ++q) {
       x[q] = 0.0;
-      if (v + q < v1) {
-        const double *src = cell_ptr(u, s.leaf, v + q);
+      if (k + q < nv) {
+        const double *src = cell_ptr(u, s.leaf, fill_var(k + q, v0, s0, s1));
         x[q] = s.kind == 1 ? restrict_cell(src, sb, g, s.idx, t.cyl != 0) : src[s.idx];
       }
     }
 #pragma unroll
     for (int q = 0; q < kChunk; ++q) {
-      if (v + q < v1) {
-        const int ax = vector_axis(v + q, nc);
-        cell_ptr(u, b, v + q)[dc] = (ax >= 0 && fl[ax]) ? -x[q] : x[q];
+      if (k + q < nv) {
+        const int v = fill_var(k + q, v0, s0, s1);
+        const int ax = vector_axis(v, nc);
+        cell_ptr(u, b, v)[dc] = (ax >= 0 && fl[ax]) ? -x[q] : x[q];
       }
     }
   }
@@ -193,7 +201,7 @@ __global__ void __launch_bounds__(256) k_amr_ghost(km_pack_t u, AmrGrid t, int v
 // count, outflow clamp, reflect mirror) send both children of an unmapped coarse zone
 // to children of one mapped C, and a level-(L-1) leaf covers either both or neither.
 __global__ void __launch_bounds__(256) k_amr_prolong(km_pack_t u, AmrGrid t, int v0, int v1,
-                                                     int nc) {
+                                                     int nc, int s0, int s1) {
   const Geo g(u);
   km_pack_t half = u;
   half.ng = (u.ng + 1) / 2;
@@ -268,7 +276,9 @@ __global__ void __launch_bounds__(256) k_amr_prolong(km_pack_t u, AmrGrid t, int
 #pragma unroll
   for (int d = 0; d < 3; ++d)
     if (d < g.ndim) df[d] = 0.5 * blk.dx[d];
-  for (int v = v0; v < v1; ++v) {
+  const int nv = (v1 - v0) - (s1 - s0);
+  for (int k = 0; k < nv; ++k) {
+    const int v = fill_var(k, v0, s0, s1);
     const int ax = vector_axis(v, nc);
     const double fc = cell_ptr(u, s.leaf, v)[s.idx];
     double gr[3] = {0., 0., 0.};
@@ -713,19 +723,20 @@ void launch_amr_emf_correct(const AmrGrid &t, const km_pack_t &u, hipStream_t st
 }
 
 void launch_amr_ghost_fill(const AmrGrid &t, const km_pack_t &u, int v0, int v1, int nc,
-                           hipStream_t st) {
+                           hipStream_t st, int s0, int s1) {
   const Geo g(u);
   const long n = ghost_count(g) * u.nblocks;
-  if (n == 0 || v0 >= v1) return;
+  if (s0 < v0 || s1 > v1 || s1 <= s0) s0 = s1 = v1;  // nothing left out
+  if (n == 0 || v0 >= v1 || (v1 - v0) == (s1 - s0)) return;
   hipLaunchKernelGGL(k_amr_ghost, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, u, t, v0,
-                     v1, nc);
+                     v1, nc, s0, s1);
   if (t.lmax > 0) {
     km_pack_t half = u;
     half.ng = (u.ng + 1) / 2;
     for (int d = 0; d < 3; ++d) half.nx[d] = d < u.ndim ? u.nx[d] / 2 : 1;
     const long nc2 = ghost_count(Geo(half)) * u.nblocks;
     hipLaunchKernelGGL(k_amr_prolong, dim3((unsigned)((nc2 + 255) / 256)), dim3(256), 0, st, u, t,
-                       v0, v1, nc);
+                       v0, v1, nc, s0, s1);
   }
 }
 

@@ -220,6 +220,10 @@ struct km_sim_t {
   // kernel of each fused sweep: -1 picked by the autotuner at the first cycle that starts
   // from conserved data, 0 the march, k >= 1 pencil-tile shape k
   int kern[3] = {-1, -1, -1};
+  // fused path on a block tree: the stages exchange only the ghost zones their sweeps read
+  // (density and the primitives); momentum and energy ghosts are filled from the base
+  // interiors when something reads them (state copies, tagging and remesh)
+  bool lazy_ghosts = false;
   void drop_graph() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -568,7 +572,10 @@ int amr_fused_stage(km_sim_t &s, int stage, double beta) {
     g1 = s.ev();
     (void)hipEventRecord(g0, s.stream);
   }
-  ghost_fill(s, md1.pack, 0, KM_TEMP(cfg.mhd), false);
+  // density and the primitives; momentum and energy are left out
+  launch_amr_ghost_fill(s.agrid, md1.pack, 0, KM_TEMP(cfg.mhd), s.nc, s.stream, 1,
+                        KM_NCONS(cfg.mhd));
+  s.lazy_ghosts = true;
   if (s.profile) {
     (void)hipEventRecord(g1, s.stream);
     s.recs.push_back({3, g0, g1});
@@ -576,6 +583,13 @@ int amr_fused_stage(km_sim_t &s, int stage, double beta) {
     s.recs.push_back({4, st0, st1});
   }
   return check_launch();
+}
+
+// the momentum and energy ghost zones the fused block-tree stages leave to be filled
+void fill_lazy_ghosts(km_sim_t &s) {
+  if (!s.lazy_ghosts) return;
+  ghost_fill(s, s.base.pack, 1, KM_NCONS(s.cfg.mhd), false);
+  s.lazy_ghosts = false;
 }
 
 int fused_stage(km_sim_t &s, int stage, double beta) {
@@ -671,6 +685,7 @@ int amr_remesh(km_sim_t &s, const std::vector<int32_t> &nl, const std::vector<in
 // InitializeBlockTimeStepsAfterAMR, then the time-step rule)
 int adapt_step(km_sim_t &s) {
   const km_amr_adapt_t &a = s.adapt;
+  fill_lazy_ghosts(s);  // the estimator and the remesh read ghost zones of every field
   std::vector<int32_t> tags(s.nb, KM_AMR_DEREFINE);
   if (a.ncrit > 0) {
     for (int c = 0; c < a.ncrit; ++c)
@@ -1324,6 +1339,10 @@ int km_sim_set_primitive(km_sim_t *s, const double *cell, const double *const fa
 int km_sim_get_state(km_sim_t *s, double *cell, double *const face[3]) {
   if (!s || !cell) return KM_EINVAL;
   KM_HIP(hipStreamSynchronize(s->stream));
+  if (s->lazy_ghosts) {
+    fill_lazy_ghosts(*s);
+    KM_HIP(hipStreamSynchronize(s->stream));  // the copies below run on another stream
+  }
   if (s->path == KM_PATH_FUSED && !s->amr) {
     // the single-level fused path keeps only the conserved interior current; fill the
     // ghost zones and derive the primitives
@@ -1405,6 +1424,8 @@ int km_sim_advance(km_sim_t *s, int32_t ncycles, double tlim) {
       if (int rc = run_cycle(*s, 0.0)) return rc;
     }
   }
+  // replayed cycles skip the host code that marks the momentum and energy ghosts stale
+  if (s->amr && s->path == KM_PATH_FUSED && ncycles > 0) s->lazy_ghosts = true;
   KM_HIP(hipStreamSynchronize(s->stream));
   return KM_OK;
 }
