@@ -469,12 +469,15 @@ def main() -> int:
     if not args.no_variants:
         var = {}
         for prec in ("fp32_mixed", "fp64_fma"):
-            dtv, _, sv, _, kv = _run_gpu(km, cell, prec, args.steps, args.warmup,
-                                         profile=False)
+            # profiled like the FP64 run (eager cycles, HIP events per sweep)
+            dtv, tv, sv, _, kv = _run_gpu(km, cell, prec, args.steps, args.warmup,
+                                          profile=True)
             var[prec] = {"value": zones * args.steps / dtv,
                          "speedup_vs_fp64": dt64 / dtv,
                          "rel_l1_vs_fp64_per_var": _rel_l1(sv, s64, lay),
-                         "sweep_kernels": kv}
+                         "sweep_kernels": kv,
+                         "kernels": {k: {"avg_ms": ms / n, "launches": n}
+                                     for k, (ms, n) in tv.items() if n}}
         out["variants"] = var
     if not args.no_cpu:
         cpu = _cpu_baseline(km, cell, args.cpu_cycles)
