@@ -25,39 +25,45 @@ N, NB, NG = 256, 64, 4
 SEED = 20261204
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_ZONE_STAGE = 8 * 3 * 5  # stage input + RK base read, stage output written
-# Per-launch figures of each FP64 sweep kernel on this workload, measured with rocprofv3
-# (profiles/fused_stage_traffic_hydro3d_256c_wenoz_hllc_rk3.md): HBM bytes read
-# (FETCH_SIZE x 2, the factor a 1 GiB copy by the library's copy kernel calibrates) and
+# Per-launch figures of the fused kernels on this workload from the rocprofv3 counter runs
+# summarised in profiles/fused_stage_traffic_hydro3d_256c_wenoz_hllc_rk3.md: HBM bytes read
+# (FETCH_SIZE x 2, the factor a 1 GiB copy by the library's copy kernel calibrates) plus
 # written (WRITE_SIZE x 1), and VALU instructions per zone (SQ_INSTS_VALU x 64 / zones).
-# The final sweep has a plain form and, on the last RK stage, the CFL-minimum form.
-# Kernels are named as _kernel_names() reports the autotuner's choices.
+# Keyed by form, then by direction and kernel name as _kernel_names() reports them; the
+# final sweep has a plain form and, on the last RK stage, the CFL-minimum form. These are
+# constants from those runs, not measured by this script.
+COUNTER_TABLE = "profiles/fused_stage_traffic_hydro3d_256c_wenoz_hllc_rk3.md"
 MEASURED = {
-    "x1": {"march": (2.3494e9 + 0.8032e9, 1535.8),
-           "tile 4 pencils x 256 lanes": (1.0908e9 + 0.6711e9, 1972.8),
-           "tile 8 pencils x 256 lanes": (1.0908e9 + 0.6711e9, 1734.4),
-           "tile 8 pencils x 512 lanes": (1.0908e9 + 0.6711e9, 1773.4),
-           "tile 16 pencils x 1024 lanes": (1.0908e9 + 0.6711e9, 1683.1)},
-    "x2": {"march": (1.6589e9 + 0.9476e9, 1433.3),
-           "tile 4 pencils x 256 lanes": (1.4997e9 + 0.6711e9, 1921.2),
-           "tile 8 pencils x 256 lanes": (1.4997e9 + 0.6711e9, 1672.1),
-           "tile 8 pencils x 512 lanes": (1.4997e9 + 0.6711e9, 1724.1),
-           "tile 16 pencils x 1024 lanes": (1.4997e9 + 0.6711e9, 1633.8)},
-    "x3": {"march": (2.4187e9 + 0.6711e9, 1745.3)},
+    "fp64": {
+        "x1": {"march": (2.3494e9 + 0.8032e9, 1535.8),
+               "tile 4 pencils x 256 lanes": (1.0908e9 + 0.6711e9, 1972.8),
+               "tile 8 pencils x 256 lanes": (1.0908e9 + 0.6711e9, 1734.4),
+               "tile 8 pencils x 512 lanes": (1.0908e9 + 0.6711e9, 1773.4),
+               "tile 16 pencils x 1024 lanes": (1.0908e9 + 0.6711e9, 1683.1)},
+        "x2": {"march": (1.6589e9 + 0.9476e9, 1433.3),
+               "tile 4 pencils x 256 lanes": (1.4997e9 + 0.6711e9, 1921.2),
+               "tile 8 pencils x 256 lanes": (1.4997e9 + 0.6711e9, 1672.1),
+               "tile 8 pencils x 512 lanes": (1.4997e9 + 0.6711e9, 1724.1),
+               "tile 16 pencils x 1024 lanes": (1.4997e9 + 0.6711e9, 1633.8)},
+        "x3": {"march": (2.4187e9 + 0.6711e9, 1745.3)},
+        "x3_dt": {"march": (2.8539e9 + 0.6712e9, 1865.8)},
+    },
 }
-MEASURED_X3_DT = {"march": (2.8539e9 + 0.6712e9, 1865.8)}
 
 
-def _stage_figures(kern):
-    """(HBM bytes, VALU instructions per zone) of one RK3 stage with the given sweep
-    kernels (two plain final sweeps and one CFL-minimum final sweep per cycle), or None
-    for a kernel without measurements."""
+def _stage_figures(form, kern):
+    """HBM bytes and VALU instructions per zone of one RK3 stage (two plain final sweeps
+    and one CFL-minimum final sweep per cycle) from the counter table, or None when the
+    form or one of its kernels has no entry."""
+    tab = MEASURED.get(form)
     try:
-        x1, x2 = MEASURED["x1"][kern["x1"]], MEASURED["x2"][kern["x2"]]
-        x3p, x3d = MEASURED["x3"][kern["x3"]], MEASURED_X3_DT[kern["x3"]]
-    except KeyError:
+        x1, x2 = tab["x1"][kern["x1"]], tab["x2"][kern["x2"]]
+        x3p, x3d = tab["x3"][kern["x3"]], tab["x3_dt"][kern["x3"]]
+    except (KeyError, TypeError):
         return None
     x3 = [(2 * x3p[i] + x3d[i]) / 3 for i in range(2)]
-    return x1[0] + x2[0] + x3[0], x1[1] + x2[1] + x3[1]
+    return {"traffic": x1[0] + x2[0] + x3[0], "valu": x1[1] + x2[1] + x3[1],
+            "source": f"counter table from {COUNTER_TABLE} (not measured by this run)"}
 
 
 # peak issue rate of FP64 VALU work on MI355X: 256 CUs x 64 lanes x 2.4 GHz (78.6 TFLOP/s
@@ -81,8 +87,19 @@ def _setup(km, precision="fp64", n=N, nb=NB):
 TILE_SHAPES = [(4, 256), (8, 256), (8, 512), (16, 1024), (32, 1024)]
 
 
+# one-launch stage kernel variants (3-D): tile of x1 columns x x2 rows, x3 segment
+STAGE_SHAPES = [(64, 4), (64, 8), (32, 8), (32, 16), (64, 16)]
+STAGE_SEGS = ["block", 32, 16]
+
+
 def _kernel_names(sim, ndim):
     names = {}
+    st = sim.fused_kernel(3) if ndim == 3 else 0
+    if st > 0:
+        tx, ty = STAGE_SHAPES[(st - 1) // 3]
+        seg = STAGE_SEGS[(st - 1) % 3]
+        names["stage"] = f"one-launch stage, {tx}x{ty} columns, x3 segment {seg}"
+        return names
     for d in range(ndim):
         k = sim.fused_kernel(d)
         names[f"x{d + 1}"] = ("march" if k <= 0 else
@@ -119,23 +136,84 @@ def _rel_l1(a, b, lay):
             for n, d in zip(num, den)]
 
 
+def _host_cpus():
+    """The host the CPU baseline runs on: lscpu model and topology, nproc and the CPUs this
+    process may run on, and the thread count the oracle uses (OMP_NUM_THREADS when the
+    environment sets it, which is the GPU box's CPU share for one GPU, else every CPU this
+    process may run on)."""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core",
+                             "CPU(s)", "NUMA node(s)"):
+                info[k.strip()] = v.strip()
+    except Exception as e:  # lscpu missing: record why
+        info["lscpu"] = f"unavailable ({e})"
+    try:
+        nproc = int(subprocess.run(["nproc"], capture_output=True, text=True,
+                                   timeout=20).stdout.strip())
+    except Exception:
+        nproc = os.cpu_count() or 1
+    affinity = len(os.sched_getaffinity(0))
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or affinity
+    return {"lscpu": info, "nproc": nproc, "affinity_cpus": affinity, "threads": threads}
+
+
 def _cpu_baseline(km, cell, cycles):
+    """The FP64 CPU restatement of the reference algorithm (the oracle) on the host cores,
+    OpenMP with close binding: one warm-up cycle, then `cycles` timed cycles of the same
+    256^3 state. Returns the baseline record and the oracle's state and (time, dt) after
+    all 1 + cycles cycles, for the full-size parity check."""
+    host = _host_cpus()
+    os.environ["OMP_NUM_THREADS"] = str(host["threads"])
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     from oracle import km_oracle  # the checker, timed as the CPU baseline only
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
-    os.environ["OMP_NUM_THREADS"] = str(threads)
     cfg, desc, lay, _ = _setup(km, "fp64")
     ref = km_oracle.CpuSimulation(cfg.c, desc, lay)
     ref.set_primitive(cell)
+    ref.advance(1)  # warm-up (first-touch of the state, thread start-up)
     t0 = time.perf_counter()
     ref.advance(cycles)
     t1 = time.perf_counter()
+    state, _ = ref.state()
+    info = ref.info()
     ref.close()
     zones = N ** 3
-    return {"value": zones * cycles / (t1 - t0), "unit": "zone-cycles/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{cycles} cycle(s) of the same {N}^3 workload on the FP64 CPU "
-                      f"restatement (OpenMP, {threads} threads), {t1 - t0:.1f} s"}
+    rec = {"value": zones * cycles / (t1 - t0), "unit": "zone-cycles/s", "cores": host["threads"],
+           "kind": "port",
+           "sample": f"{cycles} cycles after 1 warm-up cycle of the same {N}^3 workload on the "
+                     f"FP64 CPU restatement of the reference algorithm (OpenMP, "
+                     f"{host['threads']} threads, OMP_PROC_BIND={os.environ['OMP_PROC_BIND']}), "
+                     f"{t1 - t0:.1f} s",
+           "host": host}
+    return rec, state, info, 1 + cycles
+
+
+def _parity_fullsize(km, cell, ref_state, ref_info, ncycles, lay):
+    """The GPU's first ncycles cycles at 256^3 against the oracle's: the FP64 form bit for
+    bit (state, time and dt), the FP64-FMA and FP32-mixed forms as relative L1 per
+    conserved variable."""
+    out = {"cycles": ncycles}
+    sl = (slice(None), slice(0, 5)) + lay.interior()
+    for prec in ("fp64", "fp64_fma", "fp32_mixed"):
+        cfg, desc, lay, _ = _setup(km, prec)
+        sim = km.Simulation(cfg, desc, lay, path=km.PATH_FUSED)
+        sim.set_primitive(cell)
+        sim.advance(ncycles)
+        g, _ = sim.state()
+        gi = sim.info()
+        sim.close()
+        if prec == "fp64":
+            out["fp64_bitwise"] = bool(np.array_equal(g[sl], ref_state[sl]) and gi == ref_info)
+        else:
+            out[f"{prec}_rel_l1_per_var"] = _rel_l1(g, ref_state, lay)
+    out["fp64_fma_within_1e-12"] = bool(max(out["fp64_fma_rel_l1_per_var"]) < 1e-12)
+    return out
 
 
 # ---------------------------------------------------------------------------------
@@ -184,10 +262,13 @@ def _divb_max(cell, lay, mhd_slot):
 
 
 def _secondary(km, args):
+    host = _host_cpus()
+    threads = host["threads"]
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     from oracle import km_oracle  # CPU baseline leg only
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
-    os.environ["OMP_NUM_THREADS"] = str(threads)
     out = {"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "unit": "zone-cycles/s",
            "metric": "zone-cycles/s on 1 MI355X, FP64 vs FP32-mixed at equal L1 error; % HBM peak"}
@@ -217,7 +298,7 @@ def _secondary(km, args):
                          "parallelism": "single GPU"}
         ref = km_oracle.CpuSimulation(cfg.c, desc, lay)
         ref.set_primitive(cell, faces)
-        sample_cycles = 1
+        sample_cycles = 3
     elif args.workload == "sedov_amr":
         cfg, desc, lay, leaves, cell = _sedov_setup(km)
         zones = len(leaves) * 16 ** 3
@@ -252,7 +333,7 @@ def _secondary(km, args):
                          "nghost": 4, "path": "tasks", "parallelism": "single GPU"}
         ref = km_oracle.CpuSimulation(cfg.c, desc, lay, leaves=leaves)
         ref.set_primitive(cell)
-        sample_cycles = 1
+        sample_cycles = 3
     elif args.workload == "sedov_amr_adaptive":
         # the same start, then tagging on pres (Loehner, kamayan/refinement0) and remesh
         # after every cycle, as the reference's adaptive Sedov runs (sedov.py:38-44)
@@ -285,7 +366,7 @@ def _secondary(km, args):
         ref = km_oracle.CpuSimulation(cfg.c, desc, lay, leaves=leaves)
         ref.set_primitive(cell)
         ref.set_adaptive(adapt)
-        sample_cycles = 1
+        sample_cycles = 3
         cpu_zones = len(leaves) * 16 ** 3
     elif args.workload == "vortex2d":
         # the reference's CPU-runnable case: isentropic_vortex.in at 128^2 in 32^2 blocks,
@@ -382,14 +463,18 @@ def _secondary(km, args):
     if not args.no_cpu:
         if args.workload not in ("kh_ensemble", "sedov_amr_adaptive"):
             cpu_zones = zones
+        ref.advance(1)  # warm-up
         t0 = time.perf_counter()
         ref.advance(sample_cycles)
         t1 = time.perf_counter()
         ref.close()
         out["cpu_baseline"] = {"value": cpu_zones * sample_cycles / (t1 - t0),
                                "unit": "zone-cycles/s", "cores": threads, "kind": "port",
-                               "sample": f"{sample_cycles} cycle(s), {cpu_zones} zones, "
-                                         f"{t1 - t0:.1f} s on the FP64 CPU restatement"}
+                               "sample": f"{sample_cycles} cycle(s) after 1 warm-up cycle, "
+                                         f"{cpu_zones} zones, {t1 - t0:.1f} s on the FP64 CPU "
+                                         f"restatement (OpenMP, {threads} threads, "
+                                         f"OMP_PROC_BIND={os.environ['OMP_PROC_BIND']})",
+                               "host": host}
         out["gpu_vs_cpu"] = value / out["cpu_baseline"]["value"]
     print(json.dumps(out), flush=True)
     return 0
@@ -400,7 +485,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--cpu-cycles", type=int, default=1)
+    ap.add_argument("--cpu-cycles", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--workload", default="hydro3d",
@@ -425,16 +510,29 @@ def main() -> int:
 
     _, _, lay, cell = _setup(km)
     zones = N ** 3
-    dt64, times, s64, lay, kern64 = _run_gpu(km, cell, "fp64", args.steps, args.warmup,
-                                             profile=True)
-    value = zones * args.steps / dt64
+    forms = ("fp64",) if args.no_variants else ("fp64", "fp64_fma", "fp32_mixed")
+    runs = {}
+    for prec in forms:
+        # eager cycles with HIP events around each kernel on the simulation's stream
+        runs[prec] = _run_gpu(km, cell, prec, args.steps, args.warmup, profile=True)
+    cpu = parity = None
+    if not args.no_cpu:
+        cpu, ref_state, ref_info, ncyc = _cpu_baseline(km, cell, args.cpu_cycles)
+        if not args.no_variants:
+            parity = _parity_fullsize(km, cell, ref_state, ref_info, ncyc, lay)
+    # the headline is the fastest FP64 form that agrees with the oracle to 1e-12 relative L1
+    # per conserved variable at full size: the FMA form when the check ran and passed, else
+    # the bitwise form
+    head = "fp64_fma" if parity and parity["fp64_fma_within_1e-12"] and \
+        runs["fp64_fma"][0] < runs["fp64"][0] else "fp64"
+    dt_h, times, s_h, lay, kern_h = runs[head]
+    value = zones * args.steps / dt_h
     st_ms, st_n = times["stage"]
     stage_ms = st_ms / max(st_n, 1)
     achieved = BYTES_PER_ZONE_STAGE * zones / (stage_ms * 1e-3) / 1e9
     kernels = {k: {"avg_ms": (ms / n if n else 0.0), "launches": n}
                for k, (ms, n) in times.items() if n}
-    fig = _stage_figures(kern64)
-    traffic, valu_zone = fig if fig else (None, None)
+    fig = _stage_figures(head, kern_h)
     out = {
         "metric": "zone-cycles/s on 1 MI355X, FP64 vs FP32-mixed at equal L1 error; % HBM peak",
         "value": value,
@@ -442,7 +540,7 @@ def main() -> int:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": dt64 * 1e3 / args.steps,
+        "ms_per_step": dt_h * 1e3 / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -451,36 +549,42 @@ def main() -> int:
         "config": {"workload": WORKLOAD, "zones": zones, "block": [NB] * 3, "nghost": NG,
                    "reconstruction": "wenoz", "riemann": "hllc", "integrator": "rk3",
                    "cfl": 0.8, "gamma": 5.0 / 3.0, "seed": SEED, "path": "fused",
-                   "sweep_kernels": kern64, "parallelism": "single GPU"},
+                   "form": head, "kernels": kern_h, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "one RK stage: sweep_x1 + sweep_x2 + sweep_x3_update (the fused "
-                               "path needs no ghost exchange on a single-level mesh)",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": fig["traffic"] if fig else None,
+                     "traffic_source": fig["source"] if fig else
+                     "no counter measurements for these kernels in the profiles/ table",
+                     "kernel": "one RK stage of the " + head + " form (" +
+                               ", ".join(f"{k}: {v}" for k, v in kern_h.items()) +
+                               "); the fused path needs no ghost exchange on a single-level mesh",
                      "algorithmic_bytes_per_launch": BYTES_PER_ZONE_STAGE * zones,
                      "avg_launch_ms": stage_ms},
         "kernels": kernels,
     }
-    if valu_zone:
-        rate = valu_zone * zones / (stage_ms * 1e-3)
+    if fig and fig.get("valu"):
+        rate = fig["valu"] * zones / (stage_ms * 1e-3)
         out["valu"] = {"bound": "fp64 valu issue", "achieved": rate,
                        "peak": VALU_PEAK_LANE_INSTR_PER_S, "unit": "lane-instr/s",
                        "frac": rate / VALU_PEAK_LANE_INSTR_PER_S,
-                       "instr_per_zone_stage": valu_zone}
-    if not args.no_variants:
-        var = {}
-        for prec in ("fp32_mixed", "fp64_fma"):
-            # profiled like the FP64 run (eager cycles, HIP events per sweep)
-            dtv, tv, sv, _, kv = _run_gpu(km, cell, prec, args.steps, args.warmup,
-                                          profile=True)
-            var[prec] = {"value": zones * args.steps / dtv,
-                         "speedup_vs_fp64": dt64 / dtv,
-                         "rel_l1_vs_fp64_per_var": _rel_l1(sv, s64, lay),
-                         "sweep_kernels": kv,
-                         "kernels": {k: {"avg_ms": ms / n, "launches": n}
-                                     for k, (ms, n) in tv.items() if n}}
+                       "instr_per_zone_stage": fig["valu"],
+                       "instr_source": fig["source"] + "; the rate uses this run's stage time"}
+    var = {}
+    for prec in forms:
+        if prec == head:
+            continue
+        dtv, tv, sv, _, kv = runs[prec]
+        var[prec] = {"value": zones * args.steps / dtv, "speedup_vs_headline": dt_h / dtv,
+                     "speedup_vs_fp64": runs["fp64"][0] / dtv,
+                     "rel_l1_vs_fp64_per_var": _rel_l1(sv, runs["fp64"][2], lay),
+                     "kernels_used": kv,
+                     "kernels": {k: {"avg_ms": ms / n, "launches": n}
+                                 for k, (ms, n) in tv.items() if n}}
+    if var:
         out["variants"] = var
-    if not args.no_cpu:
-        cpu = _cpu_baseline(km, cell, args.cpu_cycles)
+    if parity:
+        out["parity_fullsize"] = parity
+    if cpu:
         out["cpu_baseline"] = cpu
         out["gpu_vs_cpu"] = value / cpu["value"]
     print(json.dumps(out), flush=True)

@@ -274,6 +274,49 @@ int km_ghost_fill(const km_ghost_plan_t *plan, const km_pack_t *u, int32_t v0, i
  * fine edges on it. No-op for a uniform plan. */
 int km_flux_correct(const km_ghost_plan_t *plan, const km_pack_t *u, void *stream);
 
+/* ---- one RK stage as fused kernels on caller-owned packs ---------------------------- */
+/* The body of one BuildTaskListRKStage (kamayan_driver.cpp:186-248) from the fluxes to the
+ * stage combine -- hydro::CalculateFluxes (hydro_add_flux_tasks.cpp:444-476), on block
+ * trees the flux correction (kamayan_driver.cpp:198-207), grid::FluxesToDuDt and
+ * grid::ApplyDuDt (grid.cpp:179-283) -- as the fused kernels: on a single-level mesh one
+ * stage kernel or three directional sweeps reading u0's conserved interior directly (no
+ * ghost exchange, no stored primitives or fluxes), on a block tree pencil tiles reading the
+ * stored primitives. The worklist (plan) holds what the kernels need beyond the packs: the
+ * face-neighbour table and instance map of single-level meshes or ensembles, the leaf
+ * lookup of a block tree, the FP64 flux-divergence scratch of the sweeps and the kernel
+ * choices (autotuned at the first call, as km_sim_set_fused_kernel describes). Cartesian
+ * hydro with one species. */
+typedef struct km_stage_plan_t km_stage_plan_t; /* opaque, device-resident */
+/* Single-level meshes: ninst independent uniform block grids (one for a plain mesh), their
+ * blocks one after the other in the packs, each x-fastest (as km_sim_create_ensemble). */
+int km_stage_plan_uniform(const km_cfg_t *cfg, const km_mesh_desc_t *meshes, int32_t ninst,
+                          km_stage_plan_t **plan);
+/* A block-tree mesh with the leaves described at km_ghost_plan_amr, in pack order. */
+int km_stage_plan_amr(const km_cfg_t *cfg, const km_mesh_desc_t *mesh, const int32_t *leaves,
+                      int32_t nleaves, km_stage_plan_t **plan);
+int km_stage_plan_destroy(km_stage_plan_t *plan);
+/* Kernel choice of the plan's stages: dir 0..2 (-1: all) a sweep kernel as for
+ * km_sim_set_fused_kernel; dir 3 the whole stage: -1 autotune, 0 three sweeps, k >= 1 the
+ * one-launch stage kernel variant k (3-D single-level meshes). No reference counterpart
+ * (the reference fixes its loop shapes in source). */
+int km_stage_plan_set_kernel(km_stage_plan_t *plan, int32_t dir, int32_t kernel);
+int km_stage_plan_kernel(km_stage_plan_t *plan, int32_t dir, int32_t *kernel);
+/* One stage: u1 = (beta base + (1 - beta) u0) + beta dt_dev[inst] dudt(u0) over the
+ * conserved interior (grid.cpp:232-235; u0's in-place combine is not written back, since
+ * nothing reads md0 afterwards). dt_dev: device step dt per instance. dt_est_dev (device,
+ * per instance, or NULL): on a single-level mesh the CFL estimate of u1 times cfl/ndim, the
+ * last stage's EstimateTimestep (hydro_time_step.cpp:19-65); block trees take NULL and run
+ * km_prepare_primitive then km_estimate_dt after the stage, as the task order does. u0
+ * must not be u1 (the kernels read u0 across block faces while writing u1); base may be
+ * u0. Single-level meshes read u0's conserved interior only; block trees read u0's stored
+ * primitives and ghost zones (PreparePrimitive and the ghost exchange of the previous
+ * stage) and need u0's flux tables, which receive the fluxes at level jumps. Calls on one
+ * plan are serialised; the first call (and the first after a change of method or form)
+ * runs the autotuner on the call's data and stream. */
+int km_stage_fused(const km_cfg_t *cfg, km_stage_plan_t *plan, const km_pack_t *base,
+                   const km_pack_t *u0, const km_pack_t *u1, double beta, const double *dt_dev,
+                   double *dt_est_dev, void *stream);
+
 /* ---- whole-mesh driver (KamayanDriver RK stages, kamayan_driver.cpp:121-248) ----- */
 enum {
   KM_PATH_TASKS = 0, /* one kernel per reference task, same sequence as BuildTaskListRKStage */

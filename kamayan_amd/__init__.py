@@ -107,6 +107,12 @@ def lib() -> C.CDLL:
         "km_history_vortex_errors": [P(KmCfg), P(KmPack), P(KmMeshDesc), P(dbl), i32, P(dbl),
                                      vp],
         "km_sim_set_history": [vp, C.c_char_p, dbl, C.c_char_p, P(dbl), i32],
+        "km_stage_plan_uniform": [P(KmCfg), P(KmMeshDesc), i32, P(vp)],
+        "km_stage_plan_amr": [P(KmCfg), P(KmMeshDesc), P(i32), i32, P(vp)],
+        "km_stage_plan_destroy": [vp],
+        "km_stage_plan_set_kernel": [vp, i32, i32],
+        "km_stage_plan_kernel": [vp, i32, P(i32)],
+        "km_stage_fused": [P(KmCfg), vp, P(KmPack), P(KmPack), P(KmPack), dbl, vp, vp, vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -539,6 +545,84 @@ class DevicePack:
         for p in self._allocs:
             lib().km_free(C.c_void_p(p))
         self._allocs = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceArray:
+    """A small device buffer of float64 or int32 values (dt, CFL estimates, tags)."""
+
+    def __init__(self, n: int, dtype=np.float64):
+        self.n, self.dtype = n, np.dtype(dtype)
+        ptr = C.c_void_p()
+        check(lib().km_malloc(C.byref(ptr), max(n * self.dtype.itemsize, 8)), "km_malloc")
+        self.ptr = ptr.value
+        self.set(np.zeros(n, self.dtype))
+
+    def set(self, values):
+        a = np.ascontiguousarray(values, dtype=self.dtype)
+        check(lib().km_memcpy_h2d(C.c_void_p(self.ptr), C.c_void_p(a.ctypes.data), a.nbytes),
+              "h2d")
+
+    def get(self) -> np.ndarray:
+        a = np.empty(self.n, self.dtype)
+        check(lib().km_synchronize(), "sync")
+        check(lib().km_memcpy_d2h(C.c_void_p(a.ctypes.data), C.c_void_p(self.ptr), a.nbytes),
+              "d2h")
+        return a
+
+    def close(self):
+        if self.ptr:
+            lib().km_free(C.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class StagePlan:
+    """km_stage_plan_* and km_stage_fused: one RK stage of the fused kernels on
+    caller-owned packs (DevicePack). `descs`: the uniform mesh(es) of a single-level mesh or
+    ensemble; `desc` + `leaves`: a block tree."""
+
+    def __init__(self, cfg: Config, descs=None, desc=None, leaves=None):
+        self.h = C.c_void_p()
+        if leaves is not None:
+            self._leaves = np.ascontiguousarray(leaves, dtype=np.int32)
+            check(lib().km_stage_plan_amr(C.byref(cfg.c), C.byref(desc), _i32(self._leaves),
+                                          len(self._leaves), C.byref(self.h)),
+                  "km_stage_plan_amr")
+        else:
+            arr = (KmMeshDesc * len(descs))(*descs)
+            check(lib().km_stage_plan_uniform(C.byref(cfg.c), arr, len(descs), C.byref(self.h)),
+                  "km_stage_plan_uniform")
+
+    def set_kernel(self, dir: int, kern: int):
+        check(lib().km_stage_plan_set_kernel(self.h, dir, kern), "km_stage_plan_set_kernel")
+
+    def kernel(self, dir: int) -> int:
+        v = C.c_int32()
+        check(lib().km_stage_plan_kernel(self.h, dir, C.byref(v)), "km_stage_plan_kernel")
+        return v.value
+
+    def stage(self, cfg: Config, base: "DevicePack", u0: "DevicePack", u1: "DevicePack",
+              beta: float, dt: DeviceArray, est: DeviceArray = None, stream=None):
+        check(lib().km_stage_fused(C.byref(cfg.c), self.h, C.byref(base.pack), C.byref(u0.pack),
+                                   C.byref(u1.pack), C.c_double(beta), C.c_void_p(dt.ptr),
+                                   C.c_void_p(est.ptr) if est is not None else None, stream),
+              "km_stage_fused")
+
+    def close(self):
+        if self.h:
+            lib().km_stage_plan_destroy(self.h)
+            self.h = C.c_void_p()
 
     def __del__(self):
         try:

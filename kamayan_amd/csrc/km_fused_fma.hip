@@ -7,3 +7,4 @@
 #define KM_FUSED_NS fma
 #define KM_FUSED_T double
 #include "km_fused.inc"
+#include "km_stage3.inc"

@@ -7,3 +7,4 @@
 #define KM_FUSED_NS fp32
 #define KM_FUSED_T float
 #include "km_fused.inc"
+#include "km_stage3.inc"

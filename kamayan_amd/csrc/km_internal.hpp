@@ -111,18 +111,30 @@ int launch_fused_sweep(const km_cfg_t &cfg, const FusedArgs &a, int ndim, int di
                        int kern, hipStream_t st);
 int tile_shapes();
 int launch_fine_face_fluxes(const km_cfg_t &cfg, const FusedArgs &a, hipStream_t st);
+// the whole stage in one launch (3-D, single-level meshes), kern in 1..stage3_kernels()
+int launch_fused_stage3(const km_cfg_t &cfg, const FusedArgs &a, bool with_dt, int kern,
+                        hipStream_t st);
+int stage3_kernels();
 }
 namespace fma {
 int launch_fused_sweep(const km_cfg_t &cfg, const FusedArgs &a, int ndim, int dir, bool with_dt,
                        int kern, hipStream_t st);
 int tile_shapes();
 int launch_fine_face_fluxes(const km_cfg_t &cfg, const FusedArgs &a, hipStream_t st);
+// the whole stage in one launch (3-D, single-level meshes), kern in 1..stage3_kernels()
+int launch_fused_stage3(const km_cfg_t &cfg, const FusedArgs &a, bool with_dt, int kern,
+                        hipStream_t st);
+int stage3_kernels();
 }
 namespace fp32 {
 int launch_fused_sweep(const km_cfg_t &cfg, const FusedArgs &a, int ndim, int dir, bool with_dt,
                        int kern, hipStream_t st);
 int tile_shapes();
 int launch_fine_face_fluxes(const km_cfg_t &cfg, const FusedArgs &a, hipStream_t st);
+// the whole stage in one launch (3-D, single-level meshes), kern in 1..stage3_kernels()
+int launch_fused_stage3(const km_cfg_t &cfg, const FusedArgs &a, bool with_dt, int kern,
+                        hipStream_t st);
+int stage3_kernels();
 }
 
 // thread-local error message for km_last_error()

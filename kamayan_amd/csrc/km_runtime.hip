@@ -277,6 +277,9 @@ struct km_sim_t {
   // kernel of each fused sweep: -1 picked by the autotuner at the first cycle that starts
   // from conserved data, 0 the march, k >= 1 pencil-tile shape k
   int kern[3] = {-1, -1, -1};
+  // kernel of a whole fused stage on a 3-D single-level mesh: -1 picked by the autotuner,
+  // 0 the three sweeps above, k >= 1 the one-launch stage kernel variant k (km_stage3.inc)
+  int stage_kern = -1;
   // fused path on a block tree: the stages exchange only the ghost zones their sweeps read
   // (density and the primitives); momentum and energy ghosts are filled from the base
   // interiors when something reads them (state copies, tagging and remesh)
@@ -349,6 +352,40 @@ struct km_sim_t {
     if (graph) (void)hipGraphDestroy(graph);
     if (stream) (void)hipStreamDestroy(stream);
     if (hist) std::fclose(hist);
+  }
+};
+
+// Worklist of km_stage_fused: the mesh topology the fused kernels need beyond the caller's
+// packs (face-neighbour table and instance map of single-level meshes or ensembles, the
+// leaf lookup of a block tree), the FP64 du scratch of the sweeps and the kernel choices.
+struct km_stage_plan_t {
+  km_cfg_t cfg{};            // options of the calls the kernel choices were tuned for
+  km_mesh_desc_t mesh{};     // instance 0 (ndim, nghost, cells per block)
+  int path = KM_PATH_FUSED;  // the fused kernels only
+  int nb = 0, ninst = 1;
+  bool amr = false;
+  AmrGrid agrid{};
+  int *lookup_d = nullptr, *nbr_d = nullptr, *inst_d = nullptr;
+  double *du = nullptr;
+  int kern[3] = {-1, -1, -1};
+  int stage_kern = -1;
+  hipStream_t stream = nullptr;  // the current call's
+  std::vector<hipEvent_t> ev_free;
+  std::mutex mu;  // one call at a time per plan (the kernel choices and du are shared)
+  hipEvent_t ev() {
+    hipEvent_t e;
+    if (!ev_free.empty()) {
+      e = ev_free.back();
+      ev_free.pop_back();
+    } else {
+      (void)hipEventCreate(&e);
+    }
+    return e;
+  }
+  ~km_stage_plan_t() {
+    for (auto e : ev_free) (void)hipEventDestroy(e);
+    for (void *p : {(void *)lookup_d, (void *)nbr_d, (void *)inst_d, (void *)du})
+      if (p) (void)hipFree(p);
   }
 };
 
@@ -515,7 +552,8 @@ FusedArgs fused_args(km_sim_t &s, int stage, double beta) {
   return a;
 }
 
-int launch_sweep(const km_sim_t &s, const FusedArgs &a, int dir, bool last, int kern) {
+template <class S>
+int launch_sweep(const S &s, const FusedArgs &a, int dir, bool last, int kern) {
   const km_cfg_t &cfg = s.cfg;
   const int ndim = s.mesh.ndim;
   if (cfg.precision == KM_PREC_FP32_MIXED)
@@ -525,10 +563,32 @@ int launch_sweep(const km_sim_t &s, const FusedArgs &a, int dir, bool last, int 
   return exact::launch_fused_sweep(cfg, a, ndim, dir, last, kern, s.stream);
 }
 
-int launch_fine_faces(const km_sim_t &s, const FusedArgs &a) {
+template <class S>
+int launch_fine_faces(const S &s, const FusedArgs &a) {
   if (s.cfg.precision == KM_PREC_FP32_MIXED) return fp32::launch_fine_face_fluxes(s.cfg, a, s.stream);
   if (s.cfg.precision == KM_PREC_FP64_FMA) return fma::launch_fine_face_fluxes(s.cfg, a, s.stream);
   return exact::launch_fine_face_fluxes(s.cfg, a, s.stream);
+}
+
+template <class S>
+int launch_stage3(const S &s, const FusedArgs &a, bool last, int kern) {
+  const km_cfg_t &cfg = s.cfg;
+  if (cfg.precision == KM_PREC_FP32_MIXED)
+    return fp32::launch_fused_stage3(cfg, a, last, kern, s.stream);
+  if (cfg.precision == KM_PREC_FP64_FMA) return fma::launch_fused_stage3(cfg, a, last, kern, s.stream);
+  return exact::launch_fused_stage3(cfg, a, last, kern, s.stream);
+}
+
+int stage3_kernels(const km_cfg_t &cfg) {
+  if (cfg.precision == KM_PREC_FP32_MIXED) return fp32::stage3_kernels();
+  if (cfg.precision == KM_PREC_FP64_FMA) return fma::stage3_kernels();
+  return exact::stage3_kernels();
+}
+
+// whether the one-launch stage kernel applies: fused path, 3-D hydro, single level
+template <class S>
+bool stage3_applies(const S &s) {
+  return s.path == KM_PATH_FUSED && !s.amr && s.mesh.ndim == 3 && s.cfg.mhd == KM_MHD_OFF;
 }
 
 int tile_shapes(const km_cfg_t &cfg) {
@@ -548,12 +608,41 @@ int tile_shapes(const km_cfg_t &cfg) {
 std::mutex g_tune_mu;
 std::map<std::vector<int>, int> g_tune;
 
-int autotune(km_sim_t &s) {
+// Time fn (one launch sequence on the simulation's stream): a warm-up, then the fastest
+// of three HIP-event-timed runs into *ms; *ms = -1 when fn refuses (KM_EINVAL: the
+// variant does not fit the blocks) or the device refuses the launch.
+template <class S, class Fn>
+int time_launch(S &s, Fn &&fn, float *ms) {
+  *ms = -1.f;
+  if (fn() != KM_OK || hipGetLastError() != hipSuccess) {
+    (void)hipGetLastError();
+    return KM_OK;
+  }
+  hipEvent_t e0 = s.ev(), e1 = s.ev();
+  float best = 0.f;
+  for (int r = 0; r < 3; ++r) {
+    KM_HIP(hipEventRecord(e0, s.stream));
+    if (int rc = fn()) return rc;
+    KM_HIP(hipEventRecord(e1, s.stream));
+    KM_HIP(hipEventSynchronize(e1));
+    float t = 0.f;
+    KM_HIP(hipEventElapsedTime(&t, e0, e1));
+    best = r == 0 ? t : std::min(best, t);
+  }
+  s.ev_free.push_back(e0);
+  s.ev_free.push_back(e1);
+  if (int rc = check_launch()) return rc;
+  *ms = best;
+  return KM_OK;
+}
+
+// s: a simulation or a stage plan; a: the arguments of a stage on its data
+template <class S>
+int autotune(S &s, const FusedArgs &a) {
   const km_cfg_t &cfg = s.cfg;
   const int ndim = s.mesh.ndim;
   const int nshape = tile_shapes(cfg);
   std::lock_guard<std::mutex> lock(g_tune_mu);
-  const FusedArgs a = fused_args(s, 1, 1.0);
   for (int dir = 0; dir < 3; ++dir) {
     if (s.kern[dir] >= 0) continue;
     if (s.path != KM_PATH_FUSED || dir >= ndim) {
@@ -568,41 +657,113 @@ int autotune(km_sim_t &s) {
       s.kern[dir] = it->second;
       continue;
     }
-    hipEvent_t e0 = s.ev(), e1 = s.ev();
     int best = -1;
     float best_ms = 0.f;
     for (int k = s.amr ? 1 : 0; k <= nshape; ++k) {  // block trees run tiles only
-      // warm-up; a shape whose LDS does not fit the blocks, or that the device refuses,
-      // is skipped
-      if (launch_sweep(s, a, dir, false, k)) {
-        (void)hipGetLastError();
-        continue;
-      }
-      if (k > 0 && hipGetLastError() != hipSuccess) continue;
       float ms = 0.f;
-      for (int r = 0; r < 3; ++r) {
-        KM_HIP(hipEventRecord(e0, s.stream));
-        if (int rc = launch_sweep(s, a, dir, false, k)) return rc;
-        KM_HIP(hipEventRecord(e1, s.stream));
-        KM_HIP(hipEventSynchronize(e1));
-        float t = 0.f;
-        KM_HIP(hipEventElapsedTime(&t, e0, e1));
-        ms = r == 0 ? t : std::min(ms, t);
-      }
-      if (int rc = check_launch()) return rc;
+      if (int rc = time_launch(s, [&] { return launch_sweep(s, a, dir, false, k); }, &ms))
+        return rc;
+      if (ms < 0.f) continue;  // does not fit the blocks
       if (best < 0 || ms < best_ms) {
         best = k;
         best_ms = ms;
       }
     }
-    s.ev_free.push_back(e0);
-    s.ev_free.push_back(e1);
     if (best < 0) {
       set_error("no fused sweep kernel fits these blocks");
       return KM_EINVAL;
     }
     g_tune[key] = best;
     s.kern[dir] = best;
+  }
+  // the whole stage: the three tuned sweeps against each one-launch stage variant
+  if (s.stage_kern < 0) {
+    if (!stage3_applies(s)) {
+      s.stage_kern = 0;
+      return KM_OK;
+    }
+    const std::vector<int> key = {cfg.precision, cfg.recon, cfg.limiter, cfg.riemann, ndim,
+                                  s.mesh.nx[0], s.mesh.nx[1], s.mesh.nx[2], s.nb, 3, 0};
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+      s.stage_kern = it->second;
+      return KM_OK;
+    }
+    int best = 0;
+    float best_ms = 0.f;
+    if (int rc = time_launch(s, [&] {
+          for (int dir = 0; dir < ndim; ++dir)
+            if (int r = launch_sweep(s, a, dir, false, s.kern[dir])) return r;
+          return (int)KM_OK;
+        }, &best_ms))
+      return rc;
+    for (int k = 1; k <= stage3_kernels(cfg); ++k) {
+      float ms = 0.f;
+      if (int rc = time_launch(s, [&] { return launch_stage3(s, a, false, k); }, &ms)) return rc;
+      if (ms >= 0.f && (best_ms < 0.f || ms < best_ms)) {
+        best = k;
+        best_ms = ms;
+      }
+    }
+    g_tune[key] = best;
+    s.stage_kern = best;
+  }
+  return KM_OK;
+}
+
+// HIP-event records of each sweep of a simulation that profiles (none otherwise)
+auto sweep_recorder(km_sim_t &s) {
+  return [&s](int dir, bool begin) {
+    if (!s.profile) return;
+    if (begin) {
+      s.recs.push_back({dir, s.ev(), s.ev()});
+      (void)hipEventRecord(s.recs.back().a, s.stream);
+    } else {
+      (void)hipEventRecord(s.recs.back().b, s.stream);
+    }
+  };
+}
+
+// The kernels of one fused stage on a single-level mesh (shared by the simulation driver
+// and km_stage_fused): the one-launch stage kernel or the three sweeps, and on the last
+// stage the CFL minimum of u1 into est (per instance, scaled by cfl / ndim).
+// rec(dir, begin) brackets each sweep (HIP-event profiling).
+template <class S, class Rec>
+int stage_launch(S &s, const FusedArgs &a, bool last, int ninst, double *est, Rec &&rec) {
+  const int ndim = s.mesh.ndim;
+  if (last)
+    hipLaunchKernelGGL(k_set, dim3(nb64(ninst)), dim3(64), 0, s.stream, est, DBL_MAX, ninst);
+  if (s.stage_kern > 0 && stage3_applies(s)) {
+    if (int rc = launch_stage3(s, a, last, s.stage_kern)) {
+      set_error("the one-launch stage kernel " + std::to_string(s.stage_kern) +
+                " does not fit these blocks");
+      return rc;
+    }
+  } else {
+    for (int dir = 0; dir < ndim; ++dir) {
+      rec(dir, true);
+      if (int rc = launch_sweep(s, a, dir, last, s.kern[dir] > 0 ? s.kern[dir] : 0)) return rc;
+      rec(dir, false);
+    }
+  }
+  if (last)
+    hipLaunchKernelGGL(k_scale, dim3(nb64(ninst)), dim3(64), 0, s.stream, est,
+                       s.cfg.cfl / static_cast<double>(ndim), ninst);
+  return KM_OK;
+}
+
+// The fused kernels of one stage on a block-tree mesh up to u1 (shared by the simulation
+// driver and km_stage_fused): the fluxes of faces where a block borders a coarser leaf,
+// their restriction onto the coarse face, then the pencil-tile sweeps, which take that flux
+// at their end faces.
+template <class S, class Rec>
+int amr_stage_launch(S &s, const FusedArgs &a, int nflux, Rec &&rec) {
+  if (int rc = launch_fine_faces(s, a)) return rc;
+  launch_amr_flux_correct(s.agrid, a.u0, nflux, s.stream);
+  for (int dir = 0; dir < s.mesh.ndim; ++dir) {
+    rec(dir, true);
+    if (int rc = launch_sweep(s, a, dir, false, s.kern[dir] > 0 ? s.kern[dir] : 1)) return rc;
+    rec(dir, false);
   }
   return KM_OK;
 }
@@ -620,28 +781,13 @@ int amr_fused_stage(km_sim_t &s, int stage, double beta) {
   const km_cfg_t &cfg = s.cfg;
   Cont &md1 = container(s, stage);
   const FusedArgs a = fused_args(s, stage, beta);
-  const int ndim = s.mesh.ndim;
   hipEvent_t st0 = nullptr, st1 = nullptr;
   if (s.profile) {
     st0 = s.ev();
     st1 = s.ev();
     (void)hipEventRecord(st0, s.stream);
   }
-  if (int rc = launch_fine_faces(s, a)) return rc;
-  launch_amr_flux_correct(s.agrid, a.u0, s.nflux, s.stream);
-  for (int dir = 0; dir < ndim; ++dir) {
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (s.profile) {
-      e0 = s.ev();
-      e1 = s.ev();
-      (void)hipEventRecord(e0, s.stream);
-    }
-    if (int rc = launch_sweep(s, a, dir, false, s.kern[dir] > 0 ? s.kern[dir] : 1)) return rc;
-    if (s.profile) {
-      (void)hipEventRecord(e1, s.stream);
-      s.recs.push_back({dir, e0, e1});
-    }
-  }
+  if (int rc = amr_stage_launch(s, a, s.nflux, sweep_recorder(s))) return rc;
   launch_prepare_primitive(cfg, md1.pack, s.stream);
   if (stage == cfg.nstages)
     launch_estimate_dt(cfg, s.base.pack, s.est_d, s.stream, s.inst_d, s.ninst, s.fail_d);
@@ -675,32 +821,13 @@ int fused_stage(km_sim_t &s, int stage, double beta) {
   const km_cfg_t &cfg = s.cfg;
   const bool last = stage == cfg.nstages;
   const FusedArgs a = fused_args(s, stage, beta);
-  const int ndim = s.mesh.ndim;
   hipEvent_t st0 = nullptr, st1 = nullptr;
   if (s.profile) {
     st0 = s.ev();
     st1 = s.ev();
     (void)hipEventRecord(st0, s.stream);
   }
-  if (last)
-    hipLaunchKernelGGL(k_set, dim3(nb64(s.ninst)), dim3(64), 0, s.stream, s.est_d, DBL_MAX,
-                       s.ninst);
-  for (int dir = 0; dir < ndim; ++dir) {
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (s.profile) {
-      e0 = s.ev();
-      e1 = s.ev();
-      (void)hipEventRecord(e0, s.stream);
-    }
-    if (int rc = launch_sweep(s, a, dir, last, s.kern[dir] > 0 ? s.kern[dir] : 0)) return rc;
-    if (s.profile) {
-      (void)hipEventRecord(e1, s.stream);
-      s.recs.push_back({dir, e0, e1});
-    }
-  }
-  if (last)
-    hipLaunchKernelGGL(k_scale, dim3(nb64(s.ninst)), dim3(64), 0, s.stream, s.est_d,
-                       cfg.cfl / static_cast<double>(ndim), s.ninst);
+  if (int rc = stage_launch(s, a, last, s.ninst, s.est_d, sweep_recorder(s))) return rc;
   if (rk1_fused(s)) rk1_copy_back(s);
   if (s.profile) {
     (void)hipEventRecord(st1, s.stream);
@@ -1164,6 +1291,43 @@ int km_flux_correct(const km_ghost_plan_t *plan, const km_pack_t *u, void *strea
 }
 
 // ---- whole-mesh driver ----
+// Face-neighbour table [nb][3][2] of independent uniform block grids stored one after the
+// other, each x-fastest: a block index (periodic wraps included) or KM_NBR_OUTFLOW /
+// KM_NBR_REFLECT at a physical boundary (mapped inside the fused kernels).
+static std::vector<int> uniform_neighbours(const km_mesh_desc_t *meshes, int ninst, int nb) {
+  std::vector<int> nbr((size_t)nb * 6);
+  int first = 0;
+  for (int m = 0; m < ninst; ++m) {
+    const km_mesh_desc_t &me = meshes[m];
+    const int nbm = me.nb[0] * me.nb[1] * me.nb[2];
+    for (int lb = 0; lb < nbm; ++lb) {
+      const int l[3] = {lb % me.nb[0], (lb / me.nb[0]) % me.nb[1], lb / (me.nb[0] * me.nb[1])};
+      for (int d = 0; d < 3; ++d)
+        for (int side = 0; side < 2; ++side) {
+          int mm[3] = {l[0], l[1], l[2]};
+          mm[d] += side ? 1 : -1;
+          int val;
+          if (d >= me.ndim) {
+            val = KM_NBR_OUTFLOW;
+          } else if (mm[d] < 0 || mm[d] >= me.nb[d]) {
+            const int bc = me.bc[2 * d + side];
+            if (bc == KM_BC_PERIODIC) {
+              mm[d] = (mm[d] + me.nb[d]) % me.nb[d];
+              val = first + (mm[2] * me.nb[1] + mm[1]) * me.nb[0] + mm[0];
+            } else {
+              val = bc == KM_BC_REFLECT ? KM_NBR_REFLECT : KM_NBR_OUTFLOW;
+            }
+          } else {
+            val = first + (mm[2] * me.nb[1] + mm[1]) * me.nb[0] + mm[0];
+          }
+          nbr[(size_t)(first + lb) * 6 + d * 2 + side] = val;
+        }
+    }
+    first += nbm;
+  }
+  return nbr;
+}
+
 static int sim_create(const km_cfg_t *cfg, const km_mesh_desc_t *meshes, int32_t ninst,
                       int32_t path, const AmrTreeHost *tree, km_sim_t **out) {
   if (!cfg || !meshes || !out || ninst < 1) return KM_EINVAL;
@@ -1290,36 +1454,7 @@ static int sim_create(const km_cfg_t *cfg, const km_mesh_desc_t *meshes, int32_t
     if ((rc = s->alloc(&s->du, (size_t)s->nb * 5 * nint))) return rc;
   }
   if (path == KM_PATH_FUSED && !tree) {
-    // face-neighbour table of every instance's uniform block grid (periodic wraps; other
-    // boundaries are mapped inside the sweep)
-    std::vector<int> nbr((size_t)s->nb * 6);
-    for (int m = 0; m < ninst; ++m) {
-      const km_mesh_desc_t &me = meshes[m];
-      const int nbm = me.nb[0] * me.nb[1] * me.nb[2];
-      for (int lb = 0; lb < nbm; ++lb) {
-        const int l[3] = {lb % me.nb[0], (lb / me.nb[0]) % me.nb[1], lb / (me.nb[0] * me.nb[1])};
-        for (int d = 0; d < 3; ++d)
-          for (int side = 0; side < 2; ++side) {
-            int mm[3] = {l[0], l[1], l[2]};
-            mm[d] += side ? 1 : -1;
-            int val;
-            if (d >= me.ndim) {
-              val = KM_NBR_OUTFLOW;
-            } else if (mm[d] < 0 || mm[d] >= me.nb[d]) {
-              const int bc = me.bc[2 * d + side];
-              if (bc == KM_BC_PERIODIC) {
-                mm[d] = (mm[d] + me.nb[d]) % me.nb[d];
-                val = first[m] + (mm[2] * me.nb[1] + mm[1]) * me.nb[0] + mm[0];
-              } else {
-                val = bc == KM_BC_REFLECT ? KM_NBR_REFLECT : KM_NBR_OUTFLOW;
-              }
-            } else {
-              val = first[m] + (mm[2] * me.nb[1] + mm[1]) * me.nb[0] + mm[0];
-            }
-            nbr[(size_t)(first[m] + lb) * 6 + d * 2 + side] = val;
-          }
-      }
-    }
+    const std::vector<int> nbr = uniform_neighbours(meshes, ninst, s->nb);
     if ((rc = s->alloc(&s->nbr_d, nbr.size()))) return rc;
     KM_HIP(hipMemcpy(s->nbr_d, nbr.data(), nbr.size() * sizeof(int), hipMemcpyHostToDevice));
   }
@@ -1469,8 +1604,9 @@ int km_sim_set_dt(km_sim_t *s, double dt) {
 int km_sim_advance(km_sim_t *s, int32_t ncycles, double tlim) {
   if (!s) return KM_EINVAL;
   for (int n = 0; n < ncycles; ++n) {
-    if (!s->prims_from_ic && (s->kern[0] < 0 || s->kern[1] < 0 || s->kern[2] < 0))
-      if (int rc = autotune(*s)) return rc;
+    if (!s->prims_from_ic &&
+        (s->kern[0] < 0 || s->kern[1] < 0 || s->kern[2] < 0 || s->stage_kern < 0))
+      if (int rc = autotune(*s, fused_args(*s, 1, 1.0))) return rc;
     // a cycle needs the host when it reads dt there (tlim, an override), writes history,
     // starts from the initial primitives or may remesh; every other cycle of either path
     // is one captured graph replayed (the task path keeps dt on the device as well)
@@ -1536,6 +1672,18 @@ int km_sim_advance(km_sim_t *s, int32_t ncycles, double tlim) {
 
 int km_sim_set_fused_kernel(km_sim_t *s, int32_t dir, int32_t kern) {
   if (!s) return KM_EINVAL;
+  if (dir == 3) {  // the whole stage
+    if (kern < -1 || kern > stage3_kernels(s->cfg) || (kern > 0 && !stage3_applies(*s))) {
+      set_error("stage kernel: -1 (autotune), 0 (three sweeps) or a one-launch variant 1.." +
+                std::to_string(stage3_kernels(s->cfg)) +
+                " (fused path, 3-D hydro, single-level meshes)");
+      return KM_EINVAL;
+    }
+    KM_HIP(hipStreamSynchronize(s->stream));
+    s->stage_kern = kern;
+    s->drop_graph();
+    return KM_OK;
+  }
   if (dir < -1 || dir > 2 || kern < -1 || kern > tile_shapes(s->cfg)) {
     set_error("fused sweep kernel: direction 0..2 (or -1 for all) and kernel -1 (autotune), "
               "0 (march) or a pencil-tile shape 1.." + std::to_string(tile_shapes(s->cfg)));
@@ -1549,13 +1697,15 @@ int km_sim_set_fused_kernel(km_sim_t *s, int32_t dir, int32_t kern) {
   KM_HIP(hipStreamSynchronize(s->stream));
   for (int d = 0; d < 3; ++d)
     if (dir < 0 || d == dir) s->kern[d] = kern;
+  // a fixed sweep kernel runs the stage as sweeps; autotuning picks the stage kernel too
+  s->stage_kern = kern >= 0 ? 0 : -1;
   s->drop_graph();
   return KM_OK;
 }
 
 int km_sim_fused_kernel(km_sim_t *s, int32_t dir, int32_t *kern) {
-  if (!s || !kern || dir < 0 || dir > 2) return KM_EINVAL;
-  *kern = s->kern[dir];
+  if (!s || !kern || dir < 0 || dir > 3) return KM_EINVAL;
+  *kern = dir == 3 ? s->stage_kern : s->kern[dir];
   return KM_OK;
 }
 
@@ -1732,6 +1882,196 @@ int km_amr_remesh_geom(const km_pack_t *old_u, const km_pack_t *new_u, const int
   if (faces)
     launch_amr_remesh_faces(*old_u, *new_u, origin_dev, cyl, cyl ? mesh->xmin[0] : 0.0,
                             as_stream(stream));
+  return check_launch();
+}
+
+// ---- the fused stage on caller-owned packs ----
+static int plan_checks(const km_cfg_t *cfg, const km_mesh_desc_t *mesh) {
+  if (!cfg || !mesh) return KM_EINVAL;
+  if (cfg->mhd != KM_MHD_OFF || cfg->nspecies > 1 || cfg->geometry != KM_GEOM_CARTESIAN) {
+    set_error("the fused stage is built for Cartesian hydro with one species; MHD, species "
+              "and cylindrical geometry run on the per-task entry points");
+    return KM_EINVAL;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    set_error("no GPU visible");
+    return KM_ENODEV;
+  }
+  return KM_OK;
+}
+
+static int plan_buffers(km_stage_plan_t &p) {
+  const long nint = (long)p.mesh.nx[0] * (p.mesh.ndim > 1 ? p.mesh.nx[1] : 1) *
+                    (p.mesh.ndim > 2 ? p.mesh.nx[2] : 1);
+  KM_HIP(hipMalloc(&p.du, (size_t)p.nb * 5 * nint * sizeof(double) + 256));
+  return KM_OK;
+}
+
+int km_stage_plan_uniform(const km_cfg_t *cfg, const km_mesh_desc_t *meshes, int32_t ninst,
+                          km_stage_plan_t **plan) {
+  if (!plan || ninst < 1) return KM_EINVAL;
+  if (int rc = plan_checks(cfg, meshes)) return rc;
+  auto p = std::make_unique<km_stage_plan_t>();
+  p->cfg = *cfg;
+  p->mesh = meshes[0];
+  p->cfg.ndim = meshes[0].ndim;
+  p->ninst = ninst;
+  std::vector<int> inst;
+  for (int m = 0; m < ninst; ++m) {
+    bool same = meshes[m].ndim == p->mesh.ndim && meshes[m].ng == p->mesh.ng;
+    for (int d = 0; d < p->mesh.ndim; ++d) same = same && meshes[m].nx[d] == p->mesh.nx[d];
+    for (int d = 0; d < meshes[m].ndim; ++d)
+      if ((meshes[m].bc[2 * d] == KM_BC_PERIODIC) != (meshes[m].bc[2 * d + 1] == KM_BC_PERIODIC))
+        same = false;
+    if (!same) {
+      set_error("instances must share ndim, nghost and cells per block, with paired periodic "
+                "boundaries");
+      return KM_EINVAL;
+    }
+    const int nbm = meshes[m].nb[0] * meshes[m].nb[1] * meshes[m].nb[2];
+    inst.insert(inst.end(), nbm, m);
+    p->nb += nbm;
+  }
+  const std::vector<int> nbr = uniform_neighbours(meshes, ninst, p->nb);
+  KM_HIP(hipMalloc(&p->nbr_d, nbr.size() * sizeof(int)));
+  KM_HIP(hipMemcpy(p->nbr_d, nbr.data(), nbr.size() * sizeof(int), hipMemcpyHostToDevice));
+  if (ninst > 1) {
+    KM_HIP(hipMalloc(&p->inst_d, inst.size() * sizeof(int)));
+    KM_HIP(hipMemcpy(p->inst_d, inst.data(), inst.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
+  if (int rc = plan_buffers(*p)) return rc;
+  *plan = p.release();
+  return KM_OK;
+}
+
+int km_stage_plan_amr(const km_cfg_t *cfg, const km_mesh_desc_t *mesh, const int32_t *leaves,
+                      int32_t nleaves, km_stage_plan_t **plan) {
+  if (!plan || !leaves || nleaves < 1) return KM_EINVAL;
+  if (int rc = plan_checks(cfg, mesh)) return rc;
+  AmrTreeHost t;
+  const std::string err = build_amr_tree(*mesh, leaves, nleaves, t);
+  if (!err.empty()) {
+    set_error(err);
+    return KM_EINVAL;
+  }
+  auto p = std::make_unique<km_stage_plan_t>();
+  p->cfg = *cfg;
+  p->mesh = *mesh;
+  p->cfg.ndim = mesh->ndim;
+  p->nb = nleaves;
+  p->amr = true;
+  KM_HIP(hipMalloc(&p->lookup_d, t.lookup.size() * sizeof(int)));
+  KM_HIP(hipMemcpy(p->lookup_d, t.lookup.data(), t.lookup.size() * sizeof(int),
+                   hipMemcpyHostToDevice));
+  p->agrid = t.grid(p->lookup_d);
+  if (int rc = plan_buffers(*p)) return rc;
+  *plan = p.release();
+  return KM_OK;
+}
+
+int km_stage_plan_destroy(km_stage_plan_t *plan) {
+  delete plan;
+  return KM_OK;
+}
+
+int km_stage_plan_set_kernel(km_stage_plan_t *p, int32_t dir, int32_t kern) {
+  if (!p) return KM_EINVAL;
+  std::lock_guard<std::mutex> lock(p->mu);
+  if (dir == 3) {
+    if (kern < -1 || kern > stage3_kernels(p->cfg) || (kern > 0 && !stage3_applies(*p))) {
+      set_error("stage kernel: -1 (autotune), 0 (three sweeps) or a one-launch variant 1.." +
+                std::to_string(stage3_kernels(p->cfg)) + " (3-D single-level meshes)");
+      return KM_EINVAL;
+    }
+    p->stage_kern = kern;
+    return KM_OK;
+  }
+  if (dir < -1 || dir > 2 || kern < -1 || kern > tile_shapes(p->cfg) || (kern == 0 && p->amr)) {
+    set_error("sweep kernel: direction 0..2 (or -1 for all) and kernel -1 (autotune), 0 (march, "
+              "single-level meshes) or a pencil-tile shape 1.." + std::to_string(tile_shapes(p->cfg)));
+    return KM_EINVAL;
+  }
+  for (int d = 0; d < 3; ++d)
+    if (dir < 0 || d == dir) p->kern[d] = kern;
+  p->stage_kern = kern >= 0 ? 0 : -1;
+  return KM_OK;
+}
+
+int km_stage_plan_kernel(km_stage_plan_t *p, int32_t dir, int32_t *kern) {
+  if (!p || !kern || dir < 0 || dir > 3) return KM_EINVAL;
+  std::lock_guard<std::mutex> lock(p->mu);
+  *kern = dir == 3 ? p->stage_kern : p->kern[dir];
+  return KM_OK;
+}
+
+int km_stage_fused(const km_cfg_t *cfg, km_stage_plan_t *p, const km_pack_t *base,
+                   const km_pack_t *u0, const km_pack_t *u1, double beta, const double *dt_dev,
+                   double *dt_est_dev, void *stream) {
+  if (!cfg || !p || !base || !u0 || !u1 || !dt_dev) {
+    set_error("null argument");
+    return KM_EINVAL;
+  }
+  if (int rc = check_cfg(cfg, u0)) return rc;
+  if (cfg->mhd != KM_MHD_OFF || cfg->nspecies > 1 || cfg->geometry != KM_GEOM_CARTESIAN) {
+    set_error("the fused stage is built for Cartesian hydro with one species");
+    return KM_EINVAL;
+  }
+  for (const km_pack_t *q : {base, u0, u1}) {
+    bool ok = q->nblocks == p->nb && q->ndim == p->mesh.ndim && q->ng == p->mesh.ng &&
+              q->cell && q->blocks && q->nvar >= KM_NVAR(0);
+    for (int d = 0; d < q->ndim; ++d) ok = ok && q->nx[d] == p->mesh.nx[d];
+    if (!ok) {
+      set_error("pack shape does not match the stage plan");
+      return KM_EINVAL;
+    }
+  }
+  if (u1->cell == u0->cell) {
+    set_error("u1 must not be u0: the stage reads u0 across block faces while it writes u1");
+    return KM_EINVAL;
+  }
+  if (p->amr) {
+    for (int d = 0; d < u0->ndim; ++d)
+      if (!u0->flux[d] || u0->nflux < 5) {
+        set_error("on a block tree u0 needs flux tables (the fluxes at level jumps)");
+        return KM_EINVAL;
+      }
+    if (dt_est_dev) {
+      set_error("on a block tree the CFL estimate follows PreparePrimitive: call "
+                "km_prepare_primitive and km_estimate_dt after the stage");
+      return KM_EINVAL;
+    }
+  }
+  std::lock_guard<std::mutex> lock(p->mu);
+  if (cfg->precision != p->cfg.precision || cfg->recon != p->cfg.recon ||
+      cfg->limiter != p->cfg.limiter || cfg->riemann != p->cfg.riemann) {
+    for (int d = 0; d < 3; ++d) p->kern[d] = -1;
+    p->stage_kern = -1;
+  }
+  p->cfg = *cfg;
+  p->cfg.ndim = p->mesh.ndim;
+  p->stream = as_stream(stream);
+  FusedArgs a;
+  a.u0 = *u0;
+  a.base = *base;
+  a.u1 = *u1;
+  a.du = p->du;
+  a.dt = dt_dev;
+  a.dt_est = dt_est_dev;
+  a.inst = p->inst_d;
+  a.nbr = p->nbr_d;
+  a.beta = beta;
+  a.lim = cfg->limiter;
+  a.eos = make_eos(*cfg);
+  a.amr = p->amr ? 1 : 0;
+  a.agrid = p->agrid;
+  a.fail = nullptr;
+  if (p->kern[0] < 0 || p->kern[1] < 0 || p->kern[2] < 0 || p->stage_kern < 0)
+    if (int rc = autotune(*p, a)) return rc;
+  auto none = [](int, bool) {};
+  const int rc = p->amr ? amr_stage_launch(*p, a, u0->nflux, none)
+                        : stage_launch(*p, a, dt_est_dev != nullptr, p->ninst, dt_est_dev, none);
+  if (rc) return rc;
   return check_launch();
 }
 
