@@ -150,7 +150,9 @@ def test_kernel_choice_refusals_and_one_dimension(gpu, oracle):
     with pytest.raises(km.KmError):
         sim.set_fused_kernel(0, -2)
     with pytest.raises(km.KmError):
-        sim.set_fused_kernel(3, 1)
+        sim.set_fused_kernel(4, 1)  # directions 0..2, and 3 for the whole stage
+    with pytest.raises(km.KmError):
+        sim.set_fused_kernel(3, 99)
     sim.close()
     # one dimension, RK3: x1 is both the first and the final sweep; every shape is
     # bitwise the oracle in FP64

@@ -33,12 +33,38 @@ def _packs(km, lay, blocks, nstages, nflux=0):
     return base, stages
 
 
+def _task_cycle(km, cfg, gplan, base, stages, dudt, dt, est, amr=False):
+    """The first cycle on the per-task entry points, in the reference's task order
+    (kamayan_driver.cpp:186-248): its first stage reconstructs the problem generator's
+    primitives, which the fused kernels (they recompute primitives from the conserved
+    state) would round differently. Returns the step's dt."""
+    L, p = km.lib(), C.byref
+    n = cfg.c.nstages
+    cont = [base] + stages + [base]
+    dts = dt.get()
+    temp = km.slot("TEMP")
+    for s in range(1, n + 1):
+        u0, u1 = cont[s - 1], cont[s]
+        km.check(L.km_calculate_fluxes(p(cfg.c), p(u0.pack), None), "fluxes")
+        if amr:
+            km.check(L.km_flux_correct(gplan, p(u0.pack), None), "flux correction")
+        km.check(L.km_fluxes_to_dudt(p(cfg.c), p(u0.pack), p(dudt.pack), None), "dudt")
+        km.check(L.km_apply_dudt(p(cfg.c), p(base.pack), p(u0.pack), p(u1.pack), p(dudt.pack),
+                                 BETAS[n][s - 1], float(dts[0]), None), "apply")
+        km.check(L.km_prepare_primitive(p(cfg.c), p(u1.pack), None), "prim")
+        if s == n:
+            km.check(L.km_estimate_dt(p(cfg.c), p(base.pack), C.c_void_p(est.ptr), None), "dt")
+        km.check(L.km_ghost_fill(gplan, p(u1.pack), 0, temp, 0, None), "ghost")
+    dt.set(np.minimum(2.0 * dts, est.get()))
+    return dts
+
+
 def _cycles(km, cfg, plan, base, stages, dt, est, ninst, ncyc, after_stage=None,
-            last_estimate=None):
-    """ncyc cycles; returns (time, dt) per instance on the host"""
+            last_estimate=None, time=None):
+    """ncyc cycles of km_stage_fused; returns (time, dt) per instance on the host"""
     L = km.lib()
     n = cfg.c.nstages
-    time = np.zeros(ninst)
+    time = np.zeros(ninst) if time is None else time
     cont = [base] + stages + [base]
     for _ in range(ncyc):
         dts = dt.get()
@@ -76,18 +102,24 @@ def test_uniform_3d_cycles_bitwise(gpu, oracle, choice):
         plan.set_kernel(-1, 3)
     elif choice.startswith("stage-"):
         plan.set_kernel(3, int(choice[6:]))
-    base, stages = _packs(km, lay, blocks, 3)
+    base, stages = _packs(km, lay, blocks, 3, nflux=5)
+    dudt = km.DevicePack(lay, km.nvar(0), blocks=blocks)
     base.upload(cell=cell)
-    L = km.lib()
-    km.check(L.km_prepare_conserved(C.byref(cfg.c), C.byref(base.pack), None), "prep")
+    L, p = km.lib(), C.byref
+    gplan = C.c_void_p()
+    nb, bc = (C.c_int32 * 3)(*lay.nb), (C.c_int32 * 6)(*([0] * 6))
+    km.check(L.km_ghost_plan_uniform(p(base.pack), nb, bc, p(gplan)), "ghost plan")
+    km.check(L.km_prepare_conserved(p(cfg.c), p(base.pack), None), "prep")
+    km.check(L.km_ghost_fill(gplan, p(base.pack), 0, km.slot("TEMP"), 0, None), "ghost")
     dt, est = km.DeviceArray(1), km.DeviceArray(1)
-    km.check(L.km_estimate_dt(C.byref(cfg.c), C.byref(base.pack), C.c_void_p(dt.ptr), None),
-             "dt")
+    km.check(L.km_estimate_dt(p(cfg.c), p(base.pack), C.c_void_p(dt.ptr), None), "dt")
     ref = oracle.CpuSimulation(cfg.c, desc, lay)
     ref.set_primitive(cell)
     assert dt.get()[0] == ref.info()[1]
-    t, d = _cycles(km, cfg, plan, base, stages, dt, est, 1, 3)
-    ref.advance(3)
+    t0 = _task_cycle(km, cfg, gplan, base, stages, dudt, dt, est)
+    t, d = _cycles(km, cfg, plan, base, stages, dt, est, 1, 3, time=t0)
+    L.km_ghost_plan_destroy(gplan)
+    ref.advance(4)
     rt, rd, _ = ref.info()
     assert (t[0], d[0]) == (rt, rd)
     g = base.download()[0]
@@ -113,6 +145,10 @@ def test_ensemble_2d_cycles_bitwise(gpu, oracle):
     print("km_stage_fused 2-D ensemble", flush=True)
     plan = km.StagePlan(cfg, descs=descs)
     base, stages = _packs(km, lay_all, blocks, cfg.c.nstages)
+    # start from primitives that the conserved state reproduces exactly: the fused kernels
+    # recompute primitives from the conserved state, where the reference's first stage
+    # reads the problem generator's
+    cell = _consistent_primitives(km, oracle, cfg, lay_all, blocks, cell)
     base.upload(cell=cell)
     L = km.lib()
     km.check(L.km_prepare_conserved(C.byref(cfg.c), C.byref(base.pack), None), "prep")
@@ -142,6 +178,23 @@ def test_ensemble_2d_cycles_bitwise(gpu, oracle):
     plan.close()
 
 
+def _consistent_primitives(km, oracle, cfg, lay, blocks, cell):
+    """Primitives P with cons2prim(prim2cons(P)) == P bit for bit on the interior: iterate
+    PostMeshInitialization and PreparePrimitive on the oracle until the primitives stop
+    changing (a few rounds: each round only moves values by an ulp)."""
+    nv = km.nvar(0)
+    host = oracle.HostPack(lay, nv, blocks=blocks)
+    host.cell[...] = cell
+    sl = (slice(None), slice(None)) + lay.interior()
+    for _ in range(20):
+        before = host.cell.copy()
+        oracle.call("km_cpu_prepare_conserved", C.byref(cfg.c), C.byref(host.pack), None)
+        oracle.call("km_cpu_prepare_primitive", C.byref(cfg.c), C.byref(host.pack), None)
+        if np.array_equal(before[sl], host.cell[sl]):
+            return host.cell.copy()
+    pytest.fail("primitives did not settle")
+
+
 def test_block_tree_cycles_bitwise(gpu, oracle):
     km = gpu
     cfg, desc, lay, leaves, cell = _cases.amr_case(ndim=2, nroot=2, nblock=16, ng=4, lmax=2,
@@ -152,6 +205,8 @@ def test_block_tree_cycles_bitwise(gpu, oracle):
     print(f"km_stage_fused block tree, {len(leaves)} leaves", flush=True)
     plan = km.StagePlan(cfg, desc=desc, leaves=leaves)
     base, stages = _packs(km, lay, blocks, cfg.c.nstages, nflux=5)
+    # the block-tree stage reads the stored primitives (PostMeshInitialization's, then each
+    # stage's PreparePrimitive and ghost exchange), as the reference's stages do
     base.upload(cell=cell)
     L = km.lib()
     p = C.byref
