@@ -196,17 +196,17 @@ KM_HD int recon_slot(int r) {
 }
 
 // runtime limiter choice (uniform across a wavefront) over a compile-time method
-template <int RECON, class T>
+template <int RECON, bool FAST = false, class T>
 KM_HD void reconstruct_rt(int lim, const T *w, T &vM, T &vP) {
   if constexpr (RECON == RECON_PLM || RECON == RECON_PPM) {
     if (lim == LIM_MC)
-      reconstruct<RECON, LIM_MC>(w, vM, vP);
+      reconstruct<RECON, LIM_MC, FAST>(w, vM, vP);
     else if (lim == LIM_VAN_LEER)
-      reconstruct<RECON, LIM_VAN_LEER>(w, vM, vP);
+      reconstruct<RECON, LIM_VAN_LEER, FAST>(w, vM, vP);
     else
-      reconstruct<RECON, LIM_MINMOD>(w, vM, vP);
+      reconstruct<RECON, LIM_MINMOD, FAST>(w, vM, vP);
   } else {
-    reconstruct<RECON, LIM_MINMOD>(w, vM, vP);
+    reconstruct<RECON, LIM_MINMOD, FAST>(w, vM, vP);
   }
 }
 
@@ -249,8 +249,8 @@ __device__ __forceinline__ void march(int lim, int c0, int c1, Load &&load, Face
 // overlaps the current step's reconstruction and Riemann solve.
 // fetch(c) -> R (raw loads, no arithmetic); unpack(const R&, T p[NR]).
 // fetch0 serves the window prefill (cells c0-W .. c0+W), fetch the steps after it.
-template <class T, int RECON, int NR, class R, class Fetch0, class Fetch, class Unpack,
-          class Face>
+template <class T, int RECON, int NR, class R, bool FAST = false, class Fetch0, class Fetch,
+          class Unpack, class Face>
 __device__ __forceinline__ void march_pf(int lim, int c0, int c1, Fetch0 &&fetch0, Fetch &&fetch,
                                          Unpack &&unpack, Face &&face) {
   constexpr int W = ReconWidth<RECON>::value;
@@ -275,7 +275,7 @@ __device__ __forceinline__ void march_pf(int lim, int c0, int c1, Fetch0 &&fetch
     }
     T vM[NR], vP[NR];
 #pragma unroll
-    for (int r = 0; r < NR; ++r) reconstruct_rt<RECON>(lim, win[r], vM[r], vP[r]);
+    for (int r = 0; r < NR; ++r) reconstruct_rt<RECON, FAST>(lim, win[r], vM[r], vP[r]);
     if (c > c0) face(c, vPprev, vM);
 #pragma unroll
     for (int r = 0; r < NR; ++r) {

@@ -6,5 +6,6 @@
 
 #define KM_FUSED_NS fma
 #define KM_FUSED_T double
+#define KM_FUSED_FAST 1
 #include "km_fused.inc"
 #include "km_stage3.inc"

@@ -45,15 +45,44 @@ KM_HD float ksign(float x) { return __builtin_copysignf(1.0f, x); }
 // Division and square root: IEEE for double (the reference's rounding); the FP32 form
 // uses the hardware reciprocal / square root, whose ~1 ulp error is far below the
 // FP32 form's own truncation of the states.
-KM_HD double kdiv(double a, double b) { return a / b; }
-KM_HD double krcp(double b) { return 1.0 / b; }
+// FAST (the FP64-FMA form): the hardware reciprocal estimate refined by two Newton steps
+// (each doubles the correct bits), and for a quotient one residual correction
+// q + (a - b q) / b; within an ulp or so of the IEEE result for normal operands, which
+// every divisor here is (densities, wave-speed differences, WENO weights >= 1e-36).
+template <bool FAST = false>
+KM_HD double krcp(double b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (FAST) {
+    double r = __builtin_amdgcn_rcp(b);
+    r = __builtin_fma(__builtin_fma(-b, r, 1.0), r, r);
+    r = __builtin_fma(__builtin_fma(-b, r, 1.0), r, r);
+    return r;
+  }
+#endif
+  return 1.0 / b;
+}
+template <bool FAST = false>
+KM_HD double kdiv(double a, double b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (FAST) {
+    const double r = krcp<true>(b);
+    const double q = a * r;
+    return __builtin_fma(__builtin_fma(-b, q, a), r, q);
+  }
+#endif
+  return a / b;
+}
 KM_HD double ksqrt(double x) { return __builtin_sqrt(x); }
 #if defined(__HIP_DEVICE_COMPILE__)
+template <bool FAST = false>
 KM_HD float kdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+template <bool FAST = false>
 KM_HD float krcp(float b) { return __builtin_amdgcn_rcpf(b); }
 KM_HD float ksqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 #else
+template <bool FAST = false>
 KM_HD float kdiv(float a, float b) { return a / b; }
+template <bool FAST = false>
 KM_HD float krcp(float b) { return 1.0f / b; }
 KM_HD float ksqrt(float x) { return __builtin_sqrtf(x); }
 #endif
@@ -69,12 +98,12 @@ struct ReconWidth {
 };
 
 // LimitedSlope, reconstruction.hpp:24-39
-template <int LIM, class T>
+template <int LIM, bool FAST = false, class T>
 KM_HD T limited_slope(T a, T b) {
   if constexpr (LIM == LIM_MC) {
     return (ksign(a) + ksign(b)) * kmin(kabs(a), kmin(T(.25) * kabs(a + b), kabs(b)));
   } else if constexpr (LIM == LIM_VAN_LEER) {
-    if (a * b > T(0.)) return kdiv(T(2.) * a * b, a + b);
+    if (a * b > T(0.)) return kdiv<FAST>(T(2.) * a * b, a + b);
     return T(0.);
   } else {
     return T(0.5) * (ksign(a) + ksign(b)) * kmin(kabs(a), kabs(b));
@@ -83,7 +112,7 @@ KM_HD T limited_slope(T a, T b) {
 
 // Reconstruct (reconstruction.hpp:16-142) on a window w[0..4] = cells -2..+2 around the
 // reconstructed cell (w[2]); vM at the cell's lower face, vP at its upper face.
-template <int RECON, int LIM, class T>
+template <int RECON, int LIM, bool FAST = false, class T>
 KM_HD void reconstruct(const T *w, T &vM, T &vP) {
   const T s0 = w[2];
   if constexpr (RECON == RECON_FOG) {
@@ -92,13 +121,13 @@ KM_HD void reconstruct(const T *w, T &vM, T &vP) {
   } else if constexpr (RECON == RECON_PLM) {  // :48-59
     const T dvL = s0 - w[1];
     const T dvR = w[3] - s0;
-    const T del = limited_slope<LIM>(dvL, dvR);
+    const T del = limited_slope<LIM, FAST>(dvL, dvR);
     vM = s0 - T(0.5) * del;
     vP = s0 + T(0.5) * del;
   } else if constexpr (RECON == RECON_PPM) {  // :61-88, Slope(idx) = Lim(s(i+1)-s(i), s(i)-s(i-1))
-    const T dv_p = limited_slope<LIM>(w[4] - w[3], w[3] - s0);
-    const T dv_0 = limited_slope<LIM>(w[3] - s0, s0 - w[1]);
-    const T dv_m = limited_slope<LIM>(s0 - w[1], w[1] - w[0]);
+    const T dv_p = limited_slope<LIM, FAST>(w[4] - w[3], w[3] - s0);
+    const T dv_0 = limited_slope<LIM, FAST>(w[3] - s0, s0 - w[1]);
+    const T dv_m = limited_slope<LIM, FAST>(s0 - w[1], w[1] - w[0]);
     constexpr T sixth = T(1. / 6.);
     T m = T(0.5) * (w[1] + s0) - sixth * (dv_0 - dv_m);
     T p = T(0.5) * (s0 + w[3]) - sixth * (dv_p - dv_0);
@@ -137,9 +166,10 @@ KM_HD void reconstruct(const T *w, T &vM, T &vP) {
     const T tau = kabs(b2 - b0);
     // tau / (eps + beta_k) is evaluated identically for both sides; computing it once
     // per k gives the same bits as the reference's two evaluations
-    T r0 = kdiv(tau, eps + b0);
-    T r1 = kdiv(tau, eps + b1);
-    T r2 = kdiv(tau, eps + b2);
+    // FAST: a reciprocal and a product for each ratio, as for the normalisations below
+    T r0 = FAST ? tau * krcp<FAST>(eps + b0) : kdiv(tau, eps + b0);
+    T r1 = FAST ? tau * krcp<FAST>(eps + b1) : kdiv(tau, eps + b1);
+    T r2 = FAST ? tau * krcp<FAST>(eps + b2) : kdiv(tau, eps + b2);
     if constexpr (sizeof(T) == 4) {
       // FP32: an indicator that rounds to 0 gives tau/eps ~ 1e22, whose square overflows
       // float. Clamping at 1e15 keeps every weight finite; a ratio that large already
@@ -155,13 +185,15 @@ KM_HD void reconstruct(const T *w, T &vM, T &vP) {
     {
       const T w0 = T(3.) * q0, w1 = T(6.) * q1, w2 = q2;
       const T norm = w0 + w1 + w2;
-      vM = kdiv(w0 * em0 + w1 * em1 + w2 * em2, norm);
+      vM = FAST ? (w0 * em0 + w1 * em1 + w2 * em2) * krcp<FAST>(norm)
+                : kdiv(w0 * em0 + w1 * em1 + w2 * em2, norm);
     }
     // vP: weno_weighting(+1): w0 uses beta[2], w2 uses beta[0]
     {
       const T w0 = T(3.) * q2, w1 = T(6.) * q1, w2 = q0;
       const T norm = w0 + w1 + w2;
-      vP = kdiv(w0 * ep0 + w1 * ep1 + w2 * ep2, norm);
+      vP = FAST ? (w0 * ep0 + w1 * ep1 + w2 * ep2) * krcp<FAST>(norm)
+                : kdiv(w0 * ep0 + w1 * ep1 + w2 * ep2, norm);
     }
   }
 }
@@ -188,9 +220,9 @@ KM_HD T total_pres(const T *V) {  // :21-30
   return pres;
 }
 
-template <int MHD, int D1, class T>
+template <int MHD, int D1, bool FAST = false, class T>
 KM_HD T fast_speed(const T *V) {  // :31-53
-  const T idens = krcp(V[R_DENS]);
+  const T idens = krcp<FAST>(V[R_DENS]);
   const T a2 = idens * V[R_BMOD];
   T cfast2;
   if constexpr (!MHD) {
@@ -259,15 +291,15 @@ KM_HD void prim2flux(const T *V, T *F) {  // :100-138
 }
 
 // HLL, riemann_solver.hpp:20-57
-template <int MHD, int D1, class T>
+template <int MHD, int D1, bool FAST = false, class T>
 KM_HD void riemann_hll(const T *vL, const T *vR, T *flux) {
   constexpr int NC = HV<MHD>::NC;
-  const T cfL = fast_speed<MHD, D1>(vL);
-  const T cfR = fast_speed<MHD, D1>(vR);
+  const T cfL = fast_speed<MHD, D1, FAST>(vL);
+  const T cfR = fast_speed<MHD, D1, FAST>(vR);
   const T tiny = Num<T>::tiny;
   const T sL = kmin(-tiny, kmin(vL[R_VEL + D1] - cfL, vR[R_VEL + D1] - cfR));
   const T sR = kmax(tiny, kmax(vL[R_VEL + D1] + cfL, vR[R_VEL + D1] + cfR));
-  const T sRmsLi = krcp(sR - sL);
+  const T sRmsLi = krcp<FAST>(sR - sL);
   T UL[NC], UR[NC], FL[NC], FR[NC];
   prim2cons<MHD>(vL, UL);
   prim2cons<MHD>(vR, UR);
@@ -279,13 +311,13 @@ KM_HD void riemann_hll(const T *vL, const T *vR, T *flux) {
 }
 
 // HLLC, riemann_solver.hpp:59-149
-template <int MHD, int D1, class T>
+template <int MHD, int D1, bool FAST = false, class T>
 KM_HD void riemann_hllc(const T *vL, const T *vR, T *flux) {
   constexpr int NC = HV<MHD>::NC;
   constexpr int D2 = (D1 + 1) % 3;
   constexpr int D3 = (D1 + 2) % 3;
-  const T cfL = fast_speed<MHD, D1>(vL);
-  const T cfR = fast_speed<MHD, D1>(vR);
+  const T cfL = fast_speed<MHD, D1, FAST>(vL);
+  const T cfR = fast_speed<MHD, D1, FAST>(vR);
   const T tiny = Num<T>::tiny;
   const T sL = kmin(-tiny, kmin(vL[R_VEL + D1] - cfL, vR[R_VEL + D1] - cfR));
   const T sR = kmax(tiny, kmax(vL[R_VEL + D1] + cfL, vR[R_VEL + D1] + cfR));
@@ -298,17 +330,25 @@ KM_HD void riemann_hllc(const T *vL, const T *vR, T *flux) {
   const T total_presR = total_pres<MHD>(vR);
   const T uL = vL[R_VEL + D1], uR = vR[R_VEL + D1];
   T ustar = total_presR - total_presL + UL[C_MOM + D1] * (sL - uL) - UR[C_MOM + D1] * (sR - uR);
-  ustar = kdiv(ustar * T(1.), vL[R_DENS] * (sL - uL) - vR[R_DENS] * (sR - uR));
+  ustar = kdiv<FAST>(ustar * T(1.), vL[R_DENS] * (sL - uL) - vR[R_DENS] * (sR - uR));
   const T pstar = T(0.5) * (total_presL + total_presR + vL[R_DENS] * (sL - uL) * (ustar - uL) +
                             vR[R_DENS] * (sR - uR) * (ustar - uR));
   T UstarL[NC], UstarR[NC];
   auto hllc_state = [&](const T S, const T *U, const T Pu, T *Us) {
-    const T susi = krcp(S - ustar + tiny);
+    const T susi = krcp<FAST>(S - ustar + tiny);
     Us[C_DENS] = susi * (S * U[C_DENS] - U[C_MOM + D1]);
     Us[C_MOM + D1] = Us[C_DENS] * ustar;
-    Us[C_MOM + D2] = kdiv(U[C_MOM + D2] * Us[C_DENS], U[C_DENS]);
-    Us[C_MOM + D3] = kdiv(U[C_MOM + D3] * Us[C_DENS], U[C_DENS]);
-    Us[C_ENER] = kdiv(U[C_ENER] * Us[C_DENS], U[C_DENS]) + susi * (pstar * ustar - Pu);
+    if constexpr (FAST) {
+      // one reciprocal of the state's density for its three scaled components
+      const T q = Us[C_DENS] * krcp<FAST>(U[C_DENS]);
+      Us[C_MOM + D2] = U[C_MOM + D2] * q;
+      Us[C_MOM + D3] = U[C_MOM + D3] * q;
+      Us[C_ENER] = U[C_ENER] * q + susi * (pstar * ustar - Pu);
+    } else {
+      Us[C_MOM + D2] = kdiv(U[C_MOM + D2] * Us[C_DENS], U[C_DENS]);
+      Us[C_MOM + D3] = kdiv(U[C_MOM + D3] * Us[C_DENS], U[C_DENS]);
+      Us[C_ENER] = kdiv(U[C_ENER] * Us[C_DENS], U[C_DENS]) + susi * (pstar * ustar - Pu);
+    }
     if constexpr (MHD) {  // :111-133
       const T sRsLi = krcp(sR - sL);
       auto hll = [&](int v) { return sRsLi * (sR * UR[v] - sL * UL[v] + FL[v] - FR[v]); };
@@ -339,12 +379,12 @@ KM_HD void riemann_hllc(const T *vL, const T *vR, T *flux) {
     flux[v] = biasR * (FL[v] + sL * (UstarL[v] - UL[v])) + biasL * (FR[v] + sR * (UstarR[v] - UR[v]));
 }
 
-template <int RIEMANN, int MHD, int D1, class T>
+template <int RIEMANN, int MHD, int D1, bool FAST = false, class T>
 KM_HD void riemann(const T *vL, const T *vR, T *flux) {
   if constexpr (RIEMANN == RIEMANN_HLLC)
-    riemann_hllc<MHD, D1>(vL, vR, flux);
+    riemann_hllc<MHD, D1, FAST>(vL, vR, flux);
   else
-    riemann_hll<MHD, D1>(vL, vR, flux);
+    riemann_hll<MHD, D1, FAST>(vL, vR, flux);
 }
 
 // Ideal-gas closure of singularity-eos IdealGas (equation_of_state.hpp:40-64):
