@@ -434,6 +434,16 @@ int km_amr_adapt_set(km_amr_adapt_t *adapt, int32_t mhd, const char *block_key,
  * layers, so ghost zones must be filled. Blocks the host until the tags are written. */
 int km_amr_tag_loehner(const km_pack_t *u, const km_amr_criterion_t *crit, int32_t *tags_dev,
                        int32_t reset, void *stream);
+/* km_amr_tag_loehner without allocation: no counterpart in the reference. For callers that
+ * cannot allocate device memory (code under hipGraph capture, serving loops, harnesses that
+ * forbid hipMalloc). The per-block maxima live in one library-held device array of
+ * KM_TAG_NOALLOC_MAX_BLOCKS entries (512 KiB, held by each GPU the library runs on until
+ * the process ends; cleared by a fill kernel at each call); packs with more blocks are
+ * refused with KM_EINVAL. Calls that overlap on one GPU share that array and race:
+ * serialise them. Same arguments and results as km_amr_tag_loehner; blocks the host. */
+#define KM_TAG_NOALLOC_MAX_BLOCKS 65536
+int km_amr_tag_loehner_noalloc(const km_pack_t *u, const km_amr_criterion_t *crit,
+                               int32_t *tags_dev, int32_t reset, void *stream);
 /* Parthenon's MeshRefinement::SetRefinement for every leaf (refine below level
  * numlevel-1; derefine after derefine_count consecutive derefine tags and only without a
  * finer neighbour), then Mesh::UpdateMeshBlockTree: leaves whose 2^d siblings are all

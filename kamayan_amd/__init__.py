@@ -98,6 +98,7 @@ def lib() -> C.CDLL:
         "km_amr_adapt_default": [P(KmAmrAdapt)],
         "km_amr_adapt_set": [P(KmAmrAdapt), i32, C.c_char_p, C.c_char_p],
         "km_amr_tag_loehner": [P(KmPack), P(KmAmrCriterion), vp, i32, vp],
+        "km_amr_tag_loehner_noalloc": [P(KmPack), P(KmAmrCriterion), vp, i32, vp],
         "km_amr_update_tree": [P(KmMeshDesc), i32, i32, P(i32), i32, P(i32), P(i32), P(i32),
                                P(i32), P(i32), P(i32), P(i32)],
         "km_amr_remesh": [P(KmPack), P(KmPack), vp, i32, i32, vp],

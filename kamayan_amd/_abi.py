@@ -8,6 +8,7 @@ import numpy as np
 
 KM_OK, KM_EINVAL, KM_EHIP, KM_ENOMEM, KM_ENODEV, KM_ESTATE = 0, 1, 2, 3, 4, 5
 EOS_INIT = {"dens_pres": 0, "dens_ener": 1, "dens_temp": 2}
+KM_TAG_NOALLOC_MAX_BLOCKS = 65536  # include/km_hydro.h
 
 RECON = {"fog": 0, "plm": 1, "ppm": 2, "wenoz": 3}
 LIMITER = {"minmod": 0, "van_leer": 1, "mc": 2}

@@ -1837,6 +1837,46 @@ int km_sim_set_history(km_sim_t *s, const char *path, double dt, const char *pro
   return KM_OK;
 }
 
+// Per-block Loehner maxima of km_amr_tag_loehner_noalloc: one file-scope device array
+// (each GPU holds its own copy for the life of the process). Host code uses the address
+// hipGetSymbolAddress gives for the current device, never the array's name.
+static __device__ __align__(256) unsigned long long scratch[KM_TAG_NOALLOC_MAX_BLOCKS];
+static std::mutex g_scratch_mu;
+static std::map<int, void *> g_scratch_ptr;  // device id -> address of `scratch` there
+
+int km_amr_tag_loehner_noalloc(const km_pack_t *u, const km_amr_criterion_t *crit,
+                               int32_t *tags_dev, int32_t reset, void *stream) {
+  if (!u || !crit || !tags_dev || crit->var < 0 || crit->var >= u->nvar || u->ng < 2) {
+    set_error("km_amr_tag_loehner_noalloc: bad pack, field slot or ghost width (needs 2)");
+    return KM_EINVAL;
+  }
+  if (u->nblocks > KM_TAG_NOALLOC_MAX_BLOCKS) {
+    set_error("km_amr_tag_loehner_noalloc: " + std::to_string(u->nblocks) +
+              " blocks exceed the scratch limit of " +
+              std::to_string(KM_TAG_NOALLOC_MAX_BLOCKS) + "; use km_amr_tag_loehner");
+    return KM_EINVAL;
+  }
+  int dev = 0;
+  KM_HIP(hipGetDevice(&dev));
+  void *ptr = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch_ptr.find(dev);
+    if (it == g_scratch_ptr.end()) {
+      KM_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+      g_scratch_ptr[dev] = ptr;
+    } else {
+      ptr = it->second;
+    }
+  }
+  // the maxima are cleared by a fill kernel before the zones fold into them
+  launch_amr_tag_loehner(*u, *crit, tags_dev, reset != 0, static_cast<double *>(ptr),
+                         as_stream(stream));
+  if (int rc = check_launch()) return rc;
+  KM_HIP(hipStreamSynchronize(as_stream(stream)));
+  return KM_OK;
+}
+
 int km_amr_tag_loehner(const km_pack_t *u, const km_amr_criterion_t *crit, int32_t *tags_dev,
                        int32_t reset, void *stream) {
   if (!u || !crit || !tags_dev || crit->var < 0 || crit->var >= u->nvar || u->ng < 2) {
