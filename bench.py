@@ -32,7 +32,9 @@ BYTES_PER_ZONE_STAGE = 8 * 3 * 5  # stage input + RK base read, stage output wri
 # Keyed by form, then by direction and kernel name as _kernel_names() reports them; the
 # final sweep has a plain form and, on the last RK stage, the CFL-minimum form. These are
 # constants from those runs, not measured by this script.
-COUNTER_TABLE = "profiles/fused_stage_traffic_hydro3d_256c_wenoz_hllc_rk3.md"
+COUNTER_TABLE = {"fp64": "profiles/fused_stage_traffic_hydro3d_256c_wenoz_hllc_rk3.md",
+                 "fp64_fma": "profiles/fused_stage_fma_fp32_counters_hydro3d_256c_wenoz_hllc_rk3.md",
+                 "fp32_mixed": "profiles/fused_stage_fma_fp32_counters_hydro3d_256c_wenoz_hllc_rk3.md"}
 MEASURED = {
     "fp64": {
         "x1": {"march": (2.3494e9 + 0.8032e9, 1535.8),
@@ -48,22 +50,106 @@ MEASURED = {
         "x3": {"march": (2.4187e9 + 0.6711e9, 1745.3)},
         "x3_dt": {"march": (2.8539e9 + 0.6712e9, 1865.8)},
     },
+    "fp64_fma": {
+        "x1": {
+            "march": (2.6509e9 + 0.9349e9, 1059.5),
+            "tile 16 pencils x 1024 lanes": (1.0907e9 + 0.6711e9, 1180.3),
+            "tile 4 pencils x 256 lanes": (1.0907e9 + 0.6711e9, 1375.8),
+            "tile 8 pencils x 256 lanes": (1.0907e9 + 0.6711e9, 1199.4),
+            "tile 8 pencils x 512 lanes": (1.0907e9 + 0.6711e9, 1241.9),
+        },
+        "x2": {
+            "march": (1.5259e9 + 0.6816e9, 977.9),
+            "tile 16 pencils x 1024 lanes": (1.4996e9 + 0.6711e9, 1130.9),
+            "tile 4 pencils x 256 lanes": (1.4996e9 + 0.6711e9, 1323.8),
+            "tile 8 pencils x 256 lanes": (1.4996e9 + 0.6711e9, 1136.4),
+            "tile 8 pencils x 512 lanes": (1.4996e9 + 0.6711e9, 1192.4),
+        },
+        "x3": {
+            "march": (2.3261e9 + 0.6712e9, 1199.5),
+            "tile 16 pencils x 1024 lanes": (1.7218e9 + 0.6743e9, 1207.8),
+            "tile 4 pencils x 256 lanes": (1.9510e9 + 0.6741e9, 1405.5),
+            "tile 8 pencils x 256 lanes": (1.8308e9 + 0.6761e9, 1204.8),
+            "tile 8 pencils x 512 lanes": (1.7896e9 + 0.6764e9, 1270.8),
+        },
+        "x3_dt": {
+            "march": (2.8273e9 + 0.6712e9, 1319.6),
+        },
+        "stage": {
+            "one-launch stage 32x16 columns": (6.8910e9 + 3.4823e9, 3128.6),
+            "one-launch stage 32x8 columns": (1.5990e9 + 0.7585e9, 3473.1),
+            "one-launch stage 64x16 columns": (24.4152e9 + 12.9888e9, 3012.9),
+            "one-launch stage 64x4 columns": (1.6354e9 + 0.7252e9, 3719.4),
+            "one-launch stage 64x8 columns": (7.2060e9 + 3.3367e9, 3202.3),
+        },
+    },
+    "fp32_mixed": {
+        "x1": {
+            "march": (2.6487e9 + 0.9329e9, 887.7),
+            "tile 16 pencils x 1024 lanes": (1.0907e9 + 0.6711e9, 1024.6),
+            "tile 32 pencils x 1024 lanes": (1.0907e9 + 0.6711e9, 955.4),
+            "tile 4 pencils x 256 lanes": (1.0907e9 + 0.6711e9, 1190.5),
+            "tile 8 pencils x 256 lanes": (1.0906e9 + 0.6711e9, 1035.2),
+            "tile 8 pencils x 512 lanes": (1.0907e9 + 0.6711e9, 1075.8),
+        },
+        "x2": {
+            "march": (1.5031e9 + 0.6711e9, 842.6),
+            "tile 16 pencils x 1024 lanes": (1.4996e9 + 0.6711e9, 975.0),
+            "tile 32 pencils x 1024 lanes": (1.4996e9 + 0.6711e9, 892.5),
+            "tile 4 pencils x 256 lanes": (1.4996e9 + 0.6719e9, 1138.0),
+            "tile 8 pencils x 256 lanes": (1.4996e9 + 0.6711e9, 971.5),
+            "tile 8 pencils x 512 lanes": (1.4996e9 + 0.6711e9, 1026.0),
+        },
+        "x3": {
+            "march": (2.3076e9 + 0.6764e9, 955.3),
+            "tile 16 pencils x 1024 lanes": (1.7138e9 + 0.6740e9, 1096.9),
+            "tile 32 pencils x 1024 lanes": (2.1915e9 + 0.6739e9, 1003.6),
+            "tile 4 pencils x 256 lanes": (2.1607e9 + 0.7168e9, 1264.8),
+            "tile 8 pencils x 256 lanes": (2.2328e9 + 0.6815e9, 1084.9),
+            "tile 8 pencils x 512 lanes": (2.1232e9 + 0.6902e9, 1149.4),
+        },
+        "x3_dt": {
+            "march": (2.8211e9 + 0.6712e9, 1054.8),
+        },
+        "stage": {
+            "one-launch stage 32x16 columns": (2.0214e9 + 0.8401e9, 2592.3),
+            "one-launch stage 32x8 columns": (2.0352e9 + 0.8493e9, 2797.1),
+            "one-launch stage 64x16 columns": (6.7355e9 + 3.4127e9, 2619.4),
+            "one-launch stage 64x4 columns": (2.0253e9 + 0.7849e9, 3053.7),
+            "one-launch stage 64x8 columns": (1.9585e9 + 0.7926e9, 2691.5),
+        },
+        "stage_dt": {
+            "one-launch stage 32x16 columns": (2.4435e9 + 0.8168e9, 2679.0),
+        },
+    },
 }
 
 
 def _stage_figures(form, kern):
-    """HBM bytes and VALU instructions per zone of one RK3 stage (two plain final sweeps
-    and one CFL-minimum final sweep per cycle) from the counter table, or None when the
-    form or one of its kernels has no entry."""
+    """HBM bytes and VALU instructions per zone of one RK3 stage (two plain final passes and
+    one CFL-minimum final pass per cycle) from the counter table, or None when the form or
+    one of its kernels has no entry."""
     tab = MEASURED.get(form)
     try:
-        x1, x2 = tab["x1"][kern["x1"]], tab["x2"][kern["x2"]]
-        x3p, x3d = tab["x3"][kern["x3"]], tab["x3_dt"][kern["x3"]]
-    except (KeyError, TypeError):
+        if "stage" in kern:  # the one-launch stage kernel: table keys name the tile shape
+            key = "one-launch stage " + kern["stage"].split(", ")[1]
+            plain = tab["stage"][key]
+            dt = tab.get("stage_dt", {}).get(key, plain)
+            parts = [plain, plain, dt]
+        else:
+            x1, x2 = tab["x1"][kern["x1"]], tab["x2"][kern["x2"]]
+            x3p, x3d = tab["x3"][kern["x3"]], tab["x3_dt"][kern["x3"]]
+            parts = [x1, x2, x3p, x3p, x3d]
+    except (KeyError, TypeError, IndexError):
         return None
-    x3 = [(2 * x3p[i] + x3d[i]) / 3 for i in range(2)]
-    return {"traffic": x1[0] + x2[0] + x3[0], "valu": x1[1] + x2[1] + x3[1],
-            "source": f"counter table from {COUNTER_TABLE} (not measured by this run)"}
+    if "stage" in kern:
+        traffic = sum(p[0] for p in parts) / 3
+        valu = sum(p[1] for p in parts) / 3
+    else:
+        traffic = parts[0][0] + parts[1][0] + (parts[2][0] + parts[3][0] + parts[4][0]) / 3
+        valu = parts[0][1] + parts[1][1] + (parts[2][1] + parts[3][1] + parts[4][1]) / 3
+    return {"traffic": traffic, "valu": valu,
+            "source": f"counter table from {COUNTER_TABLE[form]} (not measured by this run)"}
 
 
 # peak issue rate of FP64 VALU work on MI355X: 256 CUs x 64 lanes x 2.4 GHz (78.6 TFLOP/s
@@ -488,6 +574,9 @@ def main() -> int:
     ap.add_argument("--cpu-cycles", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--form", choices=["fp64", "fp64_fma", "fp32_mixed"], default=None,
+                    help="time this arithmetic form alone (profiling runs); the headline "
+                         "otherwise is the fastest FP64 form that passes the 1e-12 check")
     ap.add_argument("--workload", default="hydro3d",
                     choices=["hydro3d", "mhd3d", "sedov_amr", "sedov_amr_adaptive", "vortex2d",
                              "kh_ensemble"])
@@ -510,7 +599,8 @@ def main() -> int:
 
     _, _, lay, cell = _setup(km)
     zones = N ** 3
-    forms = ("fp64",) if args.no_variants else ("fp64", "fp64_fma", "fp32_mixed")
+    forms = ((args.form,) if args.form else
+             ("fp64",) if args.no_variants else ("fp64", "fp64_fma", "fp32_mixed"))
     runs = {}
     for prec in forms:
         # eager cycles with HIP events around each kernel on the simulation's stream
@@ -518,13 +608,13 @@ def main() -> int:
     cpu = parity = None
     if not args.no_cpu:
         cpu, ref_state, ref_info, ncyc = _cpu_baseline(km, cell, args.cpu_cycles)
-        if not args.no_variants:
+        if not args.no_variants and not args.form:
             parity = _parity_fullsize(km, cell, ref_state, ref_info, ncyc, lay)
     # the headline is the fastest FP64 form that agrees with the oracle to 1e-12 relative L1
     # per conserved variable at full size: the FMA form when the check ran and passed, else
     # the bitwise form
-    head = "fp64_fma" if parity and parity["fp64_fma_within_1e-12"] and \
-        runs["fp64_fma"][0] < runs["fp64"][0] else "fp64"
+    head = args.form or ("fp64_fma" if parity and parity["fp64_fma_within_1e-12"] and
+                         runs["fp64_fma"][0] < runs["fp64"][0] else "fp64")
     dt_h, times, s_h, lay, kern_h = runs[head]
     value = zones * args.steps / dt_h
     st_ms, st_n = times["stage"]
