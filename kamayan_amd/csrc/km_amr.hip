@@ -148,15 +148,19 @@ __device__ __forceinline__ int fill_var(int k, int v0, int s0, int s1) {
   return v >= s0 ? v + (s1 - s0) : v;
 }
 
+// depth > 0: only the face slabs of that depth (face_ghost_cell), else every ghost zone
 __global__ void __launch_bounds__(256) k_amr_ghost(km_pack_t u, AmrGrid t, int v0, int v1,
-                                                   int nc, int s0, int s1) {
+                                                   int nc, int s0, int s1, int depth) {
   const Geo g(u);
-  const long n = ghost_count(g);
+  const long n = depth > 0 ? face_ghost_count(g, depth) : ghost_count(g);
   const long tt = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (tt >= n * u.nblocks) return;
   const int b = (int)(tt / n);
   int idx[3];
-  ghost_cell(g, tt % n, idx);
+  if (depth > 0)
+    face_ghost_cell(g, depth, tt % n, idx);
+  else
+    ghost_cell(g, tt % n, idx);
   const km_block_t &blk = u.blocks[b];
   const int L = blk.level;
   int G0[3], G[3];
@@ -200,20 +204,26 @@ __global__ void __launch_bounds__(256) k_amr_ghost(km_pack_t u, AmrGrid t, int v
 // ceil(ng/2) ghosts per side; the physical boundary maps (periodic shift by an even
 // count, outflow clamp, reflect mirror) send both children of an unmapped coarse zone
 // to children of one mapped C, and a level-(L-1) leaf covers either both or neither.
+// depth > 0: the coarse zones of the face slabs of depth ceil(depth / 2) at half
+// resolution, whose children cover the fine face slabs of that depth.
 __global__ void __launch_bounds__(256) k_amr_prolong(km_pack_t u, AmrGrid t, int v0, int v1,
-                                                     int nc, int s0, int s1) {
+                                                     int nc, int s0, int s1, int depth) {
   const Geo g(u);
   km_pack_t half = u;
   half.ng = (u.ng + 1) / 2;
 #pragma unroll
   for (int d = 0; d < 3; ++d) half.nx[d] = d < u.ndim ? u.nx[d] / 2 : 1;
   const Geo gc(half);
-  const long n = ghost_count(gc);
+  const int hc = (depth + 1) / 2;
+  const long n = depth > 0 ? face_ghost_count(gc, hc) : ghost_count(gc);
   const long tt = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (tt >= n * u.nblocks) return;
   const int b = (int)(tt / n);
   int cz[3];
-  ghost_cell(gc, tt % n, cz);
+  if (depth > 0)
+    face_ghost_cell(gc, hc, tt % n, cz);
+  else
+    ghost_cell(gc, tt % n, cz);
   const km_block_t &blk = u.blocks[b];
   const int L = blk.level;
   if (L == 0) return;
@@ -723,20 +733,23 @@ void launch_amr_emf_correct(const AmrGrid &t, const km_pack_t &u, hipStream_t st
 }
 
 void launch_amr_ghost_fill(const AmrGrid &t, const km_pack_t &u, int v0, int v1, int nc,
-                           hipStream_t st, int s0, int s1) {
+                           hipStream_t st, int s0, int s1, int depth) {
   const Geo g(u);
-  const long n = ghost_count(g) * u.nblocks;
+  if (depth >= g.ng) depth = 0;  // the whole layer
+  const long n = (depth > 0 ? face_ghost_count(g, depth) : ghost_count(g)) * u.nblocks;
   if (s0 < v0 || s1 > v1 || s1 <= s0) s0 = s1 = v1;  // nothing left out
   if (n == 0 || v0 >= v1 || (v1 - v0) == (s1 - s0)) return;
   hipLaunchKernelGGL(k_amr_ghost, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, u, t, v0,
-                     v1, nc, s0, s1);
+                     v1, nc, s0, s1, depth);
   if (t.lmax > 0) {
     km_pack_t half = u;
     half.ng = (u.ng + 1) / 2;
     for (int d = 0; d < 3; ++d) half.nx[d] = d < u.ndim ? u.nx[d] / 2 : 1;
-    const long nc2 = ghost_count(Geo(half)) * u.nblocks;
+    const Geo gh(half);
+    const long nc2 =
+        (depth > 0 ? face_ghost_count(gh, (depth + 1) / 2) : ghost_count(gh)) * u.nblocks;
     hipLaunchKernelGGL(k_amr_prolong, dim3((unsigned)((nc2 + 255) / 256)), dim3(256), 0, st, u, t,
-                       v0, v1, nc, s0, s1);
+                       v0, v1, nc, s0, s1, depth);
   }
 }
 

@@ -241,9 +241,10 @@ void launch_amr_remesh(const km_pack_t &old_u, const km_pack_t &new_u, const int
 void launch_amr_remesh_faces(const km_pack_t &old_u, const km_pack_t &new_u,
                              const int32_t *origin, bool cyl, double x1min, hipStream_t st);
 
-// cell variables [v0, v1) except the slots [s0, s1) (none when s1 <= s0)
+// cell variables [v0, v1) except the slots [s0, s1) (none when s1 <= s0); depth > 0 fills
+// only the face slabs of that depth (what directional sweeps read), else every ghost zone
 void launch_amr_ghost_fill(const AmrGrid &t, const km_pack_t &u, int v0, int v1, int nc,
-                           hipStream_t st, int s0 = 0, int s1 = 0);
+                           hipStream_t st, int s0 = 0, int s1 = 0, int depth = 0);
 void launch_amr_flux_correct(const AmrGrid &t, const km_pack_t &u, int nc, hipStream_t st);
 // CT: face B ghost fill across levels and EMF correction at level jumps
 void launch_amr_ghost_faces(const AmrGrid &t, const km_pack_t &u, hipStream_t st);

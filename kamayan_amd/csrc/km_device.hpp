@@ -108,6 +108,17 @@ KM_HD bool unphysical(double dens, double pres, double v0, double v1, double v2)
          !(__builtin_fabs(v0) + __builtin_fabs(v1) + __builtin_fabs(v2) < inf);
 }
 
+// Minimum of positive doubles into *p by their bit patterns (which order like the values).
+// The atomic is skipped when v is not below the value this lane reads first: within a
+// launch that value can only be stale towards larger ones (the minimum only falls), so the
+// result is still the minimum of every v, while most lanes of a large launch skip the
+// contended atomic on one address.
+__device__ __forceinline__ void dt_min_push(double *p, double v) {
+  if (v < *p)
+    atomicMin(reinterpret_cast<unsigned long long *>(p),
+              static_cast<unsigned long long>(__double_as_longlong(v)));
+}
+
 // ghost zones of one block, enumerated without the interior: the two x3 slabs (full
 // x1-x2 planes), then the two x2 slabs (interior x3 rows), then the two x1 slabs
 // (interior x2-x3 rows, ng cells at each end); x1 runs fastest within each piece.
@@ -163,6 +174,40 @@ KM_HD void ghost_cell(const Geo &g, long r, int idx[3], int stag = -1) {
   idx[0] = i + (side ? ng + n[0] : 0);
   idx[1] = g.lo[1] + (int)(q % n[1]);
   idx[2] = g.lo[2] + (int)(q / n[1]);
+}
+
+// The ghost zones a directional sweep reads: for each active dimension d, the two slabs
+// of h zones next to the d faces over the interior extent of the other dimensions (no
+// edges or corners, and nothing deeper than h); x1 runs fastest within a slab.
+KM_HD long face_ghost_count(const Geo &g, int h) {
+  long n = 0;
+  for (int d = 0; d < g.ndim; ++d) {
+    long slab = h;
+    for (int e = 0; e < g.ndim; ++e)
+      if (e != d) slab *= g.nx[e];
+    n += 2 * slab;
+  }
+  return n;
+}
+KM_HD void face_ghost_cell(const Geo &g, int h, long r, int idx[3]) {
+  for (int d = 0; d < g.ndim; ++d) {
+    int m[3];
+    for (int e = 0; e < 3; ++e) m[e] = e == d ? h : (e < g.ndim ? g.nx[e] : 1);
+    const long slab = (long)m[0] * m[1] * m[2];
+    if (r < 2 * slab || d == g.ndim - 1) {
+      const int side = r >= slab;
+      long q = r - (side ? slab : 0);
+      int o[3];
+      o[0] = (int)(q % m[0]);
+      q /= m[0];
+      o[1] = (int)(q % m[1]);
+      o[2] = (int)(q / m[1]);
+      for (int e = 0; e < 3; ++e)
+        idx[e] = e == d ? (side ? g.hi[d] + 1 + o[e] : g.lo[d] - h + o[e]) : g.lo[e] + o[e];
+      return;
+    }
+    r -= 2 * slab;
+  }
 }
 
 // loads/stores through the global address space (the pack tables hold generic

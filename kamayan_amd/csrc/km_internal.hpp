@@ -61,6 +61,9 @@ struct FusedArgs {
   int amr;
   AmrGrid agrid;
   int *fail;  // device flag word (KM_FAIL_*), or null
+  // block trees: the final sweep also writes u1's primitives (PreparePrimitive); only when
+  // u1's primitive arrays are not u0's (the kernels read u0's while they write)
+  int write_prims;
 };
 
 constexpr int KM_NBR_OUTFLOW = -1;

@@ -284,6 +284,11 @@ struct km_sim_t {
   // (density and the primitives); momentum and energy ghosts are filled from the base
   // interiors when something reads them (state copies, tagging and remesh)
   bool lazy_ghosts = false;
+  // fused path on a block tree with two or more stages: every container owns its primitive
+  // arrays, so the final sweep writes u1's primitives (and, on the last stage, the CFL
+  // minimum) while other tiles still read u0's (RK1 updates u0 in place and keeps the
+  // separate kernels)
+  bool own_prims = false;
   void drop_graph() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -547,6 +552,7 @@ FusedArgs fused_args(km_sim_t &s, int stage, double beta) {
   a.amr = s.amr ? 1 : 0;
   a.agrid = s.agrid;
   a.fail = s.fail_d;
+  a.write_prims = s.own_prims ? 1 : 0;
   if (s.amr)  // fluxes at level jumps live in the task path's flux arrays
     for (int d = 0; d < 3; ++d) a.u0.flux[d] = s.scratch.pack.flux[d];
   return a;
@@ -756,15 +762,24 @@ int stage_launch(S &s, const FusedArgs &a, bool last, int ninst, double *est, Re
 // driver and km_stage_fused): the fluxes of faces where a block borders a coarser leaf,
 // their restriction onto the coarse face, then the pencil-tile sweeps, which take that flux
 // at their end faces.
+// With est, the final sweep also reduces the CFL minimum of u1 into est (one instance,
+// scaled by cfl / ndim), which a.write_prims requires to be meaningful only on the last
+// stage.
 template <class S, class Rec>
-int amr_stage_launch(S &s, const FusedArgs &a, int nflux, Rec &&rec) {
+int amr_stage_launch(S &s, const FusedArgs &a, int nflux, Rec &&rec, double *est = nullptr) {
+  const int ndim = s.mesh.ndim;
   if (int rc = launch_fine_faces(s, a)) return rc;
   launch_amr_flux_correct(s.agrid, a.u0, nflux, s.stream);
-  for (int dir = 0; dir < s.mesh.ndim; ++dir) {
+  if (est) hipLaunchKernelGGL(k_set, dim3(1), dim3(64), 0, s.stream, est, DBL_MAX, 1);
+  for (int dir = 0; dir < ndim; ++dir) {
     rec(dir, true);
-    if (int rc = launch_sweep(s, a, dir, false, s.kern[dir] > 0 ? s.kern[dir] : 1)) return rc;
+    if (int rc = launch_sweep(s, a, dir, est != nullptr, s.kern[dir] > 0 ? s.kern[dir] : 1))
+      return rc;
     rec(dir, false);
   }
+  if (est)
+    hipLaunchKernelGGL(k_scale, dim3(1), dim3(64), 0, s.stream, est,
+                       s.cfg.cfl / static_cast<double>(ndim), 1);
   return KM_OK;
 }
 
@@ -773,10 +788,12 @@ int amr_stage_launch(S &s, const FusedArgs &a, int nflux, Rec &&rec) {
 // the task path), so the sweeps read the stored primitives of every cell they need. The
 // fluxes of faces where a block borders a coarser leaf are computed first and restricted
 // onto the coarse face (AddFluxCorrectionTasks), and the coarse block's sweep takes that
-// flux at its end face: every face flux, and so du and u1, is the task path's. The final
-// sweep stops at the stage combine; PreparePrimitive, the CFL estimate and the ghost
-// exchange are the task path's kernels (the primitive arrays are shared by the stage
-// containers and still read by other tiles during the sweep).
+// flux at its end face: every face flux, and so du and u1, is the task path's. With
+// own_prims the final sweep also writes PreparePrimitive's interior values of u1 and, on
+// the last stage, reduces the CFL estimate (the task path's kernels, folded into the
+// sweep); otherwise those are the task path's kernels. The ghost exchange of density and
+// primitives follows, over the face slabs the sweeps read; the rest of the ghost zones
+// are filled when something reads them.
 int amr_fused_stage(km_sim_t &s, int stage, double beta) {
   const km_cfg_t &cfg = s.cfg;
   Cont &md1 = container(s, stage);
@@ -787,19 +804,27 @@ int amr_fused_stage(km_sim_t &s, int stage, double beta) {
     st1 = s.ev();
     (void)hipEventRecord(st0, s.stream);
   }
-  if (int rc = amr_stage_launch(s, a, s.nflux, sweep_recorder(s))) return rc;
-  launch_prepare_primitive(cfg, md1.pack, s.stream);
-  if (stage == cfg.nstages)
-    launch_estimate_dt(cfg, s.base.pack, s.est_d, s.stream, s.inst_d, s.ninst, s.fail_d);
+  const bool last = stage == cfg.nstages;
+  if (int rc = amr_stage_launch(s, a, s.nflux, sweep_recorder(s),
+                                s.own_prims && last ? s.est_d : nullptr))
+    return rc;
+  if (!s.own_prims) {
+    launch_prepare_primitive(cfg, md1.pack, s.stream);
+    if (last)
+      launch_estimate_dt(cfg, s.base.pack, s.est_d, s.stream, s.inst_d, s.ninst, s.fail_d);
+  }
   hipEvent_t g0 = nullptr, g1 = nullptr;
   if (s.profile) {
     g0 = s.ev();
     g1 = s.ev();
     (void)hipEventRecord(g0, s.stream);
   }
-  // density and the primitives; momentum and energy are left out
+  // density and the primitives in the face slabs the next sweeps read (the
+  // reconstruction's reach plus one: the cells of the end faces' stencils); momentum,
+  // energy, edges and corners are left out
+  const int reach = cfg.recon == KM_RECON_FOG ? 1 : cfg.recon == KM_RECON_PLM ? 2 : 3;
   launch_amr_ghost_fill(s.agrid, md1.pack, 0, KM_TEMP(cfg.mhd), s.nc, s.stream, 1,
-                        KM_NCONS(cfg.mhd));
+                        KM_NCONS(cfg.mhd), reach);
   s.lazy_ghosts = true;
   if (s.profile) {
     (void)hipEventRecord(g1, s.stream);
@@ -810,10 +835,11 @@ int amr_fused_stage(km_sim_t &s, int stage, double beta) {
   return check_launch();
 }
 
-// the momentum and energy ghost zones the fused block-tree stages leave to be filled
+// the ghost zones the fused block-tree stages leave to be filled: momentum and energy, and
+// density and the primitives outside the face slabs
 void fill_lazy_ghosts(km_sim_t &s) {
   if (!s.lazy_ghosts) return;
-  ghost_fill(s, s.base.pack, 1, KM_NCONS(s.cfg.mhd), false);
+  ghost_fill(s, s.base.pack, 0, KM_TEMP(s.cfg.mhd), false);
   s.lazy_ghosts = false;
 }
 
@@ -861,7 +887,9 @@ int amr_remesh(km_sim_t &s, const std::vector<int32_t> &nl, const std::vector<in
   s.agrid = t.grid(lookup_d);
   if ((rc = make_cont(s, s.base, true, nullptr, true, false))) return rc;
   for (int i = 0; i < 2; ++i)
-    if ((rc = make_cont(s, s.stage[i], false, &s.base, true, false))) return rc;
+    if ((rc = make_cont(s, s.stage[i], s.own_prims, s.own_prims ? nullptr : &s.base, true,
+                        false)))
+      return rc;
   if ((rc = make_cont(s, s.dudt, false, nullptr, true, false))) return rc;
   if ((rc = make_cont(s, s.scratch, false, nullptr, false, true))) return rc;
   if ((rc = s.alloc(&s.tags_d, n))) return rc;
@@ -1445,8 +1473,11 @@ static int sim_create(const km_cfg_t *cfg, const km_mesh_desc_t *meshes, int32_t
                      hipMemcpyHostToDevice));
   }
   if ((rc = make_cont(*s, s->base, true, nullptr, true, false))) return rc;
+  s->own_prims = path == KM_PATH_FUSED && tree && s->cfg.nstages > 1;
   for (int i = 0; i < 2; ++i)
-    if ((rc = make_cont(*s, s->stage[i], false, &s->base, true, false))) return rc;
+    if ((rc = make_cont(*s, s->stage[i], s->own_prims, s->own_prims ? nullptr : &s->base, true,
+                        false)))
+      return rc;
   if ((rc = make_cont(*s, s->dudt, false, nullptr, true, false))) return rc;
   if ((rc = make_cont(*s, s->scratch, false, nullptr, false, true))) return rc;
   if (path == KM_PATH_FUSED) {
@@ -2106,6 +2137,7 @@ int km_stage_fused(const km_cfg_t *cfg, km_stage_plan_t *p, const km_pack_t *bas
   a.amr = p->amr ? 1 : 0;
   a.agrid = p->agrid;
   a.fail = nullptr;
+  a.write_prims = 0;  // the caller's containers may share their primitive arrays
   if (p->kern[0] < 0 || p->kern[1] < 0 || p->kern[2] < 0 || p->stage_kern < 0)
     if (int rc = autotune(*p, a)) return rc;
   auto none = [](int, bool) {};

@@ -655,11 +655,7 @@ __global__ void __launch_bounds__(256) k_prepare_conserved(km_pack_t u, Eos eos,
   for (int v = 0; v < nc; ++v) cell_ptr(u, b, v)[c] = U[v];
 }
 
-__device__ __forceinline__ void atomic_min_pos(double *out, double v) {
-  // positive doubles order like their bit patterns
-  atomicMin(reinterpret_cast<unsigned long long *>(out),
-            static_cast<unsigned long long>(__double_as_longlong(v)));
-}
+__device__ __forceinline__ void atomic_min_pos(double *out, double v) { dt_min_push(out, v); }
 
 __device__ __forceinline__ double wave_min(double v) {
 #pragma unroll
