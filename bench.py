@@ -319,17 +319,43 @@ def _mhd_setup(km, precision):
     return cfg, desc, lay, cell, faces
 
 
-def _sedov_setup(km, precision="fp64"):
-    cfg = km.Config(**{"hydro/reconstruction": "plm", "hydro/slope_limiter": "mc",
-                       "hydro/riemann": "hllc", "parthenon/time/integrator": "rk2",
-                       "material/single/gamma": 1.4, "kamayan_amd/precision": precision})
+SEDOV_TREE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden",
+                          "sedov3d_density_tree.npz")
+
+
+def _sedov_setup(km, precision="fp64", pack_size=None):
+    """The 3-D Sedov blast on its density-gradient tree: 64^3 root zones in 16^3 blocks,
+    three refinement levels (numlevel 4), PLM-MC + HLLC, RK2, gamma 1.4, CFL 0.8. The leaves
+    and the radial profiles of density, radial velocity and pressure come from the
+    adaptive run that tests/golden/make_sedov_density_tree.py records (energy 1 in a
+    sphere of two finest zones, ambient density 1 and pressure 1e-5, refinement on the
+    Loehner density estimator, stopped at 3,760 leaves, t = 0.0169); the state is those
+    profiles interpolated at the zone centres of the tree."""
+    z = np.load(SEDOV_TREE)
+    keys = {"hydro/reconstruction": "plm", "hydro/slope_limiter": "mc",
+            "hydro/riemann": "hllc", "parthenon/time/integrator": "rk2",
+            "material/single/gamma": 1.4, "kamayan_amd/precision": precision}
+    if pack_size:
+        keys["parthenon/mesh/pack_size"] = pack_size
+    cfg = km.Config(**keys)
     cfg.c.ndim = 3
-    root = km.Layout(3, 4, (16, 16, 16), (4, 4, 4))
+    nx, nb, ng = 16, 4, 4
+    root = km.Layout(3, ng, (nx,) * 3, (nb,) * 3)
     desc = km.mesh_desc(root, xmin=(-0.5,) * 3, xmax=(0.5,) * 3, bc=("outflow",) * 6)
-    leaves = km.refine_ball(desc, 2, (0.0, 0.0, 0.0), 0.4)
-    lay = km.Layout(3, 4, (16, 16, 16), (4, 4, 4), nleaves=len(leaves))
-    cell, _ = km.problem_fill("sedov", cfg, desc, lay, params=[1.0, 2.0 / 256.0, 1.0, 1e-5],
-                              leaves=leaves)
+    leaves = np.ascontiguousarray(z["leaves"], dtype=np.int32)
+    lay = km.Layout(3, ng, (nx,) * 3, (nb,) * 3, nleaves=len(leaves))
+    cell = np.zeros(lay.cell_shape(km.nvar(0)))
+    dx = 1.0 / (nb * nx * 2.0 ** leaves[:, 0])
+    xs = [-0.5 + (leaves[:, 1 + d][:, None] * nx + np.arange(nx)[None, :] + 0.5) * dx[:, None]
+          for d in range(3)]
+    x1, x2, x3 = xs[0][:, None, None, :], xs[1][:, None, :, None], xs[2][:, :, None, None]
+    r = np.sqrt(x1 ** 2 + x2 ** 2 + x3 ** 2)
+    vr = np.interp(r, z["r"], z["vrad"])
+    inner = (slice(None),) + (slice(ng, ng + nx),) * 3
+    cell[(slice(None), km.slot("DENS")) + inner[1:]] = np.interp(r, z["r"], z["dens"])
+    cell[(slice(None), km.slot("PRES")) + inner[1:]] = np.interp(r, z["r"], z["pres"])
+    for d, x in enumerate((x1, x2, x3)):
+        cell[(slice(None), km.slot(f"VEL{d}")) + inner[1:]] = vr * x / np.maximum(r, 1e-300)
     return cfg, desc, lay, leaves, cell
 
 
@@ -386,49 +412,67 @@ def _secondary(km, args):
         ref.set_primitive(cell, faces)
         sample_cycles = 3
     elif args.workload == "sedov_amr":
-        cfg, desc, lay, leaves, cell = _sedov_setup(km)
+        # the block-tree configuration: every leaf in one launch per sweep on the fused path
+        # (autotuned pencil tiles), FP32-mixed, beside the same tree in FP64 on the fused
+        # path, on the task path in one pack, and on the task path with a pack per block
+        # launched from the host every cycle (parthenon/mesh/pack_size 1)
+        cfg, desc, lay, leaves, cell = _sedov_setup(km, "fp32_mixed")
         zones = len(leaves) * 16 ** 3
         stage_bytes = 8 * 3 * 5
         nst = 2
-        sim = km.Simulation(cfg, desc, lay, path=km.PATH_TASKS, leaves=leaves)
+        out["dtype"] = "fp32_mixed"
+        out["data"] = ("synthetic: the recorded Sedov blast (radial profiles) on the recorded "
+                       "density-gradient tree, tests/golden/sedov3d_density_tree.npz")
+        sim = km.Simulation(cfg, desc, lay, path=km.PATH_FUSED, leaves=leaves)
         sim.set_primitive(cell)
         dt = _timed(km, sim, args.steps, args.warmup)
-        s64, _ = sim.state()
+        kern = _kernel_names(sim, 3)
+        s32, _ = sim.state()
         sim.close()
         var = {}
-        # the same tree on the task path in FP32-mixed, and on the fused path (pencil
-        # tiles over every leaf in one launch per direction, autotuned) in both forms
-        for name, path, prec in (("tasks_fp32_mixed", km.PATH_TASKS, "fp32_mixed"),
-                                 ("fused_fp64", km.PATH_FUSED, "fp64"),
-                                 ("fused_fp32_mixed", km.PATH_FUSED, "fp32_mixed")):
-            cfgv, _, _, _, _ = _sedov_setup(km, prec)
+        for name, path, prec, ps, steps in (
+                ("fused_fp64", km.PATH_FUSED, "fp64", None, args.steps),
+                ("tasks_fp64", km.PATH_TASKS, "fp64", None, args.steps),
+                ("tasks_fp64_per_block", km.PATH_TASKS, "fp64", 1, min(args.steps, 3))):
+            cfgv, _, _, _, _ = _sedov_setup(km, prec, ps)
             sim = km.Simulation(cfgv, desc, lay, path=path, leaves=leaves)
             sim.set_primitive(cell)
-            dtv = _timed(km, sim, args.steps, args.warmup)
-            sv, _ = sim.state()
-            var[name] = {"value": zones * args.steps / dtv, "speedup_vs_fp64_tasks": dt / dtv,
-                         "rel_l1_vs_fp64_per_var": _rel_l1(sv, s64, lay)}
-            if path == km.PATH_FUSED:
-                var[name]["sweep_kernels"] = _kernel_names(sim, 3)
+            dtv = _timed(km, sim, steps, args.warmup if steps == args.steps else 1)
+            var[name] = {"value": zones * steps / dtv,
+                         "fused_fp32_mixed_speedup": (zones * args.steps / dt) / (zones * steps / dtv),
+                         "cycles": steps}
+            if steps == args.steps:
+                sv, _ = sim.state()
+                if name == "fused_fp64":
+                    var[name]["fp32_mixed_rel_l1_per_var"] = _rel_l1(s32, sv, lay)
+                    s64 = sv
+                else:
+                    var[name]["rel_l1_vs_fused_fp64_per_var"] = _rel_l1(sv, s64, lay)
             sim.close()
         out["variants"] = var
-        lv = np.bincount(leaves[:, 0], minlength=3)
-        out["config"] = {"workload": "sedov3d_amr_root64c_16b_3lev_plm_mc_hllc_rk2",
+        out["sweep_kernels"] = kern
+        lv = np.bincount(leaves[:, 0], minlength=4)
+        out["config"] = {"workload": "sedov3d_amr_root64c_16b_numlevel4_density_tree_plm_mc_hllc_rk2",
                          "zones": zones, "leaves": int(len(leaves)),
                          "leaves_per_level": [int(x) for x in lv], "block": [16] * 3,
-                         "nghost": 4, "path": "tasks", "parallelism": "single GPU"}
+                         "nghost": 4, "path": "fused", "precision": "fp32_mixed",
+                         "tree": "tests/golden/sedov3d_density_tree.npz",
+                         "parallelism": "single GPU"}
         ref = km_oracle.CpuSimulation(cfg.c, desc, lay, leaves=leaves)
         ref.set_primitive(cell)
         sample_cycles = 3
     elif args.workload == "sedov_amr_adaptive":
-        # the same start, then tagging on pres (Loehner, kamayan/refinement0) and remesh
-        # after every cycle, as the reference's adaptive Sedov runs (sedov.py:38-44)
+        # the same start with the tree adapting every cycle to the Loehner density
+        # estimator (kamayan/refinement0, field = dens), fused path, FP32-mixed
         from kamayan_amd._abi import amr_adapt, amr_criterion
-        cfg, desc, lay, leaves, cell = _sedov_setup(km)
-        adapt = amr_adapt(3, [amr_criterion(km.slot("PRES"), 3)], derefine_count=10)
+        cfg, desc, lay, leaves, cell = _sedov_setup(km, "fp32_mixed")
+        adapt = amr_adapt(4, [amr_criterion(km.slot("DENS"), 3)], derefine_count=10)
         stage_bytes = 8 * 3 * 5
         nst = 2
-        sim = km.Simulation(cfg, desc, lay, path=km.PATH_TASKS, leaves=leaves)
+        out["dtype"] = "fp32_mixed"
+        out["data"] = ("synthetic: the recorded Sedov blast (radial profiles) on the recorded "
+                       "density-gradient tree, tests/golden/sedov3d_density_tree.npz")
+        sim = km.Simulation(cfg, desc, lay, path=km.PATH_FUSED, leaves=leaves)
         sim.set_primitive(cell)
         sim.set_adaptive(adapt)
         sim.advance(args.warmup)
@@ -443,13 +487,15 @@ def _secondary(km, args):
         final = sim.leaves()
         sim.close()
         zones = sum(counts) * 16 ** 3 / args.steps  # mean zones per timed cycle
-        lv = np.bincount(final[:, 0], minlength=3)
-        out["config"] = {"workload": "sedov3d_amr_adaptive_root64c_16b_numlevel3_plm_mc_hllc_rk2",
+        lv = np.bincount(final[:, 0], minlength=4)
+        out["config"] = {"workload": "sedov3d_amr_adaptive_root64c_16b_numlevel4_dens_plm_mc_hllc_rk2",
                          "zones": zones, "leaves_first": counts[0], "leaves_last": int(len(final)),
                          "leaves_per_level_last": [int(x) for x in lv], "block": [16] * 3,
-                         "nghost": 4, "refinement": "loehner on pres, derefine_count 10",
-                         "path": "tasks", "parallelism": "single GPU"}
-        ref = km_oracle.CpuSimulation(cfg.c, desc, lay, leaves=leaves)
+                         "nghost": 4, "refinement": "loehner on dens, derefine_count 10",
+                         "path": "fused", "precision": "fp32_mixed",
+                         "parallelism": "single GPU"}
+        cfg64, _, _, _, _ = _sedov_setup(km, "fp64")
+        ref = km_oracle.CpuSimulation(cfg64.c, desc, lay, leaves=leaves)
         ref.set_primitive(cell)
         ref.set_adaptive(adapt)
         sample_cycles = 3
@@ -541,7 +587,7 @@ def _secondary(km, args):
     value = zones * args.steps / dt
     achieved = stage_bytes * nst * value / 1e9
     out.update({"value": value, "ms_per_step": dt * 1e3 / args.steps,
-                "data": "synthetic (in-repo seeded generator)",
+                "data": out.get("data", "synthetic (in-repo seeded generator)"),
                 "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                              "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                              "kernel": "whole cycle (all kernels)",

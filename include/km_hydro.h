@@ -96,6 +96,10 @@ typedef struct km_cfg_t {
    * (dt_min) and 1 (dt_max) */
   double dt_factor, dt_init, dt_ceil, dt_floor, dt_min, dt_max, dt_user, dt_force;
   int32_t dt_init_force, dt_min_cycle_limit, dt_max_cycle_limit;
+  /* parthenon/mesh/pack_size: blocks per MeshData pack on the task path, whose kernels
+   * are launched pack by pack from the host every cycle (-1, the default: one pack of
+   * every block, graph-replayed) */
+  int32_t pack_size;
 } km_cfg_t;
 
 /* ---- cell-variable slots of a hydro state pack ------------------------------------

@@ -61,7 +61,7 @@ class KmCfg(C.Structure):
         ("dt_floor", C.c_double), ("dt_min", C.c_double), ("dt_max", C.c_double),
         ("dt_user", C.c_double), ("dt_force", C.c_double),
         ("dt_init_force", C.c_int32), ("dt_min_cycle_limit", C.c_int32),
-        ("dt_max_cycle_limit", C.c_int32),
+        ("dt_max_cycle_limit", C.c_int32), ("pack_size", C.c_int32),
     ]
 
 

@@ -148,33 +148,41 @@ __device__ __forceinline__ int fill_var(int k, int v0, int s0, int s1) {
   return v >= s0 ? v + (s1 - s0) : v;
 }
 
-// depth > 0: only the face slabs of that depth (face_ghost_cell), else every ghost zone
-__global__ void __launch_bounds__(256) k_amr_ghost(km_pack_t u, AmrGrid t, int v0, int v1,
-                                                   int nc, int s0, int s1, int depth) {
-  const Geo g(u);
-  const long n = depth > 0 ? face_ghost_count(g, depth) : ghost_count(g);
-  const long tt = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (tt >= n * u.nblocks) return;
-  const int b = (int)(tt / n);
+// Ghost zone r of block b (the face slabs of depth > 0, else the whole layer): its cell,
+// the zone that covers it once mapped through the physical boundaries, and the flips
+struct GhostSrc {
+  long dc;
+  Src s;
+  bool fl[3];
+};
+__device__ __forceinline__ GhostSrc ghost_source(const km_pack_t &u, const AmrGrid &t,
+                                                 const Geo &g, int b, long r, int depth) {
   int idx[3];
   if (depth > 0)
-    face_ghost_cell(g, depth, tt % n, idx);
+    face_ghost_cell(g, depth, r, idx);
   else
-    ghost_cell(g, tt % n, idx);
+    ghost_cell(g, r, idx);
   const km_block_t &blk = u.blocks[b];
   const int L = blk.level;
   int G0[3], G[3];
-  bool fl[3];
+  GhostSrc o;
 #pragma unroll
   for (int d = 0; d < 3; ++d) G0[d] = d < g.ndim ? blk.lx[d] * g.nx[d] + (idx[d] - g.lo[d]) : 0;
-  map_zone(t, g, L, G0, G, fl);
-  const Src s = locate(u, t, g, L, G);
-  if (s.kind == 2) return;
-  const long dc = g.c(idx[2], idx[1], idx[0]);
-  // variables go in chunks: every load of a chunk is issued before its stores, so the
-  // loads of one chunk overlap instead of waiting on the previous variable's store
+  map_zone(t, g, L, G0, G, o.fl);
+  o.s = locate(u, t, g, L, G);
+  o.dc = g.c(idx[2], idx[1], idx[0]);
+  return o;
+}
+
+// Copies or restrictions of the variables [v0, v1) without [s0, s1) into zone dc of
+// block b from zone sidx of leaf sl (restr: the lower corner of the children).
+// Variables go in chunks: every load of a chunk is issued before its stores, so the loads
+// of one chunk overlap instead of waiting on the previous variable's store.
+__device__ __forceinline__ void ghost_store(const km_pack_t &u, const Geo &g, int b, long dc,
+                                            int sl, long sidx, bool restr, const bool fl[3],
+                                            int v0, int v1, int nc, int s0, int s1, bool cyl) {
   constexpr int kChunk = 4;
-  const km_block_t &sb = u.blocks[s.leaf];
+  const km_block_t &sb = u.blocks[sl];
   const int nv = (v1 - v0) - (s1 - s0);
   for (int k = 0; k < nv; k += kChunk) {
     double x[kChunk];
@@ -182,8 +190,8 @@ __global__ void __launch_bounds__(256) k_amr_ghost(km_pack_t u, AmrGrid t, int v
     for (int q = 0; q < kChunk; ++q) {
       x[q] = 0.0;
       if (k + q < nv) {
-        const double *src = cell_ptr(u, s.leaf, fill_var(k + q, v0, s0, s1));
-        x[q] = s.kind == 1 ? restrict_cell(src, sb, g, s.idx, t.cyl != 0) : src[s.idx];
+        const double *src = cell_ptr(u, sl, fill_var(k + q, v0, s0, s1));
+        x[q] = restr ? restrict_cell(src, sb, g, sidx, cyl) : src[sidx];
       }
     }
 #pragma unroll
@@ -197,6 +205,54 @@ __global__ void __launch_bounds__(256) k_amr_ghost(km_pack_t u, AmrGrid t, int v
   }
 }
 
+// depth > 0: only the face slabs of that depth (face_ghost_cell), else every ghost zone
+__global__ void __launch_bounds__(256) k_amr_ghost(km_pack_t u, AmrGrid t, int v0, int v1,
+                                                   int nc, int s0, int s1, int depth) {
+  const Geo g(u);
+  const long n = depth > 0 ? face_ghost_count(g, depth) : ghost_count(g);
+  const long tt = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (tt >= n * u.nblocks) return;
+  const int b = (int)(tt / n);
+  const GhostSrc z = ghost_source(u, t, g, b, tt % n, depth);
+  if (z.s.kind == 2) return;
+  ghost_store(u, g, b, z.dc, z.s.leaf, z.s.idx, z.s.kind == 1, z.fl, v0, v1, nc, s0, s1,
+              t.cyl != 0);
+}
+
+// The gather list of a tree: ghost_source of every zone k_amr_ghost enumerates, stored
+// once per tree (zones a coarser leaf covers get sb = -1)
+__global__ void __launch_bounds__(256) k_amr_ghost_list(km_pack_t u, AmrGrid t, int depth,
+                                                        AmrGhostEntry *out) {
+  const Geo g(u);
+  const long n = depth > 0 ? face_ghost_count(g, depth) : ghost_count(g);
+  const long tt = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (tt >= n * u.nblocks) return;
+  const int b = (int)(tt / n);
+  const GhostSrc z = ghost_source(u, t, g, b, tt % n, depth);
+  AmrGhostEntry e;
+  e.db = b;
+  e.dc = (int)z.dc;
+  e.sb = z.s.kind == 2 ? -1 : z.s.leaf;
+  e.src = (unsigned)z.s.idx | (z.s.kind == 1 ? 1u << 28 : 0u) | (z.fl[0] ? 1u << 29 : 0u) |
+          (z.fl[1] ? 1u << 30 : 0u) | (z.fl[2] ? 1u << 31 : 0u);
+  out[tt] = e;
+}
+
+// k_amr_ghost from the gather list: one entry per lane, no tree walk
+__global__ void __launch_bounds__(256) k_amr_ghost_gather(km_pack_t u, const AmrGhostEntry *list,
+                                                          long n, int v0, int v1, int nc, int s0,
+                                                          int s1, int cyl) {
+  const long tt = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (tt >= n) return;
+  const AmrGhostEntry e = list[tt];
+  if (e.sb < 0) return;
+  const Geo g(u);
+  const bool fl[3] = {((e.src >> 29) & 1u) != 0, ((e.src >> 30) & 1u) != 0,
+                      ((e.src >> 31) & 1u) != 0};
+  ghost_store(u, g, e.db, e.dc, e.sb, (long)(e.src & 0x0fffffffu), ((e.src >> 28) & 1u) != 0,
+              fl, v0, v1, nc, s0, s1, cyl != 0);
+}
+
 // Zones a level-(L-1) leaf covers: ProlongateSharedMinMod. One thread takes one coarse
 // ghost zone of a block, i.e. the up to 2^d fine ghost zones that share a coarse parent
 // C: C's value and limited slopes are formed once and each child applies its own
@@ -206,8 +262,12 @@ __global__ void __launch_bounds__(256) k_amr_ghost(km_pack_t u, AmrGrid t, int v
 // to children of one mapped C, and a level-(L-1) leaf covers either both or neither.
 // depth > 0: the coarse zones of the face slabs of depth ceil(depth / 2) at half
 // resolution, whose children cover the fine face slabs of that depth.
+// plist (np entries): the parents to fill, as b * n + r; flag: write whether each parent of
+// the enumeration is filled here (a coarser leaf covers it) and return.
 __global__ void __launch_bounds__(256) k_amr_prolong(km_pack_t u, AmrGrid t, int v0, int v1,
-                                                     int nc, int s0, int s1, int depth) {
+                                                     int nc, int s0, int s1, int depth,
+                                                     const long *plist, long np,
+                                                     unsigned char *flag) {
   const Geo g(u);
   km_pack_t half = u;
   half.ng = (u.ng + 1) / 2;
@@ -216,8 +276,10 @@ __global__ void __launch_bounds__(256) k_amr_prolong(km_pack_t u, AmrGrid t, int
   const Geo gc(half);
   const int hc = (depth + 1) / 2;
   const long n = depth > 0 ? face_ghost_count(gc, hc) : ghost_count(gc);
-  const long tt = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (tt >= n * u.nblocks) return;
+  const long t0 = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t0 >= (plist ? np : n * u.nblocks)) return;
+  const long tt = plist ? plist[t0] : t0;
+  if (flag) flag[tt] = 0;
   const int b = (int)(tt / n);
   int cz[3];
   if (depth > 0)
@@ -264,6 +326,10 @@ __global__ void __launch_bounds__(256) k_amr_prolong(km_pack_t u, AmrGrid t, int
   if (first < 0) return;
   const Src s = locate(u, t, g, L, G);
   if (s.kind != 2) return;
+  if (flag) {
+    flag[tt] = 1;
+    return;
+  }
   // the coarse zone C = G/2 and its neighbours at level L-1
   int C[3];
 #pragma unroll
@@ -749,8 +815,56 @@ void launch_amr_ghost_fill(const AmrGrid &t, const km_pack_t &u, int v0, int v1,
     const long nc2 =
         (depth > 0 ? face_ghost_count(gh, (depth + 1) / 2) : ghost_count(gh)) * u.nblocks;
     hipLaunchKernelGGL(k_amr_prolong, dim3((unsigned)((nc2 + 255) / 256)), dim3(256), 0, st, u, t,
-                       v0, v1, nc, s0, s1, depth);
+                       v0, v1, nc, s0, s1, depth, nullptr, 0L, nullptr);
   }
+}
+
+long amr_ghost_list_size(const km_pack_t &u, int depth) {
+  const Geo g(u);
+  if (depth >= g.ng) depth = 0;
+  return (depth > 0 ? face_ghost_count(g, depth) : ghost_count(g)) * u.nblocks;
+}
+
+long amr_prolong_parents(const km_pack_t &u, int depth) {
+  const Geo g(u);
+  if (depth >= g.ng) depth = 0;
+  km_pack_t half = u;
+  half.ng = (u.ng + 1) / 2;
+  for (int d = 0; d < 3; ++d) half.nx[d] = d < u.ndim ? u.nx[d] / 2 : 1;
+  const Geo gh(half);
+  return (depth > 0 ? face_ghost_count(gh, (depth + 1) / 2) : ghost_count(gh)) * u.nblocks;
+}
+
+void launch_amr_ghost_list(const AmrGrid &t, const km_pack_t &u, int depth, AmrGhostEntry *out,
+                           hipStream_t st) {
+  const Geo g(u);
+  if (depth >= g.ng) depth = 0;
+  const long n = amr_ghost_list_size(u, depth);
+  if (n > 0)
+    hipLaunchKernelGGL(k_amr_ghost_list, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, u,
+                       t, depth, out);
+}
+
+void launch_amr_prolong_flags(const AmrGrid &t, const km_pack_t &u, int depth,
+                              unsigned char *flag, hipStream_t st) {
+  const Geo g(u);
+  if (depth >= g.ng) depth = 0;
+  const long n = amr_prolong_parents(u, depth);
+  if (n > 0 && t.lmax > 0)
+    hipLaunchKernelGGL(k_amr_prolong, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, u, t, 0,
+                       0, 0, 0, 0, depth, nullptr, 0L, flag);
+}
+
+void launch_amr_ghost_fill_list(const AmrFillList &f, const AmrGrid &t, const km_pack_t &u,
+                                int v0, int v1, int nc, hipStream_t st, int s0, int s1) {
+  if (s0 < v0 || s1 > v1 || s1 <= s0) s0 = s1 = v1;  // nothing left out
+  if (v0 >= v1 || (v1 - v0) == (s1 - s0)) return;
+  if (f.n > 0)
+    hipLaunchKernelGGL(k_amr_ghost_gather, dim3((unsigned)((f.n + 255) / 256)), dim3(256), 0, st,
+                       u, f.list, f.n, v0, v1, nc, s0, s1, t.cyl);
+  if (f.np > 0)
+    hipLaunchKernelGGL(k_amr_prolong, dim3((unsigned)((f.np + 255) / 256)), dim3(256), 0, st, u,
+                       t, v0, v1, nc, s0, s1, f.depth, f.plist, f.np, nullptr);
 }
 
 void launch_amr_flux_correct(const AmrGrid &t, const km_pack_t &u, int nc, hipStream_t st) {

@@ -245,6 +245,34 @@ void launch_amr_remesh_faces(const km_pack_t &old_u, const km_pack_t &new_u,
 // only the face slabs of that depth (what directional sweeps read), else every ghost zone
 void launch_amr_ghost_fill(const AmrGrid &t, const km_pack_t &u, int v0, int v1, int nc,
                            hipStream_t st, int s0 = 0, int s1 = 0, int depth = 0);
+// Gather list of a tree's ghost fill: one entry per ghost zone of the enumeration
+// (face slabs of `depth`, or every ghost zone), the destination block and cell, the
+// source leaf (-1: a coarser leaf covers the zone) and, in src, the source cell (bits
+// 0-27), restriction of its children (bit 28) and the flips of the x1-x3 vector
+// components (bits 29-31)
+struct alignas(16) AmrGhostEntry {
+  int db, dc, sb;
+  unsigned src;
+};
+// the lists of one tree and depth: ghost entries, and the prolonged parents (as
+// b * per_block + r of k_amr_prolong's enumeration)
+struct AmrFillList {
+  int depth = 0;
+  AmrGhostEntry *list = nullptr;
+  long n = 0;
+  const long *plist = nullptr;
+  long np = 0;
+};
+long amr_ghost_list_size(const km_pack_t &u, int depth);
+long amr_prolong_parents(const km_pack_t &u, int depth);
+void launch_amr_ghost_list(const AmrGrid &t, const km_pack_t &u, int depth, AmrGhostEntry *out,
+                           hipStream_t st);
+// flag[p] = 1 for each parent p of the prolongation enumeration that a coarser leaf covers
+void launch_amr_prolong_flags(const AmrGrid &t, const km_pack_t &u, int depth,
+                              unsigned char *flag, hipStream_t st);
+// launch_amr_ghost_fill over the enumeration the lists were built for
+void launch_amr_ghost_fill_list(const AmrFillList &f, const AmrGrid &t, const km_pack_t &u,
+                                int v0, int v1, int nc, hipStream_t st, int s0 = 0, int s1 = 0);
 void launch_amr_flux_correct(const AmrGrid &t, const km_pack_t &u, int nc, hipStream_t st);
 // CT: face B ghost fill across levels and EMF correction at level jumps
 void launch_amr_ghost_faces(const AmrGrid &t, const km_pack_t &u, hipStream_t st);

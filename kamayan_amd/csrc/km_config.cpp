@@ -95,6 +95,7 @@ int km_cfg_default(km_cfg_t *cfg) {
   cfg->dt_min_cycle_limit = 10;
   cfg->dt_max = DBL_MAX;
   cfg->dt_max_cycle_limit = 1;
+  cfg->pack_size = -1;
   cfg->dt_user = DBL_MAX;
   cfg->dt_force = -DBL_MAX;
   return KM_OK;
@@ -216,11 +217,15 @@ int km_cfg_set(km_cfg_t *cfg, const char *block_key, const char *value) {
       const char *name;
       int32_t *dst;
     } ints[] = {{"parthenon/time/dt_min_cycle_limit", &cfg->dt_min_cycle_limit},
-                {"parthenon/time/dt_max_cycle_limit", &cfg->dt_max_cycle_limit}};
+                {"parthenon/time/dt_max_cycle_limit", &cfg->dt_max_cycle_limit},
+                {"parthenon/mesh/pack_size", &cfg->pack_size}};
     for (const auto &r : ints)
       if (key == r.name) {
         long iv = 0;
-        if (!parse_int(val, &iv) || iv < 0 || iv > 1000000000L) {
+        // pack_size: -1 (every block in one pack) or a positive block count
+        const bool pack = r.dst == &cfg->pack_size;
+        if (!parse_int(val, &iv) || iv < (pack ? -1 : 0) || (pack && iv == 0) ||
+            iv > 1000000000L) {
           km::set_error("invalid value '" + val + "' for " + key);
           return KM_EINVAL;
         }

@@ -90,8 +90,11 @@ void launch_apply_dudt(const km_cfg_t &cfg, const km_pack_t &base, const km_pack
 void launch_prepare_primitive(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st);
 void launch_prepare_conserved(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st);
 // fail: device flag word; KM_FAIL_STATE is or-ed in when a zone is non-physical
+// part: 1 sets dt_dev to DBL_MAX first, 2 scales the minimum by cfl / ndim after (3: both;
+// packs of one mesh reduce into one minimum with 1, 0 ..., 2)
 void launch_estimate_dt(const km_cfg_t &cfg, const km_pack_t &u, double *dt_dev, hipStream_t st,
-                        const int *inst = nullptr, int ninst = 1, int *fail = nullptr);
+                        const int *inst = nullptr, int ninst = 1, int *fail = nullptr,
+                        int part = 3);
 void launch_fill_divb(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st);
 // hydro::PrepareConserved per stage (cylindrical CT: md0's B_phi to B_phi / r)
 void launch_prepare_conserved_stage(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st);

@@ -289,6 +289,9 @@ struct km_sim_t {
   // minimum) while other tiles still read u0's (RK1 updates u0 in place and keeps the
   // separate kernels)
   bool own_prims = false;
+  // fused path on a block tree: the gather lists of the stages' ghost fill (rebuilt with
+  // the tree)
+  AmrFillList fill;
   void drop_graph() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -336,7 +339,10 @@ struct km_sim_t {
   int alloc(T **out, size_t count) {
     auto b = std::make_unique<DevBuf>();
     KM_HIP(hipMalloc(&b->p, count * sizeof(T) + 256));
+    // the fill runs on the null stream, which the simulation's non-blocking stream does
+    // not wait for: finish it before a kernel on that stream writes the buffer
     KM_HIP(hipMemset(b->p, 0, count * sizeof(T) + 256));
+    KM_HIP(hipStreamSynchronize(nullptr));
     *out = static_cast<T *>(b->p);
     target->push_back(std::move(b));
     return KM_OK;
@@ -484,7 +490,23 @@ void ghost_fill(km_sim_t &s, const km_pack_t &p, int v0, int v1, bool faces) {
     launch_ghost_fill(s.grid, p, v0, v1, s.nc, faces, s.stream, Instances{s.inst_d, s.grids_d});
 }
 
-// one RK stage as the reference's task list
+// the blocks [b0, b0 + n) of a pack
+km_pack_t sub_pack(const km_pack_t &p, int b0, int n) {
+  km_pack_t q = p;
+  q.nblocks = n;
+  q.blocks = p.blocks + b0;
+  q.cell = p.cell + (long)b0 * p.nvar;
+  for (int d = 0; d < 3; ++d) {
+    if (p.flux[d]) q.flux[d] = p.flux[d] + (long)b0 * p.nflux;
+    if (p.face[d]) q.face[d] = p.face[d] + b0;
+    if (p.edge[d]) q.edge[d] = p.edge[d] + b0;
+  }
+  return q;
+}
+
+// one RK stage as the reference's task list; with parthenon/mesh/pack_size the per-block
+// tasks run pack by pack (MeshData partitions), the boundary exchange and the flux
+// correction over the whole mesh
 int task_stage(km_sim_t &s, int stage, double beta, double dt) {
   const km_cfg_t &cfg = s.cfg;
   Cont &md0 = container(s, stage - 1), &md1 = container(s, stage);
@@ -494,8 +516,15 @@ int task_stage(km_sim_t &s, int stage, double beta, double dt) {
     p0.edge[d] = s.scratch.pack.edge[d];
   }
   const bool ct = cfg.mhd == KM_MHD_CT;
-  launch_calculate_fluxes(cfg, p0, flux_box(cfg, p0), s.stream);
-  if (ct) launch_calculate_emf(cfg, p0, s.stream);
+  const int ps = cfg.pack_size > 0 && cfg.pack_size < s.nb ? cfg.pack_size : s.nb;
+  auto packs = [&](auto &&f) {
+    for (int b0 = 0; b0 < s.nb; b0 += ps) f(b0, std::min(ps, s.nb - b0));
+  };
+  packs([&](int b0, int n) {
+    const km_pack_t q0 = sub_pack(p0, b0, n);
+    launch_calculate_fluxes(cfg, q0, flux_box(cfg, q0), s.stream);
+    if (ct) launch_calculate_emf(cfg, q0, s.stream);
+  });
   // flux correction at level jumps (AddFluxCorrectionTasks, kamayan_driver.cpp:198-207):
   // cell fluxes and, for CT, the edge EMFs
   if (s.amr) {
@@ -504,10 +533,18 @@ int task_stage(km_sim_t &s, int stage, double beta, double dt) {
   }
   // FluxesToDuDt, ApplyDuDt, PreparePrimitive
   (void)dt;
-  launch_update(cfg, s.base.pack, p0, md1.pack, s.dudt.pack, beta, s.stream, s.dt_d, s.inst_d);
+  packs([&](int b0, int n) {
+    launch_update(cfg, sub_pack(s.base.pack, b0, n), sub_pack(p0, b0, n),
+                  sub_pack(md1.pack, b0, n), sub_pack(s.dudt.pack, b0, n), beta, s.stream,
+                  s.dt_d, s.inst_d ? s.inst_d + b0 : nullptr);
+  });
   if (stage == cfg.nstages) {
-    launch_estimate_dt(cfg, s.base.pack, s.est_d, s.stream, s.inst_d, s.ninst, s.fail_d);
-    if (ct) launch_fill_divb(cfg, s.base.pack, s.stream);
+    packs([&](int b0, int n) {
+      const int part = (b0 == 0 ? 1 : 0) | (b0 + n == s.nb ? 2 : 0);
+      launch_estimate_dt(cfg, sub_pack(s.base.pack, b0, n), s.est_d, s.stream,
+                         s.inst_d ? s.inst_d + b0 : nullptr, s.ninst, s.fail_d, part);
+      if (ct) launch_fill_divb(cfg, sub_pack(s.base.pack, b0, n), s.stream);
+    });
   }
   // FillGhost fields: conserved, VELOCITY, PRES, BMOD, EINT, face B (TEMP is not), MFRAC
   ghost_fill(s, md1.pack, 0, KM_TEMP(cfg.mhd), ct);
@@ -822,9 +859,8 @@ int amr_fused_stage(km_sim_t &s, int stage, double beta) {
   // density and the primitives in the face slabs the next sweeps read (the
   // reconstruction's reach plus one: the cells of the end faces' stencils); momentum,
   // energy, edges and corners are left out
-  const int reach = cfg.recon == KM_RECON_FOG ? 1 : cfg.recon == KM_RECON_PLM ? 2 : 3;
-  launch_amr_ghost_fill(s.agrid, md1.pack, 0, KM_TEMP(cfg.mhd), s.nc, s.stream, 1,
-                        KM_NCONS(cfg.mhd), reach);
+  launch_amr_ghost_fill_list(s.fill, s.agrid, md1.pack, 0, KM_TEMP(cfg.mhd), s.nc, s.stream, 1,
+                             KM_NCONS(cfg.mhd));
   s.lazy_ghosts = true;
   if (s.profile) {
     (void)hipEventRecord(g1, s.stream);
@@ -832,6 +868,52 @@ int amr_fused_stage(km_sim_t &s, int stage, double beta) {
     (void)hipEventRecord(st1, s.stream);
     s.recs.push_back({4, st0, st1});
   }
+  return check_launch();
+}
+
+// the ghost layers the fused block-tree sweeps read: the reconstruction's reach plus one
+// (the cells of the end faces' stencils)
+int fused_reach(const km_cfg_t &cfg) {
+  return cfg.recon == KM_RECON_FOG ? 1 : cfg.recon == KM_RECON_PLM ? 2 : 3;
+}
+
+// The gather lists of the fused stages' ghost fill for the current tree (allocated into
+// the mesh's buffers): the list entries come from the same tree walk as
+// launch_amr_ghost_fill, the prolonged parents are compacted on the host.
+int build_fill_list(km_sim_t &s) {
+  const km_pack_t &u = s.base.pack;
+  const int reach = fused_reach(s.cfg);
+  AmrFillList f;
+  f.depth = reach >= s.mesh.ng ? 0 : reach;
+  if (Geo(u).ncell() >= (1L << 28)) {
+    set_error("block too large for the ghost gather list (2^28 cells)");
+    return KM_EINVAL;
+  }
+  f.n = amr_ghost_list_size(u, f.depth);
+  int rc;
+  if ((rc = s.alloc(&f.list, (size_t)f.n))) return rc;
+  launch_amr_ghost_list(s.agrid, u, f.depth, f.list, s.stream);
+  if (s.agrid.lmax > 0) {
+    const long np = amr_prolong_parents(u, f.depth);
+    DevBuf flag_d;
+    KM_HIP(hipMalloc(&flag_d.p, np > 0 ? np : 1));
+    launch_amr_prolong_flags(s.agrid, u, f.depth, static_cast<unsigned char *>(flag_d.p),
+                             s.stream);
+    if ((rc = check_launch())) return rc;
+    std::vector<unsigned char> flag(np);
+    KM_HIP(hipMemcpyAsync(flag.data(), flag_d.p, np, hipMemcpyDeviceToHost, s.stream));
+    KM_HIP(hipStreamSynchronize(s.stream));
+    std::vector<long> pl;
+    for (long p = 0; p < np; ++p)
+      if (flag[p]) pl.push_back(p);
+    f.np = (long)pl.size();
+    long *pl_d = nullptr;
+    if ((rc = s.alloc(&pl_d, pl.size() + 1))) return rc;
+    if (f.np)
+      KM_HIP(hipMemcpy(pl_d, pl.data(), pl.size() * sizeof(long), hipMemcpyHostToDevice));
+    f.plist = pl_d;
+  }
+  s.fill = f;
   return check_launch();
 }
 
@@ -898,6 +980,7 @@ int amr_remesh(km_sim_t &s, const std::vector<int32_t> &nl, const std::vector<in
     const long nint = (long)s.mesh.nx[0] * (s.mesh.ndim > 1 ? s.mesh.nx[1] : 1) *
                       (s.mesh.ndim > 2 ? s.mesh.nx[2] : 1);
     if ((rc = s.alloc(&s.du, (size_t)n * 5 * nint))) return rc;
+    if ((rc = build_fill_list(s))) return rc;
   }
   int32_t *origin_d = nullptr;
   if ((rc = s.alloc(&origin_d, origin.size()))) return rc;
@@ -1484,6 +1567,8 @@ static int sim_create(const km_cfg_t *cfg, const km_mesh_desc_t *meshes, int32_t
     const long nint = (long)shape.nx[0] * shape.nx[1] * shape.nx[2];
     if ((rc = s->alloc(&s->du, (size_t)s->nb * 5 * nint))) return rc;
   }
+  if (path == KM_PATH_FUSED && tree)
+    if ((rc = build_fill_list(*s))) return rc;
   if (path == KM_PATH_FUSED && !tree) {
     const std::vector<int> nbr = uniform_neighbours(meshes, ninst, s->nb);
     if ((rc = s->alloc(&s->nbr_d, nbr.size()))) return rc;
@@ -1672,8 +1757,10 @@ int km_sim_advance(km_sim_t *s, int32_t ncycles, double tlim) {
       }
       continue;
     }
-    // no host decisions: capture one cycle in a graph and replay it
-    if (s->use_graph && !s->profile) {
+    // no host decisions: capture one cycle in a graph and replay it (not with
+    // parthenon/mesh/pack_size, whose packs are launched from the host as the reference's
+    // task list does)
+    if (s->use_graph && !s->profile && s->cfg.pack_size <= 0) {
       if (!s->graph_exec) {
         KM_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeRelaxed));
         const int nc0 = s->ncycle;

@@ -1137,17 +1137,20 @@ void launch_prepare_conserved(const km_cfg_t &cfg, const km_pack_t &u, hipStream
 }
 
 void launch_estimate_dt(const km_cfg_t &cfg, const km_pack_t &u, double *dt_dev,
-                        hipStream_t st, const int *inst, int ninst, int *fail) {
+                        hipStream_t st, const int *inst, int ninst, int *fail, int part) {
   const Geo g(u);
-  hipLaunchKernelGGL(k_fill_value, dim3(nblk(ninst, 64)), dim3(64), 0, st, dt_dev, DBL_MAX, ninst);
+  if (part & 1)
+    hipLaunchKernelGGL(k_fill_value, dim3(nblk(ninst, 64)), dim3(64), 0, st, dt_dev, DBL_MAX,
+                       ninst);
   const unsigned grid = nblk(interior_count(g, u.nblocks));
   const int cyl = cfg.geometry == KM_GEOM_CYLINDRICAL;
   if (cfg.mhd)
     hipLaunchKernelGGL(k_estimate_dt<1>, dim3(grid), dim3(256), 0, st, u, dt_dev, inst, cyl, fail);
   else
     hipLaunchKernelGGL(k_estimate_dt<0>, dim3(grid), dim3(256), 0, st, u, dt_dev, inst, cyl, fail);
-  hipLaunchKernelGGL(k_scale_value, dim3(nblk(ninst, 64)), dim3(64), 0, st, dt_dev,
-                     cfg.cfl / static_cast<double>(g.ndim), ninst);
+  if (part & 2)
+    hipLaunchKernelGGL(k_scale_value, dim3(nblk(ninst, 64)), dim3(64), 0, st, dt_dev,
+                       cfg.cfl / static_cast<double>(g.ndim), ninst);
 }
 
 void launch_fill_divb(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st) {
