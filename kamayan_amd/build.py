@@ -42,6 +42,9 @@ UNITS = [
     ("km_fused_exact.hip", "off"),
     ("km_fused_fma.hip", "fast"),
     ("km_fused_fp32.hip", "fast"),
+    ("km_fused_mhd_exact.hip", "off"),
+    ("km_fused_mhd_fma.hip", "fast"),
+    ("km_fused_mhd_fp32.hip", "fast"),
     ("km_config.cpp", "off"),
     ("km_problems.cpp", "off"),
     ("km_amr.hip", "off"),

@@ -88,6 +88,11 @@ void launch_apply_dudt(const km_cfg_t &cfg, const km_pack_t &base, const km_pack
                        const km_pack_t &u1, const km_pack_t &dudt, double beta, double dt,
                        hipStream_t st, const double *dt_dev = nullptr, const int *inst = nullptr);
 void launch_prepare_primitive(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st);
+// FluxStokes (u0's edges into dudt's faces) and the face part of ApplyDuDt into u1, for
+// Cartesian CT in two or more dimensions
+void launch_ct_face_update(const km_pack_t &base, const km_pack_t &u0, const km_pack_t &u1,
+                           const km_pack_t &dudt, double beta, hipStream_t st,
+                           const double *dt_dev, const int *inst);
 void launch_prepare_conserved(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st);
 // fail: device flag word; KM_FAIL_STATE is or-ed in when a zone is non-physical
 // part: 1 sets dt_dev to DBL_MAX first, 2 scales the minimum by cfl / ndim after (3: both;
@@ -121,6 +126,13 @@ int launch_fine_face_fluxes(const km_cfg_t &cfg, const FusedArgs &a, hipStream_t
 int launch_fused_stage3(const km_cfg_t &cfg, const FusedArgs &a, bool with_dt, int kern,
                         hipStream_t st);
 int stage3_kernels();
+// the CT-MHD stage (3-D, Cartesian, single-level meshes; km_fused_mhd.inc): one sweep
+// along x(dir+1) with tile shape kern in 1..mhd_tile_shapes(), and the zone pass after
+// the face update (dU/dt, stage combine, cell B from faces, PreparePrimitive of u1)
+int launch_fused_mhd_sweep(const km_cfg_t &cfg, const FusedArgs &a, int dir, int kern,
+                           hipStream_t st);
+int mhd_tile_shapes();
+int launch_ct_zone(const FusedArgs &a, hipStream_t st);
 }
 namespace fma {
 int launch_fused_sweep(const km_cfg_t &cfg, const FusedArgs &a, int ndim, int dir, bool with_dt,
@@ -131,6 +143,13 @@ int launch_fine_face_fluxes(const km_cfg_t &cfg, const FusedArgs &a, hipStream_t
 int launch_fused_stage3(const km_cfg_t &cfg, const FusedArgs &a, bool with_dt, int kern,
                         hipStream_t st);
 int stage3_kernels();
+// the CT-MHD stage (3-D, Cartesian, single-level meshes; km_fused_mhd.inc): one sweep
+// along x(dir+1) with tile shape kern in 1..mhd_tile_shapes(), and the zone pass after
+// the face update (dU/dt, stage combine, cell B from faces, PreparePrimitive of u1)
+int launch_fused_mhd_sweep(const km_cfg_t &cfg, const FusedArgs &a, int dir, int kern,
+                           hipStream_t st);
+int mhd_tile_shapes();
+int launch_ct_zone(const FusedArgs &a, hipStream_t st);
 }
 namespace fp32 {
 int launch_fused_sweep(const km_cfg_t &cfg, const FusedArgs &a, int ndim, int dir, bool with_dt,
@@ -141,6 +160,13 @@ int launch_fine_face_fluxes(const km_cfg_t &cfg, const FusedArgs &a, hipStream_t
 int launch_fused_stage3(const km_cfg_t &cfg, const FusedArgs &a, bool with_dt, int kern,
                         hipStream_t st);
 int stage3_kernels();
+// the CT-MHD stage (3-D, Cartesian, single-level meshes; km_fused_mhd.inc): one sweep
+// along x(dir+1) with tile shape kern in 1..mhd_tile_shapes(), and the zone pass after
+// the face update (dU/dt, stage combine, cell B from faces, PreparePrimitive of u1)
+int launch_fused_mhd_sweep(const km_cfg_t &cfg, const FusedArgs &a, int dir, int kern,
+                           hipStream_t st);
+int mhd_tile_shapes();
+int launch_ct_zone(const FusedArgs &a, hipStream_t st);
 }
 
 // thread-local error message for km_last_error()

@@ -45,16 +45,16 @@ KM_HD float ksign(float x) { return __builtin_copysignf(1.0f, x); }
 // Division and square root: IEEE for double (the reference's rounding); the FP32 form
 // uses the hardware reciprocal / square root, whose ~1 ulp error is far below the
 // FP32 form's own truncation of the states.
-// FAST (the FP64-FMA form): the hardware reciprocal estimate refined by two Newton steps
-// (each doubles the correct bits), and for a quotient one residual correction
-// q + (a - b q) / b; within an ulp or so of the IEEE result for normal operands, which
-// every divisor here is (densities, wave-speed differences, WENO weights >= 1e-36).
+// FAST (the FP64-FMA form): the hardware reciprocal estimate refined by one Newton step,
+// which squares its relative error (a few ulps of double remain), and for a quotient one
+// residual correction q + (a - b q) / b, within an ulp or so of the IEEE result. Every
+// divisor here is a normal number (densities, wave-speed differences, WENO weights
+// >= 1e-36); the form's contract is 1e-12 relative L1 agreement with the IEEE form.
 template <bool FAST = false>
 KM_HD double krcp(double b) {
 #if defined(__HIP_DEVICE_COMPILE__)
   if constexpr (FAST) {
     double r = __builtin_amdgcn_rcp(b);
-    r = __builtin_fma(__builtin_fma(-b, r, 1.0), r, r);
     r = __builtin_fma(__builtin_fma(-b, r, 1.0), r, r);
     return r;
   }

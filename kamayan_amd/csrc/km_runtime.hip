@@ -590,7 +590,9 @@ FusedArgs fused_args(km_sim_t &s, int stage, double beta) {
   a.agrid = s.agrid;
   a.fail = s.fail_d;
   a.write_prims = s.own_prims ? 1 : 0;
-  if (s.amr)  // fluxes at level jumps live in the task path's flux arrays
+  // block trees: fluxes at level jumps live in the task path's flux arrays; CT: the
+  // sweeps store the transverse B fluxes there for the EMF
+  if (s.amr || s.cfg.mhd == KM_MHD_CT)
     for (int d = 0; d < 3; ++d) a.u0.flux[d] = s.scratch.pack.flux[d];
   return a;
 }
@@ -599,6 +601,13 @@ template <class S>
 int launch_sweep(const S &s, const FusedArgs &a, int dir, bool last, int kern) {
   const km_cfg_t &cfg = s.cfg;
   const int ndim = s.mesh.ndim;
+  if (cfg.mhd == KM_MHD_CT) {  // CT stage sweeps: pencil tiles only (no march)
+    if (cfg.precision == KM_PREC_FP32_MIXED)
+      return fp32::launch_fused_mhd_sweep(cfg, a, dir, kern, s.stream);
+    if (cfg.precision == KM_PREC_FP64_FMA)
+      return fma::launch_fused_mhd_sweep(cfg, a, dir, kern, s.stream);
+    return exact::launch_fused_mhd_sweep(cfg, a, dir, kern, s.stream);
+  }
   if (cfg.precision == KM_PREC_FP32_MIXED)
     return fp32::launch_fused_sweep(cfg, a, ndim, dir, last, kern, s.stream);
   if (cfg.precision == KM_PREC_FP64_FMA)
@@ -635,6 +644,7 @@ bool stage3_applies(const S &s) {
 }
 
 int tile_shapes(const km_cfg_t &cfg) {
+  if (cfg.mhd == KM_MHD_CT) return exact::mhd_tile_shapes();  // the same shapes in every form
   if (cfg.precision == KM_PREC_FP32_MIXED) return fp32::tile_shapes();
   if (cfg.precision == KM_PREC_FP64_FMA) return fma::tile_shapes();
   return exact::tile_shapes();
@@ -694,7 +704,7 @@ int autotune(S &s, const FusedArgs &a) {
     }
     const std::vector<int> key = {cfg.precision, cfg.recon, cfg.limiter, cfg.riemann, ndim,
                                   s.mesh.nx[0], s.mesh.nx[1], s.mesh.nx[2], s.nb, dir,
-                                  s.amr ? 1 : 0};
+                                  s.amr ? 1 : 0, cfg.mhd};
     auto it = g_tune.find(key);
     if (it != g_tune.end()) {
       s.kern[dir] = it->second;
@@ -702,7 +712,8 @@ int autotune(S &s, const FusedArgs &a) {
     }
     int best = -1;
     float best_ms = 0.f;
-    for (int k = s.amr ? 1 : 0; k <= nshape; ++k) {  // block trees run tiles only
+    // block trees and CT stages run tiles only
+    for (int k = s.amr || cfg.mhd == KM_MHD_CT ? 1 : 0; k <= nshape; ++k) {
       float ms = 0.f;
       if (int rc = time_launch(s, [&] { return launch_sweep(s, a, dir, false, k); }, &ms))
         return rc;
@@ -944,6 +955,65 @@ int fused_stage(km_sim_t &s, int stage, double beta) {
   return check_launch();
 }
 
+// One RK stage of the fused path for constrained-transport MHD (3-D, Cartesian, single
+// level). The stage input's primitives, cell-centred B and ghost zones are current (the
+// previous stage's zone pass and ghost exchange, or the initial conditions), so the three
+// pencil-tile sweeps read the stored recon variables (the task path's flux kernels read
+// the same). They leave du and the transverse B fluxes; then the edge EMFs, FluxStokes
+// and the face update (the task kernels), the zone pass (dU/dt, stage combine, cell B
+// from the new faces, PreparePrimitive of u1), on the last stage the CFL estimate and
+// divB, and the ghost exchange of the task path. With one stage u1 is the base state:
+// every kernel after the sweeps reads and writes each zone and face in the same lane.
+int ct_fused_stage(km_sim_t &s, int stage, double beta) {
+  const km_cfg_t &cfg = s.cfg;
+  Cont &md1 = container(s, stage);
+  FusedArgs a = fused_args(s, stage, beta);
+  a.u1 = md1.pack;
+  hipEvent_t st0 = nullptr, st1 = nullptr;
+  if (s.profile) {
+    st0 = s.ev();
+    st1 = s.ev();
+    (void)hipEventRecord(st0, s.stream);
+  }
+  auto rec = sweep_recorder(s);
+  for (int dir = 0; dir < 3; ++dir) {
+    rec(dir, true);
+    if (int rc = launch_sweep(s, a, dir, false, s.kern[dir] > 0 ? s.kern[dir] : 1)) {
+      set_error("CT sweep tile shape " + std::to_string(s.kern[dir]) + " along x" +
+                std::to_string(dir + 1) + " needs more LDS than a workgroup has for these blocks");
+      return rc;
+    }
+    rec(dir, false);
+  }
+  km_pack_t p0 = a.u0;  // md0 with the scratch flux arrays and edges
+  for (int d = 0; d < 3; ++d) p0.edge[d] = s.scratch.pack.edge[d];
+  launch_calculate_emf(cfg, p0, s.stream);
+  launch_ct_face_update(s.base.pack, p0, md1.pack, s.dudt.pack, beta, s.stream, s.dt_d,
+                        s.inst_d);
+  int rc = cfg.precision == KM_PREC_FP32_MIXED ? fp32::launch_ct_zone(a, s.stream)
+           : cfg.precision == KM_PREC_FP64_FMA  ? fma::launch_ct_zone(a, s.stream)
+                                                : exact::launch_ct_zone(a, s.stream);
+  if (rc) return rc;
+  if (stage == cfg.nstages) {
+    launch_estimate_dt(cfg, s.base.pack, s.est_d, s.stream, s.inst_d, s.ninst, s.fail_d);
+    launch_fill_divb(cfg, s.base.pack, s.stream);
+  }
+  hipEvent_t g0 = nullptr, g1 = nullptr;
+  if (s.profile) {
+    g0 = s.ev();
+    g1 = s.ev();
+    (void)hipEventRecord(g0, s.stream);
+  }
+  ghost_fill(s, md1.pack, 0, KM_TEMP(cfg.mhd), true);
+  if (s.profile) {
+    (void)hipEventRecord(g1, s.stream);
+    s.recs.push_back({3, g0, g1});
+    (void)hipEventRecord(st1, s.stream);
+    s.recs.push_back({4, st0, st1});
+  }
+  return check_launch();
+}
+
 // Replace the mesh of a block-tree simulation by new leaves: every leaf-sized buffer is
 // allocated anew, base data moves by km_amr_remesh, then the old buffers are released.
 int amr_remesh(km_sim_t &s, const std::vector<int32_t> &nl, const std::vector<int32_t> &origin) {
@@ -1078,7 +1148,8 @@ int history_row(km_sim_t &s) {
   KM_HIP(hipStreamSynchronize(s.stream));
   std::fprintf(s.hist, "%.14e %.14e %d %d", t, dt, s.ncycle, s.nb);
   if (s.hist_vortex) {
-    if (s.path == KM_PATH_FUSED && !s.amr && !s.prims_from_ic) {  // conserved interiors only
+    // the single-level hydro fused path keeps the conserved interiors only
+    if (s.path == KM_PATH_FUSED && !s.amr && s.cfg.mhd == KM_MHD_OFF && !s.prims_from_ic) {
       launch_ghost_fill(s.grid, s.base.pack, 0, s.nc, s.nc, false, s.stream,
                         Instances{s.inst_d, s.grids_d});
       launch_prepare_primitive(s.cfg, s.base.pack, s.stream);
@@ -1103,6 +1174,8 @@ int run_cycle(km_sim_t &s, double dt_host) {
       rc = task_stage(s, stage, betas[stage - 1], dt_host);
     else if (s.amr)
       rc = amr_fused_stage(s, stage, betas[stage - 1]);
+    else if (s.cfg.mhd == KM_MHD_CT)
+      rc = ct_fused_stage(s, stage, betas[stage - 1]);
     else
       rc = fused_stage(s, stage, betas[stage - 1]);
     s.prims_from_ic = false;
@@ -1463,8 +1536,9 @@ static int sim_create(const km_cfg_t *cfg, const km_mesh_desc_t *meshes, int32_t
   s->path = path;
   s->ninst = ninst;
   s->meshes.assign(meshes, meshes + ninst);
-  if (path == KM_PATH_FUSED && cfg->mhd != KM_MHD_OFF) {
-    set_error("the fused stage path is built for hydro (MHD off)");
+  if (path == KM_PATH_FUSED && cfg->mhd != KM_MHD_OFF && (mesh->ndim != 3 || tree)) {
+    set_error("the fused path runs CT MHD on 3-D single-level meshes; 1-D and 2-D CT and "
+              "block trees with CT run on the task path");
     return KM_EINVAL;
   }
   if (path == KM_PATH_FUSED && cfg->nspecies > 1) {
@@ -1698,8 +1772,8 @@ int km_sim_get_state(km_sim_t *s, double *cell, double *const face[3]) {
     fill_lazy_ghosts(*s);
     KM_HIP(hipStreamSynchronize(s->stream));  // the copies below run on another stream
   }
-  if (s->path == KM_PATH_FUSED && !s->amr) {
-    // the single-level fused path keeps only the conserved interior current; fill the
+  if (s->path == KM_PATH_FUSED && !s->amr && s->cfg.mhd == KM_MHD_OFF) {
+    // the single-level hydro fused path keeps only the conserved interior current; fill the
     // ghost zones and derive the primitives
     launch_ghost_fill(s->grid, s->base.pack, 0, s->nc, s->nc, false, s->stream,
                       Instances{s->inst_d, s->grids_d});
@@ -1810,6 +1884,11 @@ int km_sim_set_fused_kernel(km_sim_t *s, int32_t dir, int32_t kern) {
   if (kern == 0 && s->amr && s->path == KM_PATH_FUSED) {
     set_error("the pencil march reads face neighbours of a single-level mesh; block trees "
               "run the fused sweeps as pencil tiles");
+    return KM_EINVAL;
+  }
+  if (kern == 0 && s->cfg.mhd == KM_MHD_CT && s->path == KM_PATH_FUSED) {
+    set_error("the fused CT stage runs its sweeps as pencil tiles (1.." +
+              std::to_string(tile_shapes(s->cfg)) + ")");
     return KM_EINVAL;
   }
   KM_HIP(hipStreamSynchronize(s->stream));

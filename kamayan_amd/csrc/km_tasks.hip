@@ -1105,6 +1105,37 @@ void launch_update(const km_cfg_t &cfg, const km_pack_t &base, const km_pack_t &
   hipLaunchKernelGGL(k_prepare_primitive<1>, dim3(grid), dim3(256), 0, st, u1, eos, cyl);
 }
 
+// FluxStokes of every face family from u0's edge EMFs into dudt, then the face part of
+// ApplyDuDt into u1 (Cartesian CT, two or more dimensions): the face kernels of
+// launch_update, for the fused CT stage, whose zone pass follows
+void launch_ct_face_update(const km_pack_t &base, const km_pack_t &u0, const km_pack_t &u1,
+                           const km_pack_t &dudt, double beta, hipStream_t st,
+                           const double *dt_dev, const int *inst) {
+  const Geo g(u0);
+  if (g.ndim < 2) return;
+  for (int f = 0; f < g.ndim; ++f) {
+    const unsigned grid = nblk(face_count(g, f, u0.nblocks));
+    if (f == 0)
+      hipLaunchKernelGGL(k_stokes<0>, dim3(grid), dim3(256), 0, st, u0, dudt, 0);
+    else if (f == 1)
+      hipLaunchKernelGGL(k_stokes<1>, dim3(grid), dim3(256), 0, st, u0, dudt, 0);
+    else
+      hipLaunchKernelGGL(k_stokes<2>, dim3(grid), dim3(256), 0, st, u0, dudt, 0);
+  }
+  for (int f = 0; f < g.ndim; ++f) {
+    const unsigned grid = nblk(face_count(g, f, u0.nblocks));
+    if (f == 0)
+      hipLaunchKernelGGL(k_apply_face<0>, dim3(grid), dim3(256), 0, st, base, u0, u1, dudt,
+                         beta, 0.0, dt_dev, inst);
+    else if (f == 1)
+      hipLaunchKernelGGL(k_apply_face<1>, dim3(grid), dim3(256), 0, st, base, u0, u1, dudt,
+                         beta, 0.0, dt_dev, inst);
+    else
+      hipLaunchKernelGGL(k_apply_face<2>, dim3(grid), dim3(256), 0, st, base, u0, u1, dudt,
+                         beta, 0.0, dt_dev, inst);
+  }
+}
+
 void launch_prepare_primitive(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st) {
   const Geo g(u);
   const Eos eos = make_eos(cfg);
