@@ -123,24 +123,34 @@ __device__ __forceinline__ void dt_min_push(double *p, double v) {
 // x1-x2 planes), then the two x2 slabs (interior x3 rows), then the two x1 slabs
 // (interior x2-x3 rows, ng cells at each end); x1 runs fastest within each piece.
 // stag = d enumerates the ghost faces of the d-staggered array instead (one more
-// interior face and one more element along d); stag = -1 the cells.
-KM_HD long ghost_count(const Geo &g, int stag = -1) {
+// interior face and one more element along d); stag = -1 the cells. h in [1, ng)
+// enumerates only the ghost zones within h zones of the interior (edges and corners
+// included), h <= 0 all of them.
+KM_HD long ghost_count(const Geo &g, int stag = -1, int h = 0) {
+  if (h <= 0 || h > g.ng) h = g.ng;
   long all = 1, in = 1;
   for (int d = 0; d < 3; ++d) {
     const int s = d == stag ? 1 : 0;
-    all *= g.E[d] + s;
+    all *= d < g.ndim ? g.nx[d] + s + 2 * h : g.E[d] + s;
     in *= d < g.ndim ? g.nx[d] + s : 1;
   }
   return all - in;
 }
-KM_HD void ghost_cell(const Geo &g, long r, int idx[3], int stag = -1) {
-  const int ng = g.ng;
-  int E[3], n[3];
+KM_HD void ghost_cell(const Geo &g, long r, int idx[3], int stag = -1, int h = 0) {
+  if (h <= 0 || h > g.ng) h = g.ng;
+  const int ng = h;
+  int E[3], n[3], lo[3];
   for (int d = 0; d < 3; ++d) {
     const int s = d == stag ? 1 : 0;
-    E[d] = g.E[d] + s;
     n[d] = d < g.ndim ? g.nx[d] + s : 1;
+    E[d] = d < g.ndim ? n[d] + 2 * h : g.E[d] + s;
+    lo[d] = d < g.ndim ? h : 0;
   }
+  // the enumeration over the block shrunk to h ghost layers, then shifted by ng - h
+  const int sh = g.ng - h;
+  auto done = [&]() {
+    for (int d = 0; d < g.ndim; ++d) idx[d] += sh;
+  };
   if (g.ndim > 2) {
     const long zs = (long)ng * E[1] * E[0];
     if (r < 2 * zs) {
@@ -150,6 +160,7 @@ KM_HD void ghost_cell(const Geo &g, long r, int idx[3], int stag = -1) {
       q /= E[0];
       idx[1] = (int)(q % E[1]);
       idx[2] = (int)(q / E[1]) + (side ? ng + n[2] : 0);
+      done();
       return;
     }
     r -= 2 * zs;
@@ -162,7 +173,8 @@ KM_HD void ghost_cell(const Geo &g, long r, int idx[3], int stag = -1) {
       idx[0] = (int)(q % E[0]);
       q /= E[0];
       idx[1] = (int)(q % ng) + (side ? ng + n[1] : 0);
-      idx[2] = g.lo[2] + (int)(q / ng);
+      idx[2] = lo[2] + (int)(q / ng);
+      done();
       return;
     }
     r -= 2 * ys;
@@ -172,8 +184,9 @@ KM_HD void ghost_cell(const Geo &g, long r, int idx[3], int stag = -1) {
   const int side = (int)(q % 2);
   q /= 2;
   idx[0] = i + (side ? ng + n[0] : 0);
-  idx[1] = g.lo[1] + (int)(q % n[1]);
-  idx[2] = g.lo[2] + (int)(q / n[1]);
+  idx[1] = lo[1] + (int)(q % n[1]);
+  idx[2] = lo[2] + (int)(q / n[1]);
+  done();
 }
 
 // The ghost zones a directional sweep reads: for each active dimension d, the two slabs

@@ -88,11 +88,10 @@ void launch_apply_dudt(const km_cfg_t &cfg, const km_pack_t &base, const km_pack
                        const km_pack_t &u1, const km_pack_t &dudt, double beta, double dt,
                        hipStream_t st, const double *dt_dev = nullptr, const int *inst = nullptr);
 void launch_prepare_primitive(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st);
-// FluxStokes (u0's edges into dudt's faces) and the face part of ApplyDuDt into u1, for
-// Cartesian CT in two or more dimensions
-void launch_ct_face_update(const km_pack_t &base, const km_pack_t &u0, const km_pack_t &u1,
-                           const km_pack_t &dudt, double beta, hipStream_t st,
-                           const double *dt_dev, const int *inst);
+// EMF, FluxStokes and the face part of ApplyDuDt into u1 in one kernel (Cartesian 3-D CT;
+// the EMF from u0's stored transverse B fluxes), with the stage's dt on the device
+void launch_ct_faces(const km_pack_t &base, const km_pack_t &u0, const km_pack_t &u1,
+                     double beta, hipStream_t st, const double *dt_dev, const int *inst);
 void launch_prepare_conserved(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st);
 // fail: device flag word; KM_FAIL_STATE is or-ed in when a zone is non-physical
 // part: 1 sets dt_dev to DBL_MAX first, 2 scales the minimum by cfl / ndim after (3: both;
@@ -110,8 +109,11 @@ struct HistVortex {
 // per-block sums of |exact - q| V of DENS, VEL0..2, PRES (+ MAGC0..2): part[b * nv + v]
 void launch_history_vortex(const km_cfg_t &cfg, const km_pack_t &u, const HistVortex &p,
                            double *part, hipStream_t st);
+// depth / face_depth in [1, ng): only the ghost cells / faces within that many zones of
+// the interior (edges and corners included); <= 0: all of them
 void launch_ghost_fill(const UniformGrid &gr, const km_pack_t &u, int v0, int v1, int nc,
-                       bool faces, hipStream_t st, Instances in = Instances());
+                       bool faces, hipStream_t st, Instances in = Instances(), int depth = 0,
+                       int face_depth = 0);
 
 // fused stage sweeps: one translation unit per arithmetic form
 //   exact : FP64, -ffp-contract=off (bitwise the reference's operation order)

@@ -932,7 +932,7 @@ int build_fill_list(km_sim_t &s) {
 // density and the primitives outside the face slabs
 void fill_lazy_ghosts(km_sim_t &s) {
   if (!s.lazy_ghosts) return;
-  ghost_fill(s, s.base.pack, 0, KM_TEMP(s.cfg.mhd), false);
+  ghost_fill(s, s.base.pack, 0, KM_TEMP(s.cfg.mhd), s.cfg.mhd == KM_MHD_CT);
   s.lazy_ghosts = false;
 }
 
@@ -959,8 +959,8 @@ int fused_stage(km_sim_t &s, int stage, double beta) {
 // level). The stage input's primitives, cell-centred B and ghost zones are current (the
 // previous stage's zone pass and ghost exchange, or the initial conditions), so the three
 // pencil-tile sweeps read the stored recon variables (the task path's flux kernels read
-// the same). They leave du and the transverse B fluxes; then the edge EMFs, FluxStokes
-// and the face update (the task kernels), the zone pass (dU/dt, stage combine, cell B
+// the same). They leave du and the transverse B fluxes; then one kernel forms the edge
+// EMFs, FluxStokes and the face update, the zone pass (dU/dt, stage combine, cell B
 // from the new faces, PreparePrimitive of u1), on the last stage the CFL estimate and
 // divB, and the ghost exchange of the task path. With one stage u1 is the base state:
 // every kernel after the sweeps reads and writes each zone and face in the same lane.
@@ -985,11 +985,8 @@ int ct_fused_stage(km_sim_t &s, int stage, double beta) {
     }
     rec(dir, false);
   }
-  km_pack_t p0 = a.u0;  // md0 with the scratch flux arrays and edges
-  for (int d = 0; d < 3; ++d) p0.edge[d] = s.scratch.pack.edge[d];
-  launch_calculate_emf(cfg, p0, s.stream);
-  launch_ct_face_update(s.base.pack, p0, md1.pack, s.dudt.pack, beta, s.stream, s.dt_d,
-                        s.inst_d);
+  // md0 with the scratch flux arrays: EMF, Stokes and face update in one kernel
+  launch_ct_faces(s.base.pack, a.u0, md1.pack, beta, s.stream, s.dt_d, s.inst_d);
   int rc = cfg.precision == KM_PREC_FP32_MIXED ? fp32::launch_ct_zone(a, s.stream)
            : cfg.precision == KM_PREC_FP64_FMA  ? fma::launch_ct_zone(a, s.stream)
                                                 : exact::launch_ct_zone(a, s.stream);
@@ -1004,7 +1001,15 @@ int ct_fused_stage(km_sim_t &s, int stage, double beta) {
     g1 = s.ev();
     (void)hipEventRecord(g0, s.stream);
   }
-  ghost_fill(s, md1.pack, 0, KM_TEMP(cfg.mhd), true);
+  // the ghost zones the next sweeps read: density, cell B and the primitives within the
+  // reconstruction's reach plus one of the interior, and the face field one zone deep;
+  // the rest (momentum and energy, deeper layers) are filled when something reads them
+  const Instances in{s.inst_d, s.grids_d};
+  const int reach = fused_reach(cfg);
+  launch_ghost_fill(s.grid, md1.pack, KM_DENS, KM_DENS + 1, s.nc, false, s.stream, in, reach);
+  launch_ghost_fill(s.grid, md1.pack, KM_MAGC0, KM_TEMP(cfg.mhd), s.nc, true, s.stream, in,
+                    reach, 1);
+  s.lazy_ghosts = true;
   if (s.profile) {
     (void)hipEventRecord(g1, s.stream);
     s.recs.push_back({3, g0, g1});
@@ -1857,7 +1862,8 @@ int km_sim_advance(km_sim_t *s, int32_t ncycles, double tlim) {
     }
   }
   // replayed cycles skip the host code that marks the momentum and energy ghosts stale
-  if (s->amr && s->path == KM_PATH_FUSED && ncycles > 0) s->lazy_ghosts = true;
+  if ((s->amr || s->cfg.mhd == KM_MHD_CT) && s->path == KM_PATH_FUSED && ncycles > 0)
+    s->lazy_ghosts = true;
   KM_HIP(hipStreamSynchronize(s->stream));
   return check_failure(*s);
 }

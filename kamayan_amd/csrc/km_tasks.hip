@@ -781,14 +781,14 @@ __device__ __forceinline__ int map_face(int g, int N, int lo, int hi, bool &flip
 }
 
 __global__ void __launch_bounds__(256) k_ghost_cells(km_pack_t u, UniformGrid gr0, int v0, int v1,
-                                                     int nc, Instances in) {
+                                                     int nc, Instances in, int depth) {
   const Geo g(u);
-  const long n = ghost_count(g);
+  const long n = ghost_count(g, -1, depth);
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (t >= n * u.nblocks) return;
   const int b = (int)(t / n);
   int idx[3];
-  ghost_cell(g, t % n, idx);
+  ghost_cell(g, t % n, idx, -1, depth);
   const UniformGrid gr = in.inst ? in.grids[in.inst[b]] : gr0;
   const int lb = b - gr.first;
   const int bl[3] = {lb % gr.nb[0], (lb / gr.nb[0]) % gr.nb[1], lb / (gr.nb[0] * gr.nb[1])};
@@ -835,14 +835,14 @@ __global__ void __launch_bounds__(256) k_ghost_cells(km_pack_t u, UniformGrid gr
 }
 
 __global__ void __launch_bounds__(256) k_ghost_faces(km_pack_t u, UniformGrid gr0, int face,
-                                                     Instances in) {
+                                                     Instances in, int depth) {
   const Geo g(u);
-  const long n = ghost_count(g, face);
+  const long n = ghost_count(g, face, depth);
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (t >= n * u.nblocks) return;
   const int b = (int)(t / n);
   int idx[3];
-  ghost_cell(g, t % n, idx, face);
+  ghost_cell(g, t % n, idx, face, depth);
   const UniformGrid gr = in.inst ? in.grids[in.inst[b]] : gr0;
   const int lb = b - gr.first;
   const int bl[3] = {lb % gr.nb[0], (lb / gr.nb[0]) % gr.nb[1], lb / (gr.nb[0] * gr.nb[1])};
@@ -1105,35 +1105,84 @@ void launch_update(const km_cfg_t &cfg, const km_pack_t &base, const km_pack_t &
   hipLaunchKernelGGL(k_prepare_primitive<1>, dim3(grid), dim3(256), 0, st, u1, eos, cyl);
 }
 
-// FluxStokes of every face family from u0's edge EMFs into dudt, then the face part of
-// ApplyDuDt into u1 (Cartesian CT, two or more dimensions): the face kernels of
-// launch_update, for the fused CT stage, whose zone pass follows
-void launch_ct_face_update(const km_pack_t &base, const km_pack_t &u0, const km_pack_t &u1,
-                           const km_pack_t &dudt, double beta, hipStream_t st,
-                           const double *dt_dev, const int *inst) {
+// EMF (k_emf), FluxStokes (k_stokes) and the face part of ApplyDuDt (k_apply_face) of one
+// face in one lane, Cartesian 3-D CT. The face's four edge EMFs are averaged from u0's
+// stored transverse B fluxes with k_emf's expression, Stokes and the stage combine follow
+// with k_stokes' and k_apply_face's, so the updated field is bitwise the three kernels'.
+// u0's face field is not overwritten by the combine (u0 is not read again in the cycle).
+template <int FACE>
+__device__ __forceinline__ void ct_face(const km_pack_t &base, const km_pack_t &u0,
+                                        const km_pack_t &u1, const Geo &g, int b, int k, int j,
+                                        int i, double beta, double dt) {
+  const km_block_t &blk = u0.blocks[b];
+  const long s = g.s(k, j, i), c = g.c(k, j, i);
+  const long sj = g.sc[1], sk = g.sc[2];
+  constexpr double esign = 1.0;
+  // edge EMF of family e at the edge with the lower corner of cell offset cc
+  auto emf = [&](int e, long cc) -> double {
+    if (e == 2) {
+      const double *fyBx = flux_ptr(u0, 1, b, C_MAGC + 0), *fxBy = flux_ptr(u0, 0, b, C_MAGC + 1);
+      return esign * 0.25 * (fyBx[cc - 1] + fyBx[cc] - fxBy[cc - sj] - fxBy[cc]);
+    }
+    if (e == 0) {
+      const double *fzBy = flux_ptr(u0, 2, b, C_MAGC + 1), *fyBz = flux_ptr(u0, 1, b, C_MAGC + 2);
+      return esign * 0.25 * (fzBy[cc - sj] + fzBy[cc] - fyBz[cc - sk] - fyBz[cc]);
+    }
+    const double *fxBz = flux_ptr(u0, 0, b, C_MAGC + 2), *fzBx = flux_ptr(u0, 2, b, C_MAGC + 0);
+    return esign * 0.25 * (fxBz[cc - sk] + fxBz[cc] - fzBx[cc - 1] - fzBx[cc]);
+  };
+  constexpr int e0 = FACE == 0 ? 2 : FACE == 1 ? 2 : 0;
+  constexpr int a0 = FACE == 0 ? 1 : FACE == 1 ? 0 : 1;
+  constexpr double s0 = FACE == 0 ? 1.0 : -1.0;
+  constexpr int e1 = FACE == 0 ? 1 : FACE == 1 ? 0 : 1;
+  constexpr int a1 = FACE == 0 ? 2 : FACE == 1 ? 2 : 0;
+  constexpr double s1 = FACE == 0 ? -1.0 : 1.0;
+  constexpr double gs = 1.0;
+  double acc = 0.;
+  acc += (gs * s0) * (blk.dx[e0] * emf(e0, c) - blk.dx[e0] * emf(e0, c + g.sc[a0]));
+  acc += (gs * s1) * (blk.dx[e1] * emf(e1, c) - blk.dx[e1] * emf(e1, c + g.sc[a1]));
+  acc *= 1. / (blk.area[FACE] + 1.e-36);
+  const double v0 = beta * base.face[FACE][b][s] + (1.0 - beta) * u0.face[FACE][b][s];
+  u1.face[FACE][b][s] = v0 + beta * dt * acc;
+}
+
+__global__ void __launch_bounds__(256) k_ct_faces(km_pack_t base, km_pack_t u0, km_pack_t u1,
+                                                  double beta, const double *dtp,
+                                                  const int *inst) {
   const Geo g(u0);
-  if (g.ndim < 2) return;
-  for (int f = 0; f < g.ndim; ++f) {
-    const unsigned grid = nblk(face_count(g, f, u0.nblocks));
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  for (int f = 0; f < 3; ++f) {
+    const int ni = g.nx[0] + (f == 0), nj = g.nx[1] + (f == 1), nk = g.nx[2] + (f == 2);
+    const long n = (long)ni * nj * nk * u0.nblocks;
+    if (t >= n) {
+      t -= n;
+      continue;
+    }
+    const long per = (long)ni * nj * nk;
+    const int b = (int)(t / per);
+    long r = t % per;
+    const int i = g.lo[0] + (int)(r % ni);
+    r /= ni;
+    const int j = g.lo[1] + (int)(r % nj);
+    const int k = g.lo[2] + (int)(r / nj);
+    const double dt = dtp[inst ? inst[b] : 0];
     if (f == 0)
-      hipLaunchKernelGGL(k_stokes<0>, dim3(grid), dim3(256), 0, st, u0, dudt, 0);
+      ct_face<0>(base, u0, u1, g, b, k, j, i, beta, dt);
     else if (f == 1)
-      hipLaunchKernelGGL(k_stokes<1>, dim3(grid), dim3(256), 0, st, u0, dudt, 0);
+      ct_face<1>(base, u0, u1, g, b, k, j, i, beta, dt);
     else
-      hipLaunchKernelGGL(k_stokes<2>, dim3(grid), dim3(256), 0, st, u0, dudt, 0);
+      ct_face<2>(base, u0, u1, g, b, k, j, i, beta, dt);
+    return;
   }
-  for (int f = 0; f < g.ndim; ++f) {
-    const unsigned grid = nblk(face_count(g, f, u0.nblocks));
-    if (f == 0)
-      hipLaunchKernelGGL(k_apply_face<0>, dim3(grid), dim3(256), 0, st, base, u0, u1, dudt,
-                         beta, 0.0, dt_dev, inst);
-    else if (f == 1)
-      hipLaunchKernelGGL(k_apply_face<1>, dim3(grid), dim3(256), 0, st, base, u0, u1, dudt,
-                         beta, 0.0, dt_dev, inst);
-    else
-      hipLaunchKernelGGL(k_apply_face<2>, dim3(grid), dim3(256), 0, st, base, u0, u1, dudt,
-                         beta, 0.0, dt_dev, inst);
-  }
+}
+
+void launch_ct_faces(const km_pack_t &base, const km_pack_t &u0, const km_pack_t &u1,
+                     double beta, hipStream_t st, const double *dt_dev, const int *inst) {
+  const Geo g(u0);
+  long n = 0;
+  for (int f = 0; f < 3; ++f) n += face_count(g, f, u0.nblocks);
+  hipLaunchKernelGGL(k_ct_faces, dim3(nblk(n)), dim3(256), 0, st, base, u0, u1, beta, dt_dev,
+                     inst);
 }
 
 void launch_prepare_primitive(const km_cfg_t &cfg, const km_pack_t &u, hipStream_t st) {
@@ -1201,15 +1250,15 @@ void launch_prepare_conserved_stage(const km_cfg_t &cfg, const km_pack_t &u, hip
 }
 
 void launch_ghost_fill(const UniformGrid &gr, const km_pack_t &u, int v0, int v1, int nc,
-                       bool faces, hipStream_t st, Instances in) {
+                       bool faces, hipStream_t st, Instances in, int depth, int face_depth) {
   const Geo g(u);
   if (v1 > v0)
-    hipLaunchKernelGGL(k_ghost_cells, dim3(nblk(ghost_count(g) * u.nblocks)), dim3(256), 0, st,
-                       u, gr, v0, v1, nc, in);
+    hipLaunchKernelGGL(k_ghost_cells, dim3(nblk(ghost_count(g, -1, depth) * u.nblocks)),
+                       dim3(256), 0, st, u, gr, v0, v1, nc, in, depth);
   if (faces)
     for (int f = 0; f < (g.ndim > 2 ? 3 : g.ndim); ++f)
-      hipLaunchKernelGGL(k_ghost_faces, dim3(nblk(ghost_count(g, f) * u.nblocks)), dim3(256), 0,
-                         st, u, gr, f, in);
+      hipLaunchKernelGGL(k_ghost_faces, dim3(nblk(ghost_count(g, f, face_depth) * u.nblocks)),
+                         dim3(256), 0, st, u, gr, f, in, face_depth);
 }
 
 }  // namespace km
