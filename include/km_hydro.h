@@ -324,7 +324,7 @@ int km_stage_fused(const km_cfg_t *cfg, km_stage_plan_t *plan, const km_pack_t *
 /* ---- whole-mesh driver (KamayanDriver RK stages, kamayan_driver.cpp:121-248) ----- */
 enum {
   KM_PATH_TASKS = 0, /* one kernel per reference task, same sequence as BuildTaskListRKStage */
-  KM_PATH_FUSED = 1  /* fused directional sweeps per RK stage (hydro) */
+  KM_PATH_FUSED = 1  /* fused sweeps per RK stage (hydro; CT MHD on 3-D single-level meshes) */
 };
 
 typedef struct km_sim_t km_sim_t; /* opaque: device mesh, stage containers, dt state */
