@@ -253,6 +253,17 @@ KM_HD int recon_slot(int r) {
   return 5 + (r - R_MAGC);
 }
 
+// Ideal-gas closure: BMOD = (gm1 + 1) gm1 rho e = gamma PRES in every cell, and every
+// reconstruction here is positively homogeneous of degree one (the PLM limiters and the PPM
+// conditions scale with the data; WENO-Z up to its 1e-36 epsilon), so the reconstructed
+// BMOD equals gamma times the reconstructed PRES up to rounding. The forms held to a
+// tolerance (FP64-FMA, FP32-mixed) take it from PRES instead of reconstructing it; the
+// FP64 form reconstructs it as the reference does.
+template <class T, bool FAST>
+constexpr bool bmod_from_pres() {
+  return FAST || sizeof(T) == 4;
+}
+
 // runtime limiter choice (uniform across a wavefront) over a compile-time method
 template <int RECON, bool FAST = false, class T>
 KM_HD void reconstruct_rt(int lim, const T *w, T &vM, T &vP) {
@@ -307,10 +318,11 @@ __device__ __forceinline__ void march(int lim, int c0, int c1, Load &&load, Face
 // overlaps the current step's reconstruction and Riemann solve.
 // fetch(c) -> R (raw loads, no arithmetic); unpack(const R&, T p[NR]).
 // fetch0 serves the window prefill (cells c0-W .. c0+W), fetch the steps after it.
-template <class T, int RECON, int NR, class R, bool FAST = false, class Fetch0, class Fetch,
-          class Unpack, class Face>
+// BP: the reconstructed BMOD is gam times the reconstructed PRES (bmod_from_pres).
+template <class T, int RECON, int NR, class R, bool FAST = false, bool BP = false,
+          class Fetch0, class Fetch, class Unpack, class Face>
 __device__ __forceinline__ void march_pf(int lim, int c0, int c1, Fetch0 &&fetch0, Fetch &&fetch,
-                                         Unpack &&unpack, Face &&face) {
+                                         Unpack &&unpack, Face &&face, T gam = T(0)) {
   constexpr int W = ReconWidth<RECON>::value;
   T win[NR][5];
 #pragma unroll
@@ -333,7 +345,12 @@ __device__ __forceinline__ void march_pf(int lim, int c0, int c1, Fetch0 &&fetch
     }
     T vM[NR], vP[NR];
 #pragma unroll
-    for (int r = 0; r < NR; ++r) reconstruct_rt<RECON, FAST>(lim, win[r], vM[r], vP[r]);
+    for (int r = 0; r < NR; ++r)
+      if (!BP || r != R_BMOD) reconstruct_rt<RECON, FAST>(lim, win[r], vM[r], vP[r]);
+    if constexpr (BP) {
+      vM[R_BMOD] = gam * vM[R_PRES];
+      vP[R_BMOD] = gam * vP[R_PRES];
+    }
     if (c > c0) face(c, vPprev, vM);
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
