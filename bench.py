@@ -40,34 +40,29 @@ MEASURED = {
         "x1": {"march": (2.3494e9 + 0.8032e9, 1535.8),
                "tile 4 pencils x 256 lanes": (1.0908e9 + 0.6711e9, 1972.8),
                "tile 8 pencils x 256 lanes": (1.0908e9 + 0.6711e9, 1734.4),
-               "tile 8 pencils x 512 lanes": (1.0908e9 + 0.6711e9, 1773.4),
-               "tile 16 pencils x 1024 lanes": (1.0908e9 + 0.6711e9, 1683.1)},
+               "tile 8 pencils x 512 lanes": (1.0908e9 + 0.6711e9, 1773.4)},
         "x2": {"march": (1.6589e9 + 0.9476e9, 1433.3),
                "tile 4 pencils x 256 lanes": (1.4997e9 + 0.6711e9, 1921.2),
                "tile 8 pencils x 256 lanes": (1.4997e9 + 0.6711e9, 1672.1),
-               "tile 8 pencils x 512 lanes": (1.4997e9 + 0.6711e9, 1724.1),
-               "tile 16 pencils x 1024 lanes": (1.4997e9 + 0.6711e9, 1633.8)},
+               "tile 8 pencils x 512 lanes": (1.4997e9 + 0.6711e9, 1724.1)},
         "x3": {"march": (2.4187e9 + 0.6711e9, 1745.3)},
         "x3_dt": {"march": (2.8539e9 + 0.6712e9, 1865.8)},
     },
     "fp64_fma": {
         "x1": {
             "march": (2.6489e9 + 0.9363e9, 975.2),
-            "tile 16 pencils x 1024 lanes": (1.0907e9 + 0.6711e9, 1093.1),
             "tile 4 pencils x 256 lanes": (1.0907e9 + 0.6711e9, 1273.5),
             "tile 8 pencils x 256 lanes": (1.0907e9 + 0.6711e9, 1107.2),
             "tile 8 pencils x 512 lanes": (1.0907e9 + 0.6711e9, 1149.8),
         },
         "x2": {
             "march": (1.5287e9 + 0.6816e9, 893.5),
-            "tile 16 pencils x 1024 lanes": (1.4997e9 + 0.6711e9, 1043.5),
             "tile 4 pencils x 256 lanes": (1.4997e9 + 0.6711e9, 1220.8),
             "tile 8 pencils x 256 lanes": (1.4997e9 + 0.6711e9, 1043.4),
             "tile 8 pencils x 512 lanes": (1.4997e9 + 0.6711e9, 1099.9),
         },
         "x3": {
             "march": (2.3270e9 + 0.6718e9, 1115.2),
-            "tile 16 pencils x 1024 lanes": (1.7140e9 + 0.6745e9, 1118.4),
             "tile 4 pencils x 256 lanes": (1.9739e9 + 0.6742e9, 1300.2),
             "tile 8 pencils x 256 lanes": (1.8256e9 + 0.6748e9, 1109.6),
             "tile 8 pencils x 512 lanes": (1.7810e9 + 0.6743e9, 1176.1),
@@ -86,7 +81,6 @@ MEASURED = {
     "fp32_mixed": {
         "x1": {
             "march": (2.6487e9 + 0.9329e9, 887.7),
-            "tile 16 pencils x 1024 lanes": (1.0907e9 + 0.6711e9, 1024.6),
             "tile 32 pencils x 1024 lanes": (1.0907e9 + 0.6711e9, 955.4),
             "tile 4 pencils x 256 lanes": (1.0907e9 + 0.6711e9, 1190.5),
             "tile 8 pencils x 256 lanes": (1.0906e9 + 0.6711e9, 1035.2),
@@ -94,7 +88,6 @@ MEASURED = {
         },
         "x2": {
             "march": (1.5031e9 + 0.6711e9, 842.6),
-            "tile 16 pencils x 1024 lanes": (1.4996e9 + 0.6711e9, 975.0),
             "tile 32 pencils x 1024 lanes": (1.4996e9 + 0.6711e9, 892.5),
             "tile 4 pencils x 256 lanes": (1.4996e9 + 0.6719e9, 1138.0),
             "tile 8 pencils x 256 lanes": (1.4996e9 + 0.6711e9, 971.5),
@@ -102,7 +95,6 @@ MEASURED = {
         },
         "x3": {
             "march": (2.3076e9 + 0.6764e9, 955.3),
-            "tile 16 pencils x 1024 lanes": (1.7138e9 + 0.6740e9, 1096.9),
             "tile 32 pencils x 1024 lanes": (2.1915e9 + 0.6739e9, 1003.6),
             "tile 4 pencils x 256 lanes": (2.1607e9 + 0.7168e9, 1264.8),
             "tile 8 pencils x 256 lanes": (2.2328e9 + 0.6815e9, 1084.9),
@@ -170,7 +162,7 @@ def _setup(km, precision="fp64", n=N, nb=NB):
 
 # pencil-tile shapes of the fused sweeps (pencils, lanes per workgroup), numbered from 1;
 # 0 is the pencil march
-TILE_SHAPES = [(4, 256), (8, 256), (8, 512), (16, 1024), (32, 1024)]
+TILE_SHAPES = [(4, 256), (8, 256), (8, 512), (16, 256), (32, 1024), (64, 1024)]
 
 
 # one-launch stage kernel variants (3-D): tile of x1 columns x x2 rows, x3 segment

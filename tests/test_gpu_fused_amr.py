@@ -73,7 +73,7 @@ def test_fused_amr_run_bitwise(gpu, oracle, name, make, ncyc):
     ref.advance(ncyc)
     r, _ = ref.state()
     g, info, used = _run(gpu, cfg, desc, lay, leaves, cell, ncyc)
-    assert all(1 <= u <= 5 for u in used), used
+    assert all(1 <= u <= 6 for u in used), used
     _same_state(g, r, lay, name)
     assert info == ref.info()
 
@@ -84,14 +84,24 @@ def test_fused_amr_every_tile_shape_bitwise(gpu, oracle):
     ref.set_primitive(cell)
     ref.advance(3)
     r, _ = ref.state()
-    for shape in range(1, 6):
+    import kamayan_amd as km
+    shapes = [(4, 256), (8, 256), (8, 512), (16, 256), (32, 1024), (64, 1024)]
+    w = {0: 0, 1: 1, 2: 2, 3: 2}[cfg.c.recon]
+    for shape in range(1, 7):
         print(f"[fused amr] tile shape {shape}", flush=True)
+        rows = shapes[shape - 1][0]
+        n = lay.nx[0]
+        # LDS of a tile (FP64): primitives and face values of its pencils, plus the
+        # 328-byte per-pencil table; a shape past 160 KiB is refused
+        if rows * (7 * (n + 2 + 2 * w) + 7 * (n + 2)) * 8 + 328 * rows > 160 * 1024:
+            with pytest.raises(km.KmError):
+                _run(gpu, cfg, desc, lay, leaves, cell, 3, kern=shape)
+            continue
         g, info, used = _run(gpu, cfg, desc, lay, leaves, cell, 3, kern=shape)
         assert used == (shape,) * 2
         _same_state(g, r, lay, f"shape {shape}")
         assert info == ref.info()
     # the march reads face neighbours of a single-level mesh: refused on a block tree
-    import kamayan_amd as km
     with pytest.raises(km.KmError):
         _run(gpu, cfg, desc, lay, leaves, cell, 3, kern=0)
 

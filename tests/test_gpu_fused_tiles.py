@@ -12,9 +12,9 @@ from tests import _cases
 
 pytestmark = pytest.mark.gpu
 
-NSHAPES = 5  # pencil-tile shapes
-SHAPES = [(4, 256), (8, 256), (8, 512), (16, 1024), (32, 1024)]  # (pencils, lanes)
-INFO_BYTES = 176  # per-pencil table of a tile workgroup
+NSHAPES = 6  # pencil-tile shapes
+SHAPES = [(4, 256), (8, 256), (8, 512), (16, 256), (32, 1024), (64, 1024)]  # (pencils, lanes)
+INFO_BYTES = 328  # per-pencil table of a tile workgroup (sizeof(PencilInfo))
 WIDTH = {"wenoz": 2, "ppm": 2, "plm": 1, "fog": 0}
 
 
@@ -126,6 +126,10 @@ def test_tile_shapes_fp32_mixed(gpu, oracle, kind):
     assert np.all(l1 < 1e-5), ("march vs oracle", l1)
     for shape in list(range(1, NSHAPES + 1)) + [(3, 1, 4)]:
         print(f"[tiles fp32] {kind} kernels {shape}", flush=True)
+        if isinstance(shape, int) and not _fits(shape, lay, _recon(cfg), 4):
+            with pytest.raises(gpu.KmError):
+                _run(gpu, case, shape, ncyc)
+            continue
         g, _, used = _run(gpu, case, shape, ncyc)
         want = shape if isinstance(shape, tuple) else (shape,) * 3
         assert used[:lay.ndim] == want[:lay.ndim]
@@ -168,6 +172,10 @@ def test_kernel_choice_refusals_and_one_dimension(gpu, oracle):
     r, _ = ref.state()
     sl = (slice(None), slice(0, 5)) + lay.interior()
     for shape in range(0, NSHAPES + 1):
+        if shape > 0 and not _fits(shape, lay, "wenoz", 8):
+            with pytest.raises(km.KmError):
+                _run(gpu, (cfg, desc, lay, cell, faces), shape, 3)
+            continue
         g, info, used = _run(gpu, (cfg, desc, lay, cell, faces), shape, 3)
         assert used[0] == shape
         assert info == ref.info()
