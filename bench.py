@@ -444,10 +444,10 @@ def _secondary(km, args):
         s32, _ = sim.state()
         sim.close()
         var = {}
-        for name, path, prec, ps, steps in (
+        for name, path, prec, ps, steps in (() if args.no_variants else (
                 ("fused_fp64", km.PATH_FUSED, "fp64", None, args.steps),
                 ("tasks_fp64", km.PATH_TASKS, "fp64", None, args.steps),
-                ("tasks_fp64_per_block", km.PATH_TASKS, "fp64", 1, min(args.steps, 3))):
+                ("tasks_fp64_per_block", km.PATH_TASKS, "fp64", 1, min(args.steps, 3)))):
             cfgv, _, _, _, _ = _sedov_setup(km, prec, ps)
             sim = km.Simulation(cfgv, desc, lay, path=path, leaves=leaves)
             sim.set_primitive(cell)

@@ -142,10 +142,17 @@ __device__ __forceinline__ double child_value(double fc, double a, double bb, do
 
 // Zones a level-L or a level-L+1 leaf covers: copy or RestrictAverage. Zones a coarser
 // leaf covers are left to k_amr_prolong.
-// variable k of the range [v0, v1) with the slots [s0, s1) left out (v0 <= s0 <= s1 <= v1)
-__device__ __forceinline__ int fill_var(int k, int v0, int s0, int s1) {
-  const int v = v0 + k;
-  return v >= s0 ? v + (s1 - s0) : v;
+// variable k of the range [v0, v1) with the slots [s0, s1) (v0 <= s0 <= s1 <= v1) and the
+// slot s2 (s1 <= s2 < v1; -1: none) left out
+__device__ __forceinline__ int fill_var(int k, int v0, int s0, int s1, int s2 = -1) {
+  int v = v0 + k;
+  if (v >= s0) v += s1 - s0;
+  if (s2 >= 0 && v >= s2) ++v;
+  return v;
+}
+// how many variables that leaves
+__host__ __device__ __forceinline__ int fill_count(int v0, int v1, int s0, int s1, int s2) {
+  return (v1 - v0) - (s1 - s0) - (s2 >= 0 ? 1 : 0);
 }
 
 // Ghost zone r of block b (the face slabs of depth > 0, else the whole layer): its cell,
@@ -180,24 +187,25 @@ __device__ __forceinline__ GhostSrc ghost_source(const km_pack_t &u, const AmrGr
 // of one chunk overlap instead of waiting on the previous variable's store.
 __device__ __forceinline__ void ghost_store(const km_pack_t &u, const Geo &g, int b, long dc,
                                             int sl, long sidx, bool restr, const bool fl[3],
-                                            int v0, int v1, int nc, int s0, int s1, bool cyl) {
+                                            int v0, int v1, int nc, int s0, int s1, bool cyl,
+                                            int s2 = -1) {
   constexpr int kChunk = 4;
   const km_block_t &sb = u.blocks[sl];
-  const int nv = (v1 - v0) - (s1 - s0);
+  const int nv = fill_count(v0, v1, s0, s1, s2);
   for (int k = 0; k < nv; k += kChunk) {
     double x[kChunk];
 #pragma unroll
     for (int q = 0; q < kChunk; ++q) {
       x[q] = 0.0;
       if (k + q < nv) {
-        const double *src = cell_ptr(u, sl, fill_var(k + q, v0, s0, s1));
+        const double *src = cell_ptr(u, sl, fill_var(k + q, v0, s0, s1, s2));
         x[q] = restr ? restrict_cell(src, sb, g, sidx, cyl) : src[sidx];
       }
     }
 #pragma unroll
     for (int q = 0; q < kChunk; ++q) {
       if (k + q < nv) {
-        const int v = fill_var(k + q, v0, s0, s1);
+        const int v = fill_var(k + q, v0, s0, s1, s2);
         const int ax = vector_axis(v, nc);
         cell_ptr(u, b, v)[dc] = (ax >= 0 && fl[ax]) ? -x[q] : x[q];
       }
@@ -241,7 +249,7 @@ __global__ void __launch_bounds__(256) k_amr_ghost_list(km_pack_t u, AmrGrid t, 
 // k_amr_ghost from the gather list: one entry per lane, no tree walk
 __global__ void __launch_bounds__(256) k_amr_ghost_gather(km_pack_t u, const AmrGhostEntry *list,
                                                           long n, int v0, int v1, int nc, int s0,
-                                                          int s1, int cyl) {
+                                                          int s1, int cyl, int s2) {
   const long tt = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (tt >= n) return;
   const AmrGhostEntry e = list[tt];
@@ -250,7 +258,7 @@ __global__ void __launch_bounds__(256) k_amr_ghost_gather(km_pack_t u, const Amr
   const bool fl[3] = {((e.src >> 29) & 1u) != 0, ((e.src >> 30) & 1u) != 0,
                       ((e.src >> 31) & 1u) != 0};
   ghost_store(u, g, e.db, e.dc, e.sb, (long)(e.src & 0x0fffffffu), ((e.src >> 28) & 1u) != 0,
-              fl, v0, v1, nc, s0, s1, cyl != 0);
+              fl, v0, v1, nc, s0, s1, cyl != 0, s2);
 }
 
 // Zones a level-(L-1) leaf covers: ProlongateSharedMinMod. One thread takes one coarse
@@ -267,7 +275,7 @@ __global__ void __launch_bounds__(256) k_amr_ghost_gather(km_pack_t u, const Amr
 __global__ void __launch_bounds__(256) k_amr_prolong(km_pack_t u, AmrGrid t, int v0, int v1,
                                                      int nc, int s0, int s1, int depth,
                                                      const long *plist, long np,
-                                                     unsigned char *flag) {
+                                                     unsigned char *flag, int s2) {
   const Geo g(u);
   km_pack_t half = u;
   half.ng = (u.ng + 1) / 2;
@@ -352,9 +360,9 @@ __global__ void __launch_bounds__(256) k_amr_prolong(km_pack_t u, AmrGrid t, int
 #pragma unroll
   for (int d = 0; d < 3; ++d)
     if (d < g.ndim) df[d] = 0.5 * blk.dx[d];
-  const int nv = (v1 - v0) - (s1 - s0);
+  const int nv = fill_count(v0, v1, s0, s1, s2);
   for (int k = 0; k < nv; ++k) {
-    const int v = fill_var(k, v0, s0, s1);
+    const int v = fill_var(k, v0, s0, s1, s2);
     const int ax = vector_axis(v, nc);
     const double fc = cell_ptr(u, s.leaf, v)[s.idx];
     double gr[3] = {0., 0., 0.};
@@ -815,7 +823,7 @@ void launch_amr_ghost_fill(const AmrGrid &t, const km_pack_t &u, int v0, int v1,
     const long nc2 =
         (depth > 0 ? face_ghost_count(gh, (depth + 1) / 2) : ghost_count(gh)) * u.nblocks;
     hipLaunchKernelGGL(k_amr_prolong, dim3((unsigned)((nc2 + 255) / 256)), dim3(256), 0, st, u, t,
-                       v0, v1, nc, s0, s1, depth, nullptr, 0L, nullptr);
+                       v0, v1, nc, s0, s1, depth, nullptr, 0L, nullptr, -1);
   }
 }
 
@@ -852,19 +860,20 @@ void launch_amr_prolong_flags(const AmrGrid &t, const km_pack_t &u, int depth,
   const long n = amr_prolong_parents(u, depth);
   if (n > 0 && t.lmax > 0)
     hipLaunchKernelGGL(k_amr_prolong, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, u, t, 0,
-                       0, 0, 0, 0, depth, nullptr, 0L, flag);
+                       0, 0, 0, 0, depth, nullptr, 0L, flag, -1);
 }
 
 void launch_amr_ghost_fill_list(const AmrFillList &f, const AmrGrid &t, const km_pack_t &u,
-                                int v0, int v1, int nc, hipStream_t st, int s0, int s1) {
+                                int v0, int v1, int nc, hipStream_t st, int s0, int s1, int s2) {
   if (s0 < v0 || s1 > v1 || s1 <= s0) s0 = s1 = v1;  // nothing left out
-  if (v0 >= v1 || (v1 - v0) == (s1 - s0)) return;
+  if (s2 < s1 || s2 >= v1) s2 = -1;
+  if (v0 >= v1 || fill_count(v0, v1, s0, s1, s2) <= 0) return;
   if (f.n > 0)
     hipLaunchKernelGGL(k_amr_ghost_gather, dim3((unsigned)((f.n + 255) / 256)), dim3(256), 0, st,
-                       u, f.list, f.n, v0, v1, nc, s0, s1, t.cyl);
+                       u, f.list, f.n, v0, v1, nc, s0, s1, t.cyl, s2);
   if (f.np > 0)
     hipLaunchKernelGGL(k_amr_prolong, dim3((unsigned)((f.np + 255) / 256)), dim3(256), 0, st, u,
-                       t, v0, v1, nc, s0, s1, f.depth, f.plist, f.np, nullptr);
+                       t, v0, v1, nc, s0, s1, f.depth, f.plist, f.np, nullptr, s2);
 }
 
 void launch_amr_flux_correct(const AmrGrid &t, const km_pack_t &u, int nc, hipStream_t st) {

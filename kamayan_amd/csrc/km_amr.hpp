@@ -270,9 +270,11 @@ void launch_amr_ghost_list(const AmrGrid &t, const km_pack_t &u, int depth, AmrG
 // flag[p] = 1 for each parent p of the prolongation enumeration that a coarser leaf covers
 void launch_amr_prolong_flags(const AmrGrid &t, const km_pack_t &u, int depth,
                               unsigned char *flag, hipStream_t st);
-// launch_amr_ghost_fill over the enumeration the lists were built for
+// launch_amr_ghost_fill over the enumeration the lists were built for; s2 (>= s1): one more
+// slot to leave out (-1: none)
 void launch_amr_ghost_fill_list(const AmrFillList &f, const AmrGrid &t, const km_pack_t &u,
-                                int v0, int v1, int nc, hipStream_t st, int s0 = 0, int s1 = 0);
+                                int v0, int v1, int nc, hipStream_t st, int s0 = 0, int s1 = 0,
+                                int s2 = -1);
 void launch_amr_flux_correct(const AmrGrid &t, const km_pack_t &u, int nc, hipStream_t st);
 // CT: face B ghost fill across levels and EMF correction at level jumps
 void launch_amr_ghost_faces(const AmrGrid &t, const km_pack_t &u, hipStream_t st);

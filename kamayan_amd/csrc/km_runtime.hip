@@ -292,6 +292,10 @@ struct km_sim_t {
   // fused path on a block tree: the gather lists of the stages' ghost fill (rebuilt with
   // the tree)
   AmrFillList fill;
+  // fused path on a block tree with own_prims: the final sweeps leave TEMP (and, in the
+  // forms that reconstruct BMOD from PRES, BMOD) of the base state to be derived when
+  // something reads the state
+  bool lazy_prims = false;
   void drop_graph() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -869,10 +873,13 @@ int amr_fused_stage(km_sim_t &s, int stage, double beta) {
   }
   // density and the primitives in the face slabs the next sweeps read (the
   // reconstruction's reach plus one: the cells of the end faces' stencils); momentum,
-  // energy, edges and corners are left out
+  // energy, edges and corners are left out, and BMOD too in the forms that take the
+  // reconstructed BMOD from PRES (their sweeps never read it)
+  const int bmod = cfg.precision == KM_PREC_FP64 ? -1 : KM_VEL0(cfg.mhd) + 4;
   launch_amr_ghost_fill_list(s.fill, s.agrid, md1.pack, 0, KM_TEMP(cfg.mhd), s.nc, s.stream, 1,
-                             KM_NCONS(cfg.mhd));
+                             KM_NCONS(cfg.mhd), bmod);
   s.lazy_ghosts = true;
+  if (s.own_prims) s.lazy_prims = true;
   if (s.profile) {
     (void)hipEventRecord(g1, s.stream);
     s.recs.push_back({3, g0, g1});
@@ -931,6 +938,10 @@ int build_fill_list(km_sim_t &s) {
 // the ghost zones the fused block-tree stages leave to be filled: momentum and energy, and
 // density and the primitives outside the face slabs
 void fill_lazy_ghosts(km_sim_t &s) {
+  if (s.lazy_prims) {  // PreparePrimitive of the base interior (the same values)
+    launch_prepare_primitive(s.cfg, s.base.pack, s.stream);
+    s.lazy_prims = false;
+  }
   if (!s.lazy_ghosts) return;
   ghost_fill(s, s.base.pack, 0, KM_TEMP(s.cfg.mhd), s.cfg.mhd == KM_MHD_CT);
   s.lazy_ghosts = false;
@@ -1864,6 +1875,7 @@ int km_sim_advance(km_sim_t *s, int32_t ncycles, double tlim) {
   // replayed cycles skip the host code that marks the momentum and energy ghosts stale
   if ((s->amr || s->cfg.mhd == KM_MHD_CT) && s->path == KM_PATH_FUSED && ncycles > 0)
     s->lazy_ghosts = true;
+  if (s->amr && s->own_prims && s->path == KM_PATH_FUSED && ncycles > 0) s->lazy_prims = true;
   KM_HIP(hipStreamSynchronize(s->stream));
   return check_failure(*s);
 }
