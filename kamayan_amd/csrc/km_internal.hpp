@@ -110,10 +110,11 @@ struct HistVortex {
 void launch_history_vortex(const km_cfg_t &cfg, const km_pack_t &u, const HistVortex &p,
                            double *part, hipStream_t st);
 // depth / face_depth in [1, ng): only the ghost cells / faces within that many zones of
-// the interior (edges and corners included); <= 0: all of them
+// the interior (edges and corners included); <= 0: all of them. skip: a slot of [v0, v1)
+// to leave out (-1: none)
 void launch_ghost_fill(const UniformGrid &gr, const km_pack_t &u, int v0, int v1, int nc,
                        bool faces, hipStream_t st, Instances in = Instances(), int depth = 0,
-                       int face_depth = 0);
+                       int face_depth = 0, int skip = -1);
 
 // fused stage sweeps: one translation unit per arithmetic form
 //   exact : FP64, -ffp-contract=off (bitwise the reference's operation order)

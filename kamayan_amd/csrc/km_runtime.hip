@@ -292,9 +292,9 @@ struct km_sim_t {
   // fused path on a block tree: the gather lists of the stages' ghost fill (rebuilt with
   // the tree)
   AmrFillList fill;
-  // fused path on a block tree with own_prims: the final sweeps leave TEMP (and, in the
-  // forms that reconstruct BMOD from PRES, BMOD) of the base state to be derived when
-  // something reads the state
+  // fused path on a block tree with own_prims, or the fused CT stage: the kernels leave
+  // TEMP (and, on block trees in the forms that take the reconstructed BMOD from PRES,
+  // BMOD) of the base state to be derived when something reads the state
   bool lazy_prims = false;
   void drop_graph() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -1015,12 +1015,15 @@ int ct_fused_stage(km_sim_t &s, int stage, double beta) {
   // the ghost zones the next sweeps read: density, cell B and the primitives within the
   // reconstruction's reach plus one of the interior, and the face field one zone deep;
   // the rest (momentum and energy, deeper layers) are filled when something reads them
+  // (BMOD only in the FP64 form: the others take it from PRES after reconstruction)
   const Instances in{s.inst_d, s.grids_d};
   const int reach = fused_reach(cfg);
+  const int bmod = cfg.precision == KM_PREC_FP64 ? -1 : KM_VEL0(cfg.mhd) + 4;
   launch_ghost_fill(s.grid, md1.pack, KM_DENS, KM_DENS + 1, s.nc, false, s.stream, in, reach);
   launch_ghost_fill(s.grid, md1.pack, KM_MAGC0, KM_TEMP(cfg.mhd), s.nc, true, s.stream, in,
-                    reach, 1);
+                    reach, 1, bmod);
   s.lazy_ghosts = true;
+  s.lazy_prims = true;  // the zone pass leaves TEMP to PreparePrimitive
   if (s.profile) {
     (void)hipEventRecord(g1, s.stream);
     s.recs.push_back({3, g0, g1});
@@ -1784,7 +1787,7 @@ int km_sim_set_primitive(km_sim_t *s, const double *cell, const double *const fa
 int km_sim_get_state(km_sim_t *s, double *cell, double *const face[3]) {
   if (!s || !cell) return KM_EINVAL;
   KM_HIP(hipStreamSynchronize(s->stream));
-  if (s->lazy_ghosts) {
+  if (s->lazy_ghosts || s->lazy_prims) {
     fill_lazy_ghosts(*s);
     KM_HIP(hipStreamSynchronize(s->stream));  // the copies below run on another stream
   }
@@ -1875,7 +1878,9 @@ int km_sim_advance(km_sim_t *s, int32_t ncycles, double tlim) {
   // replayed cycles skip the host code that marks the momentum and energy ghosts stale
   if ((s->amr || s->cfg.mhd == KM_MHD_CT) && s->path == KM_PATH_FUSED && ncycles > 0)
     s->lazy_ghosts = true;
-  if (s->amr && s->own_prims && s->path == KM_PATH_FUSED && ncycles > 0) s->lazy_prims = true;
+  if (((s->amr && s->own_prims) || s->cfg.mhd == KM_MHD_CT) && s->path == KM_PATH_FUSED &&
+      ncycles > 0)
+    s->lazy_prims = true;
   KM_HIP(hipStreamSynchronize(s->stream));
   return check_failure(*s);
 }

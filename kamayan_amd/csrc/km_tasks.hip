@@ -781,7 +781,7 @@ __device__ __forceinline__ int map_face(int g, int N, int lo, int hi, bool &flip
 }
 
 __global__ void __launch_bounds__(256) k_ghost_cells(km_pack_t u, UniformGrid gr0, int v0, int v1,
-                                                     int nc, Instances in, int depth) {
+                                                     int nc, Instances in, int depth, int skip) {
   const Geo g(u);
   const long n = ghost_count(g, -1, depth);
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -814,16 +814,20 @@ __global__ void __launch_bounds__(256) k_ghost_cells(km_pack_t u, UniformGrid gr
   const int srcb = gr.first + (sb[2] * gr.nb[1] + sb[1]) * gr.nb[0] + sb[0];
   const long dc = g.c(idx[2], idx[1], idx[0]), scx = g.c(src[2], src[1], src[0]);
   const bool fl0 = flip[0], fl1 = flip[1], fl2 = flip[2] != axis;
-  // variables go in chunks whose loads are all issued before their stores
+  // variables [v0, v1) without the slot skip go in chunks whose loads are all issued
+  // before their stores
   constexpr int kChunk = 4;
-  for (int v = v0; v < v1; v += kChunk) {
+  const bool sk = skip >= v0 && skip < v1;
+  const int nv = (v1 - v0) - (sk ? 1 : 0);
+  auto var = [&](int k) { return v0 + k + (sk && v0 + k >= skip ? 1 : 0); };
+  for (int k = 0; k < nv; k += kChunk) {
     double x[kChunk];
 #pragma unroll
-    for (int q = 0; q < kChunk; ++q) x[q] = v + q < v1 ? cell_ptr(u, srcb, v + q)[scx] : 0.0;
+    for (int q = 0; q < kChunk; ++q) x[q] = k + q < nv ? cell_ptr(u, srcb, var(k + q))[scx] : 0.0;
 #pragma unroll
     for (int q = 0; q < kChunk; ++q) {
-      const int w = v + q;
-      if (w >= v1) continue;
+      if (k + q >= nv) continue;
+      const int w = var(k + q);
       int ax = -1;
       if (w >= 1 && w < 4) ax = w - 1;
       else if (w >= nc && w < nc + 3) ax = w - nc;
@@ -1250,11 +1254,12 @@ void launch_prepare_conserved_stage(const km_cfg_t &cfg, const km_pack_t &u, hip
 }
 
 void launch_ghost_fill(const UniformGrid &gr, const km_pack_t &u, int v0, int v1, int nc,
-                       bool faces, hipStream_t st, Instances in, int depth, int face_depth) {
+                       bool faces, hipStream_t st, Instances in, int depth, int face_depth,
+                       int skip) {
   const Geo g(u);
   if (v1 > v0)
     hipLaunchKernelGGL(k_ghost_cells, dim3(nblk(ghost_count(g, -1, depth) * u.nblocks)),
-                       dim3(256), 0, st, u, gr, v0, v1, nc, in, depth);
+                       dim3(256), 0, st, u, gr, v0, v1, nc, in, depth, skip);
   if (faces)
     for (int f = 0; f < (g.ndim > 2 ? 3 : g.ndim); ++f)
       hipLaunchKernelGGL(k_ghost_faces, dim3(nblk(ghost_count(g, f, face_depth) * u.nblocks)),
