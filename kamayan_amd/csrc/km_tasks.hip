@@ -1153,38 +1153,32 @@ __device__ __forceinline__ void ct_face(const km_pack_t &base, const km_pack_t &
 __global__ void __launch_bounds__(256) k_ct_faces(km_pack_t base, km_pack_t u0, km_pack_t u1,
                                                   double beta, const double *dtp,
                                                   const int *inst) {
+  // one lane per (block, k, j, i) of the block's (nx+1)^3 face box: the lane updates the
+  // x1, x2 and x3 faces at that index that exist, so the three families' EMFs read the
+  // same flux neighbourhood close together in time (L2 reuse across families)
   const Geo g(u0);
-  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  for (int f = 0; f < 3; ++f) {
-    const int ni = g.nx[0] + (f == 0), nj = g.nx[1] + (f == 1), nk = g.nx[2] + (f == 2);
-    const long n = (long)ni * nj * nk * u0.nblocks;
-    if (t >= n) {
-      t -= n;
-      continue;
-    }
-    const long per = (long)ni * nj * nk;
-    const int b = (int)(t / per);
-    long r = t % per;
-    const int i = g.lo[0] + (int)(r % ni);
-    r /= ni;
-    const int j = g.lo[1] + (int)(r % nj);
-    const int k = g.lo[2] + (int)(r / nj);
-    const double dt = dtp[inst ? inst[b] : 0];
-    if (f == 0)
-      ct_face<0>(base, u0, u1, g, b, k, j, i, beta, dt);
-    else if (f == 1)
-      ct_face<1>(base, u0, u1, g, b, k, j, i, beta, dt);
-    else
-      ct_face<2>(base, u0, u1, g, b, k, j, i, beta, dt);
-    return;
-  }
+  const int n0 = g.nx[0] + 1, n1 = g.nx[1] + 1, n2 = g.nx[2] + 1;
+  const long per = (long)n0 * n1 * n2;
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= per * u0.nblocks) return;
+  const int b = (int)(t / per);
+  long r = t % per;
+  const int ii = (int)(r % n0);
+  r /= n0;
+  const int jj = (int)(r % n1);
+  const int kk = (int)(r / n1);
+  const int i = g.lo[0] + ii, j = g.lo[1] + jj, k = g.lo[2] + kk;
+  const double dt = dtp[inst ? inst[b] : 0];
+  const bool in_i = ii < g.nx[0], in_j = jj < g.nx[1], in_k = kk < g.nx[2];
+  if (in_j && in_k) ct_face<0>(base, u0, u1, g, b, k, j, i, beta, dt);
+  if (in_i && in_k) ct_face<1>(base, u0, u1, g, b, k, j, i, beta, dt);
+  if (in_i && in_j) ct_face<2>(base, u0, u1, g, b, k, j, i, beta, dt);
 }
 
 void launch_ct_faces(const km_pack_t &base, const km_pack_t &u0, const km_pack_t &u1,
                      double beta, hipStream_t st, const double *dt_dev, const int *inst) {
   const Geo g(u0);
-  long n = 0;
-  for (int f = 0; f < 3; ++f) n += face_count(g, f, u0.nblocks);
+  const long n = (long)(g.nx[0] + 1) * (g.nx[1] + 1) * (g.nx[2] + 1) * u0.nblocks;
   hipLaunchKernelGGL(k_ct_faces, dim3(nblk(n)), dim3(256), 0, st, base, u0, u1, beta, dt_dev,
                      inst);
 }
