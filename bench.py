@@ -26,7 +26,7 @@ SEED = 20261204
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_ZONE_STAGE = 8 * 3 * 5  # stage input + RK base read, stage output written
 # Per-launch figures of the fused kernels on this workload from the rocprofv3 counter runs
-# summarised in profiles/fused_stage_traffic_hydro3d_256c_wenoz_hllc_rk3.md: HBM bytes read
+# summarised in the profiles that COUNTER_TABLE names per form: HBM bytes read
 # (FETCH_SIZE x 2, the factor a 1 GiB copy by the library's copy kernel calibrates) plus
 # written (WRITE_SIZE x 1), and VALU instructions per zone (SQ_INSTS_VALU x 64 / zones).
 # Keyed by form, then by direction and kernel name as _kernel_names() reports them; the
@@ -37,81 +37,114 @@ COUNTER_TABLE = {"fp64": "profiles/fused_stage_traffic_hydro3d_256c_wenoz_hllc_r
                  "fp32_mixed": "profiles/fused_stage_fp32_counters_hydro3d_256c_wenoz_hllc_rk3.md"}
 MEASURED = {
     "fp64": {
-        "x1": {"march": (2.3494e9 + 0.8032e9, 1535.8),
-               "tile 4 pencils x 256 lanes": (1.0908e9 + 0.6711e9, 1972.8),
-               "tile 8 pencils x 256 lanes": (1.0908e9 + 0.6711e9, 1734.4),
-               "tile 8 pencils x 512 lanes": (1.0908e9 + 0.6711e9, 1773.4)},
-        "x2": {"march": (1.6589e9 + 0.9476e9, 1433.3),
-               "tile 4 pencils x 256 lanes": (1.4997e9 + 0.6711e9, 1921.2),
-               "tile 8 pencils x 256 lanes": (1.4997e9 + 0.6711e9, 1672.1),
-               "tile 8 pencils x 512 lanes": (1.4997e9 + 0.6711e9, 1724.1)},
-        "x3": {"march": (2.4187e9 + 0.6711e9, 1745.3)},
-        "x3_dt": {"march": (2.8539e9 + 0.6712e9, 1865.8)},
+        "x1": {
+            "march": (2.3497e9 + 0.8030e9, 1535.8),
+            "tile 4 pencils x 256 lanes": (1.0908e9 + 0.6711e9, 1966.0),
+            "tile 8 pencils x 256 lanes": (1.0908e9 + 0.6711e9, 1731.0),
+            "tile 8 pencils x 512 lanes": (1.0908e9 + 0.6711e9, 1773.5),
+            "tile 16 pencils x 256 lanes": (1.0908e9 + 0.6711e9, 1619.4),
+        },
+        "x2": {
+            "march": (1.6571e9 + 0.9474e9, 1433.3),
+            "tile 4 pencils x 256 lanes": (1.4998e9 + 0.6711e9, 1914.2),
+            "tile 8 pencils x 256 lanes": (1.4998e9 + 0.6711e9, 1668.6),
+            "tile 8 pencils x 512 lanes": (1.4998e9 + 0.6711e9, 1724.1),
+            "tile 16 pencils x 256 lanes": (1.4998e9 + 0.6711e9, 1550.6),
+        },
+        "x3": {
+            "march": (2.1711e9 + 0.6712e9, 1686.3),
+            "tile 4 pencils x 256 lanes": (1.8247e9 + 0.6715e9, 2050.8),
+            "tile 8 pencils x 256 lanes": (1.8810e9 + 0.6714e9, 1791.9),
+            "tile 8 pencils x 512 lanes": (1.8307e9 + 0.6722e9, 1857.4),
+            "tile 16 pencils x 256 lanes": (1.7808e9 + 0.6713e9, 1667.4),
+        },
+        "x3_dt": {
+            "march": (2.8558e9 + 0.6711e9, 1823.5),
+        },
+        "stage": {
+            "one-launch stage 16x32 columns": (4.0077e9 + 1.4160e9, 4711.6),
+            "one-launch stage 32x8 columns": (1.5810e9 + 0.6826e9, 5157.4),
+            "one-launch stage 32x16 columns": (3.8573e9 + 1.3438e9, 4677.3),
+            "one-launch stage 64x4 columns": (1.6365e9 + 0.6711e9, 5509.8),
+            "one-launch stage 64x8 columns": (4.1386e9 + 1.3526e9, 4838.1),
+        },
+        "stage_dt": {
+            "one-launch stage 32x16 columns": (4.0053e9 + 1.5050e9, 4750.0),
+        },
     },
     "fp64_fma": {
         "x1": {
-            "march": (2.6489e9 + 0.9363e9, 975.2),
-            "tile 4 pencils x 256 lanes": (1.0907e9 + 0.6711e9, 1273.5),
-            "tile 8 pencils x 256 lanes": (1.0907e9 + 0.6711e9, 1107.2),
-            "tile 8 pencils x 512 lanes": (1.0907e9 + 0.6711e9, 1149.8),
+            "march": (2.6478e9 + 0.9360e9, 860.2),
+            "tile 4 pencils x 256 lanes": (1.0907e9 + 0.6711e9, 1169.5),
+            "tile 8 pencils x 256 lanes": (1.0907e9 + 0.6711e9, 1011.2),
+            "tile 8 pencils x 512 lanes": (1.0907e9 + 0.6711e9, 1056.8),
+            "tile 16 pencils x 256 lanes": (1.0907e9 + 0.6711e9, 937.1),
         },
         "x2": {
-            "march": (1.5287e9 + 0.6816e9, 893.5),
-            "tile 4 pencils x 256 lanes": (1.4997e9 + 0.6711e9, 1220.8),
-            "tile 8 pencils x 256 lanes": (1.4997e9 + 0.6711e9, 1043.4),
-            "tile 8 pencils x 512 lanes": (1.4997e9 + 0.6711e9, 1099.9),
+            "march": (1.5003e9 + 0.6711e9, 787.7),
+            "tile 4 pencils x 256 lanes": (1.4997e9 + 0.6711e9, 1105.2),
+            "tile 8 pencils x 256 lanes": (1.4997e9 + 0.6711e9, 941.1),
+            "tile 8 pencils x 512 lanes": (1.4997e9 + 0.6711e9, 995.6),
+            "tile 16 pencils x 256 lanes": (1.4997e9 + 0.6711e9, 862.8),
         },
         "x3": {
-            "march": (2.3270e9 + 0.6718e9, 1115.2),
-            "tile 4 pencils x 256 lanes": (1.9739e9 + 0.6742e9, 1300.2),
-            "tile 8 pencils x 256 lanes": (1.8256e9 + 0.6748e9, 1109.6),
-            "tile 8 pencils x 512 lanes": (1.7810e9 + 0.6743e9, 1176.1),
+            "march": (2.1466e9 + 0.6715e9, 924.8),
+            "tile 4 pencils x 256 lanes": (1.9782e9 + 0.6731e9, 1183.5),
+            "tile 8 pencils x 256 lanes": (1.8223e9 + 0.6738e9, 1006.2),
+            "tile 8 pencils x 512 lanes": (1.7618e9 + 0.6741e9, 1070.8),
+            "tile 16 pencils x 256 lanes": (1.6899e9 + 0.6713e9, 921.6),
         },
         "x3_dt": {
-            "march": (2.8409e9 + 0.6711e9, 1241.3),
+            "march": (2.8348e9 + 0.6711e9, 1050.9),
         },
         "stage": {
-            "one-launch stage 32x16 columns": (7.2987e9 + 3.8184e9, 2863.9),
-            "one-launch stage 32x8 columns": (1.5991e9 + 0.7587e9, 3185.6),
-            "one-launch stage 64x16 columns": (23.1709e9 + 12.2115e9, 2768.5),
-            "one-launch stage 64x4 columns": (1.6402e9 + 0.7253e9, 3406.4),
-            "one-launch stage 64x8 columns": (7.4677e9 + 3.6017e9, 2926.9),
+            "one-launch stage 16x32 columns": (2.0549e9 + 0.7058e9, 2507.6),
+            "one-launch stage 32x8 columns": (1.5876e9 + 0.6811e9, 2741.4),
+            "one-launch stage 32x16 columns": (2.0083e9 + 0.7095e9, 2521.0),
+            "one-launch stage 64x4 columns": (1.6530e9 + 0.6711e9, 2965.4),
+            "one-launch stage 64x8 columns": (2.9328e9 + 0.8017e9, 2599.5),
+        },
+        "stage_dt": {
+            "one-launch stage 16x32 columns": (2.4589e9 + 0.6870e9, 2591.9),
         },
     },
     "fp32_mixed": {
         "x1": {
-            "march": (2.6487e9 + 0.9329e9, 887.7),
-            "tile 32 pencils x 1024 lanes": (1.0907e9 + 0.6711e9, 955.4),
-            "tile 4 pencils x 256 lanes": (1.0907e9 + 0.6711e9, 1190.5),
-            "tile 8 pencils x 256 lanes": (1.0906e9 + 0.6711e9, 1035.2),
-            "tile 8 pencils x 512 lanes": (1.0907e9 + 0.6711e9, 1075.8),
+            "march": (2.6465e9 + 0.9340e9, 797.0),
+            "tile 4 pencils x 256 lanes": (1.0907e9 + 0.6711e9, 1094.0),
+            "tile 8 pencils x 256 lanes": (1.0907e9 + 0.6711e9, 945.5),
+            "tile 8 pencils x 512 lanes": (1.0907e9 + 0.6711e9, 989.5),
+            "tile 16 pencils x 256 lanes": (1.0907e9 + 0.6711e9, 877.1),
+            "tile 32 pencils x 1024 lanes": (1.0908e9 + 0.6711e9, 873.0),
         },
         "x2": {
-            "march": (1.5031e9 + 0.6711e9, 842.6),
-            "tile 32 pencils x 1024 lanes": (1.4996e9 + 0.6711e9, 892.5),
-            "tile 4 pencils x 256 lanes": (1.4996e9 + 0.6719e9, 1138.0),
-            "tile 8 pencils x 256 lanes": (1.4996e9 + 0.6711e9, 971.5),
-            "tile 8 pencils x 512 lanes": (1.4996e9 + 0.6711e9, 1026.0),
+            "march": (1.5035e9 + 0.6711e9, 739.7),
+            "tile 4 pencils x 256 lanes": (1.4996e9 + 0.6719e9, 1029.8),
+            "tile 8 pencils x 256 lanes": (1.4997e9 + 0.6711e9, 875.4),
+            "tile 8 pencils x 512 lanes": (1.4997e9 + 0.6711e9, 928.4),
+            "tile 16 pencils x 256 lanes": (1.4997e9 + 0.6711e9, 802.8),
+            "tile 32 pencils x 1024 lanes": (1.4997e9 + 0.6711e9, 804.0),
         },
         "x3": {
-            "march": (2.3076e9 + 0.6764e9, 955.3),
-            "tile 32 pencils x 1024 lanes": (2.1915e9 + 0.6739e9, 1003.6),
-            "tile 4 pencils x 256 lanes": (2.1607e9 + 0.7168e9, 1264.8),
-            "tile 8 pencils x 256 lanes": (2.2328e9 + 0.6815e9, 1084.9),
-            "tile 8 pencils x 512 lanes": (2.1232e9 + 0.6902e9, 1149.4),
+            "march": (2.3251e9 + 0.6765e9, 833.0),
+            "tile 4 pencils x 256 lanes": (2.1329e9 + 0.7296e9, 1155.0),
+            "tile 8 pencils x 256 lanes": (2.2276e9 + 0.6831e9, 987.5),
+            "tile 8 pencils x 512 lanes": (2.1077e9 + 0.6928e9, 1050.5),
+            "tile 16 pencils x 256 lanes": (2.2138e9 + 0.6733e9, 908.6),
+            "tile 32 pencils x 1024 lanes": (2.1775e9 + 0.6737e9, 914.0),
         },
         "x3_dt": {
-            "march": (2.8211e9 + 0.6712e9, 1054.8),
+            "march": (3.0053e9 + 0.6761e9, 931.2),
         },
         "stage": {
-            "one-launch stage 32x16 columns": (2.0214e9 + 0.8401e9, 2592.3),
-            "one-launch stage 32x8 columns": (2.0352e9 + 0.8493e9, 2797.1),
-            "one-launch stage 64x16 columns": (6.7355e9 + 3.4127e9, 2619.4),
-            "one-launch stage 64x4 columns": (2.0253e9 + 0.7849e9, 3053.7),
-            "one-launch stage 64x8 columns": (1.9585e9 + 0.7926e9, 2691.5),
+            "one-launch stage 16x32 columns": (2.0328e9 + 0.6905e9, 2323.4),
+            "one-launch stage 32x8 columns": (2.0596e9 + 0.6909e9, 2507.0),
+            "one-launch stage 32x16 columns": (1.9790e9 + 0.6811e9, 2338.5),
+            "one-launch stage 64x4 columns": (2.0536e9 + 0.6711e9, 2740.1),
+            "one-launch stage 64x8 columns": (1.9672e9 + 0.6711e9, 2406.5),
         },
         "stage_dt": {
-            "one-launch stage 32x16 columns": (2.4435e9 + 0.8168e9, 2679.0),
+            "one-launch stage 16x32 columns": (2.4442e9 + 0.6713e9, 2406.8),
         },
     },
 }
@@ -166,7 +199,7 @@ TILE_SHAPES = [(4, 256), (8, 256), (8, 512), (16, 256), (32, 1024), (64, 1024)]
 
 
 # one-launch stage kernel variants (3-D): tile of x1 columns x x2 rows, x3 segment
-STAGE_SHAPES = [(64, 4), (64, 8), (32, 8), (32, 16), (64, 16)]
+STAGE_SHAPES = [(64, 4), (64, 8), (32, 8), (32, 16), (16, 32)]
 STAGE_SEGS = ["block", 32, 16]
 
 
