@@ -233,9 +233,15 @@ __device__ __forceinline__ double ldg(const double *p, long i) {
   return ((gconst_double *)p)[i];
 }
 __device__ __forceinline__ void stg(double *p, long i, double v) { ((gdouble *)p)[i] = v; }
+// v into p[i] and p[i + 1] with one 16-byte store (p + i 16-byte aligned)
+typedef __attribute__((address_space(1))) double2 gdouble2;
+__device__ __forceinline__ void stg2(double *p, long i, double v) {
+  *(gdouble2 *)(p + i) = make_double2(v, v);
+}
 #else
 __host__ __device__ inline double ldg(const double *p, long i) { return p[i]; }
 __host__ __device__ inline void stg(double *p, long i, double v) { p[i] = v; }
+__host__ __device__ inline void stg2(double *p, long i, double v) { p[i] = p[i + 1] = v; }
 #endif
 KM_HD double *flux_ptr(const km_pack_t &p, int d, int b, int v) {
   return p.flux[d][(long)b * p.nflux + v];

@@ -92,8 +92,8 @@ def test_fused_amr_every_tile_shape_bitwise(gpu, oracle):
         rows = shapes[shape - 1][0]
         n = lay.nx[0]
         # LDS of a tile (FP64): primitives and face values of its pencils, plus the
-        # 328-byte per-pencil table; a shape past 160 KiB is refused
-        if rows * (7 * (n + 2 + 2 * w) + 7 * (n + 2)) * 8 + 328 * rows > 160 * 1024:
+        # 336-byte per-pencil table; a shape past 160 KiB is refused
+        if rows * (7 * (n + 2 + 2 * w) + 7 * (n + 2)) * 8 + 336 * rows > 160 * 1024:
             with pytest.raises(km.KmError):
                 _run(gpu, cfg, desc, lay, leaves, cell, 3, kern=shape)
             continue

@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 
 NSHAPES = 6  # pencil-tile shapes
 SHAPES = [(4, 256), (8, 256), (8, 512), (16, 256), (32, 1024), (64, 1024)]  # (pencils, lanes)
-INFO_BYTES = 328  # per-pencil table of a tile workgroup (sizeof(PencilInfo))
+INFO_BYTES = 336  # per-pencil table of a tile workgroup (sizeof(PencilInfo))
 WIDTH = {"wenoz": 2, "ppm": 2, "plm": 1, "fog": 0}
 
 
