@@ -314,9 +314,12 @@ int km_stage_plan_kernel(km_stage_plan_t *plan, int32_t dir, int32_t *kernel);
  * must not be u1 (the kernels read u0 across block faces while writing u1); base may be
  * u0. Single-level meshes read u0's conserved interior only; block trees read u0's stored
  * primitives and ghost zones (PreparePrimitive and the ghost exchange of the previous
- * stage) and need u0's flux tables, which receive the fluxes at level jumps. Calls on one
- * plan are serialised; the first call (and the first after a change of method or form)
- * runs the autotuner on the call's data and stream. */
+ * stage) and need u0's flux tables, which receive the fluxes at level jumps. u1's ghost
+ * zones are stale after the call, as after ApplyDuDt; some kernels write the x1 ghost
+ * zones of the rows they update with the edge zone's value (rows in whole cache lines),
+ * which the next ghost exchange overwrites. Calls on one plan are serialised; the first
+ * call (and the first after a change of method or form) runs the autotuner on the call's
+ * data and stream. */
 int km_stage_fused(const km_cfg_t *cfg, km_stage_plan_t *plan, const km_pack_t *base,
                    const km_pack_t *u0, const km_pack_t *u1, double beta, const double *dt_dev,
                    double *dt_est_dev, void *stream);
