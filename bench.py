@@ -615,10 +615,29 @@ def _secondary(km, args):
         kern32 = _kernel_names(sim, 2)
         sim.close()
         cfg.set("kamayan_amd/precision", "fp64")
+        # the same 256 runs one after another, each its own FP64 simulation on the fused
+        # path (graph replay, autotuned), timed cycles summed over the runs
+        dt_seq, same, b0 = 0.0, True, 0
+        for d, c, n in zip(descs, cells, res):
+            lay_m = km.Layout(2, 4, (32, 32, 1), (n // 32, n // 32, 1))
+            sim = km.Simulation(cfg, d, lay_m, path=km.PATH_FUSED)
+            sim.set_primitive(c)
+            dt_seq += _timed(km, sim, args.steps, args.warmup)
+            sm, _ = sim.state()
+            sim.close()
+            nb_m = lay_m.nb[0] * lay_m.nb[1]
+            same = same and bool(np.array_equal(sm, s64[b0:b0 + nb_m]))
+            b0 += nb_m
         out["variants"] = {"fp32_mixed": {"value": zones * args.steps / dt32,
                                           "speedup_vs_fp64": dt / dt32,
+                                          "speedup_vs_sequential_fp64": dt_seq / dt32,
                                           "rel_l1_vs_fp64_per_var": _rel_l1(s32, s64, lay),
-                                          "sweep_kernels": kern32}}
+                                          "sweep_kernels": kern32},
+                           "sequential_fp64": {"value": zones * args.steps / dt_seq,
+                                               "runs": len(res),
+                                               "bitwise_equal_to_ensemble_fp64": same,
+                                               "note": "each run its own simulation, one "
+                                                       "after another; timed cycles summed"}}
         out["config"] = {"workload": "kh2d_ensemble_256runs_64to512_32b_wenoz_hllc_rk3",
                          "zones": zones, "runs": len(res), "block": [32, 32], "nghost": 4,
                          "path": "fused", "sweep_kernels": kern64, "parallelism": "single GPU"}
