@@ -67,6 +67,10 @@ def test_ct_fused_bitwise(gpu, oracle, name, kw):
     assert gi == ri
     sl = (slice(None), slice(None)) + lay.interior()
     _same(g[sl], r[sl], f"{name} cells")
+    # ghost zones too, for every field the exchange fills: the stage leaves some of them
+    # stale or holding the edge zone's value, and the fill before the state is read must
+    # restore all of them
+    _same(g[:, :gpu.slot("TEMP", 1)], r[:, :gpu.slot("TEMP", 1)], f"{name} cells with ghosts")
     for d in range(3):
         _same(gf[d], rf[d], f"{name} face {d}")
     divb = g[(slice(None), gpu.slot("DIVB", 1)) + lay.interior()]
