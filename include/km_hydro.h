@@ -301,7 +301,8 @@ int km_stage_plan_amr(const km_cfg_t *cfg, const km_mesh_desc_t *mesh, const int
 int km_stage_plan_destroy(km_stage_plan_t *plan);
 /* Kernel choice of the plan's stages: dir 0..2 (-1: all) a sweep kernel as for
  * km_sim_set_fused_kernel; dir 3 the whole stage: -1 autotune, 0 three sweeps, k >= 1 the
- * one-launch stage kernel variant k (3-D single-level meshes). No reference counterpart
+ * one-launch stage kernel variant k (2-D and 3-D single-level meshes; 2-D meshes have
+ * variants 1-4, for 16-, 32- and 64-cell block rows). No reference counterpart
  * (the reference fixes its loop shapes in source). */
 int km_stage_plan_set_kernel(km_stage_plan_t *plan, int32_t dir, int32_t kernel);
 int km_stage_plan_kernel(km_stage_plan_t *plan, int32_t dir, int32_t *kernel);

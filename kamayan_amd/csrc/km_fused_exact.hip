@@ -8,3 +8,4 @@
 #define KM_FUSED_T double
 #include "km_fused.inc"
 #include "km_stage3.inc"
+#include "km_stage2.inc"

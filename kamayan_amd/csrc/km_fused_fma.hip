@@ -9,3 +9,4 @@
 #define KM_FUSED_FAST 1
 #include "km_fused.inc"
 #include "km_stage3.inc"
+#include "km_stage2.inc"

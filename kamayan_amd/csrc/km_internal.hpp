@@ -125,10 +125,12 @@ int launch_fused_sweep(const km_cfg_t &cfg, const FusedArgs &a, int ndim, int di
                        int kern, hipStream_t st);
 int tile_shapes();
 int launch_fine_face_fluxes(const km_cfg_t &cfg, const FusedArgs &a, hipStream_t st);
-// the whole stage in one launch (3-D, single-level meshes), kern in 1..stage3_kernels()
+// the whole stage in one launch (2-D and 3-D, single-level meshes), kern in
+// 1..stage3_kernels() on 3-D meshes and 1..stage2_kernels() on 2-D ones
 int launch_fused_stage3(const km_cfg_t &cfg, const FusedArgs &a, bool with_dt, int kern,
                         hipStream_t st);
 int stage3_kernels();
+int stage2_kernels();
 // the CT-MHD stage (3-D, Cartesian, single-level meshes; km_fused_mhd.inc): one sweep
 // along x(dir+1) with tile shape kern in 1..mhd_tile_shapes(), and the zone pass after
 // the face update (dU/dt, stage combine, cell B from faces, PreparePrimitive of u1)
@@ -142,10 +144,12 @@ int launch_fused_sweep(const km_cfg_t &cfg, const FusedArgs &a, int ndim, int di
                        int kern, hipStream_t st);
 int tile_shapes();
 int launch_fine_face_fluxes(const km_cfg_t &cfg, const FusedArgs &a, hipStream_t st);
-// the whole stage in one launch (3-D, single-level meshes), kern in 1..stage3_kernels()
+// the whole stage in one launch (2-D and 3-D, single-level meshes), kern in
+// 1..stage3_kernels() on 3-D meshes and 1..stage2_kernels() on 2-D ones
 int launch_fused_stage3(const km_cfg_t &cfg, const FusedArgs &a, bool with_dt, int kern,
                         hipStream_t st);
 int stage3_kernels();
+int stage2_kernels();
 // the CT-MHD stage (3-D, Cartesian, single-level meshes; km_fused_mhd.inc): one sweep
 // along x(dir+1) with tile shape kern in 1..mhd_tile_shapes(), and the zone pass after
 // the face update (dU/dt, stage combine, cell B from faces, PreparePrimitive of u1)
@@ -159,10 +163,12 @@ int launch_fused_sweep(const km_cfg_t &cfg, const FusedArgs &a, int ndim, int di
                        int kern, hipStream_t st);
 int tile_shapes();
 int launch_fine_face_fluxes(const km_cfg_t &cfg, const FusedArgs &a, hipStream_t st);
-// the whole stage in one launch (3-D, single-level meshes), kern in 1..stage3_kernels()
+// the whole stage in one launch (2-D and 3-D, single-level meshes), kern in
+// 1..stage3_kernels() on 3-D meshes and 1..stage2_kernels() on 2-D ones
 int launch_fused_stage3(const km_cfg_t &cfg, const FusedArgs &a, bool with_dt, int kern,
                         hipStream_t st);
 int stage3_kernels();
+int stage2_kernels();
 // the CT-MHD stage (3-D, Cartesian, single-level meshes; km_fused_mhd.inc): one sweep
 // along x(dir+1) with tile shape kern in 1..mhd_tile_shapes(), and the zone pass after
 // the face update (dU/dt, stage combine, cell B from faces, PreparePrimitive of u1)

@@ -635,16 +635,16 @@ int launch_stage3(const S &s, const FusedArgs &a, bool last, int kern) {
   return exact::launch_fused_stage3(cfg, a, last, kern, s.stream);
 }
 
-int stage3_kernels(const km_cfg_t &cfg) {
-  if (cfg.precision == KM_PREC_FP32_MIXED) return fp32::stage3_kernels();
-  if (cfg.precision == KM_PREC_FP64_FMA) return fma::stage3_kernels();
-  return exact::stage3_kernels();
+// one-launch stage variants of a mesh of ndim dimensions (the same in every form)
+int stage3_kernels(const km_cfg_t &cfg, int ndim) {
+  (void)cfg;
+  return ndim == 2 ? exact::stage2_kernels() : exact::stage3_kernels();
 }
 
-// whether the one-launch stage kernel applies: fused path, 3-D hydro, single level
+// whether the one-launch stage kernel applies: fused path, 2-D or 3-D hydro, single level
 template <class S>
 bool stage3_applies(const S &s) {
-  return s.path == KM_PATH_FUSED && !s.amr && s.mesh.ndim == 3 && s.cfg.mhd == KM_MHD_OFF;
+  return s.path == KM_PATH_FUSED && !s.amr && s.mesh.ndim >= 2 && s.cfg.mhd == KM_MHD_OFF;
 }
 
 int tile_shapes(const km_cfg_t &cfg) {
@@ -755,7 +755,7 @@ int autotune(S &s, const FusedArgs &a) {
           return (int)KM_OK;
         }, &best_ms))
       return rc;
-    for (int k = 1; k <= stage3_kernels(cfg); ++k) {
+    for (int k = 1; k <= stage3_kernels(cfg, ndim); ++k) {
       float ms = 0.f;
       if (int rc = time_launch(s, [&] { return launch_stage3(s, a, false, k); }, &ms)) return rc;
       if (ms >= 0.f && (best_ms < 0.f || ms < best_ms)) {
@@ -1888,10 +1888,10 @@ int km_sim_advance(km_sim_t *s, int32_t ncycles, double tlim) {
 int km_sim_set_fused_kernel(km_sim_t *s, int32_t dir, int32_t kern) {
   if (!s) return KM_EINVAL;
   if (dir == 3) {  // the whole stage
-    if (kern < -1 || kern > stage3_kernels(s->cfg) || (kern > 0 && !stage3_applies(*s))) {
-      set_error("stage kernel: -1 (autotune), 0 (three sweeps) or a one-launch variant 1.." +
-                std::to_string(stage3_kernels(s->cfg)) +
-                " (fused path, 3-D hydro, single-level meshes)");
+    const int nk = stage3_kernels(s->cfg, s->mesh.ndim);
+    if (kern < -1 || kern > nk || (kern > 0 && !stage3_applies(*s))) {
+      set_error("stage kernel: -1 (autotune), 0 (the sweeps) or a one-launch variant 1.." +
+                std::to_string(nk) + " (fused path, 2-D or 3-D hydro, single-level meshes)");
       return KM_EINVAL;
     }
     KM_HIP(hipStreamSynchronize(s->stream));
@@ -2239,9 +2239,10 @@ int km_stage_plan_set_kernel(km_stage_plan_t *p, int32_t dir, int32_t kern) {
   if (!p) return KM_EINVAL;
   std::lock_guard<std::mutex> lock(p->mu);
   if (dir == 3) {
-    if (kern < -1 || kern > stage3_kernels(p->cfg) || (kern > 0 && !stage3_applies(*p))) {
-      set_error("stage kernel: -1 (autotune), 0 (three sweeps) or a one-launch variant 1.." +
-                std::to_string(stage3_kernels(p->cfg)) + " (3-D single-level meshes)");
+    const int nk = stage3_kernels(p->cfg, p->mesh.ndim);
+    if (kern < -1 || kern > nk || (kern > 0 && !stage3_applies(*p))) {
+      set_error("stage kernel: -1 (autotune), 0 (the sweeps) or a one-launch variant 1.." +
+                std::to_string(nk) + " (2-D and 3-D single-level meshes)");
       return KM_EINVAL;
     }
     p->stage_kern = kern;

@@ -182,6 +182,6 @@ def test_stage_kernel_refusals(gpu):
     sim.close()
     cfg2, desc2, lay2, cell2 = _cases.vortex(n=64, nblock=32)[:4]
     sim = km.Simulation(cfg2, desc2, lay2, path=km.PATH_FUSED)
-    with pytest.raises(km.KmError):  # 2-D: the stage kernel is for 3-D meshes
-        sim.set_fused_kernel(3, 2)
+    with pytest.raises(km.KmError):  # 2-D meshes have four variants (km_stage2.inc)
+        sim.set_fused_kernel(3, 5)
     sim.close()

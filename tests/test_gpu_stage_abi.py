@@ -250,9 +250,9 @@ def test_refusals(gpu, oracle):
     dt = km.DeviceArray(1)
     with pytest.raises(km.KmError):  # u1 == u0
         plan.stage(cfg, base, base, base, 1.0, dt)
-    with pytest.raises(km.KmError):  # 2-D plan cannot take the stage kernel
+    with pytest.raises(km.KmError):  # a 2-D plan has four stage-kernel variants
         cfg2, desc2 = _cases.vortex(n=64, nblock=32)[:2]
-        km.StagePlan(cfg2, descs=[desc2]).set_kernel(3, 1)
+        km.StagePlan(cfg2, descs=[desc2]).set_kernel(3, 5)
     mhd = Config(**{"physics/MHD": "ct"})
     mhd.c.ndim = 3
     with pytest.raises(km.KmError):
