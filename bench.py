@@ -203,9 +203,17 @@ STAGE_SHAPES = [(64, 4), (64, 8), (32, 8), (32, 16), (16, 32)]
 STAGE_SEGS = ["block", 32, 16]
 
 
+# one-launch stage kernel variants of 2-D meshes: x1 cells per block row, lanes
+STAGE2_SHAPES = [(16, 256), (32, 256), (32, 512), (64, 256)]
+
+
 def _kernel_names(sim, ndim):
     names = {}
-    st = sim.fused_kernel(3) if ndim == 3 else 0
+    st = sim.fused_kernel(3) if ndim >= 2 else 0
+    if st > 0 and ndim == 2:
+        tx, nt = STAGE2_SHAPES[st - 1]
+        names["stage"] = f"one-launch 2-D stage, {tx}-cell rows, {nt} lanes"
+        return names
     if st > 0:
         tx, ty = STAGE_SHAPES[(st - 1) // 3]
         seg = STAGE_SEGS[(st - 1) % 3]
@@ -607,6 +615,13 @@ def _secondary(km, args):
         s64, _ = sim.state()
         kern64 = _kernel_names(sim, 2)
         sim.close()
+        cfg.set("kamayan_amd/precision", "fp64_fma")
+        sim = km.Simulation(cfg, descs[0], lay, path=km.PATH_FUSED, descs=descs)
+        sim.set_primitive(cell)
+        dtf = _timed(km, sim, args.steps, args.warmup)
+        sf, _ = sim.state()
+        kernf = _kernel_names(sim, 2)
+        sim.close()
         cfg.set("kamayan_amd/precision", "fp32_mixed")
         sim = km.Simulation(cfg, descs[0], lay, path=km.PATH_FUSED, descs=descs)
         sim.set_primitive(cell)
@@ -628,7 +643,11 @@ def _secondary(km, args):
             nb_m = lay_m.nb[0] * lay_m.nb[1]
             same = same and bool(np.array_equal(sm, s64[b0:b0 + nb_m]))
             b0 += nb_m
-        out["variants"] = {"fp32_mixed": {"value": zones * args.steps / dt32,
+        out["variants"] = {"fp64_fma": {"value": zones * args.steps / dtf,
+                                        "speedup_vs_fp64": dt / dtf,
+                                        "rel_l1_vs_fp64_per_var": _rel_l1(sf, s64, lay),
+                                        "sweep_kernels": kernf},
+                           "fp32_mixed": {"value": zones * args.steps / dt32,
                                           "speedup_vs_fp64": dt / dt32,
                                           "speedup_vs_sequential_fp64": dt_seq / dt32,
                                           "rel_l1_vs_fp64_per_var": _rel_l1(s32, s64, lay),
