@@ -570,11 +570,13 @@ def _secondary(km, args):
         stage_bytes = 8 * 3 * 5
         nst = 2
         # both paths replay a captured cycle; the faster one is the headline
-        times = {}
+        times, kern_v = {}, None
         for name, path in (("tasks", km.PATH_TASKS), ("fused", km.PATH_FUSED)):
             sim = km.Simulation(cfg, desc, lay, path=path)
             sim.set_primitive(cell)
             times[name] = _timed(km, sim, args.steps, args.warmup)
+            if path == km.PATH_FUSED:
+                kern_v = _kernel_names(sim, 2)
             sim.close()
         head = min(times, key=times.get)
         dt = times[head]
@@ -583,7 +585,7 @@ def _secondary(km, args):
                                              "ms_per_step": times[other] * 1e3 / args.steps}}
         out["config"] = {"workload": "vortex2d_128c_32b_plm_mc_hllc_rk2_cfl04", "zones": zones,
                          "block": [32, 32], "nghost": 4, "path": head,
-                         "parallelism": "single GPU"}
+                         "fused_kernels": kern_v, "parallelism": "single GPU"}
         ref = km_oracle.CpuSimulation(cfg.c, desc, lay)
         ref.set_primitive(cell)
         sample_cycles = 200
