@@ -199,7 +199,7 @@ TILE_SHAPES = [(4, 256), (8, 256), (8, 512), (16, 256), (32, 1024), (64, 1024)]
 
 
 # one-launch stage kernel variants (3-D): tile of x1 columns x x2 rows, x3 segment
-STAGE_SHAPES = [(64, 4), (64, 8), (32, 8), (32, 16), (16, 32)]
+STAGE_SHAPES = [(64, 4), (64, 8), (32, 8), (32, 16), (16, 32), (16, 16)]
 STAGE_SEGS = ["block", 32, 16]
 
 

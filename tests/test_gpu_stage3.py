@@ -14,7 +14,7 @@ from tests import _cases
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(64, 4), (64, 8), (32, 8), (32, 16), (16, 32)]  # x1 columns x x2 rows
+SHAPES = [(64, 4), (64, 8), (32, 8), (32, 16), (16, 32), (16, 16)]  # x1 columns x x2 rows
 SEGS = [0, 32, 16]  # x3 planes per workgroup (0: the whole block)
 NKERN = len(SHAPES) * len(SEGS)
 
