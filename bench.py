@@ -74,35 +74,36 @@ MEASURED = {
     },
     "fp64_fma": {
         "x1": {
-            "march": (2.6478e9 + 0.9360e9, 860.2),
-            "tile 4 pencils x 256 lanes": (1.0907e9 + 0.6711e9, 1169.5),
-            "tile 8 pencils x 256 lanes": (1.0907e9 + 0.6711e9, 1011.2),
-            "tile 8 pencils x 512 lanes": (1.0907e9 + 0.6711e9, 1056.8),
+            "march": (2.6478e9 + 0.9354e9, 860.2),
+            "tile 4 pencils x 256 lanes": (1.0907e9 + 0.6711e9, 1169.2),
+            "tile 8 pencils x 256 lanes": (1.0907e9 + 0.6711e9, 1011.1),
+            "tile 8 pencils x 512 lanes": (1.0907e9 + 0.6711e9, 1056.6),
             "tile 16 pencils x 256 lanes": (1.0907e9 + 0.6711e9, 937.1),
         },
         "x2": {
             "march": (1.5003e9 + 0.6711e9, 787.7),
-            "tile 4 pencils x 256 lanes": (1.4997e9 + 0.6711e9, 1105.2),
-            "tile 8 pencils x 256 lanes": (1.4997e9 + 0.6711e9, 941.1),
-            "tile 8 pencils x 512 lanes": (1.4997e9 + 0.6711e9, 995.6),
-            "tile 16 pencils x 256 lanes": (1.4997e9 + 0.6711e9, 862.8),
+            "tile 4 pencils x 256 lanes": (1.4997e9 + 0.6711e9, 1106.5),
+            "tile 8 pencils x 256 lanes": (1.4997e9 + 0.6711e9, 941.8),
+            "tile 8 pencils x 512 lanes": (1.4997e9 + 0.6711e9, 996.2),
+            "tile 16 pencils x 256 lanes": (1.4997e9 + 0.6711e9, 863.1),
         },
         "x3": {
-            "march": (2.1466e9 + 0.6715e9, 924.8),
-            "tile 4 pencils x 256 lanes": (1.9782e9 + 0.6731e9, 1183.5),
-            "tile 8 pencils x 256 lanes": (1.8223e9 + 0.6738e9, 1006.2),
-            "tile 8 pencils x 512 lanes": (1.7618e9 + 0.6741e9, 1070.8),
-            "tile 16 pencils x 256 lanes": (1.6899e9 + 0.6713e9, 921.6),
+            "march": (2.1496e9 + 0.6715e9, 924.8),
+            "tile 4 pencils x 256 lanes": (1.9460e9 + 0.7608e9, 1172.9),
+            "tile 8 pencils x 256 lanes": (1.8285e9 + 0.7569e9, 1034.8),
+            "tile 8 pencils x 512 lanes": (1.8092e9 + 0.7603e9, 1069.8),
+            "tile 16 pencils x 256 lanes": (1.6127e9 + 0.7617e9, 987.0),
         },
         "x3_dt": {
-            "march": (2.8348e9 + 0.6711e9, 1050.9),
+            "march": (2.8341e9 + 0.6711e9, 1050.9),
         },
         "stage": {
-            "one-launch stage 16x32 columns": (2.0549e9 + 0.7058e9, 2507.6),
-            "one-launch stage 32x8 columns": (1.5876e9 + 0.6811e9, 2741.4),
-            "one-launch stage 32x16 columns": (2.0083e9 + 0.7095e9, 2521.0),
-            "one-launch stage 64x4 columns": (1.6530e9 + 0.6711e9, 2965.4),
-            "one-launch stage 64x8 columns": (2.9328e9 + 0.8017e9, 2599.5),
+            "one-launch stage 16x16 columns": (2.9040e9 + 0.7598e9, 2687.5),
+            "one-launch stage 16x32 columns": (2.0572e9 + 0.7061e9, 2524.0),
+            "one-launch stage 32x8 columns": (2.6145e9 + 0.7457e9, 2694.4),
+            "one-launch stage 32x16 columns": (2.0069e9 + 0.7096e9, 2504.6),
+            "one-launch stage 64x4 columns": (2.4835e9 + 0.7176e9, 2943.8),
+            "one-launch stage 64x8 columns": (2.4512e9 + 0.7371e9, 2582.7),
         },
         "stage_dt": {
             "one-launch stage 16x32 columns": (2.4589e9 + 0.6870e9, 2591.9),
@@ -110,41 +111,42 @@ MEASURED = {
     },
     "fp32_mixed": {
         "x1": {
-            "march": (2.6465e9 + 0.9340e9, 797.0),
-            "tile 4 pencils x 256 lanes": (1.0907e9 + 0.6711e9, 1094.0),
-            "tile 8 pencils x 256 lanes": (1.0907e9 + 0.6711e9, 945.5),
-            "tile 8 pencils x 512 lanes": (1.0907e9 + 0.6711e9, 989.5),
+            "march": (2.6464e9 + 0.9349e9, 797.0),
+            "tile 4 pencils x 256 lanes": (1.0907e9 + 0.6711e9, 1093.8),
+            "tile 8 pencils x 256 lanes": (1.0907e9 + 0.6711e9, 945.4),
+            "tile 8 pencils x 512 lanes": (1.0907e9 + 0.6711e9, 989.4),
             "tile 16 pencils x 256 lanes": (1.0907e9 + 0.6711e9, 877.1),
             "tile 32 pencils x 1024 lanes": (1.0908e9 + 0.6711e9, 873.0),
         },
         "x2": {
             "march": (1.5035e9 + 0.6711e9, 739.7),
-            "tile 4 pencils x 256 lanes": (1.4996e9 + 0.6719e9, 1029.8),
-            "tile 8 pencils x 256 lanes": (1.4997e9 + 0.6711e9, 875.4),
-            "tile 8 pencils x 512 lanes": (1.4997e9 + 0.6711e9, 928.4),
-            "tile 16 pencils x 256 lanes": (1.4997e9 + 0.6711e9, 802.8),
-            "tile 32 pencils x 1024 lanes": (1.4997e9 + 0.6711e9, 804.0),
+            "tile 4 pencils x 256 lanes": (1.4997e9 + 0.6719e9, 1031.0),
+            "tile 8 pencils x 256 lanes": (1.4997e9 + 0.6711e9, 876.0),
+            "tile 8 pencils x 512 lanes": (1.4996e9 + 0.6711e9, 929.0),
+            "tile 16 pencils x 256 lanes": (1.4997e9 + 0.6711e9, 803.1),
+            "tile 32 pencils x 1024 lanes": (1.4997e9 + 0.6711e9, 804.1),
         },
         "x3": {
-            "march": (2.3251e9 + 0.6765e9, 833.0),
-            "tile 4 pencils x 256 lanes": (2.1329e9 + 0.7296e9, 1155.0),
-            "tile 8 pencils x 256 lanes": (2.2276e9 + 0.6831e9, 987.5),
-            "tile 8 pencils x 512 lanes": (2.1077e9 + 0.6928e9, 1050.5),
-            "tile 16 pencils x 256 lanes": (2.2138e9 + 0.6733e9, 908.6),
-            "tile 32 pencils x 1024 lanes": (2.1775e9 + 0.6737e9, 914.0),
+            "march": (2.3256e9 + 0.6765e9, 833.0),
+            "tile 4 pencils x 256 lanes": (2.1604e9 + 0.8147e9, 1142.9),
+            "tile 8 pencils x 256 lanes": (2.2257e9 + 0.7823e9, 1013.0),
+            "tile 8 pencils x 512 lanes": (2.1154e9 + 0.7826e9, 1046.5),
+            "tile 16 pencils x 256 lanes": (2.1867e9 + 0.7807e9, 968.5),
+            "tile 32 pencils x 1024 lanes": (2.1321e9 + 0.7576e9, 999.1),
         },
         "x3_dt": {
-            "march": (3.0053e9 + 0.6761e9, 931.2),
+            "march": (3.0038e9 + 0.6761e9, 931.2),
         },
         "stage": {
-            "one-launch stage 16x32 columns": (2.0328e9 + 0.6905e9, 2323.4),
-            "one-launch stage 32x8 columns": (2.0596e9 + 0.6909e9, 2507.0),
-            "one-launch stage 32x16 columns": (1.9790e9 + 0.6811e9, 2338.5),
-            "one-launch stage 64x4 columns": (2.0536e9 + 0.6711e9, 2740.1),
-            "one-launch stage 64x8 columns": (1.9672e9 + 0.6711e9, 2406.5),
+            "one-launch stage 16x16 columns": (2.7113e9 + 0.7156e9, 2502.8),
+            "one-launch stage 16x32 columns": (1.9635e9 + 0.7028e9, 2344.0),
+            "one-launch stage 32x8 columns": (2.5502e9 + 0.6921e9, 2532.2),
+            "one-launch stage 32x16 columns": (1.9775e9 + 0.6805e9, 2342.9),
+            "one-launch stage 64x4 columns": (2.1900e9 + 0.6711e9, 2736.2),
+            "one-launch stage 64x8 columns": (1.9655e9 + 0.6711e9, 2414.0),
         },
         "stage_dt": {
-            "one-launch stage 16x32 columns": (2.4442e9 + 0.6713e9, 2406.8),
+            "one-launch stage 32x8 columns": (3.0573e9 + 0.6778e9, 2652.3),
         },
     },
 }
