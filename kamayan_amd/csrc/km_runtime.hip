@@ -656,41 +656,50 @@ int tile_shapes(const km_cfg_t &cfg) {
 
 // Autotuner of the fused sweeps. For each direction every candidate kernel (the march
 // and each pencil-tile shape) runs that direction's sweep of the first stage on the
-// simulation's own data; the fastest of three timed launches wins. The sweeps only
-// write scratch the next stage rewrites before reading it: the du accumulator, and u1
-// of stage 1, which is always a stage container (with one stage too, see rk1_fused).
-// Choices are kept
+// simulation's own data; the candidates take turns over five timed rounds and the
+// fastest best time wins. The sweeps only write scratch the next stage rewrites before
+// reading it: the du accumulator, and u1 of stage 1, which is always a stage container
+// (with one stage too, see rk1_fused). Choices are kept
 // per (form, reconstruction, limiter, Riemann solver, block shape, block count,
 // direction) in a process-wide table, so later simulations of the same shape reuse them.
 std::mutex g_tune_mu;
 std::map<std::vector<int>, int> g_tune;
 
-// Time fn (one launch sequence on the simulation's stream): a warm-up, then the fastest
-// of three HIP-event-timed runs into *ms; *ms = -1 when fn refuses (KM_EINVAL: the
-// variant does not fit the blocks) or the device refuses the launch.
+// Time candidates fn(0) .. fn(n - 1) (each one launch sequence on the simulation's
+// stream) into ms[0 .. n - 1]: each runs once untimed (ms[k] = -1 when it refuses with
+// KM_EINVAL, i.e. does not fit the blocks, or the device refuses the launch), then the
+// candidates take turns for kRounds HIP-event-timed runs and each keeps its fastest. Taking
+// turns exposes every candidate to the same clock and power state, so a drift during the
+// tuning does not favour the candidates timed last.
+constexpr int kRounds = 5;
+
 template <class S, class Fn>
-int time_launch(S &s, Fn &&fn, float *ms) {
-  *ms = -1.f;
-  if (fn() != KM_OK || hipGetLastError() != hipSuccess) {
-    (void)hipGetLastError();
-    return KM_OK;
+int time_candidates(S &s, int n, Fn &&fn, std::vector<float> &ms) {
+  ms.assign(n, -1.f);
+  std::vector<char> ok(n, 0);
+  for (int k = 0; k < n; ++k) {
+    if (fn(k) != KM_OK || hipGetLastError() != hipSuccess) {
+      (void)hipGetLastError();
+      continue;
+    }
+    ok[k] = 1;
   }
+  if (int rc = check_launch()) return rc;
   hipEvent_t e0 = s.ev(), e1 = s.ev();
-  float best = 0.f;
-  for (int r = 0; r < 3; ++r) {
-    KM_HIP(hipEventRecord(e0, s.stream));
-    if (int rc = fn()) return rc;
-    KM_HIP(hipEventRecord(e1, s.stream));
-    KM_HIP(hipEventSynchronize(e1));
-    float t = 0.f;
-    KM_HIP(hipEventElapsedTime(&t, e0, e1));
-    best = r == 0 ? t : std::min(best, t);
-  }
+  for (int r = 0; r < kRounds; ++r)
+    for (int k = 0; k < n; ++k) {
+      if (!ok[k]) continue;
+      KM_HIP(hipEventRecord(e0, s.stream));
+      if (int rc = fn(k)) return rc;
+      KM_HIP(hipEventRecord(e1, s.stream));
+      KM_HIP(hipEventSynchronize(e1));
+      float t = 0.f;
+      KM_HIP(hipEventElapsedTime(&t, e0, e1));
+      ms[k] = ms[k] < 0.f ? t : std::min(ms[k], t);
+    }
   s.ev_free.push_back(e0);
   s.ev_free.push_back(e1);
-  if (int rc = check_launch()) return rc;
-  *ms = best;
-  return KM_OK;
+  return check_launch();
 }
 
 // s: a simulation or a stage plan; a: the arguments of a stage on its data
@@ -700,6 +709,7 @@ int autotune(S &s, const FusedArgs &a) {
   const int ndim = s.mesh.ndim;
   const int nshape = tile_shapes(cfg);
   std::lock_guard<std::mutex> lock(g_tune_mu);
+  std::vector<float> ms;
   for (int dir = 0; dir < 3; ++dir) {
     if (s.kern[dir] >= 0) continue;
     if (s.path != KM_PATH_FUSED || dir >= ndim) {
@@ -714,19 +724,16 @@ int autotune(S &s, const FusedArgs &a) {
       s.kern[dir] = it->second;
       continue;
     }
+    // candidates: the march (0) and each tile shape (1..nshape); block trees and CT
+    // stages run tiles only
+    const int k0 = s.amr || cfg.mhd == KM_MHD_CT ? 1 : 0;
+    if (int rc = time_candidates(
+            s, nshape + 1 - k0, [&](int i) { return launch_sweep(s, a, dir, false, k0 + i); },
+            ms))
+      return rc;
     int best = -1;
-    float best_ms = 0.f;
-    // block trees and CT stages run tiles only
-    for (int k = s.amr || cfg.mhd == KM_MHD_CT ? 1 : 0; k <= nshape; ++k) {
-      float ms = 0.f;
-      if (int rc = time_launch(s, [&] { return launch_sweep(s, a, dir, false, k); }, &ms))
-        return rc;
-      if (ms < 0.f) continue;  // does not fit the blocks
-      if (best < 0 || ms < best_ms) {
-        best = k;
-        best_ms = ms;
-      }
-    }
+    for (int i = 0; i < (int)ms.size(); ++i)
+      if (ms[i] >= 0.f && (best < 0 || ms[i] < ms[best - k0])) best = k0 + i;
     if (best < 0) {
       set_error("no fused sweep kernel fits these blocks");
       return KM_EINVAL;
@@ -734,7 +741,8 @@ int autotune(S &s, const FusedArgs &a) {
     g_tune[key] = best;
     s.kern[dir] = best;
   }
-  // the whole stage: the three tuned sweeps against each one-launch stage variant
+  // the whole stage: the three tuned sweeps (candidate 0) against each one-launch stage
+  // variant (1..)
   if (s.stage_kern < 0) {
     if (!stage3_applies(s)) {
       s.stage_kern = 0;
@@ -747,22 +755,19 @@ int autotune(S &s, const FusedArgs &a) {
       s.stage_kern = it->second;
       return KM_OK;
     }
-    int best = 0;
-    float best_ms = 0.f;
-    if (int rc = time_launch(s, [&] {
-          for (int dir = 0; dir < ndim; ++dir)
-            if (int r = launch_sweep(s, a, dir, false, s.kern[dir])) return r;
-          return (int)KM_OK;
-        }, &best_ms))
+    if (int rc = time_candidates(
+            s, stage3_kernels(cfg, ndim) + 1,
+            [&](int k) {
+              if (k > 0) return launch_stage3(s, a, false, k);
+              for (int dir = 0; dir < ndim; ++dir)
+                if (int r = launch_sweep(s, a, dir, false, s.kern[dir])) return r;
+              return (int)KM_OK;
+            },
+            ms))
       return rc;
-    for (int k = 1; k <= stage3_kernels(cfg, ndim); ++k) {
-      float ms = 0.f;
-      if (int rc = time_launch(s, [&] { return launch_stage3(s, a, false, k); }, &ms)) return rc;
-      if (ms >= 0.f && (best_ms < 0.f || ms < best_ms)) {
-        best = k;
-        best_ms = ms;
-      }
-    }
+    int best = 0;
+    for (int k = 1; k < (int)ms.size(); ++k)
+      if (ms[k] >= 0.f && (ms[best] < 0.f || ms[k] < ms[best])) best = k;
     g_tune[key] = best;
     s.stage_kern = best;
   }
