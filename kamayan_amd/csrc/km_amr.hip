@@ -298,11 +298,14 @@ __global__ void __launch_bounds__(256) k_amr_prolong(km_pack_t u, AmrGrid t, int
   const int L = blk.level;
   if (L == 0) return;
   const int nch = 1 << g.ndim;
-  // children: padded fine index, their mapped parities and flips
+  // children: padded fine index, their mapped parities and flips (bit 3 c + d of cfl: child
+  // c's physical boundary map flips axis d; bit masks rather than bool arrays, because
+  // the vector axis that reads them is a runtime value and an array indexed by one would
+  // live in scratch memory)
   long dc[8];
   int par[8];
   bool valid[8];
-  bool fl[8][3];
+  unsigned cfl = 0u;
   int first = -1;
   int G[3];
 #pragma unroll
@@ -310,7 +313,6 @@ __global__ void __launch_bounds__(256) k_amr_prolong(km_pack_t u, AmrGrid t, int
     valid[c] = false;
     dc[c] = 0;
     par[c] = 0;
-    fl[c][0] = fl[c][1] = fl[c][2] = false;
     if (c >= nch) continue;
     int fi[3], G0[3], Gc[3];
     bool in = true;
@@ -323,7 +325,9 @@ __global__ void __launch_bounds__(256) k_amr_prolong(km_pack_t u, AmrGrid t, int
     if (!in) continue;
     valid[c] = true;
     dc[c] = g.c(fi[2], fi[1], fi[0]);
-    map_zone(t, g, L, G0, Gc, fl[c]);
+    bool f3[3];
+    map_zone(t, g, L, G0, Gc, f3);
+    cfl |= (unsigned)((f3[0] ? 1 : 0) | (f3[1] ? 2 : 0) | (f3[2] ? 4 : 0)) << (3 * c);
     par[c] = (Gc[0] & 1) | (g.ndim > 1 ? (Gc[1] & 1) << 1 : 0) | (g.ndim > 2 ? (Gc[2] & 1) << 2 : 0);
     if (first < 0) {
       first = c;
@@ -343,16 +347,19 @@ __global__ void __launch_bounds__(256) k_amr_prolong(km_pack_t u, AmrGrid t, int
 #pragma unroll
   for (int d = 0; d < 3; ++d) C[d] = d < g.ndim ? (G[d] >> 1) : 0;
   Src nbm[3], nbp[3];
-  bool flm[3][3], flp[3][3];
+  unsigned flm = 0u, flp = 0u;  // bit 3 d + e: the neighbour along -d / +d flips axis e
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     if (d < g.ndim) {
       int Cm[3] = {C[0], C[1], C[2]}, Cp[3] = {C[0], C[1], C[2]}, M[3];
       Cm[d] -= 1;
       Cp[d] += 1;
-      map_zone(t, g, L - 1, Cm, M, flm[d]);
+      bool f3[3];
+      map_zone(t, g, L - 1, Cm, M, f3);
+      flm |= (unsigned)((f3[0] ? 1 : 0) | (f3[1] ? 2 : 0) | (f3[2] ? 4 : 0)) << (3 * d);
       nbm[d] = locate(u, t, g, L - 1, M);
-      map_zone(t, g, L - 1, Cp, M, flp[d]);
+      map_zone(t, g, L - 1, Cp, M, f3);
+      flp |= (unsigned)((f3[0] ? 1 : 0) | (f3[1] ? 2 : 0) | (f3[2] ? 4 : 0)) << (3 * d);
       nbp[d] = locate(u, t, g, L - 1, M);
     }
   }
@@ -371,8 +378,8 @@ __global__ void __launch_bounds__(256) k_amr_prolong(km_pack_t u, AmrGrid t, int
       if (d < g.ndim) {
         double fm = value_at(u, g, nbm[d], v, t.cyl != 0);
         double fp = value_at(u, g, nbp[d], v, t.cyl != 0);
-        if (ax >= 0 && flm[d][ax]) fm = -fm;
-        if (ax >= 0 && flp[d][ax]) fp = -fp;
+        if (ax >= 0 && ((flm >> (3 * d + ax)) & 1u)) fm = -fm;
+        if (ax >= 0 && ((flp >> (3 * d + ax)) & 1u)) fp = -fp;
         const double dxc = 2.0 * blk.dx[d];
         gr[d] = grad_minmod(fc, fm, fp, dxc, dxc);
       }
@@ -394,7 +401,7 @@ __global__ void __launch_bounds__(256) k_amr_prolong(km_pack_t u, AmrGrid t, int
       else if (!oj) y = fc + (a - bb + cc);
       else if (!oi) y = fc - (a - bb - cc);
       else y = fc + (a + bb + cc);
-      dst[dc[c]] = (ax >= 0 && fl[c][ax]) ? -y : y;
+      dst[dc[c]] = (ax >= 0 && ((cfl >> (3 * c + ax)) & 1u)) ? -y : y;
     }
   }
 }
