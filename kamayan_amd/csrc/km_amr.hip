@@ -872,15 +872,32 @@ void launch_amr_prolong_flags(const AmrGrid &t, const km_pack_t &u, int depth,
 
 void launch_amr_ghost_fill_list(const AmrFillList &f, const AmrGrid &t, const km_pack_t &u,
                                 int v0, int v1, int nc, hipStream_t st, int s0, int s1, int s2) {
+  launch_amr_ghost_fill_list(f, t, u, v0, v1, nc, st, s0, s1, s2, nullptr, nullptr, nullptr);
+}
+
+void launch_amr_ghost_fill_list(const AmrFillList &f, const AmrGrid &t, const km_pack_t &u,
+                                int v0, int v1, int nc, hipStream_t st, int s0, int s1, int s2,
+                                hipStream_t st2, hipEvent_t fork, hipEvent_t join) {
   if (s0 < v0 || s1 > v1 || s1 <= s0) s0 = s1 = v1;  // nothing left out
   if (s2 < s1 || s2 >= v1) s2 = -1;
   if (v0 >= v1 || fill_count(v0, v1, s0, s1, s2) <= 0) return;
+  const bool side = st2 && fork && join && f.n > 0 && f.np > 0;
+  hipStream_t sp = st;
+  if (side) {
+    (void)hipEventRecord(fork, st);
+    (void)hipStreamWaitEvent(st2, fork, 0);
+    sp = st2;
+  }
+  if (f.np > 0)
+    hipLaunchKernelGGL(k_amr_prolong, dim3((unsigned)((f.np + 255) / 256)), dim3(256), 0, sp, u,
+                       t, v0, v1, nc, s0, s1, f.depth, f.plist, f.np, nullptr, s2);
   if (f.n > 0)
     hipLaunchKernelGGL(k_amr_ghost_gather, dim3((unsigned)((f.n + 255) / 256)), dim3(256), 0, st,
                        u, f.list, f.n, v0, v1, nc, s0, s1, t.cyl, s2);
-  if (f.np > 0)
-    hipLaunchKernelGGL(k_amr_prolong, dim3((unsigned)((f.np + 255) / 256)), dim3(256), 0, st, u,
-                       t, v0, v1, nc, s0, s1, f.depth, f.plist, f.np, nullptr, s2);
+  if (side) {
+    (void)hipEventRecord(join, st2);
+    (void)hipStreamWaitEvent(st, join, 0);
+  }
 }
 
 void launch_amr_flux_correct(const AmrGrid &t, const km_pack_t &u, int nc, hipStream_t st) {

@@ -275,6 +275,14 @@ void launch_amr_prolong_flags(const AmrGrid &t, const km_pack_t &u, int depth,
 void launch_amr_ghost_fill_list(const AmrFillList &f, const AmrGrid &t, const km_pack_t &u,
                                 int v0, int v1, int nc, hipStream_t st, int s0 = 0, int s1 = 0,
                                 int s2 = -1);
+// The same with the prolongation on side stream st2, concurrent with the gather on st
+// (both read interior zones only and write disjoint ghost zones): fork is recorded on st
+// and waited for on st2, join recorded on st2 and waited for on st, so the fill is whole
+// on st afterwards (also inside a graph capture of st, as two branches). Sequential on st
+// when st2 is null or one of the two lists is empty.
+void launch_amr_ghost_fill_list(const AmrFillList &f, const AmrGrid &t, const km_pack_t &u,
+                                int v0, int v1, int nc, hipStream_t st, int s0, int s1, int s2,
+                                hipStream_t st2, hipEvent_t fork, hipEvent_t join);
 void launch_amr_flux_correct(const AmrGrid &t, const km_pack_t &u, int nc, hipStream_t st);
 // CT: face B ghost fill across levels and EMF correction at level jumps
 void launch_amr_ghost_faces(const AmrGrid &t, const km_pack_t &u, hipStream_t st);

@@ -265,6 +265,10 @@ struct km_sim_t {
   int *inst_d = nullptr;           // instance of each block (null for one instance)
   UniformGrid *grids_d = nullptr;  // grid of each instance
   hipStream_t stream = nullptr;
+  // fused path on a block tree: a second stream (default priority, like stream) and the
+  // fork / join events that let the stage's ghost-fill prolongation run beside its gather
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   double *dt_d = nullptr, *est_d = nullptr, *time_d = nullptr;
   // failure flag word (KM_FAIL_*) then the dt_min / dt_max cycle counters per instance
   int *fail_d = nullptr;
@@ -365,6 +369,9 @@ struct km_sim_t {
     for (auto e : ev_free) (void)hipEventDestroy(e);
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
+    if (join_ev) (void)hipEventDestroy(join_ev);
+    if (side) (void)hipStreamDestroy(side);
     if (stream) (void)hipStreamDestroy(stream);
     if (hist) std::fclose(hist);
   }
@@ -882,7 +889,7 @@ int amr_fused_stage(km_sim_t &s, int stage, double beta) {
   // reconstructed BMOD from PRES (their sweeps never read it)
   const int bmod = cfg.precision == KM_PREC_FP64 ? -1 : KM_VEL0(cfg.mhd) + 4;
   launch_amr_ghost_fill_list(s.fill, s.agrid, md1.pack, 0, KM_TEMP(cfg.mhd), s.nc, s.stream, 1,
-                             KM_NCONS(cfg.mhd), bmod);
+                             KM_NCONS(cfg.mhd), bmod, s.side, s.fork_ev, s.join_ev);
   s.lazy_ghosts = true;
   if (s.own_prims) s.lazy_prims = true;
   if (s.profile) {
@@ -1625,6 +1632,11 @@ static int sim_create(const km_cfg_t *cfg, const km_mesh_desc_t *meshes, int32_t
   }
   s->grid = s->grids_h[0];
   KM_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  if (s->path == KM_PATH_FUSED) {  // a uniform mesh may become a block tree (adaptive)
+    KM_HIP(hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking));
+    KM_HIP(hipEventCreateWithFlags(&s->fork_ev, hipEventDisableTiming));
+    KM_HIP(hipEventCreateWithFlags(&s->join_ev, hipEventDisableTiming));
+  }
   std::vector<km_block_t> blocks(s->nb);
   if (tree)
     blocks = tree->blocks;
