@@ -1031,9 +1031,21 @@ int ct_fused_stage(km_sim_t &s, int stage, double beta) {
   const Instances in{s.inst_d, s.grids_d};
   const int reach = fused_reach(cfg);
   const int bmod = cfg.precision == KM_PREC_FP64 ? -1 : KM_VEL0(cfg.mhd) + 4;
+  // the face fills run on the side stream beside the cell fills (different arrays, and
+  // each reads interiors only), joined before anything else on the simulation stream
+  const bool fork = s.side && s.fork_ev && s.join_ev;
+  if (fork) {
+    KM_HIP(hipEventRecord(s.fork_ev, s.stream));
+    KM_HIP(hipStreamWaitEvent(s.side, s.fork_ev, 0));
+  }
+  launch_ghost_fill(s.grid, md1.pack, 0, 0, s.nc, true, fork ? s.side : s.stream, in, reach, 1);
   launch_ghost_fill(s.grid, md1.pack, KM_DENS, KM_DENS + 1, s.nc, false, s.stream, in, reach);
-  launch_ghost_fill(s.grid, md1.pack, KM_MAGC0, KM_TEMP(cfg.mhd), s.nc, true, s.stream, in,
+  launch_ghost_fill(s.grid, md1.pack, KM_MAGC0, KM_TEMP(cfg.mhd), s.nc, false, s.stream, in,
                     reach, 1, bmod);
+  if (fork) {
+    KM_HIP(hipEventRecord(s.join_ev, s.side));
+    KM_HIP(hipStreamWaitEvent(s.stream, s.join_ev, 0));
+  }
   s.lazy_ghosts = true;
   s.lazy_prims = true;  // the zone pass leaves TEMP to PreparePrimitive
   if (s.profile) {
