@@ -98,15 +98,15 @@ MEASURED = {
             "march": (2.8341e9 + 0.6711e9, 1050.9),
         },
         "stage": {
-            "one-launch stage 16x16 columns": (2.5373e9 + 0.7301e9, 2675.6),
-            "one-launch stage 16x32 columns": (2.2527e9 + 0.6934e9, 2479.5),
-            "one-launch stage 32x16 columns": (1.9663e9 + 0.7062e9, 2507.0),
-            "one-launch stage 32x8 columns": (2.2381e9 + 0.7133e9, 2682.4),
-            "one-launch stage 64x4 columns": (2.6231e9 + 1.0537e9, 2921.3),
-            "one-launch stage 64x8 columns": (2.0331e9 + 0.7151e9, 2566.7),
+            "one-launch stage 16x16 columns": (2.5394e9 + 0.7301e9, 2675.6),
+            "one-launch stage 16x32 columns": (2.2434e9 + 0.6935e9, 2479.5),
+            "one-launch stage 32x16 columns": (1.9658e9 + 0.7061e9, 2507.0),
+            "one-launch stage 32x8 columns": (2.2380e9 + 0.7134e9, 2682.4),
+            "one-launch stage 64x4 columns": (2.6227e9 + 1.0536e9, 2921.3),
+            "one-launch stage 64x8 columns": (2.0329e9 + 0.7151e9, 2566.7),
         },
         "stage_dt": {
-            "one-launch stage 16x32 columns": (2.4464e9 + 0.6869e9, 2580.0),
+            "one-launch stage 16x32 columns": (2.4461e9 + 0.6870e9, 2551.0),
         },
     },
     "fp32_mixed": {
@@ -138,15 +138,15 @@ MEASURED = {
             "march": (3.0038e9 + 0.6761e9, 931.2),
         },
         "stage": {
-            "one-launch stage 16x16 columns": (3.5573e9 + 0.7688e9, 2469.2),
-            "one-launch stage 16x32 columns": (2.1789e9 + 0.6773e9, 2280.6),
-            "one-launch stage 32x16 columns": (1.9851e9 + 0.6821e9, 2309.4),
-            "one-launch stage 32x8 columns": (3.3083e9 + 0.7604e9, 2477.7),
-            "one-launch stage 64x4 columns": (2.2984e9 + 0.6882e9, 2713.0),
-            "one-launch stage 64x8 columns": (1.9756e9 + 0.6711e9, 2377.4),
+            "one-launch stage 16x16 columns": (3.5599e9 + 0.7695e9, 2469.2),
+            "one-launch stage 16x32 columns": (2.1774e9 + 0.6778e9, 2280.6),
+            "one-launch stage 32x16 columns": (1.9853e9 + 0.6822e9, 2309.4),
+            "one-launch stage 32x8 columns": (3.3074e9 + 0.7601e9, 2477.7),
+            "one-launch stage 64x4 columns": (2.2989e9 + 0.6882e9, 2713.0),
+            "one-launch stage 64x8 columns": (1.9758e9 + 0.6711e9, 2377.4),
         },
         "stage_dt": {
-            "one-launch stage 16x32 columns": (2.4486e9 + 0.6714e9, 2378.8),
+            "one-launch stage 16x32 columns": (2.4479e9 + 0.6718e9, 2349.8),
         },
     },
 }
