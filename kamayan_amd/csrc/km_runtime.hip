@@ -1014,10 +1014,8 @@ int ct_fused_stage(km_sim_t &s, int stage, double beta) {
            : cfg.precision == KM_PREC_FP64_FMA  ? fma::launch_ct_zone(a, s.stream)
                                                 : exact::launch_ct_zone(a, s.stream);
   if (rc) return rc;
-  if (stage == cfg.nstages) {
-    launch_estimate_dt(cfg, s.base.pack, s.est_d, s.stream, s.inst_d, s.ninst, s.fail_d);
-    launch_fill_divb(cfg, s.base.pack, s.stream);
-  }
+  const bool last = stage == cfg.nstages;
+  if (last) launch_fill_divb(cfg, s.base.pack, s.stream);
   hipEvent_t g0 = nullptr, g1 = nullptr;
   if (s.profile) {
     g0 = s.ev();
@@ -1031,13 +1029,17 @@ int ct_fused_stage(km_sim_t &s, int stage, double beta) {
   const Instances in{s.inst_d, s.grids_d};
   const int reach = fused_reach(cfg);
   const int bmod = cfg.precision == KM_PREC_FP64 ? -1 : KM_VEL0(cfg.mhd) + 4;
-  // the face fills run on the side stream beside the cell fills (different arrays, and
-  // each reads interiors only), joined before anything else on the simulation stream
+  // the face fills (and on the last stage the CFL estimate, which reads interior cells
+  // only) run on the side stream beside the cell fills (different arrays, and each reads
+  // interiors only), joined before anything else on the simulation stream
   const bool fork = s.side && s.fork_ev && s.join_ev;
   if (fork) {
     KM_HIP(hipEventRecord(s.fork_ev, s.stream));
     KM_HIP(hipStreamWaitEvent(s.side, s.fork_ev, 0));
   }
+  if (last)
+    launch_estimate_dt(cfg, s.base.pack, s.est_d, fork ? s.side : s.stream, s.inst_d, s.ninst,
+                       s.fail_d);
   launch_ghost_fill(s.grid, md1.pack, 0, 0, s.nc, true, fork ? s.side : s.stream, in, reach, 1);
   launch_ghost_fill(s.grid, md1.pack, KM_DENS, KM_DENS + 1, s.nc, false, s.stream, in, reach);
   launch_ghost_fill(s.grid, md1.pack, KM_MAGC0, KM_TEMP(cfg.mhd), s.nc, false, s.stream, in,
